@@ -1,0 +1,188 @@
+/*
+ * smoq.h — C ABI of the MI355X (gfx950) state-vector kernels.
+ *
+ * One shared library, libsmoq_amd.so, exports only the functions below.
+ * Every function returns an int status (SMOQ_OK = 0, negative on error);
+ * smoq_last_error() returns a thread-local message for the last failure on
+ * the calling thread.  No C++ exception crosses this boundary.
+ *
+ * Conventions (fixed by the reference, SmoQ.jl):
+ *  - Qubits are 0-based here; qubit q is bit q of the amplitude index
+ *    (little-endian; the reference's 1-based qubit k is bit k-1,
+ *    utils/cpu/cpuQuantumChannelGates.jl:48-51).
+ *  - A state holds `batch` independent vectors of 2^n amplitudes, stored
+ *    back to back (item b at offset b*2^n), interleaved (re, im): complex128
+ *    = 16 B, complex64 = 8 B per amplitude.  This is the column-major
+ *    dim x n_traj layout of the reference's MCWF batches
+ *    (utils/cpu/cpuQuantumChannelLindbladianEvolutionMonteCarloWaveFunction.jl:326-345).
+ *  - Matrices are row-major complex, (re, im) pairs.  A 2-qubit matrix acts
+ *    on the basis index b(q[0]) + 2*b(q[1]) — the |00>,|10>,|01>,|11> order of
+ *    apply_2qubit_gate! (utils/cpu/cpuQuantumChannelRandomUnitaries.jl:300-302).
+ *  - Kernels run asynchronously on the context's stream; only calls that
+ *    return host data (reductions, samples, downloads) synchronise.  Calls on
+ *    one context must be serialised by the caller.
+ */
+#ifndef SMOQ_H
+#define SMOQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMOQ_ABI_VERSION 1
+
+enum smoq_status {
+  SMOQ_OK = 0,
+  SMOQ_EINVAL = -1,    /* bad argument (qubit out of range, size mismatch ...) */
+  SMOQ_ENOMEM = -2,    /* device allocation failed or request too large */
+  SMOQ_EDEVICE = -3,   /* HIP runtime error */
+  SMOQ_ENUMERIC = -4,  /* non-finite or zero norm where a positive one is needed */
+  SMOQ_ENOTSUP = -5    /* valid request this build does not handle */
+};
+
+enum smoq_dtype { SMOQ_C128 = 0, SMOQ_C64 = 1 };
+
+/* Gate identifiers for smoq_apply_named and smoq_op.gate.  Matrices follow
+ * utils/cpu/cpuQuantumChannelGates.jl:53-71 and the per-gate docstrings. */
+enum smoq_gate_id {
+  SMOQ_G_U1 = 0,         /* generic 2x2 on q[0], m[0..7] */
+  SMOQ_G_U2 = 1,         /* generic 4x4 on (q[0], q[1]), m[0..31] */
+  SMOQ_G_H = 2,
+  SMOQ_G_X = 3,
+  SMOQ_G_Y = 4,
+  SMOQ_G_Z = 5,
+  SMOQ_G_S = 6,
+  SMOQ_G_SDG = 7,
+  SMOQ_G_T = 8,
+  SMOQ_G_TDG = 9,
+  SMOQ_G_RX = 10,        /* exp(-i theta X/2) */
+  SMOQ_G_RY = 11,
+  SMOQ_G_RZ = 12,
+  SMOQ_G_CNOT = 13,      /* q[0] control, q[1] target */
+  SMOQ_G_CZ = 14,
+  SMOQ_G_CRY = 15,       /* q[0] control, q[1] target */
+  SMOQ_G_RXX = 16,       /* exp(-i theta XX/2) */
+  SMOQ_G_RYY = 17,
+  SMOQ_G_RZZ = 18,
+  SMOQ_G_CCZ = 19,
+  SMOQ_G_TOFFOLI = 20,   /* q[0], q[1] controls, q[2] target */
+  SMOQ_G_SIGMA_MINUS = 21, /* |0><1| (non-unitary jump) */
+  SMOQ_G_SIGMA_PLUS = 22,  /* |1><0| */
+  SMOQ_G_DIAG1 = 23,     /* diag(m[0..1], m[2..3]) on q[0] */
+  SMOQ_G_DIAG2 = 24,     /* diagonal, entry b(q0)+2b(q1) = m[2e..2e+1] */
+  SMOQ_G_COUNT = 25
+};
+
+/* One operation of a gate tape (a circuit).  For the rotation gates
+ * (RX, RY, RZ, CRY, RXX, RYY, RZZ) the angle is theta[param] when
+ * param >= 0 and the `theta` field otherwise. */
+typedef struct smoq_op {
+  int32_t gate;
+  int32_t q[3];
+  int32_t param;
+  int32_t flags;         /* reserved, must be 0 */
+  double theta;
+  double m[32];
+} smoq_op;
+
+typedef struct smoq_ctx_s smoq_ctx;
+typedef struct smoq_state_s smoq_state;
+
+/* ---- library / context ------------------------------------------------ */
+/* No reference counterpart: the reference runs in-process on host arrays. */
+const char* smoq_last_error(void);
+int smoq_version(void);
+int smoq_init(smoq_ctx** out, int device);
+int smoq_fini(smoq_ctx* ctx);
+int smoq_sync(smoq_ctx* ctx);
+/* Fused-pass geometry: tile_qubits (amplitudes per LDS tile = 2^tile_qubits,
+ * 6..12) and low_qubits (contiguous low bits kept in every tile, 3..6).
+ * Chosen by the autotuner; 0 keeps the current value. */
+int smoq_set_fusion(smoq_ctx* ctx, int tile_qubits, int low_qubits);
+/* Mean kernel time (ms) and launch count of the fused-pass kernel since the
+ * last call, measured with HIP events on the context stream when timing is
+ * enabled (enable != 0).  Used to report the per-launch roofline. */
+int smoq_pass_timing(smoq_ctx* ctx, int enable, double* mean_ms, int64_t* launches);
+
+/* ---- state lifecycle --------------------------------------------------
+ * Replaces: zeros(ComplexF64, 2^N); ψ[1] = 1 and make_ket / make_initial_psi
+ * (utils/cpu/cpuQuantumStatePreparation.jl:208-243,1211-1229) for basis kets. */
+int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_state** out);
+int smoq_state_free(smoq_state* s);
+int smoq_state_info(const smoq_state* s, int* n_qubits, int* dtype, int* batch, void** device_ptr);
+int smoq_state_set_basis(smoq_state* s, uint64_t index);
+int smoq_state_upload(smoq_state* s, const void* host, size_t bytes);
+int smoq_state_download(const smoq_state* s, void* host, size_t bytes);
+int smoq_state_copy(smoq_state* dst, const smoq_state* src);
+
+/* ---- gates --------------------------------------------------------------
+ * smoq_apply_1q replaces apply_one_qubit_gate_bitwise_cpu!
+ *   (utils/cpu/cpuQuantumChannelUnitaryEvolutionTrotter.jl:77-99) and
+ *   apply_gate! (utils/cpu/cpuQuantumChannelRandomUnitaries.jl:230).
+ * smoq_apply_2q replaces apply_2qubit_gate! (cpuQuantumChannelRandomUnitaries.jl:317-370);
+ *   apply_two_qubit_gate_bitwise_cpu! (…Trotter.jl:107-142) is the call with
+ *   (min(q1,q2), max(q1,q2)).
+ * smoq_apply_named replaces every apply_*_psi! of
+ *   utils/cpu/cpuQuantumChannelGates.jl:142-1100 (rx, ry, rz, hadamard,
+ *   pauli_x/y/z, s, sdagger, t, tdagger, ccz, toffoli, sigma_minus/plus,
+ *   cnot, cz, cry, rxx, ryy, rzz). */
+int smoq_apply_1q(smoq_state* s, int q, const double u[8]);
+int smoq_apply_2q(smoq_state* s, int qa, int qb, const double u[32]);
+int smoq_apply_named(smoq_state* s, int gate, const int* qubits, double theta);
+/* Fused application of a whole gate tape: gates are grouped into HBM passes
+ * over 2^tile_qubits-amplitude LDS tiles.  Replaces apply_circuit!
+ * (utils/cpu/cpuVariationalQuantumCircuitExecutor.jl:95-117) and
+ * fuse_gates + run_fused_circuit!
+ * (SmoQ.jl_benchmarks/benchmark_random_circuit_sampling/run_benchmark_smoq_Float64.jl:167-200,356-369).
+ * theta may be NULL when no op has param >= 0.  Returns the number of HBM
+ * passes used in *n_passes when it is not NULL. */
+int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops,
+                    const double* theta, int n_theta, int* n_passes);
+
+/* ---- observables --------------------------------------------------------
+ * <P_t> for Pauli strings P_t = i^{|x_t & z_t|} X^{x_t} Z^{z_t} (x/z bit
+ * masks; Y on qubit q sets bit q in both).  out has batch*n_terms entries
+ * (item-major).  True expectation values: for strings with X/Y this is twice
+ * what get_expectation_pauli_string returns (that function omits the
+ * factor 2, utils/cpu/cpuQuantumStateObservables.jl:1044).
+ * Replaces expect_local / expect_corr / _expect_pauli_Z_string /
+ * _expect_pauli_mixed_string (cpuQuantumStateObservables.jl:175-445,840-1049)
+ * and compute_energy term by term
+ * (scripts_variational_quantum_circuits/demo_variational_quantum_eigensolver.jl:251-289). */
+int smoq_expect_pauli(smoq_state* s, int n_terms, const uint64_t* xmask,
+                      const uint64_t* zmask, double* out);
+/* The value get_expectation_pauli_string returns
+ * (utils/cpu/cpuQuantumStateObservables.jl:840-1049): for x_t = 0 the true
+ * <Z^z>; for x_t != 0 only bras with every X/Y bit 0 paired with the ket
+ * that has every X/Y bit 1,
+ *   Re[(-i)^{|x&z|} sum_{i & x = 0} (-1)^{|i & z|} conj(psi_i) psi_{i|x}]
+ * (= <P>/2 for a single X/Y operator; its :1044).  Kept for drop-in parity. */
+int smoq_pauli_string_reference(smoq_state* s, int n_terms, const uint64_t* xmask,
+                                const uint64_t* zmask, double* out);
+/* sum_i |psi_i|^2 per batch item (FP64 accumulation). */
+int smoq_norm2(smoq_state* s, double* out);
+
+/* ---- measurement sampling ---------------------------------------------- */
+enum smoq_sample_semantics {
+  /* idx = searchsortedfirst(cumsum(abs2.(ψ)), r) - 1 with Julia's pairwise
+   * cumsum, run_benchmark_smoq_Float64.jl:424-429.  When r exceeds the last
+   * prefix the reference gets idx = 2^N, whose N-bit string is all zeros;
+   * out_index then holds 2^N. */
+  SMOQ_SAMPLE_CUMSUM_SEARCH = 0,
+  /* first i with r <= sequential prefix, else the last index:
+   * sample_from_probabilities, utils/cpu/cpuQuantumStateMeasurements.jl:203-213
+   * (0-based result). */
+  SMOQ_SAMPLE_SEQUENTIAL = 1
+};
+/* One draw per uniform r[d] from batch item 0; out_index[d] 0-based. */
+int smoq_sample(smoq_state* s, const double* uniforms, int64_t n_draws,
+                int semantics, uint64_t* out_index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMOQ_H */

@@ -1,0 +1,270 @@
+"""CPU oracle for the SmoQ.jl state-vector path — TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings to ``liboracle.so`` (built from ``smoq_oracle.c`` by the
+Makefile next to this file) plus small numpy helpers.  Only the tests, the
+smoke check and bench.py's CPU-baseline leg use this module, and only as the
+checker: the product never imports it.
+
+States are complex128 numpy arrays; qubit arguments are 1-based like the
+reference functions each wrapper restates (see the C file for the
+file:line each loop follows).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_P = ctypes.c_void_p
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "or_apply_rx": [_P, _i, _d, _i],
+            "or_apply_ry": [_P, _i, _d, _i],
+            "or_apply_rz": [_P, _i, _d, _i],
+            "or_apply_h": [_P, _i, _i],
+            "or_apply_x": [_P, _i, _i],
+            "or_apply_y": [_P, _i, _i],
+            "or_apply_phase1": [_P, _i, _d, _d, _i],
+            "or_apply_cnot": [_P, _i, _i, _i],
+            "or_apply_cz": [_P, _i, _i, _i],
+            "or_apply_cry": [_P, _i, _i, _d, _i],
+            "or_apply_rxx_ryy": [_P, _i, _i, _d, _i, _i],
+            "or_apply_rzz": [_P, _i, _i, _d, _i],
+            "or_apply_ccz": [_P, _i, _i, _i, _i],
+            "or_apply_toffoli": [_P, _i, _i, _i, _i],
+            "or_apply_sigma": [_P, _i, _i, _i],
+            "or_apply_1q": [_P, _P, _i, _i],
+            "or_apply_2q_trotter": [_P, _P, _i, _i, _i],
+            "or_apply_fused_1q": [_P, _P, _i, _i],
+            "or_expect_local": [_P, _i, _i, ctypes.c_char],
+            "or_expect_corr": [_P, _i, _i, _i, ctypes.c_char],
+            "or_pauli_string_ref": [_P, ctypes.c_char_p, _P, _i, _i],
+            "or_compute_energy": [_P, _i, _d, _d, _d, _d],
+            "or_probs": [_P, _i64, _P],
+            "or_cumsum_pairwise": [_P, _i64, _P],
+            "or_cumsum_sequential": [_P, _i64, _P],
+            "or_searchsortedfirst": [_P, _i64, _d],
+            "or_sample_cumsum_search": [_P, _i, _P, _i64, _P],
+            "or_sample_sequential": [_P, _i, _P, _i64, _P],
+            "or_hpsi": [_P, _P, _i, _i, _P, _P, _P, _P],
+            "or_mcwf_decay_jump": [_P, _i, _d, _P, _d, _P],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
+        for name in ("or_expect_local", "or_expect_corr", "or_pauli_string_ref",
+                     "or_compute_energy"):
+            getattr(L, name).restype = _d
+        L.or_searchsortedfirst.restype = _i64
+        L.or_mcwf_decay_jump.restype = _i
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c128(psi):
+    assert psi.dtype == np.complex128 and psi.flags["C_CONTIGUOUS"]
+    return psi
+
+
+# ------------------------------------------------------------------ gates --
+def apply_named(psi, gate: str, qubits, theta: float, N: int):
+    """In-place gate on a complex128 state; qubits 1-based."""
+    L, p = lib(), _p(_c128(psi))
+    q = list(qubits)
+    if gate == "rx":
+        L.or_apply_rx(p, q[0], theta, N)
+    elif gate == "ry":
+        L.or_apply_ry(p, q[0], theta, N)
+    elif gate == "rz":
+        L.or_apply_rz(p, q[0], theta, N)
+    elif gate == "h":
+        L.or_apply_h(p, q[0], N)
+    elif gate == "x":
+        L.or_apply_x(p, q[0], N)
+    elif gate == "y":
+        L.or_apply_y(p, q[0], N)
+    elif gate == "z":
+        L.or_apply_phase1(p, q[0], -1.0, 0.0, N)
+    elif gate == "s":
+        L.or_apply_phase1(p, q[0], 0.0, 1.0, N)
+    elif gate == "sdg":
+        L.or_apply_phase1(p, q[0], 0.0, -1.0, N)
+    elif gate == "t":
+        L.or_apply_phase1(p, q[0], np.cos(np.pi / 4), np.sin(np.pi / 4), N)
+    elif gate == "tdg":
+        L.or_apply_phase1(p, q[0], np.cos(np.pi / 4), -np.sin(np.pi / 4), N)
+    elif gate == "cnot":
+        L.or_apply_cnot(p, q[0], q[1], N)
+    elif gate == "cz":
+        L.or_apply_cz(p, q[0], q[1], N)
+    elif gate == "cry":
+        L.or_apply_cry(p, q[0], q[1], theta, N)
+    elif gate == "rxx":
+        L.or_apply_rxx_ryy(p, q[0], q[1], theta, 0, N)
+    elif gate == "ryy":
+        L.or_apply_rxx_ryy(p, q[0], q[1], theta, 1, N)
+    elif gate == "rzz":
+        L.or_apply_rzz(p, q[0], q[1], theta, N)
+    elif gate == "ccz":
+        L.or_apply_ccz(p, q[0], q[1], q[2], N)
+    elif gate == "toffoli":
+        L.or_apply_toffoli(p, q[0], q[1], q[2], N)
+    elif gate == "sigma_minus":
+        L.or_apply_sigma(p, q[0], 0, N)
+    elif gate == "sigma_plus":
+        L.or_apply_sigma(p, q[0], 1, N)
+    else:
+        raise ValueError(gate)
+    return psi
+
+
+def apply_1q(psi, U, q: int, N: int):
+    u = np.ascontiguousarray(np.asarray(U, np.complex128).reshape(2, 2))
+    lib().or_apply_1q(_p(_c128(psi)), _p(u), q, N)
+    return psi
+
+
+def apply_2q_trotter(psi, U, q1: int, q2: int, N: int):
+    u = np.ascontiguousarray(np.asarray(U, np.complex128).reshape(4, 4))
+    lib().or_apply_2q_trotter(_p(_c128(psi)), _p(u), q1, q2, N)
+    return psi
+
+
+def apply_fused_1q(psi, U, q: int, N: int):
+    u = np.ascontiguousarray(np.asarray(U, np.complex128).reshape(2, 2))
+    lib().or_apply_fused_1q(_p(_c128(psi)), _p(u), q, N)
+    return psi
+
+
+# ------------------------------------------------------------ observables --
+def expect_local(psi, k: int, N: int, pauli: str) -> float:
+    return lib().or_expect_local(_p(_c128(psi)), k, N, pauli.encode())
+
+
+def expect_corr(psi, i: int, j: int, N: int, pair: str) -> float:
+    return lib().or_expect_corr(_p(_c128(psi)), i, j, N, pair[0].encode())
+
+
+def pauli_string_ref(psi, ops, qubits, N: int) -> float:
+    q = np.ascontiguousarray(np.asarray(qubits, dtype=np.int32))
+    s = "".join(ops).encode()
+    return lib().or_pauli_string_ref(_p(_c128(psi)), s, _p(q), len(ops), N)
+
+
+def compute_energy(psi, N, Jx, Jy, Jz, hx) -> float:
+    return lib().or_compute_energy(_p(_c128(psi)), N, Jx, Jy, Jz, hx)
+
+
+# --------------------------------------------------------------- sampling --
+def probs(psi):
+    out = np.empty(psi.size, np.float64)
+    lib().or_probs(_p(_c128(psi)), psi.size, _p(out))
+    return out
+
+
+def cumsum_pairwise(v):
+    v = np.ascontiguousarray(v, np.float64)
+    c = np.empty_like(v)
+    lib().or_cumsum_pairwise(_p(v), v.size, _p(c))
+    return c
+
+
+def cumsum_sequential(v):
+    v = np.ascontiguousarray(v, np.float64)
+    c = np.empty_like(v)
+    lib().or_cumsum_sequential(_p(v), v.size, _p(c))
+    return c
+
+
+def searchsortedfirst(v, x: float) -> int:
+    v = np.ascontiguousarray(v, np.float64)
+    return lib().or_searchsortedfirst(_p(v), v.size, float(x))
+
+
+def sample_cumsum_search(psi, N, r):
+    r = np.ascontiguousarray(r, np.float64)
+    out = np.empty(r.size, np.int64)
+    lib().or_sample_cumsum_search(_p(_c128(psi)), N, _p(r), r.size, _p(out))
+    return out
+
+
+def sample_sequential(psi, N, r):
+    r = np.ascontiguousarray(r, np.float64)
+    out = np.empty(r.size, np.int64)
+    lib().or_sample_sequential(_p(_c128(psi)), N, _p(r), r.size, _p(out))
+    return out
+
+
+# ----------------------------------------------------------- H psi, MCWF --
+def hpsi(psi, N, bonds, fields):
+    """bonds: list of (i, j, Jxx, Jyy, Jzz) 1-based; fields: N x (hx, hy, hz)."""
+    bi = np.ascontiguousarray([b[0] for b in bonds], np.int32)
+    bj = np.ascontiguousarray([b[1] for b in bonds], np.int32)
+    J = np.ascontiguousarray([b[2:5] for b in bonds], np.float64).reshape(-1)
+    h = np.ascontiguousarray(fields, np.float64).reshape(-1)
+    out = np.empty_like(psi)
+    lib().or_hpsi(_p(out), _p(_c128(psi)), N, len(bonds), _p(bi), _p(bj), _p(J), _p(h))
+    return out
+
+
+def mcwf_decay_jump(psi, N, gamma_dt, decay_pow, r):
+    dp = np.zeros(1)
+    d = np.ascontiguousarray(decay_pow, np.float64)
+    site = lib().or_mcwf_decay_jump(_p(_c128(psi)), N, gamma_dt, _p(d), float(r), _p(dp))
+    return site, float(dp[0])
+
+
+# ------------------------------------------------------ dense references --
+def dense_1q(U, q: int, N: int):
+    """Full 2^N matrix of a 1-qubit gate on 0-based qubit q (little-endian)."""
+    mats = [np.eye(2)] * N
+    mats = list(mats)
+    mats[q] = np.asarray(U, np.complex128)
+    M = np.array([[1.0 + 0j]])
+    for b in range(N - 1, -1, -1):
+        M = np.kron(M, mats[b])
+    return M
+
+
+def pauli_matrix(xmask: int, zmask: int, N: int):
+    """Dense i^{|x&z|} X^x Z^z (0-based bit masks)."""
+    X = np.array([[0, 1], [1, 0]], np.complex128)
+    Z = np.array([[1, 0], [0, -1]], np.complex128)
+    I = np.eye(2, dtype=np.complex128)
+    M = np.array([[1.0 + 0j]])
+    for b in range(N - 1, -1, -1):
+        m = I
+        if (xmask >> b) & 1 and (zmask >> b) & 1:
+            m = 1j * X @ Z
+        elif (xmask >> b) & 1:
+            m = X
+        elif (zmask >> b) & 1:
+            m = Z
+        M = np.kron(M, m)
+    return M

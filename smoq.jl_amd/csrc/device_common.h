@@ -1,0 +1,93 @@
+// Device-side helpers shared by the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace smoq {
+
+template <typename T>
+struct CT;
+template <>
+struct CT<double> {
+  using V = double2;
+  static constexpr int kRowBits = 3;  // 8 amplitudes per 128 B LDS row
+};
+template <>
+struct CT<float> {
+  using V = float2;
+  static constexpr int kRowBits = 4;  // 16 amplitudes per 128 B LDS row
+};
+
+template <typename T>
+__host__ __device__ inline typename CT<T>::V make_c(double re, double im) {
+  typename CT<T>::V v;
+  v.x = (T)re;
+  v.y = (T)im;
+  return v;
+}
+
+template <typename V>
+__device__ inline V cmul(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+template <typename V>
+__device__ inline V cadd(V a, V b) {
+  V r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+
+// u0*a + u1*b
+template <typename V>
+__device__ inline V cfma2(V u0, V a, V u1, V b) {
+  return cadd(cmul(u0, a), cmul(u1, b));
+}
+
+template <typename T>
+struct Mat2 {
+  typename CT<T>::V m[4];
+};
+template <typename T>
+struct Mat4 {
+  typename CT<T>::V m[16];
+};
+template <typename T>
+struct Diag8 {
+  typename CT<T>::V p[8];
+};
+
+// index with a zero bit inserted at position q
+__device__ inline uint64_t insert_zero(uint64_t t, int q) {
+  const uint64_t low = t & ((1ull << q) - 1ull);
+  return ((t >> q) << (q + 1)) | low;
+}
+
+// scatter the low bits of v into the set bits of mask (ascending)
+__device__ inline uint32_t pdep32(uint32_t v, uint32_t mask) {
+  uint32_t r = 0;
+  while (mask) {
+    uint32_t low = mask & (0u - mask);
+    if (v & 1u) r |= low;
+    v >>= 1;
+    mask ^= low;
+  }
+  return r;
+}
+
+// XOR swizzle of an LDS element index: the low WB bits (position inside a
+// 128 B row) are XORed with a fold of all higher WB-bit groups.  A bijection
+// on any power-of-two range; strided gathers spread over all row slots.
+template <int WB>
+__device__ inline uint32_t swz(uint32_t l) {
+  uint32_t f = (l >> WB) ^ (l >> (2 * WB)) ^ (l >> (3 * WB)) ^ (l >> (4 * WB));
+  return l ^ (f & ((1u << WB) - 1u));
+}
+
+}  // namespace smoq

@@ -1,0 +1,143 @@
+// Internal declarations shared by the HIP translation units of libsmoq_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "smoq.h"
+
+namespace smoq {
+
+// ---------------------------------------------------------------- errors --
+int set_error(int code, const char* fmt, ...);
+int hip_error(hipError_t e, const char* what);
+
+#define SMOQ_HIP(call)                                  \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return smoq::hip_error(_e, #call); \
+  } while (0)
+
+#define SMOQ_CHECK_LAUNCH(name)                         \
+  do {                                                  \
+    hipError_t _e = hipGetLastError();                  \
+    if (_e != hipSuccess) return smoq::hip_error(_e, name); \
+  } while (0)
+
+// ---------------------------------------------------- internal gate form --
+// Every gate of the public API is lowered to one of these before it reaches
+// a kernel.  Targets must be local to a fused tile; control and diagonal
+// qubits may sit anywhere because they only select or phase amplitudes.
+enum IKind : int32_t {
+  IK_U1 = 0,    // 2x2 on t0, applied where (index & ctrl) == ctrl
+  IK_U2 = 1,    // 4x4 on (t0 low role, t1 high role), same ctrl rule
+  IK_DIAG = 2,  // phase table over nd <= 3 global qubits dq[]
+};
+
+struct IGate {
+  int32_t kind = IK_U1;
+  int32_t t0 = -1, t1 = -1;
+  int32_t nd = 0;
+  int32_t dq[3] = {-1, -1, -1};
+  uint64_t ctrl = 0;
+  double m[32] = {0};
+};
+
+// Lowers one public op (angle already resolved) to internal gates.
+int lower_op(const smoq_op& op, double theta, int n_qubits, std::vector<IGate>& out);
+
+// ------------------------------------------------------- fused programs --
+constexpr int kRegQubits = 4;          // amplitudes per thread = 2^kRegQubits
+constexpr int kMaxTileQubits = 12;     // LDS tile of up to 4096 amplitudes
+constexpr int kMaxQubits = 40;
+
+struct GateDev {
+  int32_t kind;
+  int32_t a, b;        // register-qubit slots (U1: a; U2: a low, b high)
+  int32_t nd;
+  int32_t dq[3];       // diagonal: global bit positions
+  int32_t pad;
+  uint64_t ctrl;       // global control mask
+  double m[32];
+};
+
+struct StageDev {
+  int32_t q[kRegQubits];  // local bit positions held in registers
+  int32_t g0, g1;         // gate range [g0, g1) in the gate array
+};
+
+struct PassDev {
+  int32_t k;              // tile qubits
+  int32_t n_nonlocal;
+  int32_t s0, s1;         // stage range
+  int32_t map[kMaxTileQubits];       // local bit -> global bit
+  int32_t nonlocal[kMaxQubits];      // ascending global bits outside the tile
+};
+
+struct Program {
+  std::vector<PassDev> passes;
+  std::vector<StageDev> stages;
+  std::vector<GateDev> gates;
+};
+
+// Builds the pass/stage program for a gate list on an n-qubit state.
+void plan_program(const std::vector<IGate>& gates, int n_qubits, int tile_qubits,
+                  int low_qubits, Program& prog);
+
+}  // namespace smoq
+
+// ------------------------------------------------------------ handles --
+struct smoq_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int tile_qubits = 11;
+  int low_qubits = 5;
+  // grow-only device scratch, reused across calls on this context
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void* prog = nullptr;
+  size_t prog_bytes = 0;
+  void* host_pinned = nullptr;
+  size_t host_pinned_bytes = 0;
+  // fused-pass timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;   // begin/end pairs recorded on the stream
+  size_t ev_used = 0;
+  double timed_ms = 0.0;
+  int64_t timed_launches = 0;
+  // cached pairwise-cumsum tree (sampling), keyed by vector length
+  uint64_t tree_n = 0;
+  void* tree_dev = nullptr;
+  size_t tree_dev_bytes = 0;
+  std::vector<int64_t> tree_level_off;  // node offsets per level
+  int64_t tree_nodes = 0, tree_leaves = 0;
+};
+
+struct smoq_state_s {
+  smoq_ctx* ctx = nullptr;
+  int n = 0;
+  int dtype = SMOQ_C128;
+  int batch = 1;
+  uint64_t dim = 0;        // 2^n
+  void* d = nullptr;       // device amplitudes
+  size_t bytes = 0;
+  size_t amp_bytes() const { return dtype == SMOQ_C128 ? 16 : 8; }
+};
+
+namespace smoq {
+int ensure_scratch(smoq_ctx* ctx, size_t bytes);
+int ensure_prog(smoq_ctx* ctx, size_t bytes);
+int ensure_pinned(smoq_ctx* ctx, size_t bytes);
+int check_state(const smoq_state* s);
+int check_qubit(const smoq_state* s, int q, const char* what);
+// records the next pooled event on the context stream (timing mode only)
+int time_mark(smoq_ctx* ctx);
+
+// launchers implemented in the kernel translation units
+int launch_gate(smoq_state* s, const IGate& g);
+int run_program(smoq_state* s, const Program& prog);
+}  // namespace smoq

@@ -1,0 +1,115 @@
+"""ctypes binding of libsmoq_amd.so (the C ABI declared in include/smoq.h).
+
+The library is loaded from ``smoq.jl_amd/lib/libsmoq_amd.so`` inside this
+repository.  There is no CPU fallback: if the library is missing or no
+gfx950 device is visible, every call raises ``SmoqError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "lib", "libsmoq_amd.so"))
+
+SMOQ_OK = 0
+SMOQ_EINVAL = -1
+SMOQ_ENOMEM = -2
+SMOQ_EDEVICE = -3
+SMOQ_ENUMERIC = -4
+SMOQ_ENOTSUP = -5
+
+C128 = 0
+C64 = 1
+
+# smoq_gate_id
+G = dict(
+    U1=0, U2=1, H=2, X=3, Y=4, Z=5, S=6, SDG=7, T=8, TDG=9, RX=10, RY=11, RZ=12,
+    CNOT=13, CZ=14, CRY=15, RXX=16, RYY=17, RZZ=18, CCZ=19, TOFFOLI=20,
+    SIGMA_MINUS=21, SIGMA_PLUS=22, DIAG1=23, DIAG2=24,
+)
+ROTATIONS = {G["RX"], G["RY"], G["RZ"], G["CRY"], G["RXX"], G["RYY"], G["RZZ"]}
+
+SAMPLE_CUMSUM_SEARCH = 0
+SAMPLE_SEQUENTIAL = 1
+
+
+class SmoqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"smoq error {code}: {msg}")
+        self.code = code
+
+
+class smoq_op(ctypes.Structure):
+    _fields_ = [
+        ("gate", ctypes.c_int32),
+        ("q", ctypes.c_int32 * 3),
+        ("param", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("theta", ctypes.c_double),
+        ("m", ctypes.c_double * 32),
+    ]
+
+
+_P = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_u64 = ctypes.c_uint64
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+# name -> argtypes; every function returns int status unless listed in _RET
+SIGNATURES = {
+    "smoq_last_error": [],
+    "smoq_version": [],
+    "smoq_init": [ctypes.POINTER(_P), _i],
+    "smoq_fini": [_P],
+    "smoq_sync": [_P],
+    "smoq_set_fusion": [_P, _i, _i],
+    "smoq_pass_timing": [_P, _i, ctypes.POINTER(_d), ctypes.POINTER(_i64)],
+    "smoq_state_alloc": [_P, _i, _i, _i, ctypes.POINTER(_P)],
+    "smoq_state_free": [_P],
+    "smoq_state_info": [_P, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                        ctypes.POINTER(_P)],
+    "smoq_state_set_basis": [_P, _u64],
+    "smoq_state_upload": [_P, _P, _sz],
+    "smoq_state_download": [_P, _P, _sz],
+    "smoq_state_copy": [_P, _P],
+    "smoq_apply_1q": [_P, _i, _P],
+    "smoq_apply_2q": [_P, _i, _i, _P],
+    "smoq_apply_named": [_P, _i, _P, _d],
+    "smoq_apply_tape": [_P, _P, _i, _P, _i, ctypes.POINTER(_i)],
+    "smoq_expect_pauli": [_P, _i, _P, _P, _P],
+    "smoq_pauli_string_reference": [_P, _i, _P, _P, _P],
+    "smoq_norm2": [_P, _P],
+    "smoq_sample": [_P, _P, _i64, _i, _P],
+}
+_RET = {"smoq_last_error": ctypes.c_char_p, "smoq_version": _i}
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library (raises SmoqError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SmoqError(SMOQ_ENOTSUP,
+                            f"{LIB_PATH} is not built; run `make -C smoq.jl_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RET.get(name, _i)
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != SMOQ_OK:
+        msg = lib().smoq_last_error()
+        raise SmoqError(rc, msg.decode() if msg else "")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))

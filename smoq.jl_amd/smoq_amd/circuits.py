@@ -1,0 +1,156 @@
+"""Gate tapes (the op list the C ABI consumes) and the seeded workloads.
+
+A Tape is the device-facing form of the reference's circuit IR
+(`ParameterizedCircuit` / `CircuitOperation`,
+utils/cpu/cpuVariationalQuantumCircuitBuilder.jl:120-132,295-326) and of the
+benchmark's `GateOp` lists (run_benchmark_smoq_Float64.jl:114-119).  Qubits
+in a Tape are 0-based.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import streams
+from ._lib import G, ROTATIONS, call, smoq_op
+
+TWO_PI = 2.0 * math.pi
+_NQ = {"CNOT": 2, "CZ": 2, "CRY": 2, "RXX": 2, "RYY": 2, "RZZ": 2, "U2": 2, "DIAG2": 2,
+       "CCZ": 3, "TOFFOLI": 3}
+
+
+class Tape:
+    def __init__(self, n_qubits: int):
+        self.n = n_qubits
+        self.ops: list[tuple] = []  # (gate, qubits, theta, param, matrix)
+        self.n_params = 0
+
+    def add(self, gate: str, qubits, theta: float = 0.0, param: int = -1, m=None) -> "Tape":
+        gate = gate.upper()
+        qubits = tuple(int(q) for q in qubits)
+        if len(qubits) != _NQ.get(gate, 1):
+            raise ValueError(f"{gate} takes {_NQ.get(gate, 1)} qubits, got {qubits}")
+        if any(not 0 <= q < self.n for q in qubits):
+            raise ValueError(f"qubits {qubits} outside 0..{self.n - 1}")
+        if param >= 0:
+            self.n_params = max(self.n_params, param + 1)
+        self.ops.append((gate, qubits, float(theta), int(param), m))
+        return self
+
+    def __len__(self):
+        return len(self.ops)
+
+    def to_ctypes(self):
+        arr = (smoq_op * max(1, len(self.ops)))()
+        for i, (gate, qs, th, par, m) in enumerate(self.ops):
+            o = arr[i]
+            o.gate = G[gate]
+            for j in range(3):
+                o.q[j] = qs[j] if j < len(qs) else -1
+            o.param = par
+            o.flags = 0
+            o.theta = th
+            if m is not None:
+                flat = np.asarray(m, np.complex128).reshape(-1)
+                buf = np.empty(2 * flat.size)
+                buf[0::2] = flat.real
+                buf[1::2] = flat.imag
+                for j, v in enumerate(buf):
+                    o.m[j] = v
+        return arr
+
+    def apply(self, psi, theta=None) -> int:
+        """Applies the tape to a GPUState with fused passes; returns the
+        number of HBM passes."""
+        if psi.n != self.n:
+            raise ValueError(f"tape is for {self.n} qubits, state has {psi.n}")
+        arr = self.to_ctypes()
+        npass = ctypes.c_int(0)
+        if theta is None:
+            th_p, nth = None, 0
+        else:
+            th = np.ascontiguousarray(theta, np.float64)
+            th_p, nth = th.ctypes.data_as(ctypes.c_void_p), th.size
+            self._theta_keepalive = th
+        call("smoq_apply_tape", psi.handle, arr, len(self.ops), th_p, nth, ctypes.byref(npass))
+        return npass.value
+
+    def logical_gates(self) -> int:
+        return len(self.ops)
+
+    def rotation_params(self) -> list:
+        return [(i, op) for i, op in enumerate(self.ops) if G[op[0]] in ROTATIONS and op[3] >= 0]
+
+
+# ----------------------------------------------------------------- workloads --
+def layered_ry_rz_cz(N: int, layers: int, seed: int) -> Tape:
+    """Plain circuit on |0…0>: per layer Ry(θ) on every qubit, Rz(φ) on every
+    qubit, then CZ on (i, i+1) for even i, then for odd i.  Angles are
+    U[0, 2π) from the SplitMix64 stream: per layer the N θ's, then the N φ's."""
+    u = streams.uniforms(seed, 2 * N * layers)
+    t = Tape(N)
+    for l in range(layers):
+        for q in range(N):
+            t.add("RY", [q], TWO_PI * u[2 * N * l + q])
+        for q in range(N):
+            t.add("RZ", [q], TWO_PI * u[2 * N * l + N + q])
+        for q in range(0, N - 1, 2):
+            t.add("CZ", [q, q + 1])
+        for q in range(1, N - 1, 2):
+            t.add("CZ", [q, q + 1])
+    return t
+
+
+def random_rotation_brick(N: int, layers: int, seed: int) -> Tape:
+    """Random-circuit workload: per layer a rotation on every qubit with the
+    axis uniform in {X, Y, Z} and the angle uniform in [0, 2π) (draw order:
+    per qubit axis then angle), then CZ on brick pairs — (0,1),(2,3),… on
+    even layers, (1,2),(3,4),… on odd layers."""
+    u = streams.uniforms(seed, 2 * N * layers)
+    axes = ("RX", "RY", "RZ")
+    t = Tape(N)
+    for l in range(layers):
+        for q in range(N):
+            a = min(2, int(u[2 * (N * l + q)] * 3.0))
+            t.add(axes[a], [q], TWO_PI * u[2 * (N * l + q) + 1])
+        for q in range(l % 2, N - 1, 2):
+            t.add("CZ", [q, q + 1])
+    return t
+
+
+def hardware_efficient_ansatz(N: int, layers: int, entangler: str = "CZ") -> Tape:
+    """Parametrised ansatz: per layer Ry(θ_p), Rz(θ_{p+1}) on each qubit
+    (interleaved per qubit as build_ansatz_spec,
+    scripts_variational_quantum_circuits/demo_variational_quantum_eigensolver.jl:181-186),
+    then a CZ ladder (q, q+1), q = 0..N-2.  2·N·layers parameters, no final
+    rotation layer."""
+    t = Tape(N)
+    p = 0
+    for l in range(layers):
+        for q in range(N):
+            t.add("RY", [q], param=p)
+            t.add("RZ", [q], param=p + 1)
+            p += 2
+        for q in range(N - 1):
+            t.add(entangler, [q, q + 1])
+    return t
+
+
+def gate_matrix(gate: str, theta: float = 0.0) -> np.ndarray:
+    """2x2 matrices of the benchmark's gate set (run_benchmark_smoq_Float64.jl:86-98)."""
+    g = gate.upper()
+    c, s = math.cos(theta / 2), math.sin(theta / 2)
+    r = 1 / math.sqrt(2)
+    return {
+        "H": np.array([[r, r], [r, -r]], np.complex128),
+        "X": np.array([[0, 1], [1, 0]], np.complex128),
+        "Y": np.array([[0, -1j], [1j, 0]], np.complex128),
+        "Z": np.array([[1, 0], [0, -1]], np.complex128),
+        "S": np.array([[1, 0], [0, 1j]], np.complex128),
+        "T": np.array([[1, 0], [0, complex(math.cos(math.pi / 4), math.sin(math.pi / 4))]]),
+        "RX": np.array([[c, -1j * s], [-1j * s, c]], np.complex128),
+        "RY": np.array([[c, -s], [s, c]], np.complex128),
+        "RZ": np.array([[complex(c, -s), 0], [0, complex(c, s)]], np.complex128),
+    }[g]

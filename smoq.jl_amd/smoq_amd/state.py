@@ -1,0 +1,151 @@
+"""Device context and state vectors (GPUState) over the C ABI."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import C64, C128, call
+
+_DTYPES = {"c128": C128, "complex128": C128, C128: C128, np.complex128: C128,
+           "c64": C64, "complex64": C64, C64: C64, np.complex64: C64}
+_NP = {C128: np.complex128, C64: np.complex64}
+
+
+class Context:
+    """One device and one HIP stream (smoq_init / smoq_fini)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        call("smoq_init", ctypes.byref(h), device)
+        self.handle = h
+        self.device = device
+
+    def sync(self) -> None:
+        call("smoq_sync", self.handle)
+
+    def set_fusion(self, tile_qubits: int = 0, low_qubits: int = 0) -> None:
+        call("smoq_set_fusion", self.handle, tile_qubits, low_qubits)
+
+    def pass_timing(self, enable: bool):
+        """Returns (mean fused-pass kernel ms, launches) since the last call."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        call("smoq_pass_timing", self.handle, int(enable), ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if self.handle:
+            call("smoq_fini", self.handle)
+            self.handle = None
+
+
+_default: Context | None = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+class GPUState:
+    """`batch` state vectors of n qubits resident in device memory.
+
+    Mirrors the reference's `Vector{ComplexF64}` (batch = 1) and its
+    dim x n_traj trajectory matrices (batch > 1, one item per column).
+    """
+
+    def __init__(self, n_qubits: int, dtype="c128", batch: int = 1,
+                 ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.dtype = _DTYPES[dtype]
+        self.n = int(n_qubits)
+        self.batch = int(batch)
+        h = ctypes.c_void_p()
+        call("smoq_state_alloc", self.ctx.handle, self.n, self.dtype, self.batch,
+             ctypes.byref(h))
+        self.handle = h
+
+    # ---- construction helpers
+    @classmethod
+    def from_numpy(cls, psi: np.ndarray, dtype=None, ctx: Context | None = None) -> "GPUState":
+        psi = np.asarray(psi)
+        if dtype is None:
+            dtype = "c64" if psi.dtype == np.complex64 else "c128"
+        dt = _DTYPES[dtype]
+        arr = psi.reshape(-1, psi.shape[-1]) if psi.ndim > 1 else psi.reshape(1, -1)
+        dim = arr.shape[1]
+        n = dim.bit_length() - 1
+        if 1 << n != dim:
+            raise ValueError(f"length {dim} is not a power of two")
+        s = cls(n, dt, arr.shape[0], ctx)
+        s.upload(arr)
+        return s
+
+    @classmethod
+    def basis(cls, n_qubits: int, index: int = 0, dtype="c128", batch: int = 1,
+              ctx: Context | None = None) -> "GPUState":
+        s = cls(n_qubits, dtype, batch, ctx)
+        s.set_basis(index)
+        return s
+
+    # ---- properties
+    @property
+    def N(self) -> int:
+        return self.n
+
+    @property
+    def dim(self) -> int:
+        return 1 << self.n
+
+    @property
+    def np_dtype(self):
+        return _NP[self.dtype]
+
+    @property
+    def nbytes(self) -> int:
+        return self.dim * self.batch * np.dtype(self.np_dtype).itemsize
+
+    # ---- data movement
+    def set_basis(self, index: int) -> None:
+        call("smoq_state_set_basis", self.handle, ctypes.c_uint64(index))
+
+    def upload(self, psi: np.ndarray) -> None:
+        a = np.ascontiguousarray(np.asarray(psi, dtype=self.np_dtype)).reshape(-1)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"upload of {a.nbytes} bytes into a {self.nbytes}-byte state")
+        call("smoq_state_upload", self.handle, a.ctypes.data_as(ctypes.c_void_p),
+             ctypes.c_size_t(a.nbytes))
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.empty(self.dim * self.batch, dtype=self.np_dtype)
+        call("smoq_state_download", self.handle, out.ctypes.data_as(ctypes.c_void_p),
+             ctypes.c_size_t(out.nbytes))
+        return out if self.batch == 1 else out.reshape(self.batch, self.dim)
+
+    def copy_from(self, other: "GPUState") -> None:
+        call("smoq_state_copy", self.handle, other.handle)
+
+    def clone(self) -> "GPUState":
+        s = GPUState(self.n, self.dtype, self.batch, self.ctx)
+        s.copy_from(self)
+        return s
+
+    def norm2(self) -> np.ndarray:
+        out = np.empty(self.batch, np.float64)
+        call("smoq_norm2", self.handle, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def free(self) -> None:
+        if getattr(self, "handle", None):
+            _lib.lib().smoq_state_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
