@@ -91,3 +91,15 @@ __device__ inline uint32_t swz(uint32_t l) {
 }
 
 }  // namespace smoq
+
+namespace smoq {
+// component-wise select: a struct-level `c ? a : b` becomes a pointer select
+// plus copy, which keeps register arrays from being promoted out of scratch
+template <typename V>
+__device__ __forceinline__ V csel(bool c, V a, V b) {
+  V r;
+  r.x = c ? a.x : b.x;
+  r.y = c ? a.y : b.y;
+  return r;
+}
+}  // namespace smoq

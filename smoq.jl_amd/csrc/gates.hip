@@ -129,224 +129,3 @@ extern "C" int smoq_state_set_basis(smoq_state* s, uint64_t index) {
   SMOQ_CHECK_LAUNCH("k_set_basis");
   return SMOQ_OK;
 }
-
-namespace smoq {
-
-// ------------------------------------------------------ fused pass kernel --
-// One workgroup owns one tile: the 2^k amplitudes whose non-local bits equal
-// the tile id.  The tile is read from HBM into LDS once (coalesced runs of
-// 2^m amplitudes), every stage then moves 2^R amplitudes per thread from LDS
-// into registers, applies its gates there and writes them back, and the
-// tile is finally written to HBM once.  LDS rows are XOR-swizzled so that
-// the strided per-stage gathers stay free of bank conflicts.
-
-template <typename T, int A>
-__device__ inline void reg_u1(typename CT<T>::V (&x)[1 << kRegQubits], const Mat2<T>& u,
-                              const uint64_t (&gi)[1 << kRegQubits], uint64_t ctrl) {
-  using V = typename CT<T>::V;
-#pragma unroll
-  for (int j = 0; j < (1 << kRegQubits); ++j) {
-    if (j & (1 << A)) continue;
-    const int h = j | (1 << A);
-    V a = x[j], b = x[h];
-    V na = cfma2(u.m[0], a, u.m[1], b);
-    V nb = cfma2(u.m[2], a, u.m[3], b);
-    bool on = (gi[j] & ctrl) == ctrl;
-    x[j] = on ? na : a;
-    x[h] = on ? nb : b;
-  }
-}
-
-template <typename T, int A, int B>
-__device__ inline void reg_u2(typename CT<T>::V (&x)[1 << kRegQubits], const Mat4<T>& u,
-                              const uint64_t (&gi)[1 << kRegQubits], uint64_t ctrl) {
-  using V = typename CT<T>::V;
-#pragma unroll
-  for (int j = 0; j < (1 << kRegQubits); ++j) {
-    if (j & ((1 << A) | (1 << B))) continue;
-    const int id[4] = {j, j | (1 << A), j | (1 << B), j | (1 << A) | (1 << B)};
-    V in[4] = {x[id[0]], x[id[1]], x[id[2]], x[id[3]]};
-    bool on = (gi[j] & ctrl) == ctrl;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      V acc = cmul(u.m[4 * r], in[0]);
-#pragma unroll
-      for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(u.m[4 * r + c], in[c]));
-      x[id[r]] = on ? acc : in[r];
-    }
-  }
-}
-
-template <typename T>
-__device__ inline void apply_gate_regs(typename CT<T>::V (&x)[1 << kRegQubits],
-                                       const GateDev& g, const uint64_t (&gi)[1 << kRegQubits]) {
-  using V = typename CT<T>::V;
-  if (g.kind == IK_U1) {
-    Mat2<T> u;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) u.m[i] = make_c<T>(g.m[2 * i], g.m[2 * i + 1]);
-    switch (g.a) {
-      case 0: reg_u1<T, 0>(x, u, gi, g.ctrl); break;
-      case 1: reg_u1<T, 1>(x, u, gi, g.ctrl); break;
-      case 2: reg_u1<T, 2>(x, u, gi, g.ctrl); break;
-      default: reg_u1<T, 3>(x, u, gi, g.ctrl); break;
-    }
-  } else if (g.kind == IK_U2) {
-    Mat4<T> u;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) u.m[i] = make_c<T>(g.m[2 * i], g.m[2 * i + 1]);
-    switch (g.a * 4 + g.b) {
-      case 1: reg_u2<T, 0, 1>(x, u, gi, g.ctrl); break;
-      case 2: reg_u2<T, 0, 2>(x, u, gi, g.ctrl); break;
-      case 3: reg_u2<T, 0, 3>(x, u, gi, g.ctrl); break;
-      case 4: reg_u2<T, 1, 0>(x, u, gi, g.ctrl); break;
-      case 6: reg_u2<T, 1, 2>(x, u, gi, g.ctrl); break;
-      case 7: reg_u2<T, 1, 3>(x, u, gi, g.ctrl); break;
-      case 8: reg_u2<T, 2, 0>(x, u, gi, g.ctrl); break;
-      case 9: reg_u2<T, 2, 1>(x, u, gi, g.ctrl); break;
-      case 11: reg_u2<T, 2, 3>(x, u, gi, g.ctrl); break;
-      case 12: reg_u2<T, 3, 0>(x, u, gi, g.ctrl); break;
-      case 13: reg_u2<T, 3, 1>(x, u, gi, g.ctrl); break;
-      default: reg_u2<T, 3, 2>(x, u, gi, g.ctrl); break;
-    }
-  } else {
-    V p[8];
-    const int ne = 1 << g.nd;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) p[i] = make_c<T>(i < ne ? g.m[2 * i] : 1.0, i < ne ? g.m[2 * i + 1] : 0.0);
-    const int d0 = g.dq[0], d1 = g.nd > 1 ? g.dq[1] : 63, d2 = g.nd > 2 ? g.dq[2] : 63;
-#pragma unroll
-    for (int j = 0; j < (1 << kRegQubits); ++j) {
-      const uint64_t G = gi[j];
-      const int e = (int)((G >> d0) & 1) | ((int)((G >> d1) & 1) << 1) | ((int)((G >> d2) & 1) << 2);
-      V ph = p[0];
-#pragma unroll
-      for (int c = 1; c < 8; ++c) ph = (e == c) ? p[c] : ph;
-      x[j] = cmul(ph, x[j]);
-    }
-  }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_fused_pass(typename CT<T>::V* __restrict__ psi, int n, const PassDev* __restrict__ pass,
-                 const StageDev* __restrict__ stages, const GateDev* __restrict__ gates) {
-  using V = typename CT<T>::V;
-  constexpr int R = kRegQubits;
-  constexpr int NR = 1 << R;
-  constexpr int WB = CT<T>::kRowBits;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* lds = reinterpret_cast<V*>(smem_raw);
-
-  const int k = pass->k;
-  const int NT = 1 << (k - R);
-  const int tid = threadIdx.x;
-  const int nn = pass->n_nonlocal;
-  const uint64_t tile = blockIdx.x;
-  const uint64_t item = tile >> nn;
-  const uint64_t tt = tile & ((1ull << nn) - 1);
-  uint64_t gbase = 0;
-  for (int i = 0; i < nn; ++i) gbase |= ((tt >> i) & 1ull) << pass->nonlocal[i];
-  V* base = psi + (item << n);
-
-  // thread bits cover local positions 0..k-R-1, the loop index the top R
-  uint64_t off_tid = 0;
-  for (int j = 0; j < k - R; ++j) off_tid |= (uint64_t)((tid >> j) & 1) << pass->map[j];
-  uint64_t off_hi[NR];
-#pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    uint64_t o = 0;
-#pragma unroll
-    for (int j = 0; j < R; ++j) o |= (uint64_t)((i >> j) & 1) << pass->map[k - R + j];
-    off_hi[i] = o;
-  }
-  V x[NR];
-#pragma unroll
-  for (int i = 0; i < NR; ++i) x[i] = base[gbase | off_tid | off_hi[i]];
-#pragma unroll
-  for (int i = 0; i < NR; ++i) lds[swz<WB>((uint32_t)(tid + i * NT))] = x[i];
-  __syncthreads();
-
-  const uint32_t kmask = (1u << k) - 1u;
-  for (int s = pass->s0; s < pass->s1; ++s) {
-    const StageDev& st = stages[s];
-    uint32_t qm = 0;
-#pragma unroll
-    for (int j = 0; j < R; ++j) qm |= 1u << st.q[j];
-    const uint32_t lb = pdep32((uint32_t)tid, kmask & ~qm);
-    uint32_t lo[NR];
-#pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      uint32_t o = lb;
-#pragma unroll
-      for (int j = 0; j < R; ++j) o |= (uint32_t)((i >> j) & 1) << st.q[j];
-      lo[i] = o;
-    }
-#pragma unroll
-    for (int i = 0; i < NR; ++i) x[i] = lds[swz<WB>(lo[i])];
-    // global indices (diagonal phases and controls read them)
-    uint64_t gl = gbase;
-    for (int b = 0; b < k; ++b) gl |= (uint64_t)((lb >> b) & 1u) << pass->map[b];
-    uint64_t gq[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) gq[j] = 1ull << pass->map[st.q[j]];
-    uint64_t gi[NR];
-#pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      uint64_t G = gl;
-#pragma unroll
-      for (int j = 0; j < R; ++j) G |= ((i >> j) & 1) ? gq[j] : 0ull;
-      gi[i] = G;
-    }
-    for (int g = st.g0; g < st.g1; ++g) apply_gate_regs<T>(x, gates[g], gi);
-#pragma unroll
-    for (int i = 0; i < NR; ++i) lds[swz<WB>(lo[i])] = x[i];
-    __syncthreads();
-  }
-
-#pragma unroll
-  for (int i = 0; i < NR; ++i) x[i] = lds[swz<WB>((uint32_t)(tid + i * NT))];
-#pragma unroll
-  for (int i = 0; i < NR; ++i) base[gbase | off_tid | off_hi[i]] = x[i];
-}
-
-template <typename T>
-static int run_program_t(smoq_state* s, const Program& prog) {
-  using V = typename CT<T>::V;
-  smoq_ctx* ctx = s->ctx;
-  const size_t pb = prog.passes.size() * sizeof(PassDev);
-  const size_t sb = prog.stages.size() * sizeof(StageDev);
-  const size_t gb = prog.gates.size() * sizeof(GateDev);
-  const size_t total = pb + sb + gb;
-  int rc = ensure_prog(ctx, total);
-  if (rc) return rc;
-  if ((rc = ensure_pinned(ctx, total))) return rc;
-  // the pinned staging buffer may still feed an earlier copy
-  SMOQ_HIP(hipStreamSynchronize(ctx->stream));
-  unsigned char* h = (unsigned char*)ctx->host_pinned;
-  std::memcpy(h, prog.passes.data(), pb);
-  std::memcpy(h + pb, prog.stages.data(), sb);
-  std::memcpy(h + pb + sb, prog.gates.data(), gb);
-  SMOQ_HIP(hipMemcpyAsync(ctx->prog, h, total, hipMemcpyHostToDevice, ctx->stream));
-  const PassDev* dp = (const PassDev*)ctx->prog;
-  const StageDev* ds = (const StageDev*)((unsigned char*)ctx->prog + pb);
-  const GateDev* dg = (const GateDev*)((unsigned char*)ctx->prog + pb + sb);
-  for (size_t p = 0; p < prog.passes.size(); ++p) {
-    const PassDev& pd = prog.passes[p];
-    const uint64_t tiles = (uint64_t)s->batch << pd.n_nonlocal;
-    const int nt = 1 << (pd.k - kRegQubits);
-    const size_t shm = sizeof(V) << pd.k;
-    if (ctx->timing && (rc = time_mark(ctx))) return rc;
-    hipLaunchKernelGGL(k_fused_pass<T>, dim3((unsigned)tiles), dim3(nt), shm, ctx->stream,
-                       (V*)s->d, s->n, dp + p, ds, dg);
-    SMOQ_CHECK_LAUNCH("k_fused_pass");
-    if (ctx->timing && (rc = time_mark(ctx))) return rc;
-  }
-  return SMOQ_OK;
-}
-
-int run_program(smoq_state* s, const Program& prog) {
-  return s->dtype == SMOQ_C128 ? run_program_t<double>(s, prog) : run_program_t<float>(s, prog);
-}
-
-}  // namespace smoq

@@ -8,11 +8,21 @@
 // pass — so it commutes with everything it overtakes.  Diagonal gates and
 // control qubits never need to be local: they only phase or select
 // amplitudes by their global index.
+//
+// Inside a stage, runs of consecutive diagonal gates (which commute with one
+// another) are merged: CZs into one parity-sign term per qubit distance, the
+// others into one phase table indexed by the register slots and at most
+// kMaxTabBits other qubits.  2x2 matrices are classified so that real (Ry,
+// H, X) and Rx-type matrices use the cheaper arithmetic.
 #include <algorithm>
+#include <complex>
+#include <map>
 
 #include "smoq_internal.h"
 
 namespace smoq {
+
+using cd = std::complex<double>;
 
 static uint64_t touch_mask(const IGate& g) {
   uint64_t m = g.ctrl;
@@ -31,11 +41,134 @@ static uint64_t target_mask(const IGate& g) {
 
 static int popc(uint64_t x) { return __builtin_popcountll(x); }
 
+static bool is_cz(const IGate& g) {
+  return g.kind == IK_DIAG && g.nd == 2 && g.m[0] == 1 && g.m[1] == 0 && g.m[2] == 1 &&
+         g.m[3] == 0 && g.m[4] == 1 && g.m[5] == 0 && g.m[6] == -1 && g.m[7] == 0;
+}
+
+static int u1_kind(const double* m) {
+  if (m[1] == 0 && m[3] == 0 && m[5] == 0 && m[7] == 0) return GK_U1_REAL;
+  if (m[1] == 0 && m[7] == 0 && m[2] == 0 && m[4] == 0) return GK_U1_RXT;
+  return GK_U1_GEN;
+}
+
+namespace {
+struct StageCtx {
+  const int* g2l;     // global bit -> local position or -1
+  const int* l2slot;  // local position -> register slot or -1
+  int n;
+  int slot_of(int gq) const { return g2l[gq] >= 0 ? l2slot[g2l[gq]] : -1; }
+};
+}  // namespace
+
+static GateDev blank_gate() {
+  GateDev gd;
+  std::memset(&gd, 0, sizeof(gd));
+  gd.a = gd.b = -1;
+  return gd;
+}
+
+static void emit_nondiag(const IGate& g, const StageCtx& sc, Program& prog) {
+  GateDev gd = blank_gate();
+  gd.kind = g.kind == IK_U2 ? GK_U2 : u1_kind(g.m);
+  gd.a = sc.slot_of(g.t0);
+  gd.b = g.kind == IK_U2 ? sc.slot_of(g.t1) : -1;
+  for (int b = 0; b < sc.n; ++b) {
+    if (!(g.ctrl >> b & 1)) continue;
+    int sl = sc.slot_of(b);
+    if (sl >= 0)
+      gd.creg |= 1 << sl;
+    else
+      gd.cglob |= 1ull << b;
+  }
+  std::memcpy(gd.m, g.m, sizeof(gd.m));
+  prog.gates.push_back(gd);
+}
+
+// CZ gates of a diagonal run: one sign term per qubit distance
+static void emit_czs(const std::vector<const IGate*>& czs, Program& prog) {
+  std::map<int, uint64_t> by_shift;
+  for (const IGate* g : czs) {
+    int lo = std::min(g->dq[0], g->dq[1]), hi = std::max(g->dq[0], g->dq[1]);
+    by_shift[hi - lo] ^= 1ull << lo;  // CZ twice on a pair is the identity
+  }
+  GateDev gd = blank_gate();
+  gd.kind = GK_CZSET;
+  for (auto& kv : by_shift) {
+    if (kv.second == 0) continue;
+    if (gd.ncz == kMaxCzShifts) {
+      prog.gates.push_back(gd);
+      gd = blank_gate();
+      gd.kind = GK_CZSET;
+    }
+    gd.czs[gd.ncz] = kv.first;
+    gd.czm[gd.ncz] = kv.second;
+    gd.ncz++;
+  }
+  if (gd.ncz) prog.gates.push_back(gd);
+}
+
+// other diagonal gates: phase tables over (non-register qubits, slot pattern)
+static void emit_tables(const std::vector<const IGate*>& ds, const StageCtx& sc,
+                        Program& prog) {
+  size_t i = 0;
+  while (i < ds.size()) {
+    std::vector<int> ub;  // non-register global qubits of this table
+    size_t j = i;
+    for (; j < ds.size(); ++j) {
+      std::vector<int> add;
+      for (int t = 0; t < ds[j]->nd; ++t) {
+        int q = ds[j]->dq[t];
+        if (sc.slot_of(q) < 0 && std::find(ub.begin(), ub.end(), q) == ub.end() &&
+            std::find(add.begin(), add.end(), q) == add.end())
+          add.push_back(q);
+      }
+      if ((int)(ub.size() + add.size()) > kMaxTabBits) break;
+      ub.insert(ub.end(), add.begin(), add.end());
+    }
+    GateDev gd = blank_gate();
+    gd.kind = GK_DIAGTAB;
+    gd.nu = (int)ub.size();
+    for (int t = 0; t < gd.nu; ++t) gd.ubits[t] = ub[t];
+    gd.tab = (int)(prog.tabs.size() / 2);
+    const int rows = 1 << gd.nu;
+    for (int row = 0; row < rows; ++row) {
+      for (int jj = 0; jj < (1 << kRegQubits); ++jj) {
+        cd ph(1.0, 0.0);
+        for (size_t g = i; g < j; ++g) {
+          int e = 0;
+          for (int t = 0; t < ds[g]->nd; ++t) {
+            int q = ds[g]->dq[t];
+            int sl = sc.slot_of(q);
+            int bit = sl >= 0 ? (jj >> sl) & 1
+                              : (row >> (int)(std::find(ub.begin(), ub.end(), q) - ub.begin())) & 1;
+            e |= bit << t;
+          }
+          ph *= cd(ds[g]->m[2 * e], ds[g]->m[2 * e + 1]);
+        }
+        prog.tabs.push_back(ph.real());
+        prog.tabs.push_back(ph.imag());
+      }
+    }
+    prog.gates.push_back(gd);
+    i = j;
+  }
+}
+
+static void emit_diag_run(const std::vector<const IGate*>& run, const StageCtx& sc,
+                          Program& prog) {
+  std::vector<const IGate*> czs, rest;
+  for (const IGate* g : run) (is_cz(*g) ? czs : rest).push_back(g);
+  if (!czs.empty()) emit_czs(czs, prog);
+  if (!rest.empty()) emit_tables(rest, sc, prog);
+}
+
 void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int low_qubits,
                   Program& prog) {
   prog.passes.clear();
   prog.stages.clear();
   prog.gates.clear();
+  prog.tabs.clear();
   const int k = std::min(tile_qubits, n);
   // keep room for two non-low targets so that every gate fits an empty pass
   const int m = (k == n) ? k : std::min(low_qubits, k - 2);
@@ -83,7 +216,7 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
     pd.n_nonlocal = ni;
     pd.s0 = (int)prog.stages.size();
 
-    // register stages
+    // register stages: consecutive gates whose targets fit kRegQubits slots
     size_t i = 0;
     while (i < taken.size()) {
       uint32_t S = 0;  // local positions held in registers
@@ -96,7 +229,6 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
         if (popc(S | need) > kRegQubits) break;
         S |= need;
       }
-      // pad to kRegQubits positions
       for (int b = 0; b < k && popc(S) < kRegQubits; ++b) S |= 1u << b;
       StageDev sd;
       int slot = 0;
@@ -108,20 +240,22 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
           sd.q[slot++] = b;
         }
       }
+      StageCtx sc{g2l, l2slot, n};
       sd.g0 = (int)prog.gates.size();
+      std::vector<const IGate*> run;
       for (size_t t = i; t < j; ++t) {
         const IGate& g = gates[taken[t]];
-        GateDev gd;
-        std::memset(&gd, 0, sizeof(gd));
-        gd.kind = g.kind;
-        gd.a = g.kind == IK_DIAG ? -1 : l2slot[g2l[g.t0]];
-        gd.b = g.kind == IK_U2 ? l2slot[g2l[g.t1]] : -1;
-        gd.nd = g.nd;
-        for (int d = 0; d < 3; ++d) gd.dq[d] = g.dq[d];
-        gd.ctrl = g.ctrl;
-        std::memcpy(gd.m, g.m, sizeof(gd.m));
-        prog.gates.push_back(gd);
+        if (g.kind == IK_DIAG) {
+          run.push_back(&g);
+          continue;
+        }
+        if (!run.empty()) {
+          emit_diag_run(run, sc, prog);
+          run.clear();
+        }
+        emit_nondiag(g, sc, prog);
       }
+      if (!run.empty()) emit_diag_run(run, sc, prog);
       sd.g1 = (int)prog.gates.size();
       prog.stages.push_back(sd);
       i = j;

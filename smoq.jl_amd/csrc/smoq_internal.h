@@ -55,13 +55,32 @@ constexpr int kRegQubits = 4;          // amplitudes per thread = 2^kRegQubits
 constexpr int kMaxTileQubits = 12;     // LDS tile of up to 4096 amplitudes
 constexpr int kMaxQubits = 40;
 
+// Gate kinds inside a register stage.
+enum GKind : int32_t {
+  GK_U1_REAL = 0,  // 2x2 with real entries (Ry, H, X, sigma+-)
+  GK_U1_RXT = 1,   // real diagonal, imaginary off-diagonal (Rx, Y)
+  GK_U1_GEN = 2,   // general 2x2
+  GK_U2 = 3,       // general 4x4
+  GK_DIAGTAB = 4,  // merged diagonal gates: phase table[row][slot pattern]
+  GK_CZSET = 5,    // merged CZs: sign (-1)^{sum_i popc(G & (G >> s_i) & M_i)}
+};
+constexpr int kMaxTabBits = 3;   // non-register qubits of one phase table
+constexpr int kMaxCzShifts = 4;
+
+// A gate as one register stage sees it.  Control and diagonal qubits are
+// split by the planner into register slots (known per stage) and global bit
+// positions read from the thread's base index.
 struct GateDev {
   int32_t kind;
-  int32_t a, b;        // register-qubit slots (U1: a; U2: a low, b high)
-  int32_t nd;
-  int32_t dq[3];       // diagonal: global bit positions
-  int32_t pad;
-  uint64_t ctrl;       // global control mask
+  int32_t a, b;          // register-qubit slots (U1: a; U2: a low, b high)
+  int32_t creg;          // controls held in registers (slot mask)
+  uint64_t cglob;        // controls outside the registers (global mask)
+  int32_t nu;            // DIAGTAB: number of non-register qubits
+  int32_t ubits[kMaxTabBits];
+  int32_t tab;           // DIAGTAB: offset (entries) of the 2^nu x 16 table
+  int32_t ncz;           // CZSET: number of (shift, mask) terms
+  int32_t czs[kMaxCzShifts];
+  uint64_t czm[kMaxCzShifts];
   double m[32];
 };
 
@@ -82,6 +101,7 @@ struct Program {
   std::vector<PassDev> passes;
   std::vector<StageDev> stages;
   std::vector<GateDev> gates;
+  std::vector<double> tabs;   // interleaved complex phase tables
 };
 
 // Builds the pass/stage program for a gate list on an n-qubit state.
