@@ -12,12 +12,18 @@ struct CT;
 template <>
 struct CT<double> {
   using V = double2;
-  static constexpr int kRowBits = 3;  // 8 amplitudes per 128 B LDS row
+#ifndef SMOQ_ROWBITS_C128
+#define SMOQ_ROWBITS_C128 3
+#endif
+  static constexpr int kRowBits = SMOQ_ROWBITS_C128;  // amplitudes per swizzle row (log2)
 };
 template <>
 struct CT<float> {
   using V = float2;
-  static constexpr int kRowBits = 4;  // 16 amplitudes per 128 B LDS row
+#ifndef SMOQ_ROWBITS_C64
+#define SMOQ_ROWBITS_C64 4
+#endif
+  static constexpr int kRowBits = SMOQ_ROWBITS_C64;
 };
 
 template <typename T>
@@ -75,6 +81,17 @@ __device__ inline uint32_t pdep32(uint32_t v, uint32_t mask) {
   while (mask) {
     uint32_t low = mask & (0u - mask);
     if (v & 1u) r |= low;
+    v >>= 1;
+    mask ^= low;
+  }
+  return r;
+}
+
+__device__ inline uint64_t pdep64(uint64_t v, uint64_t mask) {
+  uint64_t r = 0;
+  while (mask) {
+    uint64_t low = mask & (0ull - mask);
+    if (v & 1ull) r |= low;
     v >>= 1;
     mask ^= low;
   }

@@ -17,6 +17,14 @@ namespace smoq {
 template <typename T>
 using VT = typename CT<T>::V;
 constexpr int kNR = 1 << kRegQubits;
+constexpr uint32_t kAllPatterns = (1u << kNR) - 1u;
+// bit j set for every register pattern j with slot t set
+__host__ __device__ constexpr uint32_t slot_patterns(int t) {
+  uint32_t m = 0;
+  for (int j = 0; j < kNR; ++j)
+    if ((j >> t) & 1) m |= 1u << j;
+  return m;
+}
 
 template <typename T>
 __device__ __forceinline__ VT<T> ld_c(const double* m, int e) {
@@ -50,6 +58,7 @@ __device__ __forceinline__ VT<T> u1_row(const double* m, int r, VT<T> a, VT<T> b
 // 2x2 on register slot A for every pair (j, j | 1<<A)
 template <typename T, int PT, int A, bool CTRL>
 __device__ __forceinline__ void reg_u1(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
+  if constexpr (A >= kRegQubits) return;
   const int creg = g.creg;
 #pragma unroll
   for (int j = 0; j < kNR; ++j) {
@@ -100,6 +109,7 @@ __device__ __forceinline__ VT<T> u2_row(const double* m, int r, VT<T> a0, VT<T> 
 // 4x4 on slots (A low role, B high role)
 template <typename T, int A, int B>
 __device__ __forceinline__ void reg_u2(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
+  if constexpr (A >= kRegQubits || B >= kRegQubits) return;
   const int creg = g.creg;
 #pragma unroll
   for (int j = 0; j < kNR; ++j) {
@@ -129,27 +139,46 @@ __device__ __forceinline__ void reg_diagtab(VT<T> (&x)[kNR], const GateDev& g, u
   for (int j = 0; j < kNR; ++j) x[j] = cmul(p[j], x[j]);
 }
 
-// merged CZs: negate where sum_i popc(G & (G >> s_i) & M_i) is odd
+// merged CZs: the lane builds a 16-bit mask of the slot patterns to negate
+// (lane-lane pairs, register-register pairs, register-lane pairs)
 template <typename T>
-__device__ __forceinline__ void reg_czset(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl,
-                                          const uint64_t (&gq)[kRegQubits]) {
+__device__ __forceinline__ void reg_czset(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl) {
+  int par = 0;
+  for (int i = 0; i < g.ncz; ++i) par ^= __popcll(gl & (gl >> g.czs[i]) & g.czm[i]);
+  uint32_t neg = (par & 1) ? kAllPatterns : 0u;
+  neg ^= (uint32_t)g.czreg;
+#pragma unroll
+  for (int t = 0; t < kRegQubits; ++t)
+    neg ^= (__popcll(gl & g.czcm[t]) & 1) ? slot_patterns(t) : 0u;
 #pragma unroll
   for (int j = 0; j < kNR; ++j) {
-    uint64_t G = gl;
-#pragma unroll
-    for (int s = 0; s < kRegQubits; ++s) G |= ((j >> s) & 1) ? gq[s] : 0ull;
-    int par = 0;
-    for (int i = 0; i < g.ncz; ++i) par ^= __popcll(G & (G >> g.czs[i]) & g.czm[i]);
-    if (par & 1) {
-      x[j].x = -x[j].x;
-      x[j].y = -x[j].y;
-    }
+    const bool f = (neg >> j) & 1u;
+    x[j].x = f ? -x[j].x : x[j].x;
+    x[j].y = f ? -x[j].y : x[j].y;
   }
+}
+
+// diagonal on register qubits only: uniform phase per slot pattern
+template <typename T>
+__device__ __forceinline__ void reg_diagreg(VT<T> (&x)[kNR], const GateDev& g) {
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) x[j] = cmul(ld_c<T>(g.m, j), x[j]);
+}
+
+// diagonal on non-register qubits only: one phase per lane
+template <typename T>
+__device__ __forceinline__ void reg_diaglane(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl) {
+  int row = 0;
+  for (int t = 0; t < g.nu; ++t) row |= (int)((gl >> g.ubits[t]) & 1ull) << t;
+  VT<T> ph = ld_c<T>(g.m, 0);
+#pragma unroll
+  for (int e = 1; e < (1 << kMaxTabBits); ++e) ph = csel(row == e, ld_c<T>(g.m, e), ph);
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) x[j] = cmul(ph, x[j]);
 }
 
 template <typename T>
 __device__ __forceinline__ void apply_gate(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl,
-                                           const uint64_t (&gq)[kRegQubits],
                                            const VT<T>* __restrict__ tabs) {
   switch (g.kind) {
     case GK_U1_REAL: reg_u1_any<T, GK_U1_REAL>(x, g, gl); break;
@@ -174,7 +203,9 @@ __device__ __forceinline__ void apply_gate(VT<T> (&x)[kNR], const GateDev& g, ui
       break;
     }
     case GK_DIAGTAB: reg_diagtab<T>(x, g, gl, tabs); break;
-    default: reg_czset<T>(x, g, gl, gq); break;
+    case GK_CZSET: reg_czset<T>(x, g, gl); break;
+    case GK_DIAGREG: reg_diagreg<T>(x, g); break;
+    default: reg_diaglane<T>(x, g, gl); break;
   }
 }
 
@@ -196,73 +227,73 @@ __global__ void __launch_bounds__(256)
   const uint64_t tile = blockIdx.x;
   const uint64_t item = tile >> nn;
   const uint64_t tt = tile & ((1ull << nn) - 1ull);
-  uint64_t gbase = 0;
-  for (int i = 0; i < nn; ++i) gbase |= ((tt >> i) & 1ull) << pass->nonlocal[i];
+  // tile id -> global bits outside the tile (uniform), thread index -> global
+  // bits of local positions 0..k-R-1; the unrolled register index covers the
+  // top R local positions
+  const uint64_t gbase = pdep64(tt, pass->nl_mask);
   VT<T>* __restrict__ base = psi + (item << n);
-
-  // HBM -> LDS: the thread index covers local bits 0..k-R-1, the unrolled
-  // loop index the top R local bits
-  uint64_t gt = gbase;
-  for (int j = 0; j < kr; ++j) gt |= (uint64_t)((tid >> j) & 1) << pass->map[j];
+  const uint64_t gt = gbase | pdep64((uint64_t)tid, pass->tid_mask);
+  VT<T>* __restrict__ pt = base + gt;
   uint64_t hb[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) hb[j] = 1ull << pass->map[kr + j];
+  for (int j = 0; j < R; ++j) hb[j] = pass->hb[j];
+  // The swizzle is linear over GF(2): swz(a | b) = swz(a) ^ swz(b) for
+  // disjoint a, b, so every LDS address is a per-lane part XOR a uniform part.
+  const uint32_t st_lane = swz<WB>((uint32_t)tid);
   {
     VT<T> x[kNR];
 #pragma unroll
     for (int i = 0; i < kNR; ++i) {
-      uint64_t o = gt;
+      uint64_t o = 0;  // uniform; disjoint from gt, so | is +
 #pragma unroll
       for (int j = 0; j < R; ++j) o |= ((i >> j) & 1) ? hb[j] : 0ull;
-      x[i] = base[o];
+      x[i] = pt[o];
     }
 #pragma unroll
-    for (int i = 0; i < kNR; ++i) lds[swz<WB>((uint32_t)(tid + i * NT))] = x[i];
+    for (int i = 0; i < kNR; ++i) lds[st_lane ^ swz<WB>((uint32_t)(i * NT))] = x[i];
   }
   __syncthreads();
 
-  const uint32_t kmask = (1u << k) - 1u;
   for (int s = pass->s0; s < pass->s1; ++s) {
     const StageDev& st = stages[s];
-    uint32_t b[R];
-    uint64_t gq[R];
+    // lane part of the swizzled LDS index and of the global index, from the
+    // host-precomputed images of each thread-index bit
+    uint32_t sl = 0;
+    uint64_t gl = gbase;
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      b[j] = 1u << st.q[j];
-      gq[j] = 1ull << pass->map[st.q[j]];
+    for (int i = 0; i < kMaxTileQubits - R; ++i) {
+      if (i < st.ntb && ((tid >> i) & 1)) {
+        sl ^= st.sw_tid[i];
+        gl |= st.g_tid[i];
+      }
     }
-    const uint32_t lb = pdep32((uint32_t)tid, kmask & ~(b[0] | b[1] | b[2] | b[3]));
+    uint32_t sr[kNR];
+#pragma unroll
+    for (int i = 0; i < kNR; ++i) {
+      uint32_t r = 0;
+#pragma unroll
+      for (int j = 0; j < R; ++j) r ^= ((i >> j) & 1) ? st.sw_reg[j] : 0u;
+      sr[i] = r;
+    }
     VT<T> x[kNR];
 #pragma unroll
-    for (int i = 0; i < kNR; ++i) {
-      uint32_t l = lb;
+    for (int i = 0; i < kNR; ++i) x[i] = lds[sl ^ sr[i]];
+    for (int g = st.g0; g < st.g1; ++g) apply_gate<T>(x, gates[g], gl, tabs);
 #pragma unroll
-      for (int j = 0; j < R; ++j) l |= ((i >> j) & 1) ? b[j] : 0u;
-      x[i] = lds[swz<WB>(l)];
-    }
-    uint64_t gl = gbase;
-    for (int bb = 0; bb < k; ++bb) gl |= (uint64_t)((lb >> bb) & 1u) << pass->map[bb];
-    for (int g = st.g0; g < st.g1; ++g) apply_gate<T>(x, gates[g], gl, gq, tabs);
-#pragma unroll
-    for (int i = 0; i < kNR; ++i) {
-      uint32_t l = lb;
-#pragma unroll
-      for (int j = 0; j < R; ++j) l |= ((i >> j) & 1) ? b[j] : 0u;
-      lds[swz<WB>(l)] = x[i];
-    }
+    for (int i = 0; i < kNR; ++i) lds[sl ^ sr[i]] = x[i];
     __syncthreads();
   }
 
   {
     VT<T> x[kNR];
 #pragma unroll
-    for (int i = 0; i < kNR; ++i) x[i] = lds[swz<WB>((uint32_t)(tid + i * NT))];
+    for (int i = 0; i < kNR; ++i) x[i] = lds[st_lane ^ swz<WB>((uint32_t)(i * NT))];
 #pragma unroll
     for (int i = 0; i < kNR; ++i) {
-      uint64_t o = gt;
+      uint64_t o = 0;
 #pragma unroll
       for (int j = 0; j < R; ++j) o |= ((i >> j) & 1) ? hb[j] : 0ull;
-      base[o] = x[i];
+      pt[o] = x[i];
     }
   }
 }
@@ -298,6 +329,9 @@ static int run_program_t(smoq_state* s, const Program& prog) {
     const PassDev& pd = prog.passes[p];
     const uint64_t tiles = (uint64_t)s->batch << pd.n_nonlocal;
     const int nt = 1 << (pd.k - kRegQubits);
+    if (nt > 256 || pd.k > kMaxTileQubits)
+      return set_error(SMOQ_EINVAL, "fused pass: tile of 2^%d needs %d threads (max 256)", pd.k,
+                       nt);
     const size_t shm = sizeof(VT<T>) << pd.k;
     if (ctx->timing && (rc = time_mark(ctx))) return rc;
     hipLaunchKernelGGL(k_fused_pass<T>, dim3((unsigned)tiles), dim3(nt), shm, ctx->stream,

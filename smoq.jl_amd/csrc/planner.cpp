@@ -85,15 +85,33 @@ static void emit_nondiag(const IGate& g, const StageCtx& sc, Program& prog) {
   prog.gates.push_back(gd);
 }
 
-// CZ gates of a diagonal run: one sign term per qubit distance
-static void emit_czs(const std::vector<const IGate*>& czs, Program& prog) {
+// CZ gates of a diagonal run: lane-lane pairs become one popcount term per
+// qubit distance, register-register pairs a fixed 16-bit pattern mask and
+// register-lane pairs a per-slot mask of lane qubits.
+static void emit_czs(const std::vector<const IGate*>& czs, const StageCtx& sc,
+                     Program& prog) {
   std::map<int, uint64_t> by_shift;
+  int czreg = 0;
+  uint64_t czcm[kRegQubits] = {};
   for (const IGate* g : czs) {
-    int lo = std::min(g->dq[0], g->dq[1]), hi = std::max(g->dq[0], g->dq[1]);
-    by_shift[hi - lo] ^= 1ull << lo;  // CZ twice on a pair is the identity
+    int a = g->dq[0], b = g->dq[1];
+    int sa = sc.slot_of(a), sb = sc.slot_of(b);
+    if (sa >= 0 && sb >= 0) {
+      for (int j = 0; j < (1 << kRegQubits); ++j)
+        if ((j >> sa & 1) && (j >> sb & 1)) czreg ^= 1 << j;
+    } else if (sa >= 0) {
+      czcm[sa] ^= 1ull << b;
+    } else if (sb >= 0) {
+      czcm[sb] ^= 1ull << a;
+    } else {
+      int lo = std::min(a, b), hi = std::max(a, b);
+      by_shift[hi - lo] ^= 1ull << lo;  // CZ twice on a pair is the identity
+    }
   }
   GateDev gd = blank_gate();
   gd.kind = GK_CZSET;
+  gd.czreg = czreg;
+  for (int t = 0; t < kRegQubits; ++t) gd.czcm[t] = czcm[t];
   for (auto& kv : by_shift) {
     if (kv.second == 0) continue;
     if (gd.ncz == kMaxCzShifts) {
@@ -105,7 +123,65 @@ static void emit_czs(const std::vector<const IGate*>& czs, Program& prog) {
     gd.czm[gd.ncz] = kv.second;
     gd.ncz++;
   }
-  if (gd.ncz) prog.gates.push_back(gd);
+  prog.gates.push_back(gd);
+}
+
+// diagonal gates on register qubits only: one uniform 16-entry phase vector
+static void emit_diag_reg(const std::vector<const IGate*>& ds, const StageCtx& sc,
+                          Program& prog) {
+  GateDev gd = blank_gate();
+  gd.kind = GK_DIAGREG;
+  for (int j = 0; j < (1 << kRegQubits); ++j) {
+    cd ph(1.0, 0.0);
+    for (const IGate* g : ds) {
+      int e = 0;
+      for (int t = 0; t < g->nd; ++t) e |= ((j >> sc.slot_of(g->dq[t])) & 1) << t;
+      ph *= cd(g->m[2 * e], g->m[2 * e + 1]);
+    }
+    gd.m[2 * j] = ph.real();
+    gd.m[2 * j + 1] = ph.imag();
+  }
+  prog.gates.push_back(gd);
+}
+
+// diagonal gates on non-register qubits only: one phase per lane, chosen
+// from 2^nu values by nu <= kMaxTabBits index bits
+static void emit_diag_lane(const std::vector<const IGate*>& ds, Program& prog) {
+  size_t i = 0;
+  while (i < ds.size()) {
+    std::vector<int> ub;
+    size_t j = i;
+    for (; j < ds.size(); ++j) {
+      std::vector<int> add;
+      for (int t = 0; t < ds[j]->nd; ++t) {
+        int q = ds[j]->dq[t];
+        if (std::find(ub.begin(), ub.end(), q) == ub.end() &&
+            std::find(add.begin(), add.end(), q) == add.end())
+          add.push_back(q);
+      }
+      if ((int)(ub.size() + add.size()) > kMaxTabBits) break;
+      ub.insert(ub.end(), add.begin(), add.end());
+    }
+    GateDev gd = blank_gate();
+    gd.kind = GK_DIAGLANE;
+    gd.nu = (int)ub.size();
+    for (int t = 0; t < gd.nu; ++t) gd.ubits[t] = ub[t];
+    for (int row = 0; row < (1 << gd.nu); ++row) {
+      cd ph(1.0, 0.0);
+      for (size_t g = i; g < j; ++g) {
+        int e = 0;
+        for (int t = 0; t < ds[g]->nd; ++t) {
+          int pos = (int)(std::find(ub.begin(), ub.end(), ds[g]->dq[t]) - ub.begin());
+          e |= ((row >> pos) & 1) << t;
+        }
+        ph *= cd(ds[g]->m[2 * e], ds[g]->m[2 * e + 1]);
+      }
+      gd.m[2 * row] = ph.real();
+      gd.m[2 * row + 1] = ph.imag();
+    }
+    prog.gates.push_back(gd);
+    i = j;
+  }
 }
 
 // other diagonal gates: phase tables over (non-register qubits, slot pattern)
@@ -155,16 +231,33 @@ static void emit_tables(const std::vector<const IGate*>& ds, const StageCtx& sc,
   }
 }
 
+// A run of diagonal gates commutes internally, so it is regrouped by where
+// its qubits live: CZs, register-only, lane-only and mixed gates.
 static void emit_diag_run(const std::vector<const IGate*>& run, const StageCtx& sc,
                           Program& prog) {
-  std::vector<const IGate*> czs, rest;
-  for (const IGate* g : run) (is_cz(*g) ? czs : rest).push_back(g);
-  if (!czs.empty()) emit_czs(czs, prog);
-  if (!rest.empty()) emit_tables(rest, sc, prog);
+  std::vector<const IGate*> czs, reg, lane, mixed;
+  for (const IGate* g : run) {
+    if (is_cz(*g)) {
+      czs.push_back(g);
+      continue;
+    }
+    int nreg = 0;
+    for (int t = 0; t < g->nd; ++t) nreg += sc.slot_of(g->dq[t]) >= 0;
+    if (nreg == g->nd)
+      reg.push_back(g);
+    else if (nreg == 0)
+      lane.push_back(g);
+    else
+      mixed.push_back(g);
+  }
+  if (!czs.empty()) emit_czs(czs, sc, prog);
+  if (!reg.empty()) emit_diag_reg(reg, sc, prog);
+  if (!lane.empty()) emit_diag_lane(lane, prog);
+  if (!mixed.empty()) emit_tables(mixed, sc, prog);
 }
 
 void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int low_qubits,
-                  Program& prog) {
+                  int row_bits, Program& prog) {
   prog.passes.clear();
   prog.stages.clear();
   prog.gates.clear();
@@ -210,10 +303,13 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
         pd.map[li++] = b;
       } else {
         g2l[b] = -1;
-        pd.nonlocal[ni++] = b;
+        pd.nl_mask |= 1ull << b;
+        ni++;
       }
     }
     pd.n_nonlocal = ni;
+    for (int j = 0; j < k - kRegQubits; ++j) pd.tid_mask |= 1ull << pd.map[j];
+    for (int j = 0; j < kRegQubits; ++j) pd.hb[j] = 1ull << pd.map[k - kRegQubits + j];
     pd.s0 = (int)prog.stages.size();
 
     // register stages: consecutive gates whose targets fit kRegQubits slots
@@ -231,15 +327,21 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
       }
       for (int b = 0; b < k && popc(S) < kRegQubits; ++b) S |= 1u << b;
       StageDev sd;
-      int slot = 0;
+      std::memset(&sd, 0, sizeof(sd));
+      int slot = 0, ti = 0;
       int l2slot[kMaxTileQubits];
       for (int b = 0; b < k; ++b) {
         l2slot[b] = -1;
         if (S >> b & 1) {
           l2slot[b] = slot;
+          sd.sw_reg[slot] = swz_host(1u << b, row_bits);
           sd.q[slot++] = b;
+        } else {
+          sd.sw_tid[ti] = swz_host(1u << b, row_bits);
+          sd.g_tid[ti++] = 1ull << pd.map[b];
         }
       }
+      sd.ntb = ti;
       StageCtx sc{g2l, l2slot, n};
       sd.g0 = (int)prog.gates.size();
       std::vector<const IGate*> run;

@@ -51,7 +51,13 @@ struct IGate {
 int lower_op(const smoq_op& op, double theta, int n_qubits, std::vector<IGate>& out);
 
 // ------------------------------------------------------- fused programs --
-constexpr int kRegQubits = 4;          // amplitudes per thread = 2^kRegQubits
+// 8 amplitudes per thread: at 16 per thread the complex128 stage kernel needs
+// ~255 VGPRs (one wave per SIMD) and LDS latency between stages goes
+// unhidden; at 8 it fits in ~112 VGPRs (four waves per SIMD).
+#ifndef SMOQ_REG_QUBITS
+#define SMOQ_REG_QUBITS 3
+#endif
+constexpr int kRegQubits = SMOQ_REG_QUBITS;  // amplitudes per thread = 2^kRegQubits
 constexpr int kMaxTileQubits = 12;     // LDS tile of up to 4096 amplitudes
 constexpr int kMaxQubits = 40;
 
@@ -62,7 +68,9 @@ enum GKind : int32_t {
   GK_U1_GEN = 2,   // general 2x2
   GK_U2 = 3,       // general 4x4
   GK_DIAGTAB = 4,  // merged diagonal gates: phase table[row][slot pattern]
-  GK_CZSET = 5,    // merged CZs: sign (-1)^{sum_i popc(G & (G >> s_i) & M_i)}
+  GK_CZSET = 5,    // merged CZs: per-lane 16-bit negate mask
+  GK_DIAGREG = 6,  // diagonal over register qubits only: m = 16 phases
+  GK_DIAGLANE = 7, // diagonal over non-register qubits only: m = 2^nu phases
 };
 constexpr int kMaxTabBits = 3;   // non-register qubits of one phase table
 constexpr int kMaxCzShifts = 4;
@@ -78,23 +86,40 @@ struct GateDev {
   int32_t nu;            // DIAGTAB: number of non-register qubits
   int32_t ubits[kMaxTabBits];
   int32_t tab;           // DIAGTAB: offset (entries) of the 2^nu x 16 table
-  int32_t ncz;           // CZSET: number of (shift, mask) terms
+  // CZSET: a CZ between two non-register qubits adds popc(G & (G >> s) & M)
+  // terms evaluated on the lane's base index; a CZ between register slots
+  // flips the slot patterns in czreg; a CZ between slot t and a non-register
+  // qubit q puts q in czcm[t] (the lane flips pattern j when parity(gl &
+  // czcm[t]) and bit t of j are both 1).
+  int32_t ncz;
+  int32_t czreg;
   int32_t czs[kMaxCzShifts];
   uint64_t czm[kMaxCzShifts];
+  uint64_t czcm[kRegQubits];
   double m[32];
 };
 
+// Per-stage addressing, precomputed on the host so that a stage costs the
+// device one descriptor load: thread-index bit i selects local position
+// p_i (the positions not held in registers, ascending).
 struct StageDev {
-  int32_t q[kRegQubits];  // local bit positions held in registers
-  int32_t g0, g1;         // gate range [g0, g1) in the gate array
+  int32_t q[kRegQubits];             // local bit positions held in registers
+  int32_t g0, g1;                    // gate range [g0, g1) in the gate array
+  int32_t ntb;                       // thread-index bits (k - kRegQubits)
+  int32_t pad;
+  uint32_t sw_tid[kMaxTileQubits - kRegQubits];  // swizzled LDS index of bit p_i
+  uint32_t sw_reg[kRegQubits];       // swizzled LDS index of slot bit q_j
+  uint64_t g_tid[kMaxTileQubits - kRegQubits];   // global index bit of p_i
 };
 
 struct PassDev {
   int32_t k;              // tile qubits
   int32_t n_nonlocal;
   int32_t s0, s1;         // stage range
+  uint64_t nl_mask;       // global bits outside the tile (tile id deposits here)
+  uint64_t tid_mask;      // global bits of local positions 0..k-R-1 (thread index)
+  uint64_t hb[kRegQubits];  // global bits of local positions k-R..k-1
   int32_t map[kMaxTileQubits];       // local bit -> global bit
-  int32_t nonlocal[kMaxQubits];      // ascending global bits outside the tile
 };
 
 struct Program {
@@ -105,8 +130,14 @@ struct Program {
 };
 
 // Builds the pass/stage program for a gate list on an n-qubit state.
+// row_bits: log2 of the amplitudes per LDS swizzle row of the state's dtype.
 void plan_program(const std::vector<IGate>& gates, int n_qubits, int tile_qubits,
-                  int low_qubits, Program& prog);
+                  int low_qubits, int row_bits, Program& prog);
+// LDS index swizzle (the same function the kernels use)
+inline uint32_t swz_host(uint32_t l, int wb) {
+  uint32_t f = (l >> wb) ^ (l >> (2 * wb)) ^ (l >> (3 * wb)) ^ (l >> (4 * wb));
+  return l ^ (f & ((1u << wb) - 1u));
+}
 
 }  // namespace smoq
 
@@ -114,8 +145,8 @@ void plan_program(const std::vector<IGate>& gates, int n_qubits, int tile_qubits
 struct smoq_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
-  int tile_qubits = 11;
-  int low_qubits = 5;
+  int tile_qubits = 10;
+  int low_qubits = 4;
   // grow-only device scratch, reused across calls on this context
   void* scratch = nullptr;
   size_t scratch_bytes = 0;

@@ -10,7 +10,8 @@ import ctypes
 import os
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "lib", "libsmoq_amd.so"))
+LIB_PATH = os.environ.get("SMOQ_LIB") or os.path.normpath(
+    os.path.join(_PKG, "..", "lib", "libsmoq_amd.so"))
 
 SMOQ_OK = 0
 SMOQ_EINVAL = -1

@@ -159,7 +159,7 @@ def oracle_run_tape(tape, psi, theta=None):
 
 
 @pytest.mark.parametrize("N,tile,low", [(4, 6, 3), (5, 8, 3), (9, 8, 3), (12, 8, 4),
-                                        (14, 10, 5), (16, 11, 5), (16, 12, 5), (18, 12, 6)])
+                                        (14, 10, 5), (16, 11, 5), (16, 11, 6), (18, 10, 4)])
 def test_fused_tape_parity(ctx, N, tile, low):
     from smoq_amd import GPUState
 
@@ -173,7 +173,30 @@ def test_fused_tape_parity(ctx, N, tile, low):
         assert 1 <= npass <= len(tape)
         assert rel(s.to_numpy(), ref) < TOL128
     finally:
-        ctx.set_fusion(11, 5)
+        ctx.set_fusion(10, 4)
+
+
+@pytest.mark.parametrize("N,tile,low", [(10, 6, 3), (14, 8, 4), (15, 10, 5), (17, 11, 5)])
+def test_fused_brick_circuit_parity(ctx, N, tile, low):
+    # rotation layers + CZ bricks: exercises merged CZ signs between register
+    # slots, between a slot and a lane qubit and between lane qubits, and
+    # merged register-only / lane-only diagonal phases
+    from smoq_amd import GPUState
+    from smoq_amd.circuits import random_rotation_brick
+
+    ctx.set_fusion(tile, low)
+    try:
+        tape = random_rotation_brick(N, 8, seed=N + tile)
+        tape.add("CZ", [0, N - 1]).add("RZZ", [1, N - 2], 0.7).add("CCZ", [0, 2, N - 1])
+        tape.add("S", [N - 1]).add("T", [3]).add("RZ", [N - 3], 1.3).add("CZ", [2, 3])
+        ref = np.zeros(1 << N, np.complex128)
+        ref[0] = 1
+        oracle_run_tape(tape, ref)
+        s = GPUState.basis(N, 0)
+        tape.apply(s)
+        assert rel(s.to_numpy(), ref) < TOL128
+    finally:
+        ctx.set_fusion(10, 4)
 
 
 def test_fused_batch_and_c64(ctx):
