@@ -70,8 +70,39 @@ def cpu_baseline(tape, N, budget_s=20.0):
                       f"(reference apply_*_psi! restated), {dt:.1f} s, {cpu_model()}"}
 
 
+def traffic_probe(N=28):
+    """Program for the rocprofv3 FETCH_SIZE / WRITE_SIZE runs: one Ry with the
+    one-gate kernel (reads and writes the state once: the calibration), then
+    the fused passes of the workload circuit at N qubits."""
+    import smoq_amd
+    import smoq_amd.gates as g
+    from smoq_amd import GPUState
+    from smoq_amd.circuits import random_rotation_brick
+
+    ctx = smoq_amd.default_context()
+    s = GPUState(N, "c128")
+    s.set_basis(0)
+    g.apply_ry_psi(s, 20, 0.3, N)
+    ctx.sync()
+    n = random_rotation_brick(N, 20, 30).apply(s)
+    ctx.sync()
+    print(f"traffic probe: N={N}, {n} fused passes, state {s.nbytes} bytes", flush=True)
+
+
+def measured_traffic_ratio():
+    """HBM traffic / algorithmic bytes of the fused pass, from the committed
+    counter summary (profiles/k_fused_pass_traffic.json), or None."""
+    p = os.path.join(ROOT, "profiles", "k_fused_pass_traffic.json")
+    try:
+        with open(p) as f:
+            return float(json.load(f)["traffic_over_algorithmic"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--probe", choices=["traffic"], default=None)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -85,6 +116,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     args = ap.parse_args()
 
+    if args.probe == "traffic":
+        traffic_probe()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -144,7 +178,9 @@ def main():
                    "n_qubits": N, "layers": args.layers, "gates": n_gates,
                    "fused_passes": n_passes, "state_bytes": S},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": (None if measured_traffic_ratio() is None
+                                 else measured_traffic_ratio() * 2 * S),
                      "kernel": "k_fused_pass<double>", "bytes_per_launch": 2 * S,
                      "mean_launch_ms": pass_ms},
         "hbm_GBps_whole_step": n_passes * 2.0 * S / (ms_per_step * 1e-3) / 1e9,
