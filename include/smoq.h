@@ -142,6 +142,44 @@ int smoq_apply_named(smoq_state* s, int gate, const int* qubits, double theta);
 int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops,
                     const double* theta, int n_theta, int* n_passes);
 
+/* ---- compiled tapes ------------------------------------------------------
+ * A plan is a tape planned once (fused passes, device-resident program) for
+ * states of one qubit count and dtype; every run replays the same launches,
+ * captured in a hipGraph after the first run on a given state.  It replaces
+ * the per-step gate loop of apply_fast_trotter_step_cpu! / evolve_trotter_psi_cpu!
+ * (utils/cpu/cpuQuantumChannelUnitaryEvolutionTrotter.jl:150-162,250-256). */
+typedef struct smoq_plan_s smoq_plan;
+int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops, int n_ops,
+                     const double* theta, int n_theta, smoq_plan** out);
+int smoq_plan_passes(const smoq_plan* p, int* n_passes);
+/* applies the tape `repeat` times */
+int smoq_plan_run(smoq_plan* p, smoq_state* s, int repeat);
+int smoq_plan_free(smoq_plan* p);
+/* Runs n_steps applications of the plan and, after every obs_every-th one,
+ * records <P_t> for the n_terms Pauli strings: out[r][item][t] with
+ * r = 0 .. n_steps/obs_every - 1.  solve_trotter_bitwise_cpu's time loop
+ * (…UnitaryEvolutionTrotter.jl:293-311) with the observables taken on the
+ * device instead of storing states. */
+int smoq_evolve_observe(smoq_plan* p, smoq_state* s, int n_steps, int obs_every, int n_terms,
+                        const uint64_t* xmask, const uint64_t* zmask, double* out);
+
+/* ---- Hamiltonian action and Chebyshev propagation -------------------------
+ * H = sum_t c_t P_t with real c_t and Pauli strings P_t given as (x, z)
+ * masks as in smoq_expect_pauli.
+ * smoq_hpsi: out = H in (pull form: one write per amplitude), replacing
+ *   apply_hamiltonian_bitwise! (utils/cpu/cpuQuantumChannelUnitaryEvolutionChebyshev.jl:55-152).
+ * smoq_chebyshev_evolve: psi <- exp(-i H t) psi by the Chebyshev-Bessel
+ *   series on the spectrum rescaled from [emin, emax] (widened by 1% each
+ *   side), stopping after the first order k >= 2 with |J_k(a t)| < tol or at
+ *   max_order; replaces chebyshev_evolve_psi! (…Chebyshev.jl:262-346) with
+ *   the spectral bounds supplied.  *orders_used receives the reference's
+ *   M_used (last order + 1).  Needs two scratch states of the same shape. */
+int smoq_hpsi(const smoq_state* in, smoq_state* out, int n_terms, const uint64_t* xmask,
+              const uint64_t* zmask, const double* coeff);
+int smoq_chebyshev_evolve(smoq_state* s, int n_terms, const uint64_t* xmask,
+                          const uint64_t* zmask, const double* coeff, double t, double emin,
+                          double emax, double tol, int max_order, int* orders_used);
+
 /* ---- observables --------------------------------------------------------
  * <P_t> for Pauli strings P_t = i^{|x_t & z_t|} X^{x_t} Z^{z_t} (x/z bit
  * masks; Y on qubit q sets bit q in both).  out has batch*n_terms entries

@@ -240,6 +240,46 @@ def mcwf_decay_jump(psi, N, gamma_dt, decay_pow, r):
     return site, float(dp[0])
 
 
+def trotter_step(psi, gates, N):
+    """apply_fast_trotter_step_cpu! (utils/cpu/cpuQuantumChannelUnitaryEvolutionTrotter.jl:150-162):
+    gates = [(indices 1-based, U)] applied in order."""
+    for idx, U in gates:
+        if len(idx) == 1:
+            apply_1q(psi, U, idx[0], N)
+        else:
+            apply_2q_trotter(psi, U, idx[0], idx[1], N)
+    return psi
+
+
+def chebyshev_evolve(psi, N, bonds, fields, t, emin, emax, tol=1e-12, max_order=1000):
+    """chebyshev_evolve_psi! (utils/cpu/cpuQuantumChannelUnitaryEvolutionChebyshev.jl:262-346)
+    with supplied bounds; H·ψ by the C pull loop (or_hpsi), Bessel values
+    from scipy.special.jv (the reference calls SpecialFunctions.besselj).
+    Returns (psi_out, M_used)."""
+    from scipy.special import jv
+
+    width = emax - emin
+    lo, hi = emin - 0.01 * abs(width), emax + 0.01 * abs(width)
+    a, b = (hi - lo) / 2.0, (hi + lo) / 2.0
+    tau = a * t
+    psi_in = psi.copy()
+    T_prev = psi_in.copy()
+    H = hpsi(psi_in, N, bonds, fields)
+    T_curr = (H - b * psi_in) / a
+    out = jv(0, tau) * T_prev
+    out = out + (-2j * jv(1, tau)) * T_curr
+    M_used = 2
+    for k in range(2, max_order + 1):
+        H = hpsi(T_curr, N, bonds, fields)
+        T_next = 2 * ((H - b * T_curr) / a) - T_prev
+        out = out + (2 * (1, -1j, -1, 1j)[k % 4] * jv(k, tau)) * T_next
+        M_used = k + 1
+        if abs(jv(k, tau)) < tol:
+            break
+        T_prev, T_curr = T_curr, T_next
+    return out * np.exp(-1j * b * t), M_used
+
+
 # ------------------------------------------------------ dense references --
 def dense_1q(U, q: int, N: int):
     """Full 2^N matrix of a 1-qubit gate on 0-based qubit q (little-endian)."""

@@ -27,6 +27,9 @@ struct CT<float> {
 };
 
 template <typename T>
+using VT_alias = typename CT<T>::V;
+
+template <typename T>
 __host__ __device__ inline typename CT<T>::V make_c(double re, double im) {
   typename CT<T>::V v;
   v.x = (T)re;

@@ -1,0 +1,213 @@
+// Hamiltonian action (pull form) and Chebyshev-Bessel propagation.
+//
+// H = sum_t c_t P_t, P_t = i^{|x&z|} X^x Z^z.  Row k of P_t has one entry,
+// at column k ^ x: i^{|x&z|} (-1)^{|(k ^ x) & z|}.  Each thread owns output
+// amplitudes and writes each of them once, as apply_hamiltonian_bitwise!
+// (utils/cpu/cpuQuantumChannelUnitaryEvolutionChebyshev.jl:55-152) does.
+#include <cmath>
+
+#include "device_common.h"
+#include "smoq_internal.h"
+
+namespace smoq {
+
+struct TermDev {
+  uint64_t x, z;
+  double fre, fim;  // c_t * i^{|x&z|}
+};
+
+enum HMode : int {
+  HM_PLAIN = 0,  // out = H in
+  HM_CHEB1 = 1,  // out = (H in - b in)/a ;  acc = J0 in + a1 out
+  HM_CHEBK = 2,  // out = 2 (H in - b in)/a - prev ;  acc += ak out   (out may alias prev)
+};
+
+struct ChebArgs {
+  double a, b;
+  double c0;         // J0 (mode 1)
+  double cre, cim;   // a1 (mode 1) or ak (mode 2)
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_hpsi(const VT_alias<T>* __restrict__ in, const VT_alias<T>* prev, VT_alias<T>* out,
+           VT_alias<T>* __restrict__ acc, uint64_t total, uint64_t dim, int nt,
+           const TermDev* __restrict__ terms, int mode, ChebArgs ca) {
+#pragma clang fp contract(off)
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t k = i & (dim - 1);
+  const uint64_t base = i - k;
+  double hr = 0.0, hi = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    const TermDev tm = terms[t];
+    const uint64_t j = k ^ tm.x;
+    const VT_alias<T> v = in[base + j];
+    const double vr = (double)v.x, vi = (double)v.y;
+    const bool neg = __popcll(j & tm.z) & 1;
+    const double sr = neg ? -vr : vr, si = neg ? -vi : vi;
+    hr += tm.fre * sr - tm.fim * si;
+    hi += tm.fre * si + tm.fim * sr;
+  }
+  if (mode == HM_PLAIN) {
+    out[i] = make_c<T>(hr, hi);
+    return;
+  }
+  const VT_alias<T> x = in[i];
+  const double xr = (double)x.x, xi = (double)x.y;
+  // (H x - b x) / a
+  double tr = (hr - ca.b * xr) / ca.a, ti = (hi - ca.b * xi) / ca.a;
+  if (mode == HM_CHEBK) {
+    const VT_alias<T> p = prev[i];
+    tr = 2.0 * tr - (double)p.x;
+    ti = 2.0 * ti - (double)p.y;
+  }
+  const VT_alias<T> tv = make_c<T>(tr, ti);
+  out[i] = tv;
+  // accumulate with the rounded T value, as the reference stores T then adds
+  const double ur = (double)tv.x, ui = (double)tv.y;
+  const double ar = ca.cre * ur - ca.cim * ui, ai = ca.cre * ui + ca.cim * ur;
+  if (mode == HM_CHEB1) {
+    acc[i] = make_c<T>(ca.c0 * xr + ar, ca.c0 * xi + ai);
+  } else {
+    const VT_alias<T> o = acc[i];
+    acc[i] = make_c<T>((double)o.x + ar, (double)o.y + ai);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_scale(VT_alias<T>* __restrict__ psi, uint64_t total, double re, double im) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const VT_alias<T> v = psi[i];
+  psi[i] = make_c<T>(re * (double)v.x - im * (double)v.y, re * (double)v.y + im * (double)v.x);
+}
+
+static int check_terms(const smoq_state* s, int n_terms, const uint64_t* x, const uint64_t* z,
+                       const double* c, std::vector<TermDev>& tv) {
+  if (n_terms < 0 || (n_terms > 0 && (!x || !z || !c)))
+    return set_error(SMOQ_EINVAL, "Pauli sum: bad term arrays");
+  const uint64_t lim = s->dim - 1;
+  for (int t = 0; t < n_terms; ++t) {
+    if ((x[t] | z[t]) & ~lim)
+      return set_error(SMOQ_EINVAL, "Pauli sum: term %d acts outside the %d qubits", t, s->n);
+    const int cnt = __builtin_popcountll(x[t] & z[t]) & 3;
+    // c * i^cnt
+    const double re[4] = {1, 0, -1, 0}, im[4] = {0, 1, 0, -1};
+    tv.push_back({x[t], z[t], c[t] * re[cnt], c[t] * im[cnt]});
+  }
+  return SMOQ_OK;
+}
+
+template <typename T>
+static int hpsi_launch(smoq_state* s, const void* in, const void* prev, void* out, void* acc,
+                       const TermDev* dterms, int nt, int mode, ChebArgs ca) {
+  const uint64_t total = s->dim * (uint64_t)s->batch;
+  hipLaunchKernelGGL(k_hpsi<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     s->ctx->stream, (const VT_alias<T>*)in, (const VT_alias<T>*)prev,
+                     (VT_alias<T>*)out, (VT_alias<T>*)acc, total, s->dim, nt, dterms, mode, ca);
+  SMOQ_CHECK_LAUNCH("k_hpsi");
+  return SMOQ_OK;
+}
+
+static int hpsi_any(smoq_state* s, const void* in, const void* prev, void* out, void* acc,
+                    const TermDev* dterms, int nt, int mode, ChebArgs ca) {
+  return s->dtype == SMOQ_C128 ? hpsi_launch<double>(s, in, prev, out, acc, dterms, nt, mode, ca)
+                               : hpsi_launch<float>(s, in, prev, out, acc, dterms, nt, mode, ca);
+}
+
+static int upload_terms(smoq_ctx* ctx, const std::vector<TermDev>& tv, TermDev** d) {
+  const size_t bytes = tv.size() * sizeof(TermDev) + 64;
+  int rc = ensure_scratch(ctx, bytes);
+  if (rc) return rc;
+  *d = (TermDev*)ctx->scratch;
+  if (!tv.empty())
+    SMOQ_HIP(hipMemcpyAsync(*d, tv.data(), tv.size() * sizeof(TermDev), hipMemcpyHostToDevice,
+                            ctx->stream));
+  // tv is host memory owned by the caller's frame: finish the copy now
+  SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+  return SMOQ_OK;
+}
+
+}  // namespace smoq
+
+using namespace smoq;
+
+extern "C" int smoq_hpsi(const smoq_state* in, smoq_state* out, int n_terms,
+                         const uint64_t* xmask, const uint64_t* zmask, const double* coeff) {
+  int rc = check_state(in);
+  if (rc) return rc;
+  if ((rc = check_state(out))) return rc;
+  if (in->n != out->n || in->dtype != out->dtype || in->batch != out->batch || in == out ||
+      in->d == out->d)
+    return set_error(SMOQ_EINVAL, "smoq_hpsi: in/out must be distinct states of one shape");
+  std::vector<TermDev> tv;
+  if ((rc = check_terms(in, n_terms, xmask, zmask, coeff, tv))) return rc;
+  TermDev* d = nullptr;
+  if ((rc = upload_terms(out->ctx, tv, &d))) return rc;
+  return hpsi_any(out, in->d, nullptr, out->d, nullptr, d, n_terms, HM_PLAIN, ChebArgs{});
+}
+
+extern "C" int smoq_chebyshev_evolve(smoq_state* s, int n_terms, const uint64_t* xmask,
+                                     const uint64_t* zmask, const double* coeff, double t,
+                                     double emin, double emax, double tol, int max_order,
+                                     int* orders_used) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!(emax > emin)) return set_error(SMOQ_EINVAL, "smoq_chebyshev_evolve: emax <= emin");
+  std::vector<TermDev> tv;
+  if ((rc = check_terms(s, n_terms, xmask, zmask, coeff, tv))) return rc;
+  smoq_ctx* ctx = s->ctx;
+  // get_rescaling_factors (…Chebyshev.jl:232-243), safety margin 0.01
+  const double width = emax - emin;
+  const double lo = emin - 0.01 * std::fabs(width), hi = emax + 0.01 * std::fabs(width);
+  const double a = (hi - lo) / 2.0, b = (hi + lo) / 2.0;
+  const double tau = a * t;
+  void* A = nullptr;
+  void* B = nullptr;
+  SMOQ_HIP(hipMalloc(&A, s->bytes));
+  hipError_t e = hipMalloc(&B, s->bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(A);
+    return hip_error(e, "hipMalloc(chebyshev)");
+  }
+  auto done = [&](int code) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(A);
+    (void)hipFree(B);
+    return code;
+  };
+  TermDev* dt = nullptr;
+  if ((rc = upload_terms(ctx, tv, &dt))) return done(rc);
+  // T0 = psi (copy in A); T1 -> B; psi = J0 T0 - 2i J1 T1
+  if (hipMemcpyAsync(A, s->d, s->bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+    return done(set_error(SMOQ_EDEVICE, "smoq_chebyshev_evolve: copy failed"));
+  ChebArgs ca{a, b, jn(0, tau), 0.0, -2.0 * jn(1, tau)};
+  if ((rc = hpsi_any(s, A, nullptr, B, s->d, dt, n_terms, HM_CHEB1, ca))) return done(rc);
+  void* prev = A;  // T_{k-2}, overwritten by T_k
+  void* cur = B;   // T_{k-1}
+  int m_used = 2;
+  for (int k = 2; k <= max_order; ++k) {
+    const double jk = jn(k, tau);
+    // 2 (-i)^k J_k
+    const double pre[4] = {1, 0, -1, 0}, pim[4] = {0, -1, 0, 1};
+    ChebArgs ck{a, b, 0.0, 2.0 * jk * pre[k & 3], 2.0 * jk * pim[k & 3]};
+    if ((rc = hpsi_any(s, cur, prev, prev, s->d, dt, n_terms, HM_CHEBK, ck))) return done(rc);
+    m_used = k + 1;
+    if (std::fabs(jk) < tol) break;
+    std::swap(prev, cur);
+  }
+  // global phase exp(-i b t)
+  const uint64_t total = s->dim * (uint64_t)s->batch;
+  if (s->dtype == SMOQ_C128)
+    hipLaunchKernelGGL(k_scale<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (double2*)s->d, total, std::cos(b * t), -std::sin(b * t));
+  else
+    hipLaunchKernelGGL(k_scale<float>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (float2*)s->d, total, std::cos(b * t), -std::sin(b * t));
+  if (hipGetLastError() != hipSuccess)
+    return done(set_error(SMOQ_EDEVICE, "smoq_chebyshev_evolve: k_scale launch failed"));
+  if (orders_used) *orders_used = m_used;
+  return done(SMOQ_OK);
+}

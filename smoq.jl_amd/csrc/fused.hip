@@ -298,33 +298,49 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <typename T>
-static int run_program_t(smoq_state* s, const Program& prog) {
-  smoq_ctx* ctx = s->ctx;
-  const size_t pb = prog.passes.size() * sizeof(PassDev);
-  const size_t sb = prog.stages.size() * sizeof(StageDev);
-  const size_t gb = prog.gates.size() * sizeof(GateDev);
-  const size_t ntab = prog.tabs.size() / 2;
-  const size_t tb = (ntab + 1) * sizeof(VT<T>);
-  const size_t off_t = (pb + sb + gb + 255) & ~(size_t)255;
-  const size_t total = off_t + tb;
-  int rc = ensure_prog(ctx, total);
+ProgLayout program_layout(const Program& prog, int dtype) {
+  ProgLayout L;
+  L.pb = prog.passes.size() * sizeof(PassDev);
+  L.sb = prog.stages.size() * sizeof(StageDev);
+  L.gb = prog.gates.size() * sizeof(GateDev);
+  L.ntab = prog.tabs.size() / 2;
+  const size_t amp = dtype == SMOQ_C128 ? sizeof(double2) : sizeof(float2);
+  L.off_t = (L.pb + L.sb + L.gb + 255) & ~(size_t)255;
+  L.total = L.off_t + (L.ntab + 1) * amp;
+  return L;
+}
+
+// copies the program into device buffer d (L.total bytes) through the
+// context's pinned staging buffer, ordered on the context stream
+int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayout& L, void* d) {
+  int rc = ensure_pinned(ctx, L.total);
   if (rc) return rc;
-  if ((rc = ensure_pinned(ctx, total))) return rc;
   // the pinned staging buffer may still feed an earlier copy
   SMOQ_HIP(hipStreamSynchronize(ctx->stream));
   unsigned char* h = (unsigned char*)ctx->host_pinned;
-  std::memcpy(h, prog.passes.data(), pb);
-  std::memcpy(h + pb, prog.stages.data(), sb);
-  std::memcpy(h + pb + sb, prog.gates.data(), gb);
-  VT<T>* ht = (VT<T>*)(h + off_t);
-  for (size_t i = 0; i < ntab; ++i) ht[i] = make_c<T>(prog.tabs[2 * i], prog.tabs[2 * i + 1]);
-  SMOQ_HIP(hipMemcpyAsync(ctx->prog, h, total, hipMemcpyHostToDevice, ctx->stream));
-  unsigned char* d = (unsigned char*)ctx->prog;
+  std::memcpy(h, prog.passes.data(), L.pb);
+  std::memcpy(h + L.pb, prog.stages.data(), L.sb);
+  std::memcpy(h + L.pb + L.sb, prog.gates.data(), L.gb);
+  for (size_t i = 0; i < L.ntab; ++i) {
+    if (dtype == SMOQ_C128)
+      ((double2*)(h + L.off_t))[i] = make_c<double>(prog.tabs[2 * i], prog.tabs[2 * i + 1]);
+    else
+      ((float2*)(h + L.off_t))[i] = make_c<float>(prog.tabs[2 * i], prog.tabs[2 * i + 1]);
+  }
+  SMOQ_HIP(hipMemcpyAsync(d, h, L.total, hipMemcpyHostToDevice, ctx->stream));
+  return SMOQ_OK;
+}
+
+template <typename T>
+static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout& L,
+                            const void* dbuf) {
+  smoq_ctx* ctx = s->ctx;
+  int rc = SMOQ_OK;
+  const unsigned char* d = (const unsigned char*)dbuf;
   const PassDev* dp = (const PassDev*)d;
-  const StageDev* ds = (const StageDev*)(d + pb);
-  const GateDev* dg = (const GateDev*)(d + pb + sb);
-  const VT<T>* dt = (const VT<T>*)(d + off_t);
+  const StageDev* ds = (const StageDev*)(d + L.pb);
+  const GateDev* dg = (const GateDev*)(d + L.pb + L.sb);
+  const VT<T>* dt = (const VT<T>*)(d + L.off_t);
   for (size_t p = 0; p < prog.passes.size(); ++p) {
     const PassDev& pd = prog.passes[p];
     const uint64_t tiles = (uint64_t)s->batch << pd.n_nonlocal;
@@ -342,8 +358,17 @@ static int run_program_t(smoq_state* s, const Program& prog) {
   return SMOQ_OK;
 }
 
+int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf) {
+  return s->dtype == SMOQ_C128 ? launch_program_t<double>(s, prog, L, dbuf)
+                               : launch_program_t<float>(s, prog, L, dbuf);
+}
+
 int run_program(smoq_state* s, const Program& prog) {
-  return s->dtype == SMOQ_C128 ? run_program_t<double>(s, prog) : run_program_t<float>(s, prog);
+  const ProgLayout L = program_layout(prog, s->dtype);
+  int rc = ensure_prog(s->ctx, L.total);
+  if (rc) return rc;
+  if ((rc = upload_program(s->ctx, prog, s->dtype, L, s->ctx->prog))) return rc;
+  return launch_program(s, prog, L, s->ctx->prog);
 }
 
 }  // namespace smoq

@@ -1,0 +1,168 @@
+// Compiled tapes: planned once, program resident on the device, launches
+// replayed through a hipGraph.
+#include "device_common.h"
+#include "smoq_internal.h"
+
+struct smoq_plan_s {
+  smoq_ctx* ctx = nullptr;
+  int n = 0;
+  int dtype = SMOQ_C128;
+  smoq::Program prog;
+  smoq::ProgLayout layout;
+  void* dev = nullptr;  // device image of prog
+  std::vector<smoq::IGate> small;  // n < kRegQubits: replayed gate by gate
+  // captured launches of one application, valid for one (state, batch)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* graph_state = nullptr;
+  int graph_batch = 0;
+};
+
+namespace smoq {
+
+static void drop_graph(smoq_plan* p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  p->graph_state = nullptr;
+}
+
+static int run_once(smoq_plan* p, smoq_state* s) {
+  if (!p->small.empty()) {
+    for (auto& g : p->small) {
+      int rc = launch_gate(s, g);
+      if (rc) return rc;
+    }
+    return SMOQ_OK;
+  }
+  return launch_program(s, p->prog, p->layout, p->dev);
+}
+
+// capture one application of the plan on state s into an executable graph
+static int capture(smoq_plan* p, smoq_state* s) {
+  drop_graph(p);
+  hipStream_t st = p->ctx->stream;
+  SMOQ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = run_once(p, s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+  p->graph = g;
+  SMOQ_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+  p->graph_state = s->d;
+  p->graph_batch = s->batch;
+  return SMOQ_OK;
+}
+
+static int plan_run(smoq_plan* p, smoq_state* s, int repeat) {
+  if (repeat <= 0) return SMOQ_OK;
+  // pass timing records events between launches, which a graph cannot hold
+  if (p->ctx->timing) {
+    for (int r = 0; r < repeat; ++r) {
+      int rc = run_once(p, s);
+      if (rc) return rc;
+    }
+    return SMOQ_OK;
+  }
+  if (!p->exec || p->graph_state != s->d || p->graph_batch != s->batch) {
+    int rc = capture(p, s);
+    if (rc) return rc;
+  }
+  for (int r = 0; r < repeat; ++r) SMOQ_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
+  return SMOQ_OK;
+}
+
+}  // namespace smoq
+
+using namespace smoq;
+
+extern "C" {
+
+int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops, int n_ops,
+                     const double* theta, int n_theta, smoq_plan** out) {
+  if (!ctx || !out) return set_error(SMOQ_EINVAL, "smoq_plan_create: null argument");
+  *out = nullptr;
+  if (n_qubits < 1 || n_qubits > 36)
+    return set_error(SMOQ_EINVAL, "smoq_plan_create: n_qubits %d outside [1, 36]", n_qubits);
+  if (dtype != SMOQ_C128 && dtype != SMOQ_C64)
+    return set_error(SMOQ_EINVAL, "smoq_plan_create: unknown dtype %d", dtype);
+  std::vector<IGate> gs;
+  int rc = lower_tape(ops, n_ops, theta, n_theta, n_qubits, gs);
+  if (rc) return rc;
+  smoq_plan* p = new smoq_plan();
+  p->ctx = ctx;
+  p->n = n_qubits;
+  p->dtype = dtype;
+  if (n_qubits < kRegQubits) {
+    p->small = gs;
+    *out = p;
+    return SMOQ_OK;
+  }
+  plan_program(gs, n_qubits, ctx->tile_qubits, ctx->low_qubits,
+               dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog);
+  p->layout = program_layout(p->prog, dtype);
+  hipError_t e = hipMalloc(&p->dev, p->layout.total);
+  if (e != hipSuccess) {
+    delete p;
+    return hip_error(e, "hipMalloc(plan)");
+  }
+  if ((rc = upload_program(ctx, p->prog, dtype, p->layout, p->dev))) {
+    (void)hipFree(p->dev);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return SMOQ_OK;
+}
+
+int smoq_plan_passes(const smoq_plan* p, int* n_passes) {
+  if (!p || !n_passes) return set_error(SMOQ_EINVAL, "smoq_plan_passes: null argument");
+  *n_passes = p->small.empty() ? (int)p->prog.passes.size() : (int)p->small.size();
+  return SMOQ_OK;
+}
+
+int smoq_plan_run(smoq_plan* p, smoq_state* s, int repeat) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!p) return set_error(SMOQ_EINVAL, "smoq_plan_run: null plan");
+  if (s->n != p->n || s->dtype != p->dtype || s->ctx != p->ctx)
+    return set_error(SMOQ_EINVAL, "smoq_plan_run: plan is for %d qubits dtype %d", p->n,
+                     p->dtype);
+  return plan_run(p, s, repeat);
+}
+
+int smoq_plan_free(smoq_plan* p) {
+  if (!p) return SMOQ_OK;
+  (void)hipStreamSynchronize(p->ctx->stream);
+  drop_graph(p);
+  if (p->dev) (void)hipFree(p->dev);
+  delete p;
+  return SMOQ_OK;
+}
+
+int smoq_evolve_observe(smoq_plan* p, smoq_state* s, int n_steps, int obs_every, int n_terms,
+                        const uint64_t* xmask, const uint64_t* zmask, double* out) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!p) return set_error(SMOQ_EINVAL, "smoq_evolve_observe: null plan");
+  if (n_steps < 0 || obs_every < 1)
+    return set_error(SMOQ_EINVAL, "smoq_evolve_observe: n_steps %d, obs_every %d", n_steps,
+                     obs_every);
+  if (s->n != p->n || s->dtype != p->dtype)
+    return set_error(SMOQ_EINVAL, "smoq_evolve_observe: plan/state shape mismatch");
+  const int records = n_steps / obs_every;
+  for (int r = 0; r < records; ++r) {
+    if ((rc = plan_run(p, s, obs_every))) return rc;
+    if (n_terms > 0 &&
+        (rc = reduce_pauli(s, n_terms, xmask, zmask, out + (size_t)r * s->batch * n_terms, 0)))
+      return rc;
+  }
+  return plan_run(p, s, n_steps - records * obs_every);
+}
+
+}  // extern "C"

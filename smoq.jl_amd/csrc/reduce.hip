@@ -229,6 +229,13 @@ static int expect_entry(smoq_state* s, int n_terms, const uint64_t* xmask,
                                : expect_t<float>(s, n_terms, xmask, zmask, out, ref_mode);
 }
 
+namespace smoq {
+int reduce_pauli(smoq_state* s, int n_terms, const uint64_t* xm, const uint64_t* zm,
+                 double* out_host, int ref_mode) {
+  return expect_entry(s, n_terms, xm, zm, out_host, ref_mode);
+}
+}  // namespace smoq
+
 extern "C" int smoq_expect_pauli(smoq_state* s, int n_terms, const uint64_t* xmask,
                                  const uint64_t* zmask, double* out) {
   return expect_entry(s, n_terms, xmask, zmask, out, 0);

@@ -269,6 +269,25 @@ static bool is_rotation(int gate) {
          gate == SMOQ_G_RXX || gate == SMOQ_G_RYY || gate == SMOQ_G_RZZ;
 }
 
+int lower_tape(const smoq_op* ops, int n_ops, const double* theta, int n_theta, int n,
+               std::vector<IGate>& gs) {
+  if (n_ops < 0 || (n_ops > 0 && !ops)) return set_error(SMOQ_EINVAL, "bad tape");
+  gs.reserve(gs.size() + n_ops);
+  for (int i = 0; i < n_ops; ++i) {
+    const smoq_op& op = ops[i];
+    double th = op.theta;
+    if (is_rotation(op.gate) && op.param >= 0) {
+      if (!theta || op.param >= n_theta)
+        return set_error(SMOQ_EINVAL, "tape op %d uses theta[%d], %d given", i, op.param,
+                         n_theta);
+      th = theta[op.param];
+    }
+    int rc = lower_op(op, th, n, gs);
+    if (rc) return rc;
+  }
+  return SMOQ_OK;
+}
+
 }  // namespace smoq
 
 using namespace smoq;
@@ -500,20 +519,8 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
                     int n_theta, int* n_passes) {
   int rc = check_state(s);
   if (rc) return rc;
-  if (n_ops < 0 || (n_ops > 0 && !ops)) return set_error(SMOQ_EINVAL, "smoq_apply_tape: bad tape");
   std::vector<IGate> gs;
-  gs.reserve(n_ops);
-  for (int i = 0; i < n_ops; ++i) {
-    const smoq_op& op = ops[i];
-    double th = op.theta;
-    if (is_rotation(op.gate) && op.param >= 0) {
-      if (!theta || op.param >= n_theta)
-        return set_error(SMOQ_EINVAL, "smoq_apply_tape: op %d uses theta[%d], %d given", i,
-                         op.param, n_theta);
-      th = theta[op.param];
-    }
-    if ((rc = lower_op(op, th, s->n, gs))) return rc;
-  }
+  if ((rc = lower_tape(ops, n_ops, theta, n_theta, s->n, gs))) return rc;
   Program prog;
   plan_program(gs, s->n, s->ctx->tile_qubits, s->ctx->low_qubits,
                s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog);

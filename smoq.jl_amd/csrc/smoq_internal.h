@@ -191,4 +191,17 @@ int time_mark(smoq_ctx* ctx);
 // launchers implemented in the kernel translation units
 int launch_gate(smoq_state* s, const IGate& g);
 int run_program(smoq_state* s, const Program& prog);
+
+// device image of a Program: passes | stages | gates | phase tables
+struct ProgLayout {
+  size_t pb = 0, sb = 0, gb = 0, ntab = 0, off_t = 0, total = 0;
+};
+ProgLayout program_layout(const Program& prog, int dtype);
+int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayout& L, void* d);
+int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf);
+// lowers a tape (angles resolved from theta) to internal gates
+int lower_tape(const smoq_op* ops, int n_ops, const double* theta, int n_theta, int n_qubits,
+               std::vector<IGate>& out);
+int reduce_pauli(smoq_state* s, int n_terms, const uint64_t* xm, const uint64_t* zm,
+                 double* out_host, int ref_mode);
 }  // namespace smoq

@@ -82,6 +82,13 @@ SIGNATURES = {
     "smoq_apply_tape": [_P, _P, _i, _P, _i, ctypes.POINTER(_i)],
     "smoq_expect_pauli": [_P, _i, _P, _P, _P],
     "smoq_pauli_string_reference": [_P, _i, _P, _P, _P],
+    "smoq_plan_create": [_P, _i, _i, _P, _i, _P, _i, ctypes.POINTER(_P)],
+    "smoq_plan_passes": [_P, ctypes.POINTER(_i)],
+    "smoq_plan_run": [_P, _P, _i],
+    "smoq_plan_free": [_P],
+    "smoq_evolve_observe": [_P, _P, _i, _i, _i, _P, _P, _P],
+    "smoq_hpsi": [_P, _P, _i, _P, _P, _P],
+    "smoq_chebyshev_evolve": [_P, _i, _P, _P, _P, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
     "smoq_norm2": [_P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
 }

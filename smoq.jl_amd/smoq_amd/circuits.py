@@ -77,11 +77,68 @@ class Tape:
         call("smoq_apply_tape", psi.handle, arr, len(self.ops), th_p, nth, ctypes.byref(npass))
         return npass.value
 
+    def compile(self, dtype="c128", theta=None, ctx=None) -> "Plan":
+        return Plan(self, dtype, theta, ctx)
+
     def logical_gates(self) -> int:
         return len(self.ops)
 
     def rotation_params(self) -> list:
         return [(i, op) for i, op in enumerate(self.ops) if G[op[0]] in ROTATIONS and op[3] >= 0]
+
+
+class Plan:
+    """A tape planned once (smoq_plan_*): device-resident fused-pass program,
+    launches replayed through a hipGraph on every run."""
+
+    def __init__(self, tape: Tape, dtype="c128", theta=None, ctx=None):
+        from .state import _DTYPES, default_context
+
+        self.ctx = ctx or default_context()
+        self.n = tape.n
+        self.dtype = _DTYPES[dtype]
+        arr = tape.to_ctypes()
+        th = None if theta is None else np.ascontiguousarray(theta, np.float64)
+        h = ctypes.c_void_p()
+        call("smoq_plan_create", self.ctx.handle, tape.n, self.dtype, arr, len(tape.ops),
+             None if th is None else th.ctypes.data_as(ctypes.c_void_p),
+             0 if th is None else th.size, ctypes.byref(h))
+        self.handle = h
+        self.gates = len(tape.ops)
+
+    @property
+    def passes(self) -> int:
+        n = ctypes.c_int()
+        call("smoq_plan_passes", self.handle, ctypes.byref(n))
+        return n.value
+
+    def run(self, psi, repeat: int = 1):
+        call("smoq_plan_run", self.handle, psi.handle, int(repeat))
+        return psi
+
+    def evolve_observe(self, psi, n_steps: int, obs_every: int, xmask, zmask) -> np.ndarray:
+        """Runs n_steps applications, recording <P_t> after every obs_every-th:
+        array (n_steps // obs_every, [batch,] n_terms)."""
+        xm = np.ascontiguousarray(np.asarray(xmask, np.uint64))
+        zm = np.ascontiguousarray(np.asarray(zmask, np.uint64))
+        rec = n_steps // obs_every
+        out = np.zeros((rec, psi.batch, xm.size), np.float64)
+        call("smoq_evolve_observe", self.handle, psi.handle, int(n_steps), int(obs_every),
+             xm.size, xm.ctypes.data_as(ctypes.c_void_p), zm.ctypes.data_as(ctypes.c_void_p),
+             out.ctypes.data_as(ctypes.c_void_p))
+        return out[:, 0, :] if psi.batch == 1 else out
+
+    def free(self):
+        if getattr(self, "handle", None):
+            from . import _lib
+            _lib.lib().smoq_plan_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ----------------------------------------------------------------- workloads --
