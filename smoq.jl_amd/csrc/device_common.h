@@ -38,6 +38,14 @@ __host__ __device__ inline typename CT<T>::V make_c(double re, double im) {
 }
 
 template <typename V>
+__device__ inline double2 to_d2(V a) {
+  double2 r;
+  r.x = (double)a.x;
+  r.y = (double)a.y;
+  return r;
+}
+
+template <typename V>
 __device__ inline V cmul(V a, V b) {
   V r;
   r.x = a.x * b.x - a.y * b.y;

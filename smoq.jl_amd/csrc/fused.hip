@@ -209,15 +209,18 @@ __device__ __forceinline__ void apply_gate(VT<T> (&x)[kNR], const GateDev& g, ui
   }
 }
 
+// T is the storage type of the state in HBM; the tile lives in LDS and in
+// registers as complex128 whatever T is, so a complex64 state is rounded
+// once per pass (at the store) instead of after every gate.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_fused_pass(VT<T>* __restrict__ psi, int n, const PassDev* __restrict__ pass,
                  const StageDev* __restrict__ stages, const GateDev* __restrict__ gates,
-                 const VT<T>* __restrict__ tabs) {
+                 const double2* __restrict__ tabs) {
   constexpr int R = kRegQubits;
-  constexpr int WB = CT<T>::kRowBits;
+  constexpr int WB = CT<double>::kRowBits;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  VT<T>* lds = reinterpret_cast<VT<T>*>(smem_raw);
+  double2* lds = reinterpret_cast<double2*>(smem_raw);
 
   const int k = pass->k;
   const int kr = k - R;
@@ -250,7 +253,7 @@ __global__ void __launch_bounds__(256)
       x[i] = pt[o];
     }
 #pragma unroll
-    for (int i = 0; i < kNR; ++i) lds[st_lane ^ swz<WB>((uint32_t)(i * NT))] = x[i];
+    for (int i = 0; i < kNR; ++i) lds[st_lane ^ swz<WB>((uint32_t)(i * NT))] = to_d2(x[i]);
   }
   __syncthreads();
 
@@ -275,17 +278,17 @@ __global__ void __launch_bounds__(256)
       for (int j = 0; j < R; ++j) r ^= ((i >> j) & 1) ? st.sw_reg[j] : 0u;
       sr[i] = r;
     }
-    VT<T> x[kNR];
+    VT<double> x[kNR];
 #pragma unroll
     for (int i = 0; i < kNR; ++i) x[i] = lds[sl ^ sr[i]];
-    for (int g = st.g0; g < st.g1; ++g) apply_gate<T>(x, gates[g], gl, tabs);
+    for (int g = st.g0; g < st.g1; ++g) apply_gate<double>(x, gates[g], gl, tabs);
 #pragma unroll
     for (int i = 0; i < kNR; ++i) lds[sl ^ sr[i]] = x[i];
     __syncthreads();
   }
 
   {
-    VT<T> x[kNR];
+    double2 x[kNR];
 #pragma unroll
     for (int i = 0; i < kNR; ++i) x[i] = lds[st_lane ^ swz<WB>((uint32_t)(i * NT))];
 #pragma unroll
@@ -293,7 +296,7 @@ __global__ void __launch_bounds__(256)
       uint64_t o = 0;
 #pragma unroll
       for (int j = 0; j < R; ++j) o |= ((i >> j) & 1) ? hb[j] : 0ull;
-      pt[o] = x[i];
+      pt[o] = make_c<T>(x[i].x, x[i].y);
     }
   }
 }
@@ -304,9 +307,9 @@ ProgLayout program_layout(const Program& prog, int dtype) {
   L.sb = prog.stages.size() * sizeof(StageDev);
   L.gb = prog.gates.size() * sizeof(GateDev);
   L.ntab = prog.tabs.size() / 2;
-  const size_t amp = dtype == SMOQ_C128 ? sizeof(double2) : sizeof(float2);
+  (void)dtype;  // phase tables are complex128 for every state dtype
   L.off_t = (L.pb + L.sb + L.gb + 255) & ~(size_t)255;
-  L.total = L.off_t + (L.ntab + 1) * amp;
+  L.total = L.off_t + (L.ntab + 1) * sizeof(double2);
   return L;
 }
 
@@ -321,12 +324,8 @@ int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayo
   std::memcpy(h, prog.passes.data(), L.pb);
   std::memcpy(h + L.pb, prog.stages.data(), L.sb);
   std::memcpy(h + L.pb + L.sb, prog.gates.data(), L.gb);
-  for (size_t i = 0; i < L.ntab; ++i) {
-    if (dtype == SMOQ_C128)
-      ((double2*)(h + L.off_t))[i] = make_c<double>(prog.tabs[2 * i], prog.tabs[2 * i + 1]);
-    else
-      ((float2*)(h + L.off_t))[i] = make_c<float>(prog.tabs[2 * i], prog.tabs[2 * i + 1]);
-  }
+  (void)dtype;
+  if (L.ntab) std::memcpy(h + L.off_t, prog.tabs.data(), L.ntab * sizeof(double2));
   SMOQ_HIP(hipMemcpyAsync(d, h, L.total, hipMemcpyHostToDevice, ctx->stream));
   return SMOQ_OK;
 }
@@ -340,7 +339,7 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
   const PassDev* dp = (const PassDev*)d;
   const StageDev* ds = (const StageDev*)(d + L.pb);
   const GateDev* dg = (const GateDev*)(d + L.pb + L.sb);
-  const VT<T>* dt = (const VT<T>*)(d + L.off_t);
+  const double2* dt = (const double2*)(d + L.off_t);
   for (size_t p = 0; p < prog.passes.size(); ++p) {
     const PassDev& pd = prog.passes[p];
     const uint64_t tiles = (uint64_t)s->batch << pd.n_nonlocal;
@@ -348,7 +347,7 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
     if (nt > 256 || pd.k > kMaxTileQubits)
       return set_error(SMOQ_EINVAL, "fused pass: tile of 2^%d needs %d threads (max 256)", pd.k,
                        nt);
-    const size_t shm = sizeof(VT<T>) << pd.k;
+    const size_t shm = sizeof(double2) << pd.k;
     if (ctx->timing && (rc = time_mark(ctx))) return rc;
     hipLaunchKernelGGL(k_fused_pass<T>, dim3((unsigned)tiles), dim3(nt), shm, ctx->stream,
                        (VT<T>*)s->d, s->n, dp + p, ds, dg, dt);

@@ -1,91 +1,89 @@
-// Gate kernels for gfx950: one-gate streaming kernels and the fused
-// LDS-tile pass kernel that applies a whole stage program per HBM pass.
-#include "smoq_internal.h"
-#include "device_common.h"
-
-namespace smoq {
-
-// ----------------------------------------------------- one-gate kernels --
+// One-gate streaming kernels for gfx950 and the basis-state fill.
+//
 // Each thread owns one amplitude pair (quad for 2-qubit gates).  Pair index
 // t enumerates the indices with the target bit cleared; inserting a zero at
 // the target keeps lanes on consecutive addresses for both halves of the
 // pair, so every load and store is a coalesced 16 B (c128) / 8 B (c64)
 // access.  Batch items are contiguous, so the bits above n carry the item.
+// T is the storage type; the arithmetic is complex128 for both dtypes.
+#include "device_common.h"
+#include "smoq_internal.h"
+
+namespace smoq {
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_u1(typename CT<T>::V* __restrict__ psi, uint64_t npairs, int q, uint64_t ctrl, Mat2<T> u) {
-  using V = typename CT<T>::V;
-  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    k_u1(VT_alias<T>* __restrict__ psi, uint64_t npairs, int q, uint64_t ctrl, Mat2<double> u) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npairs) return;
-  uint64_t lo = insert_zero(t, q);
+  const uint64_t lo = insert_zero(t, q);
   if ((lo & ctrl) != ctrl) return;
-  uint64_t hi = lo | (1ull << q);
-  V a = psi[lo], b = psi[hi];
-  psi[lo] = cfma2(u.m[0], a, u.m[1], b);
-  psi[hi] = cfma2(u.m[2], a, u.m[3], b);
+  const uint64_t hi = lo | (1ull << q);
+  const double2 a = to_d2(psi[lo]), b = to_d2(psi[hi]);
+  const double2 na = cfma2(u.m[0], a, u.m[1], b);
+  const double2 nb = cfma2(u.m[2], a, u.m[3], b);
+  psi[lo] = make_c<T>(na.x, na.y);
+  psi[hi] = make_c<T>(nb.x, nb.y);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_u2(typename CT<T>::V* __restrict__ psi, uint64_t nquads, int qa, int qb, uint64_t ctrl,
-         Mat4<T> u) {
-  using V = typename CT<T>::V;
-  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    k_u2(VT_alias<T>* __restrict__ psi, uint64_t nquads, int qa, int qb, uint64_t ctrl,
+         Mat4<double> u) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nquads) return;
-  int lo_b = qa < qb ? qa : qb, hi_b = qa < qb ? qb : qa;
-  uint64_t i0 = insert_zero(insert_zero(t, lo_b), hi_b);
+  const int lo_b = qa < qb ? qa : qb, hi_b = qa < qb ? qb : qa;
+  const uint64_t i0 = insert_zero(insert_zero(t, lo_b), hi_b);
   if ((i0 & ctrl) != ctrl) return;
-  uint64_t ma = 1ull << qa, mb = 1ull << qb;
-  uint64_t idx[4] = {i0, i0 | ma, i0 | mb, i0 | ma | mb};
-  V x[4];
+  const uint64_t ma = 1ull << qa, mb = 1ull << qb;
+  const uint64_t idx[4] = {i0, i0 | ma, i0 | mb, i0 | ma | mb};
+  double2 x[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) x[e] = psi[idx[e]];
+  for (int e = 0; e < 4; ++e) x[e] = to_d2(psi[idx[e]]);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    V acc = cmul(u.m[4 * r], x[0]);
+    double2 acc = cmul(u.m[4 * r], x[0]);
 #pragma unroll
     for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(u.m[4 * r + c], x[c]));
-    psi[idx[r]] = acc;
+    psi[idx[r]] = make_c<T>(acc.x, acc.y);
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_diag(typename CT<T>::V* __restrict__ psi, uint64_t total, uint64_t dim_mask, int nd, int q0,
-           int q1, int q2, Diag8<T> ph) {
-  using V = typename CT<T>::V;
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    k_diag(VT_alias<T>* __restrict__ psi, uint64_t total, uint64_t dim_mask, int nd, int q0,
+           int q1, int q2, Diag8<double> ph) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
-  uint64_t g = i & dim_mask;
+  const uint64_t g = i & dim_mask;
   int e = (int)((g >> q0) & 1);
   if (nd > 1) e |= (int)((g >> q1) & 1) << 1;
   if (nd > 2) e |= (int)((g >> q2) & 1) << 2;
-  psi[i] = cmul(ph.p[e], psi[i]);
+  const double2 v = cmul(ph.p[e], to_d2(psi[i]));
+  psi[i] = make_c<T>(v.x, v.y);
 }
 
 template <typename T>
 static int launch_gate_t(smoq_state* s, const IGate& g) {
-  using V = typename CT<T>::V;
-  V* psi = (V*)s->d;
+  VT_alias<T>* psi = (VT_alias<T>*)s->d;
   hipStream_t st = s->ctx->stream;
   const uint64_t total = s->dim * (uint64_t)s->batch;
   const int B = 256;
   if (g.kind == IK_U1) {
-    Mat2<T> u;
-    for (int i = 0; i < 4; ++i) u.m[i] = make_c<T>(g.m[2 * i], g.m[2 * i + 1]);
-    uint64_t np = total >> 1;
+    Mat2<double> u;
+    for (int i = 0; i < 4; ++i) u.m[i] = make_c<double>(g.m[2 * i], g.m[2 * i + 1]);
+    const uint64_t np = total >> 1;
     hipLaunchKernelGGL(k_u1<T>, dim3((unsigned)((np + B - 1) / B)), dim3(B), 0, st, psi, np, g.t0,
                        g.ctrl, u);
   } else if (g.kind == IK_U2) {
-    Mat4<T> u;
-    for (int i = 0; i < 16; ++i) u.m[i] = make_c<T>(g.m[2 * i], g.m[2 * i + 1]);
-    uint64_t nq = total >> 2;
+    Mat4<double> u;
+    for (int i = 0; i < 16; ++i) u.m[i] = make_c<double>(g.m[2 * i], g.m[2 * i + 1]);
+    const uint64_t nq = total >> 2;
     hipLaunchKernelGGL(k_u2<T>, dim3((unsigned)((nq + B - 1) / B)), dim3(B), 0, st, psi, nq, g.t0,
                        g.t1, g.ctrl, u);
   } else {
-    Diag8<T> ph;
-    for (int i = 0; i < (1 << g.nd); ++i) ph.p[i] = make_c<T>(g.m[2 * i], g.m[2 * i + 1]);
+    Diag8<double> ph;
+    for (int i = 0; i < (1 << g.nd); ++i) ph.p[i] = make_c<double>(g.m[2 * i], g.m[2 * i + 1]);
     hipLaunchKernelGGL(k_diag<T>, dim3((unsigned)((total + B - 1) / B)), dim3(B), 0, st, psi, total,
                        s->dim - 1, g.nd, g.dq[0], g.dq[1] < 0 ? 0 : g.dq[1],
                        g.dq[2] < 0 ? 0 : g.dq[2], ph);
@@ -102,7 +100,7 @@ int launch_gate(smoq_state* s, const IGate& g) {
 // can sit inside a captured graph.
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_set_basis(typename CT<T>::V* __restrict__ psi, uint64_t total, uint64_t dim_mask,
+    k_set_basis(VT_alias<T>* __restrict__ psi, uint64_t total, uint64_t dim_mask,
                 uint64_t index) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;

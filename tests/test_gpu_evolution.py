@@ -138,9 +138,14 @@ def test_trotter_converges_to_chebyshev(ctx):
 
     N = 12
     params, _ = xxz(N, 7)
-    plan = trotter_tape(second_order_gates(params, 0.01), N).compile()
-    a = GPUState.from_numpy(neel(N))
-    plan.run(a, 100)
     b = GPUState.from_numpy(neel(N))
     chebyshev_evolve_psi(b, params, 1.0)
-    assert rel(a.to_numpy(), b.to_numpy()) < 1e-4
+    exact = b.to_numpy()
+    errs = []
+    for dt, steps in ((0.02, 50), (0.01, 100)):
+        plan = trotter_tape(second_order_gates(params, dt), N).compile()
+        a = GPUState.from_numpy(neel(N))
+        plan.run(a, steps)
+        errs.append(rel(a.to_numpy(), exact))
+    assert errs[1] < 1e-3
+    assert 3.5 < errs[0] / errs[1] < 4.5, errs
