@@ -180,6 +180,29 @@ int smoq_chebyshev_evolve(smoq_state* s, int n_terms, const uint64_t* xmask,
                           const uint64_t* zmask, const double* coeff, double t, double emin,
                           double emax, double tol, int max_order, int* orders_used);
 
+/* ---- parametrised circuits: adjoint gradient -----------------------------
+ * E_b(θ) = <0|U(θ_b)† H U(θ_b)|0> for every batch item b of `s`, with
+ * H = sum_t coeff_t P_t (Pauli masks as in smoq_expect_pauli) and the tape's
+ * rotation gates (RX, RY, RZ, RXX, RYY, RZZ, CRY) taking theta[b*n_params +
+ * op.param] when op.param >= 0 (R(θ) = exp(-iθG/2)).  One forward sweep,
+ * λ = Hψ, then one reverse sweep that un-applies each gate to ψ and λ and,
+ * for a parametrised gate, accumulates dE/dθ = Im<λ|G|ψ> in the same pass.
+ * On return s holds U(θ_b)|0> per item; energy[b], grad[b*n_params + p].
+ * Replaces calculate_gradients (Enzyme) and calculate_gradients_parameter_shift
+ * (utils/cpu/cpuVariationalQuantumCircuitGradients.jl:88-110,138-163). */
+int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta,
+                          int n_params, int n_terms, const uint64_t* xmask,
+                          const uint64_t* zmask, const double* coeff, double* energy,
+                          double* grad);
+/* <H> after the tape for every item: the cost a parameter-shift or finite
+ * difference evaluates (per-item theta as above; s receives U(θ_b)|0>).
+ * flags & SMOQ_ENERGY_ONE_GATE_KERNELS: apply the tape gate by gate with the
+ * one-gate kernels instead of fused passes. */
+#define SMOQ_ENERGY_ONE_GATE_KERNELS 1
+int smoq_tape_energy(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta,
+                     int n_params, int n_terms, const uint64_t* xmask, const uint64_t* zmask,
+                     const double* coeff, int flags, double* energy);
+
 /* ---- observables --------------------------------------------------------
  * <P_t> for Pauli strings P_t = i^{|x_t & z_t|} X^{x_t} Z^{z_t} (x/z bit
  * masks; Y on qubit q sets bit q in both).  out has batch*n_terms entries

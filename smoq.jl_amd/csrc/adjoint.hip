@@ -1,0 +1,437 @@
+// Adjoint gradients of parametrised circuits.
+//
+// A parametrised gate is R(θ) = exp(-iθG/2) with G = Π_C ⊗ P: a Pauli string
+// P = i^{|x&z|} X^x Z^z, optionally restricted to indices whose control bits
+// C are all set (CRY).  R(θ) = cos(θ/2) I - i sin(θ/2) G on the controlled
+// subspace.  The reverse sweep keeps ψ_g (state after gate g) and
+// λ_g = U_{>g}† H ψ_N; dE/dθ_g = 2 Re<λ_g|(-i/2) G ψ_g> = Im<λ_g|G|ψ_g>,
+// then both vectors are un-rotated by R(-θ_g) in the same pass.
+#include <algorithm>
+#include <cmath>
+
+#include "device_common.h"
+#include "smoq_internal.h"
+
+namespace smoq {
+
+constexpr int kAdjBlock = 256;
+
+struct PauliRot {
+  uint64_t x, z, ctrl;
+  int cnt;  // |x & z| mod 4
+  int h;    // highest bit of x (x != 0)
+};
+
+__device__ inline double2 ipow_c(int cnt) {
+  double2 r;
+  r.x = (cnt == 0) ? 1.0 : (cnt == 2) ? -1.0 : 0.0;
+  r.y = (cnt == 1) ? 1.0 : (cnt == 3) ? -1.0 : 0.0;
+  return r;
+}
+
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Rotates the item's vector(s) by R(sign*θ) and, when lam is given,
+// accumulates Im<λ|G|ψ> (before the rotation) into partial[item*bpi + block].
+template <typename T>
+__global__ void __launch_bounds__(kAdjBlock)
+    k_prot(VT_alias<T>* __restrict__ psi, VT_alias<T>* __restrict__ lam, int n, PauliRot g,
+           const double2* __restrict__ cs, int n_params, int pidx, double sign,
+           double* __restrict__ partial) {
+  __shared__ double sh[kAdjBlock / 64];
+  const uint64_t item = blockIdx.y;
+  const uint64_t dim = 1ull << n;
+  VT_alias<T>* __restrict__ p = psi + item * dim;
+  VT_alias<T>* __restrict__ l = lam ? lam + item * dim : nullptr;
+  const double2 ang = cs[item * n_params + pidx];
+  const double c = ang.x, s = sign * ang.y;
+  const double2 ic = ipow_c(g.cnt);
+  double ov = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  if (g.x) {
+    const uint64_t half = dim >> 1;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < half; t += stride) {
+      const uint64_t k = insert_zero(t, g.h);
+      if ((k & g.ctrl) != g.ctrl) continue;
+      const uint64_t j = k ^ g.x;
+      // <k|G|j> = i^cnt (-1)^{|j&z|},  <j|G|k> = i^cnt (-1)^{|k&z|}
+      double2 fkj = ic, fjk = ic;
+      if (__popcll(j & g.z) & 1) fkj = make_c<double>(-ic.x, -ic.y);
+      if (__popcll(k & g.z) & 1) fjk = make_c<double>(-ic.x, -ic.y);
+      // -i s f
+      const double2 mkj = make_c<double>(s * fkj.y, -s * fkj.x);
+      const double2 mjk = make_c<double>(s * fjk.y, -s * fjk.x);
+      const double2 a = to_d2(p[k]), b = to_d2(p[j]);
+      if (l) {
+        const double2 la = to_d2(l[k]), lb = to_d2(l[j]);
+        // conj(λk) fkj ψj + conj(λj) fjk ψk : imaginary part
+        const double2 u = cmul(fkj, b), v = cmul(fjk, a);
+        ov += (la.x * u.y - la.y * u.x) + (lb.x * v.y - lb.y * v.x);
+        const double2 nla = cadd(make_c<double>(c * la.x, c * la.y), cmul(mkj, lb));
+        const double2 nlb = cadd(make_c<double>(c * lb.x, c * lb.y), cmul(mjk, la));
+        l[k] = make_c<T>(nla.x, nla.y);
+        l[j] = make_c<T>(nlb.x, nlb.y);
+      }
+      const double2 na = cadd(make_c<double>(c * a.x, c * a.y), cmul(mkj, b));
+      const double2 nb = cadd(make_c<double>(c * b.x, c * b.y), cmul(mjk, a));
+      p[k] = make_c<T>(na.x, na.y);
+      p[j] = make_c<T>(nb.x, nb.y);
+    }
+  } else {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < dim; k += stride) {
+      if ((k & g.ctrl) != g.ctrl) continue;
+      const double sg = (__popcll(k & g.z) & 1) ? -1.0 : 1.0;
+      // (c - i s sg)
+      const double2 f = make_c<double>(c, -s * sg);
+      const double2 a = to_d2(p[k]);
+      if (l) {
+        const double2 la = to_d2(l[k]);
+        ov += sg * (la.x * a.y - la.y * a.x);
+        const double2 nl = cmul(f, la);
+        l[k] = make_c<T>(nl.x, nl.y);
+      }
+      const double2 na = cmul(f, a);
+      p[k] = make_c<T>(na.x, na.y);
+    }
+  }
+  if (!partial) return;
+  ov = wave_sum_d(ov);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = ov;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += sh[w];
+    partial[item * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// Re <ψ|λ> per item, block partials
+template <typename T>
+__global__ void __launch_bounds__(kAdjBlock)
+    k_dot_re(const VT_alias<T>* __restrict__ psi, const VT_alias<T>* __restrict__ lam, int n,
+             double* __restrict__ partial) {
+  __shared__ double sh[kAdjBlock / 64];
+  const uint64_t item = blockIdx.y;
+  const uint64_t dim = 1ull << n;
+  const VT_alias<T>* p = psi + item * dim;
+  const VT_alias<T>* l = lam + item * dim;
+  double acc = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < dim; k += stride) {
+    const double2 a = to_d2(p[k]), b = to_d2(l[k]);
+    acc += a.x * b.x + a.y * b.y;
+  }
+  acc = wave_sum_d(acc);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += sh[w];
+    partial[item * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// sum of bpi consecutive partials, fixed order: out[r] = Σ_b in[r*bpi + b]
+__global__ void k_rowsum(const double* __restrict__ in, int bpi, int rows, double* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  double v = 0.0;
+  for (int b = 0; b < bpi; ++b) v += in[(size_t)r * bpi + b];
+  out[r] = v;
+}
+
+namespace {
+
+// one step of a sweep: a fixed gate (and its inverse) or a Pauli rotation
+struct Step {
+  bool param = false;
+  int pidx = -1;
+  PauliRot rot{};
+  IGate fwd, inv;
+};
+
+IGate inverse_of(const IGate& g) {
+  IGate r = g;
+  if (g.kind == IK_U1 || g.kind == IK_U2) {
+    const int d = g.kind == IK_U1 ? 2 : 4;
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) {
+        r.m[2 * (i * d + j)] = g.m[2 * (j * d + i)];
+        r.m[2 * (i * d + j) + 1] = -g.m[2 * (j * d + i) + 1];
+      }
+  } else {
+    for (int e = 0; e < (1 << g.nd); ++e) r.m[2 * e + 1] = -g.m[2 * e + 1];
+  }
+  return r;
+}
+
+int build_steps(const smoq_op* ops, int n_ops, int n_params, int n, std::vector<Step>& steps) {
+  if (n_ops < 0 || (n_ops > 0 && !ops)) return set_error(SMOQ_EINVAL, "adjoint: bad tape");
+  for (int i = 0; i < n_ops; ++i) {
+    const smoq_op& op = ops[i];
+    const bool rot = op.gate == SMOQ_G_RX || op.gate == SMOQ_G_RY || op.gate == SMOQ_G_RZ ||
+                     op.gate == SMOQ_G_RXX || op.gate == SMOQ_G_RYY || op.gate == SMOQ_G_RZZ ||
+                     op.gate == SMOQ_G_CRY;
+    if (rot && op.param >= 0) {
+      if (op.param >= n_params)
+        return set_error(SMOQ_EINVAL, "adjoint: op %d uses parameter %d of %d", i, op.param,
+                         n_params);
+      // validate the qubits through the ordinary lowering
+      std::vector<IGate> tmp;
+      int rc = lower_op(op, 0.0, n, tmp);
+      if (rc) return rc;
+      Step st;
+      st.param = true;
+      st.pidx = op.param;
+      PauliRot& g = st.rot;
+      const uint64_t a = 1ull << op.q[0], b = op.q[1] >= 0 ? 1ull << op.q[1] : 0;
+      g.ctrl = 0;
+      switch (op.gate) {
+        case SMOQ_G_RX: g.x = a; g.z = 0; break;
+        case SMOQ_G_RY: g.x = a; g.z = a; break;
+        case SMOQ_G_RZ: g.x = 0; g.z = a; break;
+        case SMOQ_G_RXX: g.x = a | b; g.z = 0; break;
+        case SMOQ_G_RYY: g.x = a | b; g.z = a | b; break;
+        case SMOQ_G_RZZ: g.x = 0; g.z = a | b; break;
+        default: g.x = b; g.z = b; g.ctrl = a; break;  // CRY: control q0, Y on q1
+      }
+      g.cnt = __builtin_popcountll(g.x & g.z) & 3;
+      g.h = g.x ? 63 - __builtin_clzll(g.x) : 0;
+      steps.push_back(st);
+    } else {
+      std::vector<IGate> gs;
+      int rc = lower_op(op, op.theta, n, gs);
+      if (rc) return rc;
+      for (auto& g : gs) {
+        Step st;
+        st.fwd = g;
+        st.inv = inverse_of(g);
+        steps.push_back(st);
+      }
+    }
+  }
+  return SMOQ_OK;
+}
+
+int blocks_per_item(const smoq_state* s) {
+  const uint64_t work = s->dim >> 1;
+  uint64_t b = (work + kAdjBlock * 4 - 1) / (kAdjBlock * 4);
+  const uint64_t cap = std::max<uint64_t>(1, 4096 / (uint64_t)s->batch);
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(b, std::min<uint64_t>(cap, 1024)));
+}
+
+template <typename T>
+int launch_prot(smoq_state* s, void* lam, const PauliRot& g, const double2* cs, int n_params,
+                int pidx, double sign, double* partial, int bpi) {
+  hipLaunchKernelGGL(k_prot<T>, dim3(bpi, s->batch), dim3(kAdjBlock), 0, s->ctx->stream,
+                     (VT_alias<T>*)s->d, (VT_alias<T>*)lam, s->n, g, cs, n_params, pidx, sign,
+                     partial);
+  SMOQ_CHECK_LAUNCH("k_prot");
+  return SMOQ_OK;
+}
+
+int prot(smoq_state* s, void* lam, const PauliRot& g, const double2* cs, int n_params, int pidx,
+         double sign, double* partial, int bpi) {
+  return s->dtype == SMOQ_C128 ? launch_prot<double>(s, lam, g, cs, n_params, pidx, sign, partial, bpi)
+                               : launch_prot<float>(s, lam, g, cs, n_params, pidx, sign, partial, bpi);
+}
+
+// device memory owned by one adjoint / energy call
+struct Work {
+  void* lam = nullptr;
+  double2* cs = nullptr;
+  double* partial = nullptr;
+  double* sums = nullptr;
+  ~Work() {
+    if (lam) (void)hipFree(lam);
+    if (cs) (void)hipFree(cs);
+    if (partial) (void)hipFree(partial);
+    if (sums) (void)hipFree(sums);
+  }
+};
+
+// forward sweep from |0> on every item; per-item angles from cs
+int forward(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta, int n_params,
+            const std::vector<Step>& steps, const double2* cs, bool one_gate) {
+  int rc = smoq_state_set_basis(s, 0);
+  if (rc) return rc;
+  if (!one_gate && s->batch == 1 && s->n >= kRegQubits) {
+    // one item: angles can be baked into a fused program
+    std::vector<IGate> gs;
+    if ((rc = lower_tape(ops, n_ops, theta, n_params, s->n, gs))) return rc;
+    Program prog;
+    plan_program(gs, s->n, s->ctx->tile_qubits, s->ctx->low_qubits, CT<double>::kRowBits, prog);
+    return run_program(s, prog);
+  }
+  for (const Step& st : steps) {
+    if (st.param)
+      rc = prot(s, nullptr, st.rot, cs, n_params, st.pidx, 1.0, nullptr, blocks_per_item(s));
+    else
+      rc = launch_gate(s, st.fwd);
+    if (rc) return rc;
+  }
+  return SMOQ_OK;
+}
+
+int setup(smoq_state* s, const double* theta, int n_params, Work& w, std::vector<double2>& hcs) {
+  const size_t ncs = (size_t)s->batch * std::max(1, n_params);
+  hcs.resize(ncs);
+  for (size_t i = 0; i < ncs; ++i) {
+    const double th = n_params > 0 ? theta[i] : 0.0;
+    hcs[i] = make_c<double>(std::cos(0.5 * th), std::sin(0.5 * th));
+  }
+  SMOQ_HIP(hipMalloc(&w.cs, ncs * sizeof(double2)));
+  SMOQ_HIP(hipMemcpyAsync(w.cs, hcs.data(), ncs * sizeof(double2), hipMemcpyHostToDevice,
+                          s->ctx->stream));
+  return SMOQ_OK;
+}
+
+// λ = H ψ into w.lam and E_b = Re<ψ|λ>
+int energy_of(smoq_state* s, int n_terms, const uint64_t* x, const uint64_t* z, const double* c,
+              Work& w, double* energy) {
+  SMOQ_HIP(hipMalloc(&w.lam, s->bytes));
+  smoq_state lam;
+  lam.ctx = s->ctx;
+  lam.n = s->n;
+  lam.dtype = s->dtype;
+  lam.batch = s->batch;
+  lam.dim = s->dim;
+  lam.d = w.lam;
+  lam.bytes = s->bytes;
+  int rc = smoq_hpsi(s, &lam, n_terms, x, z, c);
+  lam.d = nullptr;
+  if (rc) return rc;
+  const int bpi = blocks_per_item(s);
+  double* part = nullptr;
+  SMOQ_HIP(hipMalloc(&part, (size_t)bpi * s->batch * sizeof(double) + 16));
+  double* esum = nullptr;
+  hipError_t e = hipMalloc(&esum, (size_t)s->batch * sizeof(double) + 16);
+  if (e != hipSuccess) {
+    (void)hipFree(part);
+    return hip_error(e, "hipMalloc(energy)");
+  }
+  if (s->dtype == SMOQ_C128)
+    hipLaunchKernelGGL(k_dot_re<double>, dim3(bpi, s->batch), dim3(kAdjBlock), 0, s->ctx->stream,
+                       (const double2*)s->d, (const double2*)w.lam, s->n, part);
+  else
+    hipLaunchKernelGGL(k_dot_re<float>, dim3(bpi, s->batch), dim3(kAdjBlock), 0, s->ctx->stream,
+                       (const float2*)s->d, (const float2*)w.lam, s->n, part);
+  hipLaunchKernelGGL(k_rowsum, dim3((s->batch + 255) / 256), dim3(256), 0, s->ctx->stream, part,
+                     bpi, s->batch, esum);
+  hipError_t le = hipGetLastError();
+  hipError_t ce = hipMemcpyAsync(energy, esum, s->batch * sizeof(double), hipMemcpyDeviceToHost,
+                                 s->ctx->stream);
+  hipError_t se = hipStreamSynchronize(s->ctx->stream);
+  (void)hipFree(part);
+  (void)hipFree(esum);
+  if (le != hipSuccess) return hip_error(le, "energy kernels");
+  if (ce != hipSuccess) return hip_error(ce, "energy copy");
+  if (se != hipSuccess) return hip_error(se, "energy sync");
+  return SMOQ_OK;
+}
+
+}  // namespace
+}  // namespace smoq
+
+using namespace smoq;
+
+extern "C" int smoq_tape_energy(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta,
+                                int n_params, int n_terms, const uint64_t* xmask,
+                                const uint64_t* zmask, const double* coeff, int flags,
+                                double* energy) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!energy || n_params < 0 || (n_params > 0 && !theta))
+    return set_error(SMOQ_EINVAL, "smoq_tape_energy: bad arguments");
+  std::vector<Step> steps;
+  if ((rc = build_steps(ops, n_ops, n_params, s->n, steps))) return rc;
+  Work w;
+  std::vector<double2> hcs;
+  if ((rc = setup(s, theta, n_params, w, hcs))) return rc;
+  if ((rc = forward(s, ops, n_ops, theta, n_params, steps, w.cs,
+                    (flags & SMOQ_ENERGY_ONE_GATE_KERNELS) != 0)))
+    return rc;
+  return energy_of(s, n_terms, xmask, zmask, coeff, w, energy);
+}
+
+extern "C" int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_ops,
+                                     const double* theta, int n_params, int n_terms,
+                                     const uint64_t* xmask, const uint64_t* zmask,
+                                     const double* coeff, double* energy, double* grad) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!energy || !grad || n_params < 0 || (n_params > 0 && !theta))
+    return set_error(SMOQ_EINVAL, "smoq_adjoint_gradient: bad arguments");
+  std::vector<Step> steps;
+  if ((rc = build_steps(ops, n_ops, n_params, s->n, steps))) return rc;
+  Work w;
+  std::vector<double2> hcs;
+  if ((rc = setup(s, theta, n_params, w, hcs))) return rc;
+  if ((rc = forward(s, ops, n_ops, theta, n_params, steps, w.cs, false))) return rc;
+  if ((rc = energy_of(s, n_terms, xmask, zmask, coeff, w, energy))) return rc;
+
+  // reverse sweep on a working copy of ψ, so that s keeps U(θ)|0>
+  smoq_state ps = *s;
+  void* pcopy = nullptr;
+  SMOQ_HIP(hipMalloc(&pcopy, s->bytes));
+  SMOQ_HIP(hipMemcpyAsync(pcopy, s->d, s->bytes, hipMemcpyDeviceToDevice, s->ctx->stream));
+  ps.d = pcopy;
+  smoq_state ls = *s;
+  ls.d = w.lam;
+  int n_slots = 0;
+  for (const Step& st : steps) n_slots += st.param;
+  const int bpi = blocks_per_item(s);
+  const size_t rows = (size_t)n_slots * s->batch;
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(pcopy);
+    ps.d = nullptr;
+    ls.d = nullptr;
+    return code;
+  };
+  if (hipMalloc(&w.partial, rows * bpi * sizeof(double) + 16) != hipSuccess ||
+      hipMalloc(&w.sums, rows * sizeof(double) + 16) != hipSuccess)
+    return fail(set_error(SMOQ_ENOMEM, "smoq_adjoint_gradient: partial buffers"));
+  std::vector<int> slot_param;
+  int slot = n_slots;
+  for (int i = (int)steps.size() - 1; i >= 0; --i) {
+    const Step& st = steps[i];
+    if (st.param) {
+      --slot;
+      double* part = w.partial + (size_t)slot * s->batch * bpi;
+      rc = prot(&ps, w.lam, st.rot, w.cs, n_params, st.pidx, -1.0, part, bpi);
+    } else {
+      rc = launch_gate(&ps, st.inv);
+      if (!rc) rc = launch_gate(&ls, st.inv);
+    }
+    if (rc) return fail(rc);
+  }
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s->ctx->stream,
+                     w.partial, bpi, (int)rows, w.sums);
+  if (hipGetLastError() != hipSuccess)
+    return fail(set_error(SMOQ_EDEVICE, "smoq_adjoint_gradient: reduction launch failed"));
+  std::vector<double> sums(rows);
+  if (rows &&
+      hipMemcpyAsync(sums.data(), w.sums, rows * sizeof(double), hipMemcpyDeviceToHost,
+                     s->ctx->stream) != hipSuccess)
+    return fail(set_error(SMOQ_EDEVICE, "smoq_adjoint_gradient: copy failed"));
+  if (hipStreamSynchronize(s->ctx->stream) != hipSuccess)
+    return fail(set_error(SMOQ_EDEVICE, "smoq_adjoint_gradient: sync failed"));
+  // sum the slots of each parameter in gate order (deterministic)
+  for (int b = 0; b < s->batch; ++b)
+    for (int p = 0; p < n_params; ++p) grad[(size_t)b * n_params + p] = 0.0;
+  slot = 0;
+  for (const Step& st : steps) {
+    if (!st.param) continue;
+    for (int b = 0; b < s->batch; ++b)
+      grad[(size_t)b * n_params + st.pidx] += sums[(size_t)slot * s->batch + b];
+    ++slot;
+  }
+  return fail(SMOQ_OK);
+}

@@ -88,6 +88,8 @@ SIGNATURES = {
     "smoq_plan_free": [_P],
     "smoq_evolve_observe": [_P, _P, _i, _i, _i, _P, _P, _P],
     "smoq_hpsi": [_P, _P, _i, _P, _P, _P],
+    "smoq_adjoint_gradient": [_P, _P, _i, _P, _i, _i, _P, _P, _P, _P, _P],
+    "smoq_tape_energy": [_P, _P, _i, _P, _i, _i, _P, _P, _P, _i, _P],
     "smoq_chebyshev_evolve": [_P, _i, _P, _P, _P, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
     "smoq_norm2": [_P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],

@@ -1,0 +1,120 @@
+"""Adjoint gradients on the device against the oracle's parameter-shift
+gradient (exact for exp(-iθP/2) with P a Pauli string), central finite
+differences and the batched / complex64 variants."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def tfim_terms(N, J=1.0, h=1.0):
+    from smoq_amd.observables import xxz_terms
+
+    return xxz_terms(N, 0.0, 0.0, J, h)
+
+
+def oracle_energy(tape, theta, N, J=1.0, h=1.0):
+    psi = np.zeros(1 << N, np.complex128)
+    psi[0] = 1.0
+    for gate, qs, th, par, m in tape.ops:
+        if par >= 0:
+            th = theta[par]
+        orc.apply_named(psi, gate.lower(), [q + 1 for q in qs], th, N)
+    return orc.compute_energy(psi, N, 0.0, 0.0, J, h)
+
+
+def oracle_param_shift(tape, theta, N):
+    g = np.zeros_like(theta)
+    for k in range(theta.size):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += math.pi / 2
+        tm[k] -= math.pi / 2
+        g[k] = (oracle_energy(tape, tp, N) - oracle_energy(tape, tm, N)) / 2.0
+    return g
+
+
+def test_adjoint_matches_oracle_parameter_shift(ctx):
+    from smoq_amd.circuits import hardware_efficient_ansatz
+    from smoq_amd.gradients import VQEEnergy
+
+    N, L = 7, 3
+    tape = hardware_efficient_ansatz(N, L)
+    theta = np.random.default_rng(1).uniform(0, 2 * math.pi, tape.n_params)
+    xs, zs, cs = tfim_terms(N)
+    v = VQEEnergy(tape, xs, zs, cs)
+    e, g = v.energy_and_grad(theta)
+    assert abs(e[0] - oracle_energy(tape, theta, N)) < 1e-12
+    want = oracle_param_shift(tape, theta, N)
+    assert np.max(np.abs(g[0] - want)) < 1e-10
+
+
+def test_all_rotation_kinds_against_parameter_shift_and_fd(ctx):
+    from smoq_amd.circuits import Tape
+    from smoq_amd.gradients import VQEEnergy, calculate_gradients_finite_difference
+
+    N = 6
+    rng = np.random.default_rng(3)
+    t = Tape(N)
+    p = 0
+    for rep in range(2):
+        for q in range(N):
+            t.add("H", [q]) if rep == 0 else None
+            t.add(["RX", "RY", "RZ"][(q + rep) % 3], [q], param=p)
+            p += 1
+        for q in range(0, N - 1):
+            t.add(["RXX", "RYY", "RZZ"][(q + rep) % 3], [q, q + 1], param=p)
+            p += 1
+        t.add("CZ", [0, N - 1]).add("T", [2])
+    n_ps = p
+    t.add("CRY", [1, 4], param=p)
+    theta = rng.uniform(0, 2 * math.pi, t.n_params)
+    xs, zs, cs = tfim_terms(N, 0.7, 1.3)
+    v = VQEEnergy(t, xs, zs, cs)
+    e, g = v.energy_and_grad(theta)
+    # parameter shift is exact for every Pauli-string rotation (all but CRY)
+    ps = np.zeros(n_ps)
+    for k in range(n_ps):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += math.pi / 2
+        tm[k] -= math.pi / 2
+        ps[k] = (oracle_energy(t, tp, N, 0.7, 1.3) - oracle_energy(t, tm, N, 0.7, 1.3)) / 2
+    assert np.max(np.abs(g[0][:n_ps] - ps)) < 1e-10
+    fd = calculate_gradients_finite_difference(v.cost_fn(), theta, 1e-6)
+    assert np.max(np.abs(g[0] - fd)) < 1e-7
+
+
+def test_batched_adjoint_equals_single(ctx):
+    from smoq_amd.circuits import hardware_efficient_ansatz
+    from smoq_amd.gradients import VQEEnergy
+
+    N, L, B = 9, 2, 3
+    tape = hardware_efficient_ansatz(N, L)
+    thetas = np.random.default_rng(5).uniform(0, 2 * math.pi, (B, tape.n_params))
+    xs, zs, cs = tfim_terms(N)
+    vb = VQEEnergy(tape, xs, zs, cs, batch=B)
+    eb, gb = vb.energy_and_grad(thetas)
+    v1 = VQEEnergy(tape, xs, zs, cs)
+    for b in range(B):
+        e1, g1 = v1.energy_and_grad(thetas[b])
+        assert abs(eb[b] - e1[0]) < 1e-12
+        assert np.max(np.abs(gb[b] - g1[0])) < 1e-12
+    # plain one-gate-kernel energies agree with the fused forward pass
+    assert abs(v1.energy(thetas[0], one_gate_kernels=True)[0] - v1.energy(thetas[0])[0]) < 1e-12
+
+
+def test_adjoint_complex64(ctx):
+    from smoq_amd.circuits import hardware_efficient_ansatz
+    from smoq_amd.gradients import VQEEnergy
+
+    N, L = 10, 3
+    tape = hardware_efficient_ansatz(N, L)
+    theta = np.random.default_rng(7).uniform(0, 2 * math.pi, tape.n_params)
+    xs, zs, cs = tfim_terms(N)
+    e128, g128 = VQEEnergy(tape, xs, zs, cs).energy_and_grad(theta)
+    e64, g64 = VQEEnergy(tape, xs, zs, cs, dtype="c64").energy_and_grad(theta)
+    assert abs(e64[0] - e128[0]) < 1e-5
+    assert np.max(np.abs(g64[0] - g128[0])) < 1e-5
