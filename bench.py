@@ -100,9 +100,76 @@ def measured_traffic_ratio():
         return None
 
 
+def adjoint_vs_parameter_shift(N=24, layers=20, shift_params=4, batch_n=20, batch=64, seed=24):
+    """Adjoint gradient of the TFIM energy (J = h = 1, open chain) for the
+    N-qubit hardware-efficient ansatz (Ry, Rz on every qubit + CZ ladder per
+    layer, 2·N·layers parameters) against parameter shift with the one-gate
+    kernels.  Parameter shift runs 2 energy evaluations per parameter; it is
+    timed on `shift_params` parameters and scaled to all of them."""
+    import smoq_amd
+    from smoq_amd import streams
+    from smoq_amd.circuits import hardware_efficient_ansatz
+    from smoq_amd.gradients import VQEEnergy
+    from smoq_amd.observables import xxz_terms
+
+    ctx = smoq_amd.default_context()
+    tape = hardware_efficient_ansatz(N, layers)
+    P = tape.n_params
+    theta = 2 * np.pi * streams.uniforms(seed, P)
+    xs, zs, cs = xxz_terms(N, 0.0, 0.0, 1.0, 1.0)
+    v = VQEEnergy(tape, xs, zs, cs)
+    v.energy_and_grad(theta)
+    ctx.sync()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e, g = v.energy_and_grad(theta)
+    t_adj = (time.perf_counter() - t0) / reps
+    cost = v.cost_fn(one_gate_kernels=True)
+    cost(theta)
+    sub = list(range(shift_params))
+    t0 = time.perf_counter()
+    # the reference's two-evaluation shift rule on a subset of parameters
+    gsub = np.zeros(len(sub))
+    for i, k in enumerate(sub):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += np.pi / 2
+        tm[k] -= np.pi / 2
+        gsub[i] = (cost(tp) - cost(tm)) / 2.0
+    t_ps_sub = time.perf_counter() - t0
+    t_ps = t_ps_sub / len(sub) * P
+    v.free()
+    # batched parameter sets at batch_n qubits in one launch per gate
+    tb = hardware_efficient_ansatz(batch_n, layers)
+    thb = (2 * np.pi * streams.uniforms(seed + 1, batch * tb.n_params)).reshape(batch, -1)
+    xb, zb, cb = xxz_terms(batch_n, 0.0, 0.0, 1.0, 1.0)
+    vb = VQEEnergy(tb, xb, zb, cb, batch=batch)
+    vb.energy_and_grad(thb)
+    ctx.sync()
+    t0 = time.perf_counter()
+    eb, gb = vb.energy_and_grad(thb)
+    t_batch = time.perf_counter() - t0
+    vb.free()
+    return {
+        "workload": f"TFIM energy gradient, N={N}, {layers} layers, {P} parameters, c128",
+        "adjoint_s_per_gradient": t_adj,
+        "adjoint_grads_per_s": 1.0 / t_adj,
+        "param_shift_s_per_gradient": t_ps,
+        "param_shift_sample": f"{len(sub)} of {P} parameters timed "
+                              f"({t_ps_sub:.2f} s, one-gate kernels), scaled to {P}",
+        "speedup_vs_param_shift": t_ps / t_adj,
+        "max_abs_diff_vs_param_shift_on_sample": float(np.max(np.abs(g[0][sub] - gsub))),
+        "energy": float(e[0]),
+        "batched": {"n_qubits": batch_n, "batch": batch, "params_per_set": tb.n_params,
+                    "s_per_batched_gradient": t_batch,
+                    "gradients_per_s": batch / t_batch},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", choices=["traffic"], default=None)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -218,6 +285,10 @@ def main():
         }
     else:
         psi.free()
+
+    if not args.no_adjoint and not args.no_extras:
+        log("adjoint gradient vs parameter shift ...")
+        result.setdefault("extras", {})["adjoint"] = adjoint_vs_parameter_shift()
 
     if not args.no_cpu:
         log("CPU baseline ...")

@@ -209,18 +209,18 @@ __device__ __forceinline__ void apply_gate(VT<T> (&x)[kNR], const GateDev& g, ui
   }
 }
 
-// T is the storage type of the state in HBM; the tile lives in LDS and in
-// registers as complex128 whatever T is, so a complex64 state is rounded
-// once per pass (at the store) instead of after every gate.
+// T is the storage type of the state in HBM and LDS; in registers the
+// amplitudes are complex128 whatever T is, so a complex64 state is rounded
+// once per register stage instead of after every gate.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_fused_pass(VT<T>* __restrict__ psi, int n, const PassDev* __restrict__ pass,
                  const StageDev* __restrict__ stages, const GateDev* __restrict__ gates,
                  const double2* __restrict__ tabs) {
   constexpr int R = kRegQubits;
-  constexpr int WB = CT<double>::kRowBits;
+  constexpr int WB = CT<T>::kRowBits;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double2* lds = reinterpret_cast<double2*>(smem_raw);
+  VT<T>* lds = reinterpret_cast<VT<T>*>(smem_raw);
 
   const int k = pass->k;
   const int kr = k - R;
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(256)
       x[i] = pt[o];
     }
 #pragma unroll
-    for (int i = 0; i < kNR; ++i) lds[st_lane ^ swz<WB>((uint32_t)(i * NT))] = to_d2(x[i]);
+    for (int i = 0; i < kNR; ++i) lds[st_lane ^ swz<WB>((uint32_t)(i * NT))] = x[i];
   }
   __syncthreads();
 
@@ -280,15 +280,15 @@ __global__ void __launch_bounds__(256)
     }
     VT<double> x[kNR];
 #pragma unroll
-    for (int i = 0; i < kNR; ++i) x[i] = lds[sl ^ sr[i]];
+    for (int i = 0; i < kNR; ++i) x[i] = to_d2(lds[sl ^ sr[i]]);
     for (int g = st.g0; g < st.g1; ++g) apply_gate<double>(x, gates[g], gl, tabs);
 #pragma unroll
-    for (int i = 0; i < kNR; ++i) lds[sl ^ sr[i]] = x[i];
+    for (int i = 0; i < kNR; ++i) lds[sl ^ sr[i]] = make_c<T>(x[i].x, x[i].y);
     __syncthreads();
   }
 
   {
-    double2 x[kNR];
+    VT<T> x[kNR];
 #pragma unroll
     for (int i = 0; i < kNR; ++i) x[i] = lds[st_lane ^ swz<WB>((uint32_t)(i * NT))];
 #pragma unroll
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(256)
       uint64_t o = 0;
 #pragma unroll
       for (int j = 0; j < R; ++j) o |= ((i >> j) & 1) ? hb[j] : 0ull;
-      pt[o] = make_c<T>(x[i].x, x[i].y);
+      pt[o] = x[i];
     }
   }
 }
@@ -347,7 +347,7 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
     if (nt > 256 || pd.k > kMaxTileQubits)
       return set_error(SMOQ_EINVAL, "fused pass: tile of 2^%d needs %d threads (max 256)", pd.k,
                        nt);
-    const size_t shm = sizeof(double2) << pd.k;
+    const size_t shm = sizeof(VT<T>) << pd.k;
     if (ctx->timing && (rc = time_mark(ctx))) return rc;
     hipLaunchKernelGGL(k_fused_pass<T>, dim3((unsigned)tiles), dim3(nt), shm, ctx->stream,
                        (VT<T>*)s->d, s->n, dp + p, ds, dg, dt);

@@ -104,7 +104,7 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
     return SMOQ_OK;
   }
   plan_program(gs, n_qubits, ctx->tile_qubits, ctx->low_qubits,
-               CT<double>::kRowBits, p->prog);  // the LDS tile is complex128
+               dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog);
   p->layout = program_layout(p->prog, dtype);
   hipError_t e = hipMalloc(&p->dev, p->layout.total);
   if (e != hipSuccess) {

@@ -523,7 +523,7 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
   if ((rc = lower_tape(ops, n_ops, theta, n_theta, s->n, gs))) return rc;
   Program prog;
   plan_program(gs, s->n, s->ctx->tile_qubits, s->ctx->low_qubits,
-               CT<double>::kRowBits, prog);  // the LDS tile is complex128
+               s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog);
   if (n_passes) *n_passes = (int)prog.passes.size();
   if (gs.empty()) return SMOQ_OK;
   if (s->n < kRegQubits) {
