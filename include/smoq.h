@@ -180,6 +180,36 @@ int smoq_chebyshev_evolve(smoq_state* s, int n_terms, const uint64_t* xmask,
                           const uint64_t* zmask, const double* coeff, double t, double emin,
                           double emax, double tol, int max_order, int* orders_used);
 
+/* ---- Monte Carlo wave-function trajectories (σ⁻ jumps) -------------------
+ * One group = the trajectories of one qubit count: `psi` holds them as batch
+ * items.  Each step of lindblad_mcwf_batched_step!
+ * (utils/cpu/cpuQuantumChannelLindbladianEvolutionMonteCarloWaveFunction.jl:340-390)
+ * is applied to every trajectory:
+ *   1. the first-order Trotter step: the diagonal 2-qubit bond gates in
+ *      order, then the 2x2 site gates (the TFIM gate list of
+ *      precompute_trotter_gates_bitwise_cpu; bonds must be diagonal);
+ *   2. ψ_i *= decay_pow[popcount(i)]   (decay_pow[m] = sqrt(1-γdt)^m);
+ *   3. dp = 1 - ||ψ||²; with r = uniforms[traj*n_steps + step]: if r < dp
+ *      and dp > 0, site k is the first with r < Σ_{j<=k} γdt pop_j/||ψ||²
+ *      (site 0 when none), σ⁻_k is applied and jumps[...] = k, else -1;
+ *   4. ψ /= ||ψ||.
+ * After every obs_every-th step z_out[(traj*(n_steps/obs_every) + r)*n + i]
+ * receives <Z_i>.  All groups run in one persistent launch, one workgroup
+ * per trajectory, largest qubit count first.  n <= 20. */
+typedef struct smoq_mcwf_group {
+  smoq_state* psi;
+  int n_bonds;
+  const int32_t* bond_q;      /* 2 per bond, 0-based; basis b(q0) + 2 b(q1) */
+  const double* bond_diag;    /* 4 complex per bond: the bond gate's diagonal */
+  const double* site_u;       /* n x 2x2 complex (row-major), one per site */
+  const double* decay_pow;    /* n + 1 values */
+  const double* uniforms;     /* [traj][n_steps] */
+  int32_t* jumps;             /* out [traj][n_steps] */
+  double* z_out;              /* out [traj][n_steps/obs_every][n], may be NULL */
+} smoq_mcwf_group;
+int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, int n_groups, double gamma_dt,
+                     int n_steps, int obs_every);
+
 /* ---- parametrised circuits: adjoint gradient -----------------------------
  * E_b(θ) = <0|U(θ_b)† H U(θ_b)|0> for every batch item b of `s`, with
  * H = sum_t coeff_t P_t (Pauli masks as in smoq_expect_pauli) and the tape's

@@ -240,6 +240,24 @@ def mcwf_decay_jump(psi, N, gamma_dt, decay_pow, r):
     return site, float(dp[0])
 
 
+def mcwf_trajectory(psi, gates, N, gamma_dt, decay_pow, r_seq, obs_every=0):
+    """One trajectory of evolve_mcwf_batched_trotter!
+    (…MonteCarloWaveFunction.jl:412-423): per step the gate list in order
+    (apply_fast_trotter_step_cpu!), then or_mcwf_decay_jump with r_seq[step]
+    in place of rand().  Returns (jump sites 0-based or -1, <Z_i> after
+    every obs_every-th step as [record][i])."""
+    idx = np.arange(1 << N)
+    jumps, zs = [], []
+    for s, r in enumerate(r_seq):
+        trotter_step(psi, gates, N)
+        site, _ = mcwf_decay_jump(psi, N, gamma_dt, decay_pow, float(r))
+        jumps.append(site - 1)
+        if obs_every and (s + 1) % obs_every == 0:
+            p = np.abs(psi) ** 2
+            zs.append([float(np.sum(p * (1 - 2 * ((idx >> k) & 1)))) for k in range(N)])
+    return np.array(jumps, np.int32), np.array(zs, np.float64).reshape(-1, N)
+
+
 def trotter_step(psi, gates, N):
     """apply_fast_trotter_step_cpu! (utils/cpu/cpuQuantumChannelUnitaryEvolutionTrotter.jl:150-162):
     gates = [(indices 1-based, U)] applied in order."""

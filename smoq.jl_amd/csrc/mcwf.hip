@@ -1,0 +1,524 @@
+// Batched Monte Carlo wave-function trajectories with σ⁻ jumps in one
+// persistent launch: one workgroup per trajectory runs all of its steps.
+//
+// A trajectory of n <= 12 qubits lives in LDS for the whole run.  Larger
+// ones stay in HBM and every step makes two tiled passes: pass A over the
+// contiguous 4096-amplitude chunks (local qubits 0..11) applies the pending
+// renormalisation, the diagonal bond gates and the site gates of qubits
+// 0..11; pass B over tiles made of the low 12-h qubits and the high h = n-12
+// qubits applies the remaining site gates, the no-jump decay and the norm
+// reduction.  The jump decision is taken by one thread from block sums;
+// site populations are reduced only on steps that jump or record <Z>.
+// The 1/||ψ|| of step s is folded into the first pass of step s+1.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_common.h"
+#include "smoq_internal.h"
+
+namespace smoq {
+
+constexpr int kMcBlock = 256;
+constexpr int kMcTileQ = 12;
+constexpr int kMcMaxQ = 20;
+constexpr int kMcWaves = kMcBlock / 64;
+
+struct McwfGroupDev {
+  void* psi;
+  int n, n_bonds, uniform_chain, pad;
+  const int32_t* bond_q;
+  const double2* bond_diag;
+  const double2* site_u;
+  const double* decay_pow;
+  const double2* chain_tab;  // open uniform chain: product phase by #unequal neighbours
+  const double* uniforms;
+  int32_t* jumps;
+  double* z_out;
+};
+
+struct McwfTrajDev {
+  int group, idx;
+};
+
+struct McwfArgs {
+  const McwfGroupDev* groups;
+  const McwfTrajDev* trajs;
+  double gamma_dt;
+  int n_steps, obs_every;
+};
+
+__device__ inline double mc_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// block sums of v[0..m), m <= M; every thread reads the results from out[]
+template <int M>
+__device__ inline void mc_block_sum(const double (&v)[M], int m, double* red, double* out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    if (i < m) {
+      const double s = mc_wave_sum(v[i]);
+      if (lane == 0) red[i * kMcWaves + wid] = s;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < m) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kMcWaves; ++w) s += red[threadIdx.x * kMcWaves + w];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// global index of tile position l in tile t
+struct TileMap {
+  int pass_b;  // 0: chunk t of 2^tq contiguous amplitudes; 1: low 12-h bits + high h bits
+  int tq, h;
+  __device__ inline uint64_t g(uint64_t t, uint32_t l) const {
+    if (!pass_b) return (t << tq) | l;
+    const int lb = kMcTileQ - h;
+    const uint64_t low = l & ((1u << lb) - 1u);
+    return low | (t << lb) | ((uint64_t)(l >> lb) << kMcTileQ);
+  }
+};
+
+struct StageOpts {
+  double pre_scale;  // applied on load
+  bool bonds;        // bond phases before the site gates
+  bool decay;        // decay + norm after them
+};
+
+// One LDS round trip over the tile: 8 amplitudes per thread spanning three
+// tile positions, site gates on the positions with site[s] >= 0.
+template <typename T>
+__device__ void mc_stage(VT_alias<T>* lds, int tq, const int (&pos)[3], const int (&site)[3],
+                         const McwfGroupDev& G, const TileMap& tm, uint64_t t, const StageOpts& o,
+                         uint64_t dmask, double& norm_acc) {
+  constexpr int WB = CT<T>::kRowBits;
+  const uint32_t tmask = (1u << tq) - 1u;
+  const uint32_t pm = (1u << pos[0]) | (1u << pos[1]) | (1u << pos[2]);
+  const uint32_t groups = 1u << (tq - 3);
+  for (uint32_t grp = threadIdx.x; grp < groups; grp += kMcBlock) {
+    const uint32_t base = pdep32(grp, tmask & ~pm);
+    double2 x[8];
+    uint32_t li[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      li[i] = base | ((i & 1) ? (1u << pos[0]) : 0u) | ((i & 2) ? (1u << pos[1]) : 0u) |
+              ((i & 4) ? (1u << pos[2]) : 0u);
+      x[i] = to_d2(lds[swz<WB>(li[i])]);
+      x[i].x *= o.pre_scale;
+      x[i].y *= o.pre_scale;
+    }
+    if (o.bonds) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint64_t gi = tm.g(t, li[i]) & dmask;
+        if (G.uniform_chain) {
+          const int nd = __popcll((gi ^ (gi >> 1)) & (dmask >> 1));
+          x[i] = cmul(G.chain_tab[nd], x[i]);
+        } else {
+          for (int b = 0; b < G.n_bonds; ++b) {
+            const int e = (int)((gi >> G.bond_q[2 * b]) & 1) |
+                          ((int)((gi >> G.bond_q[2 * b + 1]) & 1) << 1);
+            x[i] = cmul(G.bond_diag[4 * b + e], x[i]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      if (site[s] < 0) continue;
+      const double2* u = G.site_u + 4 * site[s];
+      const double2 u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        if (i & (1 << s)) continue;
+        const int j = i | (1 << s);
+        const double2 a = x[i], b = x[j];
+        x[i] = cfma2(u0, a, u1, b);
+        x[j] = cfma2(u2, a, u3, b);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (o.decay) {
+        // the norm is that of the amplitudes as stored
+        const int ne = __popcll(tm.g(t, li[i]) & dmask);
+        const double d = ne > 0 ? G.decay_pow[ne] : 1.0;
+        const VT_alias<T> r = make_c<T>(x[i].x * d, x[i].y * d);
+        lds[swz<WB>(li[i])] = r;
+        norm_acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+      } else {
+        lds[swz<WB>(li[i])] = make_c<T>(x[i].x, x[i].y);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// site gates site0 + k on tile positions pos0 + k, k < cnt, three per stage;
+// the pending scale and the bonds ride on the first stage, decay + norm on
+// the last
+template <typename T>
+__device__ void mc_gates(VT_alias<T>* lds, int tq, int pos0, int site0, int cnt,
+                         const McwfGroupDev& G, const TileMap& tm, uint64_t t, StageOpts o,
+                         uint64_t dmask, double& norm_acc) {
+  const int stages = cnt > 3 ? (cnt + 2) / 3 : 1;
+  for (int st = 0; st < stages; ++st) {
+    int pos[3], site[3];
+    uint32_t used = 0;
+    for (int s = 0; s < 3; ++s) {
+      const int k = st * 3 + s;
+      pos[s] = k < cnt ? pos0 + k : -1;
+      site[s] = k < cnt ? site0 + k : -1;
+      if (pos[s] >= 0) used |= 1u << pos[s];
+    }
+    for (int s = 0; s < 3; ++s) {  // pad with free positions that carry no gate
+      if (pos[s] >= 0) continue;
+      const int b = __ffs(~used & ((1u << tq) - 1u)) - 1;
+      pos[s] = b;
+      used |= 1u << b;
+    }
+    StageOpts so{st == 0 ? o.pre_scale : 1.0, o.bonds && st == 0, o.decay && st == stages - 1};
+    mc_stage<T>(lds, tq, pos, site, G, tm, t, so, dmask, norm_acc);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
+  using V = VT_alias<T>;
+  constexpr int WB = CT<T>::kRowBits;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* lds = reinterpret_cast<V*>(smem_raw);
+  __shared__ double red[(kMcMaxQ + 1) * kMcWaves];
+  __shared__ double sums[kMcMaxQ + 1];
+  __shared__ double s_scale;
+  __shared__ int s_site;
+
+  const McwfTrajDev tr = A.trajs[blockIdx.x];
+  const McwfGroupDev G = A.groups[tr.group];
+  const int n = G.n;
+  const uint64_t D = 1ull << n, dmask = D - 1;
+  V* psi = reinterpret_cast<V*>(G.psi) + (uint64_t)tr.idx * D;
+  const bool resident = n <= kMcTileQ;
+  const int tq = resident ? (n < 3 ? 3 : n) : kMcTileQ;  // tiny states pad the tile with zeros
+  const int h = resident ? 0 : n - kMcTileQ;
+  const uint64_t ntiles = 1ull << h;
+  const uint32_t tile = 1u << tq;
+  const int rec_n = A.obs_every > 0 ? A.n_steps / A.obs_every : 0;
+  const TileMap tmA{0, tq, h}, tmB{1, tq, h};
+
+  const int cntA = n < kMcTileQ ? n : kMcTileQ;
+
+  auto load_tile = [&](const TileMap& tm, uint64_t t) {
+    for (uint32_t l = threadIdx.x; l < tile; l += kMcBlock) {
+      const uint64_t gi = tm.g(t, l);
+      lds[swz<WB>(l)] = gi < D ? psi[gi] : make_c<T>(0.0, 0.0);
+    }
+    __syncthreads();
+  };
+  auto store_tile = [&](const TileMap& tm, uint64_t t) {
+    for (uint32_t l = threadIdx.x; l < tile; l += kMcBlock) psi[tm.g(t, l)] = lds[swz<WB>(l)];
+    __syncthreads();
+  };
+  // sums[i] = Σ_{bit i set} |ψ|² for i < n, sums[n] = ||ψ||², over the
+  // stored state.  Index k = tid + 256 j: bits 0..7 come from tid, 8.. from j.
+  auto pops = [&]() {
+    double tot = 0.0, hi[kMcMaxQ - 8];
+#pragma unroll
+    for (int i = 0; i < kMcMaxQ - 8; ++i) hi[i] = 0.0;
+    for (uint64_t j = 0; (j << 8) < D; ++j) {
+      const uint64_t k = (j << 8) | threadIdx.x;
+      if (k >= D) break;
+      const V v = resident ? lds[swz<WB>((uint32_t)k)] : psi[k];
+      const double p = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+      tot += p;
+#pragma unroll
+      for (int i = 0; i < kMcMaxQ - 8; ++i)
+        if ((j >> i) & 1) hi[i] += p;
+    }
+    double v[kMcMaxQ + 1];
+#pragma unroll
+    for (int i = 0; i <= kMcMaxQ; ++i) {
+      double b = i < 8 ? (((threadIdx.x >> i) & 1) ? tot : 0.0) : hi[i < 8 ? 0 : i - 8];
+      if (i == kMcMaxQ) b = 0.0;
+      v[i] = (i == n) ? tot : b;  // the total goes to slot n
+    }
+    mc_block_sum<kMcMaxQ + 1>(v, n + 1, red, sums);
+  };
+
+  if (threadIdx.x == 0) s_scale = 1.0;
+  if (resident) load_tile(tmA, 0);
+  __syncthreads();
+
+  for (int step = 0; step < A.n_steps; ++step) {
+    const double sc = s_scale;
+    double norm_acc = 0.0;
+    if (resident) {
+      mc_gates<T>(lds, tq, 0, 0, cntA, G, tmA, 0, StageOpts{sc, true, true}, dmask,
+                  norm_acc);
+    } else {
+      double unused = 0.0;
+      for (uint64_t t = 0; t < ntiles; ++t) {
+        load_tile(tmA, t);
+        mc_gates<T>(lds, tq, 0, 0, cntA, G, tmA, t, StageOpts{sc, true, false}, dmask,
+                    unused);
+        store_tile(tmA, t);
+      }
+      for (uint64_t t = 0; t < ntiles; ++t) {
+        load_tile(tmB, t);
+        mc_gates<T>(lds, tq, kMcTileQ - h, kMcTileQ, h, G, tmB, t, StageOpts{1.0, false, true}, dmask,
+                    norm_acc);
+        store_tile(tmB, t);
+      }
+    }
+    double nv[1] = {norm_acc};
+    mc_block_sum<1>(nv, 1, red, sums);
+    const double norm_sq = sums[0];
+    const double dp = 1.0 - norm_sq;
+    const double r = G.uniforms[(uint64_t)tr.idx * A.n_steps + step];
+    const bool jump = (r < dp) && (dp > 0.0);
+    double new_norm_sq = norm_sq;
+    if (jump) {  // uniform over the block: every thread read the same sums[0] and r
+      pops();
+      if (threadIdx.x == 0) {
+        double cum = 0.0;
+        int sel = 0;
+        for (int k = 0; k < n; ++k) {
+          cum += A.gamma_dt * sums[k] / norm_sq;
+          if (r < cum) {
+            sel = k;
+            break;
+          }
+        }
+        s_site = sel;
+      }
+      __syncthreads();
+      const int k = s_site;
+      new_norm_sq = sums[k];
+      const uint64_t m = 1ull << k;
+      for (uint64_t p = threadIdx.x; p < (D >> 1); p += kMcBlock) {
+        const uint64_t i0 = insert_zero(p, k), i1 = i0 | m;
+        if (resident) {
+          lds[swz<WB>((uint32_t)i0)] = lds[swz<WB>((uint32_t)i1)];
+          lds[swz<WB>((uint32_t)i1)] = make_c<T>(0.0, 0.0);
+        } else {
+          psi[i0] = psi[i1];
+          psi[i1] = make_c<T>(0.0, 0.0);
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      G.jumps[(uint64_t)tr.idx * A.n_steps + step] = jump ? s_site : -1;
+      s_scale = 1.0 / sqrt(new_norm_sq);
+    }
+    __syncthreads();
+    if (rec_n > 0 && G.z_out && (step + 1) % A.obs_every == 0) {
+      // <Z_i> = (||ψ||² - 2 pop_i) / ||ψ||², independent of the pending scale
+      pops();
+      const int rix = (step + 1) / A.obs_every - 1;
+      if ((int)threadIdx.x < n) {
+        const double nn = sums[n];
+        G.z_out[((uint64_t)tr.idx * rec_n + rix) * n + threadIdx.x] =
+            (nn - 2.0 * sums[threadIdx.x]) / nn;
+      }
+      __syncthreads();
+    }
+  }
+  // apply the last renormalisation and write the state back
+  const double sc = s_scale;
+  for (uint64_t k = threadIdx.x; k < D; k += kMcBlock) {
+    const V v = resident ? lds[swz<WB>((uint32_t)k)] : psi[k];
+    psi[k] = make_c<T>(sc * (double)v.x, sc * (double)v.y);
+  }
+}
+
+}  // namespace smoq
+
+using namespace smoq;
+
+extern "C" int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, int n_groups,
+                                double gamma_dt, int n_steps, int obs_every) {
+  if (!ctx || !groups || n_groups < 1)
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_evolve: null context/groups or n_groups < 1");
+  if (n_steps < 0 || obs_every < 0)
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_evolve: n_steps %d obs_every %d", n_steps,
+                     obs_every);
+  if (!(gamma_dt >= 0.0 && gamma_dt <= 1.0))
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_evolve: gamma_dt %g outside [0, 1]", gamma_dt);
+  int dtype = -1;
+  size_t total_traj = 0;
+  for (int g = 0; g < n_groups; ++g) {
+    const smoq_mcwf_group& G = groups[g];
+    int rc = check_state(G.psi);
+    if (rc) return rc;
+    if (dtype < 0) dtype = G.psi->dtype;
+    if (G.psi->ctx != ctx || G.psi->dtype != dtype)
+      return set_error(SMOQ_EINVAL, "smoq_mcwf_evolve: groups must share context and dtype");
+    const int n = G.psi->n;
+    if (n > kMcMaxQ)
+      return set_error(SMOQ_ENOTSUP, "smoq_mcwf_evolve: group %d has %d qubits (max %d)", g, n,
+                       kMcMaxQ);
+    if (G.n_bonds < 0 || (G.n_bonds > 0 && (!G.bond_q || !G.bond_diag)) || !G.site_u ||
+        !G.decay_pow || (n_steps > 0 && (!G.uniforms || !G.jumps)))
+      return set_error(SMOQ_EINVAL, "smoq_mcwf_evolve: group %d has a null array", g);
+    for (int b = 0; b < G.n_bonds; ++b) {
+      const int a = G.bond_q[2 * b], c = G.bond_q[2 * b + 1];
+      if (a < 0 || a >= n || c < 0 || c >= n || a == c)
+        return set_error(SMOQ_EINVAL, "smoq_mcwf_evolve: group %d bond %d on qubits (%d, %d)", g,
+                         b, a, c);
+    }
+    total_traj += (size_t)G.psi->batch;
+  }
+  if (n_steps == 0) return SMOQ_OK;
+  if (total_traj > 0x7fffffff)
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_evolve: %zu trajectories", total_traj);
+  const int rec_n = obs_every > 0 ? n_steps / obs_every : 0;
+
+  // one device buffer: tables, descriptors, uniforms, outputs
+  std::vector<unsigned char> blob;
+  auto reserve = [&](size_t bytes) {
+    const size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + bytes);
+    return off;
+  };
+  auto put = [&](const void* p, size_t bytes) {
+    const size_t off = reserve(bytes);
+    if (bytes) std::memcpy(blob.data() + off, p, bytes);
+    return off;
+  };
+  struct Off {
+    size_t bond_q, bond_diag, site_u, decay, chain, uni, jumps, z;
+    bool chain_ok;
+  };
+  std::vector<Off> offs(n_groups);
+  size_t device_bytes = 0;  // uniforms and outputs, placed after the blob
+  for (int g = 0; g < n_groups; ++g) {
+    const smoq_mcwf_group& G = groups[g];
+    const int n = G.psi->n;
+    Off& o = offs[g];
+    o.bond_q = put(G.bond_q, sizeof(int32_t) * 2 * G.n_bonds);
+    o.bond_diag = put(G.bond_diag, sizeof(double) * 8 * G.n_bonds);
+    o.site_u = put(G.site_u, sizeof(double) * 8 * n);
+    o.decay = put(G.decay_pow, sizeof(double) * (n + 1));
+    // open chain (q, q+1), q = 0..n-2, every bond diagonal (s, d, d, s) with
+    // the same s and d: the product of the bond phases depends only on the
+    // number of unequal neighbours
+    bool chain = n >= 2 && G.n_bonds == n - 1;
+    for (int b = 0; chain && b < G.n_bonds; ++b) {
+      chain = G.bond_q[2 * b] == b && G.bond_q[2 * b + 1] == b + 1;
+      for (int e = 0; chain && e < 8; ++e) chain = G.bond_diag[8 * b + e] == G.bond_diag[e];
+    }
+    if (chain)
+      chain = G.bond_diag[0] == G.bond_diag[6] && G.bond_diag[1] == G.bond_diag[7] &&
+              G.bond_diag[2] == G.bond_diag[4] && G.bond_diag[3] == G.bond_diag[5];
+    std::vector<double> tab(2 * (size_t)n + 2, 0.0);
+    if (chain) {
+      for (int nd = 0; nd < n; ++nd) {
+        double re = 1.0, im = 0.0;
+        for (int b = 0; b < n - 1; ++b) {
+          const double* f = b < n - 1 - nd ? &G.bond_diag[0] : &G.bond_diag[2];
+          const double r2 = re * f[0] - im * f[1], i2 = re * f[1] + im * f[0];
+          re = r2;
+          im = i2;
+        }
+        tab[2 * nd] = re;
+        tab[2 * nd + 1] = im;
+      }
+    }
+    o.chain = put(tab.data(), sizeof(double) * tab.size());
+    o.chain_ok = chain;
+    const size_t T = (size_t)G.psi->batch;
+    o.uni = device_bytes;
+    device_bytes += (T * n_steps * sizeof(double) + 255) & ~(size_t)255;
+    o.jumps = device_bytes;
+    device_bytes += (T * n_steps * sizeof(int32_t) + 255) & ~(size_t)255;
+    o.z = device_bytes;
+    if (G.z_out && rec_n > 0)
+      device_bytes += (T * rec_n * n * sizeof(double) + 255) & ~(size_t)255;
+  }
+  const size_t grp_off = reserve(sizeof(McwfGroupDev) * n_groups);
+  const size_t trj_off = reserve(sizeof(McwfTrajDev) * total_traj);
+  const size_t dyn_off = (blob.size() + 255) & ~(size_t)255;
+
+  unsigned char* dev = nullptr;
+  SMOQ_HIP(hipMalloc(&dev, dyn_off + device_bytes));
+  auto done = [&](int code) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dev);
+    return code;
+  };
+
+  std::vector<McwfGroupDev> gd(n_groups);
+  std::vector<std::pair<int, int>> order;  // (n, group): largest states start first
+  for (int g = 0; g < n_groups; ++g) {
+    const smoq_mcwf_group& G = groups[g];
+    const Off& o = offs[g];
+    McwfGroupDev& d = gd[g];
+    d.psi = G.psi->d;
+    d.n = G.psi->n;
+    d.n_bonds = G.n_bonds;
+    d.uniform_chain = o.chain_ok ? 1 : 0;
+    d.pad = 0;
+    d.bond_q = (const int32_t*)(dev + o.bond_q);
+    d.bond_diag = (const double2*)(dev + o.bond_diag);
+    d.site_u = (const double2*)(dev + o.site_u);
+    d.decay_pow = (const double*)(dev + o.decay);
+    d.chain_tab = (const double2*)(dev + o.chain);
+    d.uniforms = (const double*)(dev + dyn_off + o.uni);
+    d.jumps = (int32_t*)(dev + dyn_off + o.jumps);
+    d.z_out = (G.z_out && rec_n > 0) ? (double*)(dev + dyn_off + o.z) : nullptr;
+    order.push_back({d.n, g});
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                     return a.first > b.first;
+                   });
+  std::vector<McwfTrajDev> trajs;
+  trajs.reserve(total_traj);
+  for (const auto& og : order)
+    for (int i = 0; i < groups[og.second].psi->batch; ++i) trajs.push_back({og.second, i});
+  std::memcpy(blob.data() + grp_off, gd.data(), sizeof(McwfGroupDev) * n_groups);
+  std::memcpy(blob.data() + trj_off, trajs.data(), sizeof(McwfTrajDev) * total_traj);
+
+  hipError_t e =
+      hipMemcpyAsync(dev, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
+  for (int g = 0; g < n_groups && e == hipSuccess; ++g) {
+    const size_t T = (size_t)groups[g].psi->batch;
+    e = hipMemcpyAsync(dev + dyn_off + offs[g].uni, groups[g].uniforms,
+                       T * n_steps * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+  }
+  if (e != hipSuccess) return done(hip_error(e, "smoq_mcwf_evolve: upload"));
+  McwfArgs args{(const McwfGroupDev*)(dev + grp_off), (const McwfTrajDev*)(dev + trj_off),
+                gamma_dt, n_steps, obs_every};
+  const size_t shm = (dtype == SMOQ_C128 ? sizeof(double2) : sizeof(float2)) << kMcTileQ;
+  if (dtype == SMOQ_C128)
+    hipLaunchKernelGGL(k_mcwf<double>, dim3((unsigned)total_traj), dim3(kMcBlock), shm,
+                       ctx->stream, args);
+  else
+    hipLaunchKernelGGL(k_mcwf<float>, dim3((unsigned)total_traj), dim3(kMcBlock), shm,
+                       ctx->stream, args);
+  e = hipGetLastError();
+  if (e != hipSuccess) return done(hip_error(e, "k_mcwf launch"));
+  for (int g = 0; g < n_groups && e == hipSuccess; ++g) {
+    const smoq_mcwf_group& G = groups[g];
+    const size_t T = (size_t)G.psi->batch;
+    e = hipMemcpyAsync(G.jumps, dev + dyn_off + offs[g].jumps, T * n_steps * sizeof(int32_t),
+                       hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && gd[g].z_out)
+      e = hipMemcpyAsync(G.z_out, gd[g].z_out, T * rec_n * G.psi->n * sizeof(double),
+                         hipMemcpyDeviceToHost, ctx->stream);
+  }
+  if (e != hipSuccess) return done(hip_error(e, "smoq_mcwf_evolve: download"));
+  e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return done(hip_error(e, "k_mcwf"));
+  return done(SMOQ_OK);
+}

@@ -52,6 +52,20 @@ class smoq_op(ctypes.Structure):
     ]
 
 
+class smoq_mcwf_group(ctypes.Structure):
+    _fields_ = [
+        ("psi", ctypes.c_void_p),
+        ("n_bonds", ctypes.c_int),
+        ("bond_q", ctypes.c_void_p),
+        ("bond_diag", ctypes.c_void_p),
+        ("site_u", ctypes.c_void_p),
+        ("decay_pow", ctypes.c_void_p),
+        ("uniforms", ctypes.c_void_p),
+        ("jumps", ctypes.c_void_p),
+        ("z_out", ctypes.c_void_p),
+    ]
+
+
 _P = ctypes.c_void_p
 _i = ctypes.c_int
 _d = ctypes.c_double
@@ -93,6 +107,7 @@ SIGNATURES = {
     "smoq_chebyshev_evolve": [_P, _i, _P, _P, _P, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
     "smoq_norm2": [_P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
+    "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
 }
 _RET = {"smoq_last_error": ctypes.c_char_p, "smoq_version": _i}
 

@@ -1,0 +1,161 @@
+"""Monte Carlo wave-function trajectories with σ⁻ jumps, with the reference's
+names (utils/cpu/cpuQuantumChannelLindbladianEvolutionMonteCarloWaveFunction.jl),
+on the device.
+
+Every step of every trajectory runs inside one persistent kernel launch
+(smoq_mcwf_evolve): the first-order Trotter step (diagonal bond gates, then
+site gates), the no-jump decay, the jump decision, σ⁻ and the
+renormalisation.  The reference's `rand()` per trajectory and step is
+replaced by a caller-supplied uniform (SplitMix64 by default), so a run is
+reproducible and can be replayed on the CPU.
+
+Groups of different qubit counts share one launch (`evolve_mcwf_groups`),
+the largest states starting first.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import call, smoq_mcwf_group
+from .hamiltonian import build_hamiltonian_parameters
+from .state import GPUState
+from .streams import uniforms as splitmix_uniforms
+from .trotter import FastTrotterGate, precompute_trotter_gates_bitwise_cpu
+
+
+@dataclass
+class MCWFGates:
+    """A first-order Trotter step in the form the trajectory kernel takes:
+    bond_q (B x 2, 0-based, ascending), bond_diag (B x 4, the diagonal in
+    the basis b(q0) + 2 b(q1)), site_u (N x 2 x 2, identity where a site
+    has no gate)."""
+    N: int
+    bond_q: np.ndarray
+    bond_diag: np.ndarray
+    site_u: np.ndarray
+
+
+def mcwf_gates(gates, N: int) -> MCWFGates:
+    """Splits a gate list (FastTrotterGate, 1-based indices) into the kernel's
+    form.  The kernel applies every 2-qubit gate before every 1-qubit gate,
+    which is the order precompute_trotter_gates_bitwise_cpu emits
+    (…UnitaryEvolutionTrotter.jl:330-351); 2-qubit gates must be diagonal
+    (ZZ couplings: the transverse-field Ising chain of the MCWF studies)."""
+    bq, bd = [], []
+    site = np.tile(np.eye(2, dtype=np.complex128), (N, 1, 1))
+    seen_site = False
+    for g in gates:
+        idx = list(g.indices)
+        U = np.asarray(g.U, np.complex128)
+        if len(idx) == 2:
+            if seen_site:
+                raise ValueError("mcwf_gates: a 2-qubit gate follows a 1-qubit gate")
+            if np.any(U - np.diag(np.diag(U))):
+                raise ValueError(f"mcwf_gates: bond {idx} is not diagonal")
+            lo, hi = sorted(idx)
+            bq.append((lo - 1, hi - 1))
+            bd.append(np.diag(U))
+        elif len(idx) == 1:
+            seen_site = True
+            k = idx[0] - 1
+            site[k] = U @ site[k]
+        else:
+            raise ValueError("Gates with more than 2 qubits not supported")
+    return MCWFGates(N, np.asarray(bq, np.int32).reshape(-1, 2),
+                     np.asarray(bd, np.complex128).reshape(-1, 4), site)
+
+
+def decay_powers(N: int, gamma: float, dt: float) -> np.ndarray:
+    """decay_factor^m for m = 0..N, decay_factor = sqrt(1 - γ dt)
+    (…MonteCarloWaveFunction.jl:344, 352-357)."""
+    if not 0.0 <= gamma * dt <= 1.0:
+        raise ValueError(f"gamma*dt = {gamma * dt} outside [0, 1]")
+    f = np.sqrt(1.0 - gamma * dt)
+    return np.array([f ** m for m in range(N + 1)], np.float64)
+
+
+@dataclass
+class MCWFResult:
+    jumps: np.ndarray    # [traj, step] jump site (0-based) or -1
+    z: np.ndarray | None  # [traj, record, site] <Z_i> after every obs_every-th step
+
+
+class MCWFGroup:
+    """The trajectories of one qubit count: the device batch, the step's
+    gates, the uniforms and the output arrays of one launch."""
+
+    def __init__(self, psi: GPUState, gates: MCWFGates, gamma: float, dt: float,
+                 uniforms: np.ndarray):
+        if gates.N != psi.n:
+            raise ValueError(f"gates for {gates.N} qubits, state has {psi.n}")
+        self.psi = psi
+        self.gates = gates
+        self.decay = decay_powers(psi.n, gamma, dt)
+        self.uniforms = np.ascontiguousarray(uniforms, np.float64)
+        self._bq = np.ascontiguousarray(gates.bond_q, np.int32)
+        self._bd = np.ascontiguousarray(gates.bond_diag, np.complex128)
+        self._su = np.ascontiguousarray(gates.site_u, np.complex128)
+
+    def _struct(self, n_steps: int, obs_every: int):
+        T = self.psi.batch
+        if self.uniforms.shape != (T, n_steps):
+            raise ValueError(f"uniforms of shape {self.uniforms.shape}, want {(T, n_steps)}")
+        self.jumps = np.empty((T, n_steps), np.int32)
+        rec = n_steps // obs_every if obs_every > 0 else 0
+        self.z = np.empty((T, rec, self.psi.n), np.float64) if rec > 0 else None
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+        return smoq_mcwf_group(self.psi.handle, len(self._bq), p(self._bq), p(self._bd),
+                               p(self._su), p(self.decay), p(self.uniforms), p(self.jumps),
+                               p(self.z))
+
+
+def evolve_mcwf_groups(groups, gamma: float, dt: float, n_steps: int, obs_every: int = 0):
+    """Runs n_steps of every trajectory of every group in one launch and
+    returns one MCWFResult per group."""
+    groups = list(groups)
+    arr = (smoq_mcwf_group * len(groups))(*[g._struct(n_steps, obs_every) for g in groups])
+    call("smoq_mcwf_evolve", groups[0].psi.ctx.handle, arr, len(groups), gamma * dt, n_steps,
+         obs_every)
+    return [MCWFResult(g.jumps, g.z) for g in groups]
+
+
+def evolve_mcwf_batched_trotter(psi: GPUState, gates, N: int, n_traj: int, n_steps: int,
+                                gamma: float, dt: float, *, uniforms=None, seed: int = 0,
+                                obs_every: int = 0) -> MCWFResult:
+    """evolve_mcwf_batched_trotter! (…MonteCarloWaveFunction.jl:412-423) with
+    apply_trotter! fixed to the gate list applied in order, rand() replaced
+    by uniforms[traj, step] (SplitMix64 stream `seed` when None) and the
+    observables callback by <Z_i> per trajectory every obs_every steps."""
+    if psi.n != N or psi.batch != n_traj:
+        raise ValueError(f"state holds {psi.batch} x {psi.n} qubits, want {n_traj} x {N}")
+    g = gates if isinstance(gates, MCWFGates) else mcwf_gates(gates, N)
+    if uniforms is None:
+        uniforms = splitmix_uniforms(seed, n_traj * n_steps).reshape(n_traj, n_steps)
+    return evolve_mcwf_groups([MCWFGroup(psi, g, gamma, dt, uniforms)], gamma, dt, n_steps,
+                              obs_every)[0]
+
+
+def lindblad_mcwf_batched_step(psi: GPUState, gates, N: int, n_traj: int, gamma: float,
+                               dt: float, *, uniforms=None, seed: int = 0) -> np.ndarray:
+    """lindblad_mcwf_batched_step! (…MonteCarloWaveFunction.jl:340-390): one
+    step of every trajectory; returns the jump sites (0-based, -1 for none)."""
+    u = None if uniforms is None else np.asarray(uniforms, np.float64).reshape(n_traj, 1)
+    return evolve_mcwf_batched_trotter(psi, gates, N, n_traj, 1, gamma, dt, uniforms=u,
+                                       seed=seed).jumps[:, 0]
+
+
+def tfim_mcwf_gates(N: int, J: float, h: float, dt: float) -> MCWFGates:
+    """The transverse-field Ising chain H = J Σ Z_i Z_{i+1} + h Σ X_i (open
+    boundary; build_hamiltonian_parameters(N, 1, :TFIM_ZZ_X)) as one
+    first-order step: exp(-i J ZZ dt) per bond, then exp(-i h X dt) per site,
+    from precompute_trotter_gates_bitwise_cpu."""
+    params = build_hamiltonian_parameters(N, 1, "TFIM_ZZ_X", J=J, h=h)
+    return mcwf_gates(precompute_trotter_gates_bitwise_cpu(params, dt), N)
+
+
+__all__ = ["MCWFGates", "MCWFGroup", "MCWFResult", "FastTrotterGate", "decay_powers",
+           "evolve_mcwf_batched_trotter", "evolve_mcwf_groups", "lindblad_mcwf_batched_step",
+           "mcwf_gates", "tfim_mcwf_gates"]
