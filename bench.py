@@ -9,9 +9,11 @@ LDS tiles.  value = gate applications per second.
 
 Also reported (outside the timed steps): 1e6 Born samples with the
 reference's cumsum/searchsortedfirst rule, the complex64 sweep and its error
-against complex128, the per-launch roofline of the fused-pass kernel and the
-CPU baseline (the oracle's restatement of the reference loop, one core, on a
-bounded sample of the same circuit).
+against complex128, the per-launch roofline of the fused-pass kernel, the
+adjoint gradient against parameter shift (N=24, 960 parameters; 64 sets at
+N=20), the batched MCWF run (4096 trajectories, N in {12,14,16,18}, 2000
+steps, one launch) and the CPU baseline (the oracle's restatement of the
+reference loop, one core, on a bounded sample of the same circuit).
 """
 from __future__ import annotations
 
@@ -166,10 +168,86 @@ def adjoint_vs_parameter_shift(N=24, layers=20, shift_params=4, batch_n=20, batc
     }
 
 
+def mcwf_batch(sizes=(12, 14, 16, 18), n_traj=4096, steps=2000, dt=0.005, gamma=0.1, J=1.0,
+               h=0.8, obs_every=20, seed=4, cpu_budget_s=8.0):
+    """Batched MCWF for the transverse-field Ising chain with σ⁻ damping on
+    every site: n_traj trajectories split equally over `sizes`, all spins
+    excited at t = 0, every step of every trajectory in one launch.  The CPU
+    figure runs the oracle's trajectory loop for a few steps per size and
+    scales to the whole batch (one core)."""
+    import smoq_amd
+    from smoq_amd import GPUState
+    from smoq_amd import streams
+    from smoq_amd.mcwf import MCWFGroup, evolve_mcwf_groups, tfim_mcwf_gates
+
+    ctx = smoq_amd.default_context()
+    per = n_traj // len(sizes)
+    groups, states = [], []
+    for i, N in enumerate(sizes):
+        s = GPUState.basis(N, (1 << N) - 1, "c128", batch=per)
+        states.append(s)
+        U = streams.uniforms(seed * 1000 + N, per * steps).reshape(per, steps)
+        groups.append(MCWFGroup(s, tfim_mcwf_gates(N, J, h, dt), gamma, dt, U))
+    # warm the kernel on a short run
+    evolve_mcwf_groups(groups, gamma, dt, 1, 0)
+    for s, N in zip(states, sizes):
+        s.set_basis((1 << N) - 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    res = evolve_mcwf_groups(groups, gamma, dt, steps, obs_every)
+    t = time.perf_counter() - t0
+    traj_steps = per * len(sizes) * steps
+    two_pass = sum(per * steps * 4 * (1 << N) * 16 for N in sizes if N > 12)
+    out = {
+        "workload": f"TFIM chain J={J} h={h}, sigma- gamma={gamma} on every site, "
+                    f"{n_traj} trajectories N in {list(sizes)} (equal shares), dt={dt} x {steps}, "
+                    "one launch, c128",
+        "seconds": t,
+        "trajectory_steps_per_s": traj_steps / t,
+        "hbm_GBps_two_pass_sizes": two_pass / t / 1e9,
+        "jumps_per_trajectory": {str(N): float((r.jumps >= 0).sum() / per)
+                                 for N, r in zip(sizes, res)},
+        "mean_Z_site1_final": {str(N): float(r.z[:, -1, 0].mean()) for N, r in zip(sizes, res)},
+    }
+    for s in states:
+        s.free()
+    # CPU: the oracle's step (gate list, decay, decision, σ⁻, renormalise)
+    try:
+        from oracle import oracle as orc
+        from smoq_amd.mcwf import decay_powers
+        from smoq_amd.trotter import precompute_trotter_gates_bitwise_cpu
+        from smoq_amd.hamiltonian import build_hamiltonian_parameters
+
+        est, sample = 0.0, []
+        for N in sizes:
+            gl = [(list(g.indices), g.U) for g in precompute_trotter_gates_bitwise_cpu(
+                build_hamiltonian_parameters(N, 1, "TFIM_ZZ_X", J=J, h=h), dt)]
+            psi = np.zeros(1 << N, np.complex128)
+            psi[-1] = 1.0
+            dec = decay_powers(N, gamma, dt)
+            n_done, t0 = 0, time.perf_counter()
+            while n_done < 2 or (time.perf_counter() - t0 < cpu_budget_s / len(sizes)
+                                 and n_done < steps):
+                orc.mcwf_trajectory(psi, gl, N, gamma * dt, dec, [0.5], 0)
+                n_done += 1
+            per_step = (time.perf_counter() - t0) / n_done
+            est += per_step * per * steps
+            sample.append(f"N={N}: {n_done} steps")
+        out["cpu_oracle_estimate_s"] = est
+        out["cpu_sample"] = "oracle trajectory loop, 1 core; " + ", ".join(sample)
+        out["speedup_vs_cpu_one_core"] = est / t
+    except Exception as e:  # noqa: BLE001 - the figure is informative only
+        out["cpu_oracle_estimate_s"] = None
+        out["cpu_error"] = str(e)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", choices=["traffic"], default=None)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-mcwf", action="store_true")
+    ap.add_argument("--mcwf-only", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -189,6 +267,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.mcwf_only:
+        print(json.dumps({"mcwf": mcwf_batch()}), flush=True)
+        return
 
     import smoq_amd
     from smoq_amd import GPUState
@@ -289,6 +370,9 @@ def main():
     if not args.no_adjoint and not args.no_extras:
         log("adjoint gradient vs parameter shift ...")
         result.setdefault("extras", {})["adjoint"] = adjoint_vs_parameter_shift()
+    if not args.no_mcwf and not args.no_extras:
+        log("batched MCWF ...")
+        result.setdefault("extras", {})["mcwf"] = mcwf_batch()
 
     if not args.no_cpu:
         log("CPU baseline ...")

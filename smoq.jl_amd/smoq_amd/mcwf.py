@@ -101,14 +101,17 @@ class MCWFGroup:
 
     def _struct(self, n_steps: int, obs_every: int):
         T = self.psi.batch
-        if self.uniforms.shape != (T, n_steps):
-            raise ValueError(f"uniforms of shape {self.uniforms.shape}, want {(T, n_steps)}")
+        u = self.uniforms
+        if u.ndim != 2 or u.shape[0] != T or u.shape[1] < n_steps:
+            raise ValueError(f"uniforms of shape {u.shape}, want ({T}, >= {n_steps})")
+        # the first n_steps columns, contiguous; kept alive until the call returns
+        self._u = u if u.shape[1] == n_steps else np.ascontiguousarray(u[:, :n_steps])
         self.jumps = np.empty((T, n_steps), np.int32)
         rec = n_steps // obs_every if obs_every > 0 else 0
         self.z = np.empty((T, rec, self.psi.n), np.float64) if rec > 0 else None
         p = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
         return smoq_mcwf_group(self.psi.handle, len(self._bq), p(self._bq), p(self._bd),
-                               p(self._su), p(self.decay), p(self.uniforms), p(self.jumps),
+                               p(self._su), p(self.decay), p(self._u), p(self.jumps),
                                p(self.z))
 
 

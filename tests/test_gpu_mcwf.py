@@ -214,9 +214,43 @@ def test_mcwf_rejects_bad_input(ctx):
     with pytest.raises(ValueError):  # γ dt > 1
         evolve_mcwf_groups([MCWFGroup(dev, g, 20.0, 0.1, np.zeros((2, 3)))], 20.0, 0.1, 3)
     with pytest.raises(ValueError):
-        evolve_mcwf_groups([MCWFGroup(dev, g, 0.1, 0.1, np.zeros((2, 4)))], 0.1, 0.1, 3)
+        evolve_mcwf_groups([MCWFGroup(dev, g, 0.1, 0.1, np.zeros((2, 2)))], 0.1, 0.1, 3)
     big = GPUState(21)
     g21 = MCWFGates(21, np.zeros((0, 2), np.int32), np.zeros((0, 4), np.complex128),
                     np.tile(np.eye(2, dtype=np.complex128), (21, 1, 1)))
     with pytest.raises(SmoqError):
         evolve_mcwf_groups([MCWFGroup(big, g21, 0.1, 0.1, np.zeros((1, 1)))], 0.1, 0.1, 1)
+
+
+def test_mcwf_tfim_damping_vs_density_matrix(ctx):
+    """The damped TFIM chain (J = 1, h = 0.8, γ = 0.1, dt = 0.005, 2000
+    steps) at N = 8 and N = 10, 4096 trajectories each in one launch: the
+    trajectory mean of <Z_i>(t) at t = 1..10 against the exact
+    density-matrix solution (tests/golden/lindblad_tfim_damping.json),
+    within 5 standard errors + 0.01 (first-order splitting)."""
+    import json
+    import os
+
+    from smoq_amd import GPUState
+    from smoq_amd.mcwf import MCWFGroup, evolve_mcwf_groups, tfim_mcwf_gates
+    from smoq_amd.streams import uniforms
+
+    with open(os.path.join(os.path.dirname(__file__), "golden",
+                           "lindblad_tfim_damping.json")) as f:
+        ref = json.load(f)
+    J, h, gamma = ref["J"], ref["h"], ref["gamma"]
+    dt, steps, every, T = 0.005, 2000, 200, 4096
+    assert ref["times"] == [every * dt * (r + 1) for r in range(steps // every)]
+    sizes = ref["sizes"]
+    groups = []
+    for N in sizes:
+        s = GPUState.basis(N, (1 << N) - 1, batch=T)
+        U = uniforms(31 + N, T * steps).reshape(T, steps)
+        groups.append(MCWFGroup(s, tfim_mcwf_gates(N, J, h, dt), gamma, dt, U))
+    res = evolve_mcwf_groups(groups, gamma, dt, steps, every)
+    for N, r in zip(sizes, res):
+        z_dm = np.array(ref["z"][str(N)])
+        z_mc = r.z.mean(0)
+        se = r.z.std(0) / np.sqrt(T)
+        assert z_mc.shape == z_dm.shape == (steps // every, N)
+        assert np.all(np.abs(z_mc - z_dm) < 5 * se + 0.01), (N, np.max(np.abs(z_mc - z_dm)))
