@@ -47,6 +47,9 @@ struct McwfArgs {
   const McwfTrajDev* trajs;
   double gamma_dt;
   int n_steps, obs_every;
+  // smallest blockIdx.x * n_steps + step whose renormalisation met a zero or
+  // non-finite norm; all ones when none did
+  unsigned long long* fault;
 };
 
 __device__ inline double mc_wave_sum(double v) {
@@ -319,8 +322,12 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
     if (threadIdx.x == 0) {
       G.jumps[(uint64_t)tr.idx * A.n_steps + step] = jump ? s_site : -1;
       s_scale = 1.0 / sqrt(new_norm_sq);
+      if (!(new_norm_sq > 0.0) || !isfinite(s_scale))
+        atomicMin(A.fault, (unsigned long long)blockIdx.x * A.n_steps + step);
     }
     __syncthreads();
+    // the trajectory cannot continue; the host reports the first such step
+    if (!(new_norm_sq > 0.0) || !isfinite(s_scale)) break;
     if (rec_n > 0 && G.z_out && (step + 1) % A.obs_every == 0) {
       // <Z_i> = (||ψ||² - 2 pop_i) / ||ψ||², independent of the pending scale
       pops();
@@ -447,6 +454,7 @@ extern "C" int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, in
   }
   const size_t grp_off = reserve(sizeof(McwfGroupDev) * n_groups);
   const size_t trj_off = reserve(sizeof(McwfTrajDev) * total_traj);
+  const size_t fault_off = reserve(sizeof(unsigned long long));
   const size_t dyn_off = (blob.size() + 255) & ~(size_t)255;
 
   unsigned char* dev = nullptr;
@@ -488,6 +496,8 @@ extern "C" int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, in
     for (int i = 0; i < groups[og.second].psi->batch; ++i) trajs.push_back({og.second, i});
   std::memcpy(blob.data() + grp_off, gd.data(), sizeof(McwfGroupDev) * n_groups);
   std::memcpy(blob.data() + trj_off, trajs.data(), sizeof(McwfTrajDev) * total_traj);
+  const unsigned long long no_fault = ~0ull;
+  std::memcpy(blob.data() + fault_off, &no_fault, sizeof(no_fault));
 
   hipError_t e =
       hipMemcpyAsync(dev, blob.data(), blob.size(), hipMemcpyHostToDevice, ctx->stream);
@@ -498,7 +508,7 @@ extern "C" int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, in
   }
   if (e != hipSuccess) return done(hip_error(e, "smoq_mcwf_evolve: upload"));
   McwfArgs args{(const McwfGroupDev*)(dev + grp_off), (const McwfTrajDev*)(dev + trj_off),
-                gamma_dt, n_steps, obs_every};
+                gamma_dt, n_steps, obs_every, (unsigned long long*)(dev + fault_off)};
   const size_t shm = (dtype == SMOQ_C128 ? sizeof(double2) : sizeof(float2)) << kMcTileQ;
   if (dtype == SMOQ_C128)
     hipLaunchKernelGGL(k_mcwf<double>, dim3((unsigned)total_traj), dim3(kMcBlock), shm,
@@ -517,8 +527,20 @@ extern "C" int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, in
       e = hipMemcpyAsync(G.z_out, gd[g].z_out, T * rec_n * G.psi->n * sizeof(double),
                          hipMemcpyDeviceToHost, ctx->stream);
   }
+  unsigned long long fault = no_fault;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&fault, dev + fault_off, sizeof(fault), hipMemcpyDeviceToHost,
+                       ctx->stream);
   if (e != hipSuccess) return done(hip_error(e, "smoq_mcwf_evolve: download"));
   e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return done(hip_error(e, "k_mcwf"));
+  if (fault != no_fault) {
+    const McwfTrajDev& t = trajs[fault / (unsigned long long)n_steps];
+    return done(set_error(SMOQ_ENUMERIC,
+                          "smoq_mcwf_evolve: zero or non-finite norm at step %d of trajectory %d "
+                          "in group %d (%d qubits); that trajectory stopped there",
+                          (int)(fault % (unsigned long long)n_steps), t.idx, t.group,
+                          groups[t.group].psi->n));
+  }
   return done(SMOQ_OK);
 }

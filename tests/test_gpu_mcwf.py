@@ -254,3 +254,27 @@ def test_mcwf_tfim_damping_vs_density_matrix(ctx):
         se = r.z.std(0) / np.sqrt(T)
         assert z_mc.shape == z_dm.shape == (steps // every, N)
         assert np.all(np.abs(z_mc - z_dm) < 5 * se + 0.01), (N, np.max(np.abs(z_mc - z_dm)))
+
+
+def test_mcwf_zero_norm_is_reported(ctx):
+    """(|00> + |10>)/√2 with γdt = 0.5 and r = 0.2: the decision jumps but the
+    site weights (0, 1/6) stay below r, so the fallback site 0 is chosen,
+    whose population is 0: σ⁻ leaves a zero vector (NaN in the reference
+    loop).  The call reports SMOQ_ENUMERIC naming the trajectory and step;
+    the other trajectory completes."""
+    from smoq_amd import GPUState, SmoqError
+    from smoq_amd import _lib
+    from smoq_amd.mcwf import MCWFGates, MCWFGroup, evolve_mcwf_groups
+
+    psi0 = np.zeros((2, 4), np.complex128)
+    psi0[0, 0] = 1.0
+    psi0[1, 0] = psi0[1, 2] = 1.0 / np.sqrt(2.0)
+    dev = GPUState.from_numpy(psi0)
+    g = MCWFGates(2, np.zeros((0, 2), np.int32), np.zeros((0, 4), np.complex128),
+                  np.tile(np.eye(2, dtype=np.complex128), (2, 1, 1)))
+    grp = MCWFGroup(dev, g, 5.0, 0.1, np.full((2, 3), 0.2))
+    with pytest.raises(SmoqError) as e:
+        evolve_mcwf_groups([grp], 5.0, 0.1, 3)
+    assert e.value.code == _lib.SMOQ_ENUMERIC
+    assert "step 0 of trajectory 1" in str(e.value)
+    assert grp.jumps[0].tolist() == [-1, -1, -1] and grp.jumps[1, 0] == 0

@@ -91,3 +91,101 @@ def test_oracle_chebyshev_matches_expm_multiply():
     want = expm_multiply(-1j * 2.0 * xxz_dense(N, J, delta, h), psi0)
     assert np.linalg.norm(got - want) < 1e-11
     assert m > 10
+
+
+# ------------------------------------------------------------------ MCWF --
+SM = np.array([[0, 1], [0, 0]], np.complex128)  # σ⁻ = |0><1|
+
+
+def dense_decay_jump(psi, N, gdt, r):
+    """One decay/decision/jump/renormalise of the batched MCWF step with
+    dense operators: exp(-γdt/2 · n_exc) as a diagonal, σ⁻ as a matrix."""
+    idx = np.arange(1 << N)
+    pc = np.array([bin(i).count("1") for i in idx])
+    psi = psi * np.sqrt(1.0 - gdt) ** pc
+    nsq = np.vdot(psi, psi).real
+    dp = 1.0 - nsq
+    site = 0
+    if r < dp and dp > 0:
+        pops = np.array([np.vdot(psi, op_on({k: SM.conj().T @ SM}, N) @ psi).real
+                         for k in range(N)])
+        hits = np.nonzero(r < np.cumsum(gdt * pops / nsq))[0]
+        k = int(hits[0]) if hits.size else 0
+        psi = op_on({k: SM}, N) @ psi
+        site = k + 1
+    return psi / np.linalg.norm(psi), site, dp
+
+
+def test_mcwf_decay_jump_known_cases():
+    """|11>, γdt = 0.2: dp = 1 - 0.8² = 0.36, cumulative site weights
+    0.2 and 0.4; r picks no jump / site 1 / site 2 / (none reachable)."""
+    dec = np.sqrt(0.8) ** np.arange(3)
+    for r, want_site, want_idx in [(0.5, 0, 3), (0.1, 1, 2), (0.3, 2, 1)]:
+        psi = np.zeros(4, np.complex128)
+        psi[3] = 1.0
+        site, dp = orc.mcwf_decay_jump(psi, 2, 0.2, dec, r)
+        assert site == want_site and abs(dp - 0.36) < 1e-15
+        assert abs(psi[want_idx] - 1.0) < 1e-15 and np.sum(np.abs(psi) ** 2) == pytest.approx(1)
+
+
+def test_mcwf_decay_jump_matches_dense_operators():
+    rng = np.random.default_rng(8)
+    N, gdt = 4, 0.3
+    dec = np.sqrt(1.0 - gdt) ** np.arange(N + 1)
+    sites = set()
+    for _ in range(200):
+        psi = rng.normal(size=16) + 1j * rng.normal(size=16)
+        psi /= np.linalg.norm(psi)
+        r = rng.uniform(0, 0.5)
+        want, wsite, wdp = dense_decay_jump(psi.copy(), N, gdt, r)
+        got = psi.copy()
+        site, dp = orc.mcwf_decay_jump(got, N, gdt, dec, r)
+        assert site == wsite and abs(dp - wdp) < 1e-14
+        assert np.max(np.abs(got - want)) < 1e-13
+        sites.add(site)
+    assert sites == {0, 1, 2, 3, 4}
+
+
+def test_mcwf_fallback_site():
+    """(|0> + |1>)/√2, γdt = 0.5: dp = 0.25 but the site weight is only
+    γdt·pop/||ψ||² = 1/6, so r = 0.2 jumps through the fallback (site 1)."""
+    dec = np.sqrt(0.5) ** np.arange(2)
+    for r in (0.1, 0.2):
+        psi = np.array([1.0, 1.0], np.complex128) / np.sqrt(2.0)
+        site, dp = orc.mcwf_decay_jump(psi, 1, 0.5, dec, r)
+        assert site == 1 and abs(dp - 0.25) < 1e-15
+        assert np.max(np.abs(psi - [1.0, 0.0])) < 1e-15
+
+
+def test_mcwf_trajectory_matches_dense_step():
+    """Ten-step trajectories of the oracle (gate list + decay/jump) against
+    dense matrices: bond diagonals, site rotations, decay, σ⁻."""
+    N, gamma_dt = 3, 0.25
+    rng = np.random.default_rng(2)
+    dt = 0.1
+    gates = [([1, 2], expm(-1j * 0.7 * np.kron(Z, Z) * dt)),
+             ([2, 3], expm(-1j * 1.3 * np.kron(Z, Z) * dt))]
+    gates += [([k + 1], expm(-1j * (0.8 * X + 0.3 * Z) * dt)) for k in range(N)]
+    # dense step: the 2-qubit gates are diagonal, applied as one diagonal
+    idx = np.arange(8)
+    diag = np.ones(8, np.complex128)
+    for q, G in gates[:2]:
+        e = ((idx >> (q[0] - 1)) & 1) | (((idx >> (q[1] - 1)) & 1) << 1)
+        diag *= np.diag(G)[e]
+    U = np.diag(diag)
+    for q, G in gates[2:]:
+        U = op_on({q[0] - 1: G}, N) @ U
+    dec = np.sqrt(1.0 - gamma_dt) ** np.arange(N + 1)
+    for traj in range(20):
+        psi0 = rng.normal(size=8) + 1j * rng.normal(size=8)
+        psi0 /= np.linalg.norm(psi0)
+        rs = rng.uniform(0, 1, 10) * 0.6
+        got = psi0.copy()
+        jumps, _ = orc.mcwf_trajectory(got, gates, N, gamma_dt, dec, rs)
+        ref = psi0.copy()
+        want = []
+        for r in rs:
+            ref, site, _ = dense_decay_jump(U @ ref, N, gamma_dt, r)
+            want.append(site - 1)
+        np.testing.assert_array_equal(jumps, want)
+        assert np.max(np.abs(got - ref)) < 1e-12
