@@ -288,6 +288,7 @@ def main():
 
     psi = GPUState(N, "c128")
     S = psi.nbytes
+    n_passes = None
     for _ in range(args.warmup):
         psi.set_basis(0)
         n_passes = tape.apply(psi)
@@ -298,7 +299,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         psi.set_basis(0)
-        tape.apply(psi)
+        n_passes = tape.apply(psi)
     ctx.sync()
     t1 = time.perf_counter()
     pass_ms, launches = ctx.pass_timing(False)

@@ -102,6 +102,13 @@ int smoq_sync(smoq_ctx* ctx);
  * 6..12) and low_qubits (contiguous low bits kept in every tile, 3..6).
  * Chosen by the autotuner; 0 keeps the current value. */
 int smoq_set_fusion(smoq_ctx* ctx, int tile_qubits, int low_qubits);
+/* Fused passes compiled at run time for one circuit structure (hiprtc,
+ * cached per context by structure; gate angles stay run-time data):
+ * mode 0 never, 1 automatically for states of >= 16 qubits with >= 16
+ * gates (the default; the precompiled pass kernel is kept if compilation
+ * fails), 2 always (compilation failure is an error).  The SMOQ_JIT
+ * environment variable sets the initial mode of new contexts. */
+int smoq_set_jit(smoq_ctx* ctx, int mode);
 /* Mean kernel time (ms) and launch count of the fused-pass kernel since the
  * last call, measured with HIP events on the context stream when timing is
  * enabled (enable != 0).  Used to report the per-launch roofline. */

@@ -1,7 +1,11 @@
 // Device-side helpers shared by the kernel translation units.
 #pragma once
 
+// Also compiled by the run-time compiler (as embedded text), which brings its
+// own HIP runtime declarations.
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 
 #include <cstdint>
 

@@ -1,0 +1,307 @@
+// Device code of the fused gate passes: the register-stage gate helpers and
+// the tile/stage frame shared by the precompiled pass kernel (fused.hip) and
+// the pass kernels compiled at run time for one circuit structure (jit.hip),
+// which see this file as embedded text.
+#pragma once
+
+#include "device_common.h"
+#include "fused_types.h"
+
+namespace smoq {
+
+template <typename T>
+using VT = typename CT<T>::V;
+constexpr int kNR = 1 << kRegQubits;
+constexpr uint32_t kAllPatterns = (1u << kNR) - 1u;
+// bit j set for every register pattern j with slot t set
+__host__ __device__ constexpr uint32_t slot_patterns(int t) {
+  uint32_t m = 0;
+  for (int j = 0; j < kNR; ++j)
+    if ((j >> t) & 1) m |= 1u << j;
+  return m;
+}
+
+template <typename T>
+__device__ __forceinline__ VT<T> ld_c(const double* m, int e) {
+  return make_c<T>(m[2 * e], m[2 * e + 1]);
+}
+
+// u*a for the entry kinds of the specialised 2x2 forms
+template <typename T, int PT>
+__device__ __forceinline__ VT<T> u1_row(const double* m, int r, VT<T> a, VT<T> b) {
+  VT<T> o;
+  if (PT == GK_U1_REAL) {
+    const T p = (T)m[4 * r], q = (T)m[4 * r + 2];
+    o.x = p * a.x + q * b.x;
+    o.y = p * a.y + q * b.y;
+  } else if (PT == GK_U1_RXT) {
+    if (r == 0) {  // (u00 real) a + (i u01i) b
+      const T p = (T)m[0], q = (T)m[3];
+      o.x = p * a.x - q * b.y;
+      o.y = p * a.y + q * b.x;
+    } else {  // (i u10i) a + (u11 real) b
+      const T p = (T)m[5], q = (T)m[6];
+      o.x = q * b.x - p * a.y;
+      o.y = q * b.y + p * a.x;
+    }
+  } else {
+    o = cfma2(ld_c<T>(m, 2 * r), a, ld_c<T>(m, 2 * r + 1), b);
+  }
+  return o;
+}
+
+// 2x2 on register slot A for every pair (j, j | 1<<A)
+template <typename T, int PT, int A, bool CTRL>
+__device__ __forceinline__ void reg_u1(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
+  if constexpr (A >= kRegQubits) return;
+  const int creg = g.creg;
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) {
+    if (j & (1 << A)) continue;
+    const int h = j | (1 << A);
+    const VT<T> a = x[j], b = x[h];
+    const VT<T> na = u1_row<T, PT>(g.m, 0, a, b);
+    const VT<T> nb = u1_row<T, PT>(g.m, 1, a, b);
+    if (CTRL) {
+      const bool on = gon && ((j & creg) == creg);
+      x[j] = csel(on, na, a);
+      x[h] = csel(on, nb, b);
+    } else {
+      x[j] = na;
+      x[h] = nb;
+    }
+  }
+}
+
+template <typename T, int PT, bool CTRL>
+__device__ __forceinline__ void reg_u1_slot(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
+  switch (g.a) {
+    case 0: reg_u1<T, PT, 0, CTRL>(x, g, gon); break;
+    case 1: reg_u1<T, PT, 1, CTRL>(x, g, gon); break;
+    case 2: reg_u1<T, PT, 2, CTRL>(x, g, gon); break;
+    default: reg_u1<T, PT, 3, CTRL>(x, g, gon); break;
+  }
+}
+
+template <typename T, int PT>
+__device__ __forceinline__ void reg_u1_any(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl) {
+  if ((g.creg | g.cglob) == 0) {
+    reg_u1_slot<T, PT, false>(x, g, true);
+  } else {
+    reg_u1_slot<T, PT, true>(x, g, (gl & g.cglob) == g.cglob);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ VT<T> u2_row(const double* m, int r, VT<T> a0, VT<T> a1, VT<T> a2,
+                                        VT<T> a3) {
+  VT<T> acc = cmul(ld_c<T>(m, 4 * r), a0);
+  acc = cadd(acc, cmul(ld_c<T>(m, 4 * r + 1), a1));
+  acc = cadd(acc, cmul(ld_c<T>(m, 4 * r + 2), a2));
+  return cadd(acc, cmul(ld_c<T>(m, 4 * r + 3), a3));
+}
+
+// 4x4 on slots (A low role, B high role)
+template <typename T, int A, int B>
+__device__ __forceinline__ void reg_u2(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
+  if constexpr (A >= kRegQubits || B >= kRegQubits) return;
+  const int creg = g.creg;
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) {
+    if (j & ((1 << A) | (1 << B))) continue;
+    const int id0 = j, id1 = j | (1 << A), id2 = j | (1 << B), id3 = j | (1 << A) | (1 << B);
+    const VT<T> in0 = x[id0], in1 = x[id1], in2 = x[id2], in3 = x[id3];
+    const bool on = gon && ((j & creg) == creg);
+    const VT<T> o0 = u2_row<T>(g.m, 0, in0, in1, in2, in3);
+    const VT<T> o1 = u2_row<T>(g.m, 1, in0, in1, in2, in3);
+    const VT<T> o2 = u2_row<T>(g.m, 2, in0, in1, in2, in3);
+    const VT<T> o3 = u2_row<T>(g.m, 3, in0, in1, in2, in3);
+    x[id0] = csel(on, o0, in0);
+    x[id1] = csel(on, o1, in1);
+    x[id2] = csel(on, o2, in2);
+    x[id3] = csel(on, o3, in3);
+  }
+}
+
+// merged diagonal gates: x[j] *= table[row][j], row from non-register bits
+template <typename T>
+__device__ __forceinline__ void reg_diagtab(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl,
+                                            const VT<T>* __restrict__ tabs) {
+  int row = 0;
+  for (int t = 0; t < g.nu; ++t) row |= (int)((gl >> g.ubits[t]) & 1ull) << t;
+  const VT<T>* p = tabs + g.tab + row * kNR;
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) x[j] = cmul(p[j], x[j]);
+}
+
+// merged CZs: the lane builds a 16-bit mask of the slot patterns to negate
+// (lane-lane pairs, register-register pairs, register-lane pairs)
+template <typename T>
+__device__ __forceinline__ void reg_czset(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl) {
+  int par = 0;
+  for (int i = 0; i < g.ncz; ++i) par ^= __popcll(gl & (gl >> g.czs[i]) & g.czm[i]);
+  uint32_t neg = (par & 1) ? kAllPatterns : 0u;
+  neg ^= (uint32_t)g.czreg;
+#pragma unroll
+  for (int t = 0; t < kRegQubits; ++t)
+    neg ^= (__popcll(gl & g.czcm[t]) & 1) ? slot_patterns(t) : 0u;
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) {
+    const bool f = (neg >> j) & 1u;
+    x[j].x = f ? -x[j].x : x[j].x;
+    x[j].y = f ? -x[j].y : x[j].y;
+  }
+}
+
+// diagonal on register qubits only: uniform phase per slot pattern
+template <typename T>
+__device__ __forceinline__ void reg_diagreg(VT<T> (&x)[kNR], const GateDev& g) {
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) x[j] = cmul(ld_c<T>(g.m, j), x[j]);
+}
+
+// diagonal on non-register qubits only: one phase per lane
+template <typename T>
+__device__ __forceinline__ void reg_diaglane(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl) {
+  int row = 0;
+  for (int t = 0; t < g.nu; ++t) row |= (int)((gl >> g.ubits[t]) & 1ull) << t;
+  VT<T> ph = ld_c<T>(g.m, 0);
+#pragma unroll
+  for (int e = 1; e < (1 << kMaxTabBits); ++e) ph = csel(row == e, ld_c<T>(g.m, e), ph);
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) x[j] = cmul(ph, x[j]);
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_gate(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl,
+                                           const VT<T>* __restrict__ tabs) {
+  switch (g.kind) {
+    case GK_U1_REAL: reg_u1_any<T, GK_U1_REAL>(x, g, gl); break;
+    case GK_U1_RXT: reg_u1_any<T, GK_U1_RXT>(x, g, gl); break;
+    case GK_U1_GEN: reg_u1_any<T, GK_U1_GEN>(x, g, gl); break;
+    case GK_U2: {
+      const bool gon = (gl & g.cglob) == g.cglob;
+      switch (g.a * 4 + g.b) {
+        case 1: reg_u2<T, 0, 1>(x, g, gon); break;
+        case 2: reg_u2<T, 0, 2>(x, g, gon); break;
+        case 3: reg_u2<T, 0, 3>(x, g, gon); break;
+        case 4: reg_u2<T, 1, 0>(x, g, gon); break;
+        case 6: reg_u2<T, 1, 2>(x, g, gon); break;
+        case 7: reg_u2<T, 1, 3>(x, g, gon); break;
+        case 8: reg_u2<T, 2, 0>(x, g, gon); break;
+        case 9: reg_u2<T, 2, 1>(x, g, gon); break;
+        case 11: reg_u2<T, 2, 3>(x, g, gon); break;
+        case 12: reg_u2<T, 3, 0>(x, g, gon); break;
+        case 13: reg_u2<T, 3, 1>(x, g, gon); break;
+        default: reg_u2<T, 3, 2>(x, g, gon); break;
+      }
+      break;
+    }
+    case GK_DIAGTAB: reg_diagtab<T>(x, g, gl, tabs); break;
+    case GK_CZSET: reg_czset<T>(x, g, gl); break;
+    case GK_DIAGREG: reg_diagreg<T>(x, g); break;
+    default: reg_diaglane<T>(x, g, gl); break;
+  }
+}
+
+// ---------------------------------------------------------------- frames --
+// One workgroup, one tile of 2^k amplitudes: the tile id picks the batch
+// item and the global bits outside the tile (uniform), the thread index the
+// global bits of local positions 0..k-R-1; the unrolled register index
+// covers the top R local positions (uniform offsets).
+template <typename T>
+struct PassFrame {
+  VT<T>* pt;         // amplitude of register index 0 of this thread
+  uint64_t gbase;    // global bits outside the tile
+  uint64_t off[kNR]; // offsets of register index i
+  uint32_t st_lane;  // swizzled LDS index of the thread's lane part
+  int nt;            // threads per tile
+};
+
+// HBM -> LDS for the workgroup's tile; ends with a barrier
+template <typename T>
+__device__ __forceinline__ void pass_load(VT<T>* lds, VT<T>* __restrict__ psi, int n,
+                                          const PassDev* __restrict__ pass, PassFrame<T>& f) {
+  constexpr int R = kRegQubits;
+  constexpr int WB = CT<T>::kRowBits;
+  const int nn = pass->n_nonlocal;
+  const uint64_t tile = blockIdx.x;
+  const uint64_t item = tile >> nn;
+  const uint64_t tt = tile & ((1ull << nn) - 1ull);
+  f.nt = 1 << (pass->k - R);
+  f.gbase = pdep64(tt, pass->nl_mask);
+  f.pt = psi + (item << n) + (f.gbase | pdep64((uint64_t)threadIdx.x, pass->tid_mask));
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) o |= ((i >> j) & 1) ? pass->hb[j] : 0ull;
+    f.off[i] = o;
+  }
+  // The swizzle is linear over GF(2): swz(a | b) = swz(a) ^ swz(b) for
+  // disjoint a, b, so every LDS address is a per-lane part XOR a uniform part.
+  f.st_lane = swz<WB>((uint32_t)threadIdx.x);
+  VT<T> x[kNR];
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) x[i] = f.pt[f.off[i]];
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))] = x[i];
+  __syncthreads();
+}
+
+// LDS -> HBM for the tile (after the last stage's barrier)
+template <typename T>
+__device__ __forceinline__ void pass_store(const VT<T>* lds, const PassFrame<T>& f) {
+  constexpr int WB = CT<T>::kRowBits;
+  VT<T> x[kNR];
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))];
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) f.pt[f.off[i]] = x[i];
+}
+
+// One register stage: 2^R amplitudes of the thread (its lane part plus the
+// stage's register qubits) move from LDS into complex128 registers.
+struct StageFrame {
+  uint32_t sl;       // lane part of the swizzled LDS index
+  uint64_t gl;       // global index of the thread's register index 0
+  uint32_t sr[kNR];  // swizzled LDS offsets of the register patterns
+};
+
+template <typename T>
+__device__ __forceinline__ void stage_load(const VT<T>* lds, const StageDev& st, uint64_t gbase,
+                                           VT<double> (&x)[kNR], StageFrame& sf) {
+  constexpr int R = kRegQubits;
+  const int tid = threadIdx.x;
+  // lane part of the swizzled LDS index and of the global index, from the
+  // host-precomputed images of each thread-index bit
+  sf.sl = 0;
+  sf.gl = gbase;
+#pragma unroll
+  for (int i = 0; i < kMaxTileQubits - R; ++i) {
+    if (i < st.ntb && ((tid >> i) & 1)) {
+      sf.sl ^= st.sw_tid[i];
+      sf.gl |= st.g_tid[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) r ^= ((i >> j) & 1) ? st.sw_reg[j] : 0u;
+    sf.sr[i] = r;
+  }
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) x[i] = to_d2(lds[sf.sl ^ sf.sr[i]]);
+}
+
+// registers -> LDS (rounded to T), then a barrier
+template <typename T>
+__device__ __forceinline__ void stage_store(VT<T>* lds, const StageFrame& sf,
+                                            const VT<double> (&x)[kNR]) {
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) lds[sf.sl ^ sf.sr[i]] = make_c<T>(x[i].x, x[i].y);
+  __syncthreads();
+}
+
+}  // namespace smoq

@@ -11,6 +11,7 @@ struct smoq_plan_s {
   smoq::ProgLayout layout;
   void* dev = nullptr;  // device image of prog
   std::vector<smoq::IGate> small;  // n < kRegQubits: replayed gate by gate
+  std::shared_ptr<smoq::JitModule> jit;  // compiled pass kernels, when used
   // captured launches of one application, valid for one (state, batch)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -36,7 +37,7 @@ static int run_once(smoq_plan* p, smoq_state* s) {
     }
     return SMOQ_OK;
   }
-  return launch_program(s, p->prog, p->layout, p->dev);
+  return launch_program(s, p->prog, p->layout, p->dev, p->jit.get());
 }
 
 // capture one application of the plan on state s into an executable graph
@@ -106,6 +107,14 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
   plan_program(gs, n_qubits, ctx->tile_qubits, ctx->low_qubits,
                dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog);
   p->layout = program_layout(p->prog, dtype);
+  if (jit_wanted(ctx, p->prog, n_qubits)) {
+    rc = jit_get(ctx, p->prog, dtype, p->jit);
+    // automatic mode keeps the precompiled kernel when compilation fails
+    if (rc && ctx->jit_mode >= 2) {
+      delete p;
+      return rc;
+    }
+  }
   hipError_t e = hipMalloc(&p->dev, p->layout.total);
   if (e != hipSuccess) {
     delete p;

@@ -316,6 +316,10 @@ int smoq_init(smoq_ctx** out, int device) {
                      device, prop.gcnArchName);
   smoq_ctx* ctx = new smoq_ctx();
   ctx->device = device;
+  if (const char* j = std::getenv("SMOQ_JIT")) {
+    const int m = std::atoi(j);
+    if (m >= 0 && m <= 2) ctx->jit_mode = m;
+  }
   hipError_t se = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (se != hipSuccess) {
     delete ctx;

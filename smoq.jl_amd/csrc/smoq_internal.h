@@ -5,10 +5,13 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "smoq.h"
+#include "fused_types.h"
 
 namespace smoq {
 
@@ -50,77 +53,7 @@ struct IGate {
 // Lowers one public op (angle already resolved) to internal gates.
 int lower_op(const smoq_op& op, double theta, int n_qubits, std::vector<IGate>& out);
 
-// ------------------------------------------------------- fused programs --
-// 8 amplitudes per thread: at 16 per thread the complex128 stage kernel needs
-// ~255 VGPRs (one wave per SIMD) and LDS latency between stages goes
-// unhidden; at 8 it fits in ~112 VGPRs (four waves per SIMD).
-#ifndef SMOQ_REG_QUBITS
-#define SMOQ_REG_QUBITS 3
-#endif
-constexpr int kRegQubits = SMOQ_REG_QUBITS;  // amplitudes per thread = 2^kRegQubits
-constexpr int kMaxTileQubits = 12;     // LDS tile of up to 4096 amplitudes
-constexpr int kMaxQubits = 40;
 
-// Gate kinds inside a register stage.
-enum GKind : int32_t {
-  GK_U1_REAL = 0,  // 2x2 with real entries (Ry, H, X, sigma+-)
-  GK_U1_RXT = 1,   // real diagonal, imaginary off-diagonal (Rx, Y)
-  GK_U1_GEN = 2,   // general 2x2
-  GK_U2 = 3,       // general 4x4
-  GK_DIAGTAB = 4,  // merged diagonal gates: phase table[row][slot pattern]
-  GK_CZSET = 5,    // merged CZs: per-lane 16-bit negate mask
-  GK_DIAGREG = 6,  // diagonal over register qubits only: m = 16 phases
-  GK_DIAGLANE = 7, // diagonal over non-register qubits only: m = 2^nu phases
-};
-constexpr int kMaxTabBits = 3;   // non-register qubits of one phase table
-constexpr int kMaxCzShifts = 4;
-
-// A gate as one register stage sees it.  Control and diagonal qubits are
-// split by the planner into register slots (known per stage) and global bit
-// positions read from the thread's base index.
-struct GateDev {
-  int32_t kind;
-  int32_t a, b;          // register-qubit slots (U1: a; U2: a low, b high)
-  int32_t creg;          // controls held in registers (slot mask)
-  uint64_t cglob;        // controls outside the registers (global mask)
-  int32_t nu;            // DIAGTAB: number of non-register qubits
-  int32_t ubits[kMaxTabBits];
-  int32_t tab;           // DIAGTAB: offset (entries) of the 2^nu x 16 table
-  // CZSET: a CZ between two non-register qubits adds popc(G & (G >> s) & M)
-  // terms evaluated on the lane's base index; a CZ between register slots
-  // flips the slot patterns in czreg; a CZ between slot t and a non-register
-  // qubit q puts q in czcm[t] (the lane flips pattern j when parity(gl &
-  // czcm[t]) and bit t of j are both 1).
-  int32_t ncz;
-  int32_t czreg;
-  int32_t czs[kMaxCzShifts];
-  uint64_t czm[kMaxCzShifts];
-  uint64_t czcm[kRegQubits];
-  double m[32];
-};
-
-// Per-stage addressing, precomputed on the host so that a stage costs the
-// device one descriptor load: thread-index bit i selects local position
-// p_i (the positions not held in registers, ascending).
-struct StageDev {
-  int32_t q[kRegQubits];             // local bit positions held in registers
-  int32_t g0, g1;                    // gate range [g0, g1) in the gate array
-  int32_t ntb;                       // thread-index bits (k - kRegQubits)
-  int32_t pad;
-  uint32_t sw_tid[kMaxTileQubits - kRegQubits];  // swizzled LDS index of bit p_i
-  uint32_t sw_reg[kRegQubits];       // swizzled LDS index of slot bit q_j
-  uint64_t g_tid[kMaxTileQubits - kRegQubits];   // global index bit of p_i
-};
-
-struct PassDev {
-  int32_t k;              // tile qubits
-  int32_t n_nonlocal;
-  int32_t s0, s1;         // stage range
-  uint64_t nl_mask;       // global bits outside the tile (tile id deposits here)
-  uint64_t tid_mask;      // global bits of local positions 0..k-R-1 (thread index)
-  uint64_t hb[kRegQubits];  // global bits of local positions k-R..k-1
-  int32_t map[kMaxTileQubits];       // local bit -> global bit
-};
 
 struct Program {
   std::vector<PassDev> passes;
@@ -141,12 +74,28 @@ inline uint32_t swz_host(uint32_t l, int wb) {
 
 }  // namespace smoq
 
+namespace smoq {
+// pass kernels compiled at run time for one circuit structure (jit.hip)
+struct JitModule {
+  hipModule_t mod = nullptr;
+  std::vector<hipFunction_t> fn;  // one per pass
+  ~JitModule();
+};
+// automatic mode compiles for states of at least this many qubits and
+// programs of at least this many stage gates
+constexpr int kJitMinQubits = 16;
+constexpr int kJitMinGates = 16;
+}  // namespace smoq
+
 // ------------------------------------------------------------ handles --
 struct smoq_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
   int tile_qubits = 10;
   int low_qubits = 4;
+  // run-time compiled pass kernels: 0 off, 1 automatic, 2 always
+  int jit_mode = 1;
+  std::unordered_map<std::string, std::shared_ptr<smoq::JitModule>> jit_cache;
   // grow-only device scratch, reused across calls on this context
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -198,7 +147,13 @@ struct ProgLayout {
 };
 ProgLayout program_layout(const Program& prog, int dtype);
 int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayout& L, void* d);
-int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf);
+// jit: the program's compiled pass kernels, or null for the precompiled one
+int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf,
+                   const JitModule* jit);
+// run-time compiled pass kernels (jit.hip)
+std::string jit_source(const Program& prog, int dtype);
+bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits);
+int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out);
 // lowers a tape (angles resolved from theta) to internal gates
 int lower_tape(const smoq_op* ops, int n_ops, const double* theta, int n_theta, int n_qubits,
                std::vector<IGate>& out);

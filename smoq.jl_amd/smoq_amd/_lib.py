@@ -108,6 +108,7 @@ SIGNATURES = {
     "smoq_norm2": [_P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
+    "smoq_set_jit": [_P, _i],
 }
 _RET = {"smoq_last_error": ctypes.c_char_p, "smoq_version": _i}
 

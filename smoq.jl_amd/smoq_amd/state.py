@@ -28,6 +28,10 @@ class Context:
     def set_fusion(self, tile_qubits: int = 0, low_qubits: int = 0) -> None:
         call("smoq_set_fusion", self.handle, tile_qubits, low_qubits)
 
+    def set_jit(self, mode: int) -> None:
+        """Run-time compiled fused passes: 0 never, 1 automatic, 2 always."""
+        call("smoq_set_jit", self.handle, int(mode))
+
     def pass_timing(self, enable: bool):
         """Returns (mean fused-pass kernel ms, launches) since the last call."""
         ms = ctypes.c_double()
