@@ -100,7 +100,9 @@ int smoq_fini(smoq_ctx* ctx);
 int smoq_sync(smoq_ctx* ctx);
 /* Fused-pass geometry: tile_qubits (amplitudes per LDS tile = 2^tile_qubits,
  * 6..12) and low_qubits (contiguous low bits kept in every tile, 3..6).
- * Chosen by the autotuner; 0 keeps the current value. */
+ * 0 keeps the current value; tile_qubits < 0 returns to the automatic
+ * choice (the default): 2^10 tiles for the precompiled pass kernel, the
+ * largest tile that leaves >= 2048 workgroups for compiled passes. */
 int smoq_set_fusion(smoq_ctx* ctx, int tile_qubits, int low_qubits);
 /* Fused passes compiled at run time for one circuit structure (hiprtc,
  * cached per context by structure; gate angles stay run-time data):

@@ -266,7 +266,7 @@ int forward(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta, i
     std::vector<IGate> gs;
     if ((rc = lower_tape(ops, n_ops, theta, n_params, s->n, gs))) return rc;
     Program prog;
-    plan_program(gs, s->n, s->ctx->tile_qubits, s->ctx->low_qubits,
+    plan_program(gs, s->n, pass_tile(s->ctx, s->n, s->batch, gs.size()), s->ctx->low_qubits,
                  s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog);
     return run_program(s, prog);
   }

@@ -20,7 +20,7 @@ namespace smoq {
 // its stage and gate lists from the program at run time; jit.hip generates
 // the same kernel with both lists unrolled for one circuit structure.
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kMaxPassThreads)
     k_fused_pass(VT<T>* __restrict__ psi, int n, const PassDev* __restrict__ pass,
                  const StageDev* __restrict__ stages, const GateDev* __restrict__ gates,
                  const double2* __restrict__ tabs) {
@@ -87,9 +87,9 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
     const PassDev& pd = prog.passes[p];
     const uint64_t tiles = (uint64_t)s->batch << pd.n_nonlocal;
     const int nt = 1 << (pd.k - kRegQubits);
-    if (nt > 256 || pd.k > kMaxTileQubits)
-      return set_error(SMOQ_EINVAL, "fused pass: tile of 2^%d needs %d threads (max 256)", pd.k,
-                       nt);
+    if (nt > kMaxPassThreads || pd.k > kMaxTileQubits)
+      return set_error(SMOQ_EINVAL, "fused pass: tile of 2^%d needs %d threads (max %d)", pd.k,
+                       nt, kMaxPassThreads);
     const size_t shm = sizeof(VT<T>) << pd.k;
     if (ctx->timing && (rc = time_mark(ctx))) return rc;
     if (jit) {

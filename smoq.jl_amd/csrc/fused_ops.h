@@ -124,14 +124,28 @@ __device__ __forceinline__ void reg_u2(VT<T> (&x)[kNR], const GateDev& g, bool g
 }
 
 // merged diagonal gates: x[j] *= table[row][j], row from non-register bits
-template <typename T>
+// (NU >= 0: the number of those bits, known when the kernel is compiled)
+template <typename T, int NU = -1>
 __device__ __forceinline__ void reg_diagtab(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl,
                                             const VT<T>* __restrict__ tabs) {
+  const int nu = NU >= 0 ? NU : g.nu;
   int row = 0;
-  for (int t = 0; t < g.nu; ++t) row |= (int)((gl >> g.ubits[t]) & 1ull) << t;
+  for (int t = 0; t < nu; ++t) row |= (int)((gl >> g.ubits[t]) & 1ull) << t;
   const VT<T>* p = tabs + g.tab + row * kNR;
 #pragma unroll
   for (int j = 0; j < kNR; ++j) x[j] = cmul(p[j], x[j]);
+}
+
+// negates x[j] for every register pattern j set in neg: the sign bit of
+// both components flips by an integer XOR of the high word
+template <typename T>
+__device__ __forceinline__ void czset_apply(VT<T> (&x)[kNR], uint32_t neg) {
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) {
+    const int s = (int)((neg >> j) & 1u) << 31;
+    x[j].x = __hiloint2double(__double2hiint(x[j].x) ^ s, __double2loint(x[j].x));
+    x[j].y = __hiloint2double(__double2hiint(x[j].y) ^ s, __double2loint(x[j].y));
+  }
 }
 
 // merged CZs: the lane builds a 16-bit mask of the slot patterns to negate
@@ -145,12 +159,7 @@ __device__ __forceinline__ void reg_czset(VT<T> (&x)[kNR], const GateDev& g, uin
 #pragma unroll
   for (int t = 0; t < kRegQubits; ++t)
     neg ^= (__popcll(gl & g.czcm[t]) & 1) ? slot_patterns(t) : 0u;
-#pragma unroll
-  for (int j = 0; j < kNR; ++j) {
-    const bool f = (neg >> j) & 1u;
-    x[j].x = f ? -x[j].x : x[j].x;
-    x[j].y = f ? -x[j].y : x[j].y;
-  }
+  czset_apply<T>(x, neg);
 }
 
 // diagonal on register qubits only: uniform phase per slot pattern
@@ -160,14 +169,16 @@ __device__ __forceinline__ void reg_diagreg(VT<T> (&x)[kNR], const GateDev& g) {
   for (int j = 0; j < kNR; ++j) x[j] = cmul(ld_c<T>(g.m, j), x[j]);
 }
 
-// diagonal on non-register qubits only: one phase per lane
-template <typename T>
+// diagonal on non-register qubits only: one phase per lane (NU as above)
+template <typename T, int NU = -1>
 __device__ __forceinline__ void reg_diaglane(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl) {
+  const int nu = NU >= 0 ? NU : g.nu;
+  constexpr int kRows = NU >= 0 ? (1 << NU) : (1 << kMaxTabBits);
   int row = 0;
-  for (int t = 0; t < g.nu; ++t) row |= (int)((gl >> g.ubits[t]) & 1ull) << t;
+  for (int t = 0; t < nu; ++t) row |= (int)((gl >> g.ubits[t]) & 1ull) << t;
   VT<T> ph = ld_c<T>(g.m, 0);
 #pragma unroll
-  for (int e = 1; e < (1 << kMaxTabBits); ++e) ph = csel(row == e, ld_c<T>(g.m, e), ph);
+  for (int e = 1; e < kRows; ++e) ph = csel(row == e, ld_c<T>(g.m, e), ph);
 #pragma unroll
   for (int j = 0; j < kNR; ++j) x[j] = cmul(ph, x[j]);
 }
@@ -268,20 +279,23 @@ struct StageFrame {
   uint32_t sr[kNR];  // swizzled LDS offsets of the register patterns
 };
 
-template <typename T>
+// NTB >= 0: the stage's thread-index bits, known when the kernel is compiled
+template <typename T, int NTB = -1>
 __device__ __forceinline__ void stage_load(const VT<T>* lds, const StageDev& st, uint64_t gbase,
                                            VT<double> (&x)[kNR], StageFrame& sf) {
   constexpr int R = kRegQubits;
-  const int tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x;
+  const int ntb = NTB >= 0 ? NTB : st.ntb;
   // lane part of the swizzled LDS index and of the global index, from the
-  // host-precomputed images of each thread-index bit
+  // host-precomputed images of each thread-index bit (masks, no branches)
   sf.sl = 0;
   sf.gl = gbase;
 #pragma unroll
   for (int i = 0; i < kMaxTileQubits - R; ++i) {
-    if (i < st.ntb && ((tid >> i) & 1)) {
-      sf.sl ^= st.sw_tid[i];
-      sf.gl |= st.g_tid[i];
+    if (i < ntb) {
+      const uint32_t b = (tid >> i) & 1u;
+      sf.sl ^= st.sw_tid[i] & (0u - b);
+      sf.gl |= st.g_tid[i] & (0ull - (uint64_t)b);
     }
   }
 #pragma unroll
@@ -293,6 +307,22 @@ __device__ __forceinline__ void stage_load(const VT<T>* lds, const StageDev& st,
   }
 #pragma unroll
   for (int i = 0; i < kNR; ++i) x[i] = to_d2(lds[sf.sl ^ sf.sr[i]]);
+}
+
+// The same with the stage's addressing known when the kernel is compiled:
+// l_lane is the thread's local index (register positions zero), gl its
+// global index, sr the swizzled offsets of the register patterns.
+template <typename T>
+__device__ __forceinline__ void stage_load_c(const VT<T>* lds, uint32_t l_lane, uint64_t gl,
+                                             const uint32_t (&sr)[kNR], VT<double> (&x)[kNR],
+                                             StageFrame& sf) {
+  sf.sl = swz<CT<T>::kRowBits>(l_lane);
+  sf.gl = gl;
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) {
+    sf.sr[i] = sr[i];
+    x[i] = to_d2(lds[sf.sl ^ sr[i]]);
+  }
 }
 
 // registers -> LDS (rounded to T), then a barrier

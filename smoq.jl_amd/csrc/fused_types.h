@@ -19,6 +19,8 @@ namespace smoq {
 #endif
 constexpr int kRegQubits = SMOQ_REG_QUBITS;  // amplitudes per thread = 2^kRegQubits
 constexpr int kMaxTileQubits = 12;     // LDS tile of up to 4096 amplitudes
+// a tile of 2^k amplitudes runs 2^(k - kRegQubits) threads
+constexpr int kMaxPassThreads = 1 << (kMaxTileQubits - kRegQubits);
 constexpr int kMaxQubits = 40;
 
 // Gate kinds inside a register stage.

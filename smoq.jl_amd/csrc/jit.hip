@@ -8,9 +8,9 @@
 // pass that calls the gate helpers with the gate kind and register slots as
 // template arguments, compiled with hiprtc for gfx950 and cached per context
 // by its source text.  The source holds only the structure (kinds, slots,
-// control presence, stage and gate indices); passes, stages, gate matrices
-// and phase tables are still read from the device program, so plans with
-// new angles reuse the compiled module.
+// control presence, table widths, stage and gate indices); passes, stages,
+// gate matrices and phase tables are still read from the device program, so
+// plans with new angles reuse the compiled module.
 #include <hip/hiprtc.h>
 
 #include <cstdlib>
@@ -40,13 +40,34 @@ static const char* u1_kind_name(int kind) {
   }
 }
 
+// Bit-field expression for a linear map of the thread index: thread bit i
+// goes to bit dst[i]; runs of consecutive bits move as one field.
+static std::string bit_map_expr(const std::vector<int>& dst, bool wide) {
+  std::ostringstream o;
+  bool first = true;
+  size_t i = 0;
+  while (i < dst.size()) {
+    size_t j = i + 1;
+    while (j < dst.size() && dst[j] == dst[j - 1] + 1) ++j;
+    const int len = (int)(j - i);
+    if (!first) o << " | ";
+    first = false;
+    o << "(" << (wide ? "(uint64_t)" : "") << "((tid >> " << i << ") & 0x" << std::hex
+      << ((1u << len) - 1u) << std::dec << "u) << " << dst[i] << ")";
+    i = j;
+  }
+  if (first) o << (wide ? "(uint64_t)0" : "0u");
+  return o.str();
+}
+
 std::string jit_source(const Program& prog, int dtype) {
   std::ostringstream o;
   const char* T = dtype == SMOQ_C128 ? "double" : "float";
   o << "#include \"fused_ops.h\"\nusing namespace smoq;\n";
   for (size_t p = 0; p < prog.passes.size(); ++p) {
     const PassDev& pd = prog.passes[p];
-    o << "extern \"C\" __global__ void __launch_bounds__(256) smoq_pass_" << p << "(VT<" << T
+    o << "extern \"C\" __global__ void __launch_bounds__(" << (1 << (pd.k - kRegQubits))
+      << ") smoq_pass_" << p << "(VT<" << T
       << ">* __restrict__ psi, int n, const PassDev* __restrict__ pass,"
          " const StageDev* __restrict__ stages, const GateDev* __restrict__ gates,"
          " const double2* __restrict__ tabs) {\n"
@@ -57,8 +78,27 @@ std::string jit_source(const Program& prog, int dtype) {
       << "  pass_load<T>(lds, psi, n, pass, f);\n";
     for (int s = pd.s0; s < pd.s1; ++s) {
       const StageDev& st = prog.stages[s];
+      // thread bit i -> the i-th local position outside the register slots
+      // (ascending) -> its global bit
+      std::vector<int> lpos, gpos;
+      for (int b = 0; b < pd.k; ++b) {
+        bool reg = false;
+        for (int r = 0; r < kRegQubits; ++r) reg |= st.q[r] == b;
+        if (!reg) {
+          lpos.push_back(b);
+          gpos.push_back(pd.map[b]);
+        }
+      }
       o << "  {\n    VT<double> x[kNR];\n    StageFrame sf;\n"
-        << "    stage_load<T>(lds, stages[" << s << "], f.gbase, x, sf);\n";
+        << "    const uint32_t tid = threadIdx.x;\n    const uint32_t sr[kNR] = {";
+      for (int i = 0; i < (1 << kRegQubits); ++i) {
+        uint32_t r = 0;
+        for (int j = 0; j < kRegQubits; ++j)
+          if ((i >> j) & 1) r ^= st.sw_reg[j];
+        o << (i ? ", " : "") << r << "u";
+      }
+      o << "};\n    stage_load_c<T>(lds, " << bit_map_expr(lpos, false) << ", f.gbase | "
+        << bit_map_expr(gpos, true) << ", sr, x, sf);\n";
       for (int g = st.g0; g < st.g1; ++g) {
         const GateDev& G = prog.gates[g];
         const std::string gd = "gates[" + std::to_string(g) + "]";
@@ -77,10 +117,25 @@ std::string jit_source(const Program& prog, int dtype) {
             o << "    reg_u2<double, " << G.a << ", " << G.b << ">(x, " << gd << ", (sf.gl & " << gd
               << ".cglob) == " << gd << ".cglob);\n";
             break;
-          case GK_DIAGTAB: o << "    reg_diagtab<double>(x, " << gd << ", sf.gl, tabs);\n"; break;
-          case GK_CZSET: o << "    reg_czset<double>(x, " << gd << ", sf.gl);\n"; break;
+          case GK_DIAGTAB:
+            o << "    reg_diagtab<double, " << G.nu << ">(x, " << gd << ", sf.gl, tabs);\n";
+            break;
+          case GK_CZSET: {
+            // the CZ pattern is structure: shifts and masks become immediates
+            o << "    {\n      int par = 0;\n";
+            for (int i = 0; i < G.ncz; ++i)
+              o << "      par ^= __popcll(sf.gl & (sf.gl >> " << G.czs[i] << ") & 0x" << std::hex
+                << G.czm[i] << std::dec << "ull);\n";
+            o << "      uint32_t neg = ((par & 1) ? kAllPatterns : 0u) ^ " << G.czreg << "u;\n";
+            for (int t = 0; t < kRegQubits; ++t)
+              if (G.czcm[t])
+                o << "      neg ^= (__popcll(sf.gl & 0x" << std::hex << G.czcm[t] << std::dec
+                  << "ull) & 1) ? slot_patterns(" << t << ") : 0u;\n";
+            o << "      czset_apply<double>(x, neg);\n    }\n";
+            break;
+          }
           case GK_DIAGREG: o << "    reg_diagreg<double>(x, " << gd << ");\n"; break;
-          default: o << "    reg_diaglane<double>(x, " << gd << ", sf.gl);\n"; break;
+          default: o << "    reg_diaglane<double, " << G.nu << ">(x, " << gd << ", sf.gl);\n"; break;
         }
       }
       o << "    stage_store<T>(lds, sf, x);\n  }\n";
@@ -94,7 +149,7 @@ JitModule::~JitModule() {
   if (mod) (void)hipModuleUnload(mod);
 }
 
-static int compile(const std::string& src, size_t n_passes, std::shared_ptr<JitModule>& out) {
+static int compile(const std::string& src, const Program& pg, std::shared_ptr<JitModule>& out) {
   const char* headers[3] = {kSrcDeviceCommon, kSrcFusedTypes, kSrcFusedOps};
   const char* names[3] = {"device_common.h", "fused_types.h", "fused_ops.h"};
   hiprtcProgram prog;
@@ -131,13 +186,25 @@ static int compile(const std::string& src, size_t n_passes, std::shared_ptr<JitM
   if (code.empty()) return set_error(SMOQ_EDEVICE, "pass kernel compilation produced no code");
   auto m = std::make_shared<JitModule>();
   SMOQ_HIP(hipModuleLoadData(&m->mod, code.data()));
-  m->fn.resize(n_passes);
-  for (size_t p = 0; p < n_passes; ++p) {
+  m->fn.resize(pg.passes.size());
+  for (size_t p = 0; p < pg.passes.size(); ++p) {
     const std::string name = "smoq_pass_" + std::to_string(p);
     SMOQ_HIP(hipModuleGetFunction(&m->fn[p], m->mod, name.c_str()));
   }
   out = m;
   return SMOQ_OK;
+}
+
+int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates) {
+  if (ctx->tile_qubits > 0) return ctx->tile_qubits;
+  const bool jit = ctx->jit_mode >= 2 || (ctx->jit_mode == 1 && n_qubits >= kJitMinQubits &&
+                                          n_gates >= (size_t)kJitMinGates);
+  // the precompiled kernel is fastest on 2^10 tiles; compiled passes go to
+  // the largest tile that still leaves >= 2048 workgroups (fewer passes)
+  if (!jit) return 10;
+  for (int k = kMaxTileQubits; k > 10; --k)
+    if (((uint64_t)(batch > 0 ? batch : 1) << (n_qubits > k ? n_qubits - k : 0)) >= 2048) return k;
+  return 10;
 }
 
 bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits) {
@@ -157,7 +224,7 @@ int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitMo
     return SMOQ_OK;
   }
   std::shared_ptr<JitModule> m;
-  const int rc = compile(src, prog.passes.size(), m);
+  const int rc = compile(src, prog, m);
   if (rc) return rc;
   ctx->jit_cache.emplace(std::move(src), m);
   out = m;

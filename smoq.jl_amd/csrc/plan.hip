@@ -104,7 +104,7 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
     *out = p;
     return SMOQ_OK;
   }
-  plan_program(gs, n_qubits, ctx->tile_qubits, ctx->low_qubits,
+  plan_program(gs, n_qubits, pass_tile(ctx, n_qubits, 1, gs.size()), ctx->low_qubits,
                dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog);
   p->layout = program_layout(p->prog, dtype);
   if (jit_wanted(ctx, p->prog, n_qubits)) {

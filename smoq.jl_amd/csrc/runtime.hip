@@ -351,14 +351,14 @@ int smoq_sync(smoq_ctx* ctx) {
 
 int smoq_set_fusion(smoq_ctx* ctx, int tile_qubits, int low_qubits) {
   if (!ctx) return set_error(SMOQ_EINVAL, "smoq_set_fusion: null context");
-  // a tile of 2^k amplitudes runs 2^(k - kRegQubits) threads; the kernel is
-  // built for at most 256
-  const int kmax = std::min(kMaxTileQubits, kRegQubits + 8);
-  if (tile_qubits != 0 && (tile_qubits < 6 || tile_qubits > kmax))
+  // a tile of 2^k amplitudes runs 2^(k - kRegQubits) threads
+  const int kmax = kMaxTileQubits;
+  if (tile_qubits > 0 && (tile_qubits < 6 || tile_qubits > kmax))
     return set_error(SMOQ_EINVAL, "tile_qubits %d outside [6, %d]", tile_qubits, kmax);
   if (low_qubits != 0 && (low_qubits < 3 || low_qubits > 6))
     return set_error(SMOQ_EINVAL, "low_qubits %d outside [3, 6]", low_qubits);
-  if (tile_qubits) ctx->tile_qubits = tile_qubits;
+  if (tile_qubits > 0) ctx->tile_qubits = tile_qubits;
+  if (tile_qubits < 0) ctx->tile_qubits = 0;  // automatic again
   if (low_qubits) ctx->low_qubits = low_qubits;
   return SMOQ_OK;
 }
@@ -526,7 +526,7 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
   std::vector<IGate> gs;
   if ((rc = lower_tape(ops, n_ops, theta, n_theta, s->n, gs))) return rc;
   Program prog;
-  plan_program(gs, s->n, s->ctx->tile_qubits, s->ctx->low_qubits,
+  plan_program(gs, s->n, pass_tile(s->ctx, s->n, s->batch, gs.size()), s->ctx->low_qubits,
                s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog);
   if (n_passes) *n_passes = (int)prog.passes.size();
   if (gs.empty()) return SMOQ_OK;

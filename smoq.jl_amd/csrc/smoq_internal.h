@@ -91,7 +91,7 @@ constexpr int kJitMinGates = 16;
 struct smoq_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
-  int tile_qubits = 10;
+  int tile_qubits = 0;  // 0: chosen per gate list (pass_tile)
   int low_qubits = 4;
   // run-time compiled pass kernels: 0 off, 1 automatic, 2 always
   int jit_mode = 1;
@@ -153,6 +153,8 @@ int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, cons
 // run-time compiled pass kernels (jit.hip)
 std::string jit_source(const Program& prog, int dtype);
 bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits);
+// tile qubits of the fused passes for a gate list (automatic unless set)
+int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates);
 int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out);
 // lowers a tape (angles resolved from theta) to internal gates
 int lower_tape(const smoq_op* ops, int n_ops, const double* theta, int n_theta, int n_qubits,

@@ -173,10 +173,36 @@ def test_fused_tape_parity(ctx, N, tile, low):
         assert 1 <= npass <= len(tape)
         assert rel(s.to_numpy(), ref) < TOL128
     finally:
-        ctx.set_fusion(10, 4)
+        ctx.set_fusion(-1, 4)
 
 
-@pytest.mark.parametrize("N,tile,low", [(10, 6, 3), (14, 8, 4), (15, 10, 5), (17, 11, 5)])
+@pytest.mark.parametrize("jit", [0, 2])
+@pytest.mark.parametrize("N,tile", [(9, 6), (13, 10), (14, 12), (16, 11)])
+def test_fused_kernels_precompiled_and_run_time(ctx, N, tile, jit):
+    """The precompiled pass kernel (jit 0) and the pass kernels compiled for
+    the circuit structure (jit 2) against the oracle on the same tape:
+    rotations, CZ bricks, controlled and 2-qubit gates, diagonals."""
+    from smoq_amd import GPUState
+
+    ctx.set_fusion(tile, 4)
+    ctx.set_jit(jit)
+    try:
+        tape = random_tape(N, 150, seed=7 * N + tile + jit)
+        psi = rand_state(N, N + 1)
+        ref = oracle_run_tape(tape, psi.copy())
+        s = GPUState.from_numpy(psi)
+        tape.apply(s)
+        assert rel(s.to_numpy(), ref) < TOL128
+        s64 = GPUState.from_numpy(psi.astype(np.complex64))
+        tape.apply(s64)
+        assert rel(s64.to_numpy(), ref) < 1e-5
+    finally:
+        ctx.set_fusion(-1, 4)
+        ctx.set_jit(1)
+
+
+@pytest.mark.parametrize("N,tile,low", [(10, 6, 3), (14, 8, 4), (15, 10, 5), (17, 11, 5),
+                                        (16, 12, 5), (18, 12, 4)])
 def test_fused_brick_circuit_parity(ctx, N, tile, low):
     # rotation layers + CZ bricks: exercises merged CZ signs between register
     # slots, between a slot and a lane qubit and between lane qubits, and
@@ -196,7 +222,7 @@ def test_fused_brick_circuit_parity(ctx, N, tile, low):
         tape.apply(s)
         assert rel(s.to_numpy(), ref) < TOL128
     finally:
-        ctx.set_fusion(10, 4)
+        ctx.set_fusion(-1, 4)
 
 
 def test_fused_batch_and_c64(ctx):
