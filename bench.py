@@ -93,8 +93,8 @@ def traffic_probe(N=28):
 
 def measured_traffic_ratio():
     """HBM traffic / algorithmic bytes of the fused pass, from the committed
-    counter summary (profiles/k_fused_pass_traffic.json), or None."""
-    p = os.path.join(ROOT, "profiles", "k_fused_pass_traffic.json")
+    counter summary (profiles/fused_passes_traffic.json), or None."""
+    p = os.path.join(ROOT, "profiles", "fused_passes_traffic.json")
     try:
         with open(p) as f:
             return float(json.load(f)["traffic_over_algorithmic"])
@@ -330,7 +330,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": (None if measured_traffic_ratio() is None
                                  else measured_traffic_ratio() * 2 * S),
-                     "kernel": "k_fused_pass<double>", "bytes_per_launch": 2 * S,
+                     "kernel": f"smoq_pass_0..{n_passes - 1} (fused passes compiled for the "
+                               "circuit structure)", "bytes_per_launch": 2 * S,
                      "mean_launch_ms": pass_ms},
         "hbm_GBps_whole_step": n_passes * 2.0 * S / (ms_per_step * 1e-3) / 1e9,
     }
