@@ -219,6 +219,32 @@ typedef struct smoq_mcwf_group {
 int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, int n_groups, double gamma_dt,
                      int n_steps, int obs_every);
 
+/* MCWF with any coherent step: the dissipative half of one step, for every
+ * batch item of s (one trajectory each), after the coherent step has been
+ * applied (e.g. a compiled plan).  Jump operators on every site:
+ *  SMOQ_JUMP_SIGMA_MINUS — lindblad_mcwf_batched_step!
+ *    (…MonteCarloWaveFunction.jl:352-386): ψ_i *= decay_pow[popcount(i)]
+ *    (decay_pow[m] = sqrt(1-γdt)^m), site = the first k with
+ *    r < Σ_{j<=k} γdt pop_j/||ψ||² (0 when none), σ⁻_k;
+ *  SMOQ_JUMP_SIGMA_Z — lindblad_mcwf_step_trotter! (…:279-306): ψ *=
+ *    decay_pow[0] (= sqrt(1-nγdt)), site = min(n-1, floor(u2·n)) (the
+ *    reference draws rand(1:N)), σ_z_k flips the sign where bit k is set.
+ * Both: dp = 1 - ||ψ||², jump iff r < dp and dp > 0, then ψ /= ||ψ||.
+ * uniforms[2b] = r and uniforms[2b+1] = u2 for item b; jumps[b] = the
+ * 0-based site or -1.  A zero or non-finite norm returns SMOQ_ENUMERIC. */
+enum smoq_jump_kind { SMOQ_JUMP_SIGMA_MINUS = 0, SMOQ_JUMP_SIGMA_Z = 1 };
+int smoq_mcwf_jump(smoq_state* s, int kind, double gamma_dt, const double* decay_pow,
+                   const double* uniforms, int32_t* jumps);
+/* n_steps of [the plan, then smoq_mcwf_jump] on every trajectory of s;
+ * after every obs_every-th step <P_t> per trajectory is recorded as in
+ * smoq_evolve_observe: out[r][item][t].  uniforms[item][step][2],
+ * jumps[item][step].  This is evolve_mcwf_batched_trotter!
+ * (…:412-423) with apply_trotter! = the plan. */
+int smoq_mcwf_trotter_evolve(smoq_plan* p, smoq_state* s, int kind, double gamma_dt,
+                             const double* decay_pow, int n_steps, const double* uniforms,
+                             int32_t* jumps, int obs_every, int n_terms, const uint64_t* xmask,
+                             const uint64_t* zmask, double* out);
+
 /* ---- parametrised circuits: adjoint gradient -----------------------------
  * E_b(θ) = <0|U(θ_b)† H U(θ_b)|0> for every batch item b of `s`, with
  * H = sum_t coeff_t P_t (Pauli masks as in smoq_expect_pauli) and the tape's

@@ -68,6 +68,7 @@ def lib():
             "or_sample_sequential": [_P, _i, _P, _i64, _P],
             "or_hpsi": [_P, _P, _i, _i, _P, _P, _P, _P],
             "or_mcwf_decay_jump": [_P, _i, _d, _P, _d, _P],
+            "or_mcwf_dephasing": [_P, _i, _d, _d, _d, _P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -78,6 +79,7 @@ def lib():
             getattr(L, name).restype = _d
         L.or_searchsortedfirst.restype = _i64
         L.or_mcwf_decay_jump.restype = _i
+        L.or_mcwf_dephasing.restype = _i
         _lib = L
     return _lib
 
@@ -238,6 +240,25 @@ def mcwf_decay_jump(psi, N, gamma_dt, decay_pow, r):
     d = np.ascontiguousarray(decay_pow, np.float64)
     site = lib().or_mcwf_decay_jump(_p(_c128(psi)), N, gamma_dt, _p(d), float(r), _p(dp))
     return site, float(dp[0])
+
+
+def mcwf_dephasing(psi, N, decay, r, u2):
+    dp = np.zeros(1)
+    site = lib().or_mcwf_dephasing(_p(_c128(psi)), N, float(decay), float(r), float(u2), _p(dp))
+    return site, float(dp[0])
+
+
+def mcwf_dephasing_trajectory(psi, gates, N, gamma_dt, r_seq, u2_seq):
+    """lindblad_mcwf_step_trotter! (…MonteCarloWaveFunction.jl:279-306)
+    repeated: the gate list in order, then or_mcwf_dephasing with decay
+    sqrt(1 - N γdt) and the supplied uniforms.  Returns 0-based sites or -1."""
+    decay = np.sqrt(1.0 - N * gamma_dt)
+    jumps = []
+    for r, u2 in zip(r_seq, u2_seq):
+        trotter_step(psi, gates, N)
+        site, _ = mcwf_dephasing(psi, N, decay, r, u2)
+        jumps.append(site - 1)
+    return np.array(jumps, np.int32)
 
 
 def mcwf_trajectory(psi, gates, N, gamma_dt, decay_pow, r_seq, obs_every=0):

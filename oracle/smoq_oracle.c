@@ -578,3 +578,33 @@ int or_mcwf_decay_jump(double* p, int N, double gamma_dt, const double* decay_po
   for (int64_t i = 0; i < dim; ++i) psi[i] = mk(psi[i].re / nn, psi[i].im / nn);
   return site;
 }
+
+/* One step of lindblad_mcwf_step_trotter! / lindblad_mcwf_step!
+ * (…MonteCarloWaveFunction.jl:222-306) after the coherent step, σ_z
+ * dephasing: ψ *= decay (= sqrt(1 - Nγdt), uniform), dp = 1 - ||ψ||²,
+ * jump if r < dp && dp > 0 on site k = floor(u2·N) (clamped to N-1; the
+ * reference draws rand(1:N)), σ_z_k flips the sign where bit k is set
+ * (apply_sigma_z!, :170-180), renormalise.  Returns the 1-based site or 0. */
+int or_mcwf_dephasing(double* p, int N, double decay, double r, double u2, double* dp_out) {
+  cx* psi = PSI(p);
+  int64_t dim = (int64_t)1 << N;
+  for (int64_t i = 0; i < dim; ++i) psi[i] = rmul(decay, psi[i]);
+  double nsq = 0.0;
+  for (int64_t i = 0; i < dim; ++i) nsq += abs2(psi[i]);
+  double dp = 1.0 - nsq;
+  if (dp_out) *dp_out = dp;
+  int site = 0;
+  if (r < dp && dp > 0) {
+    int k = (int)(u2 * N);
+    if (k > N - 1) k = N - 1;
+    int64_t mask = (int64_t)1 << k;
+    for (int64_t i = 0; i < dim; ++i)
+      if (i & mask) psi[i] = mk(-psi[i].re, -psi[i].im);
+    site = k + 1;
+  }
+  double nn = 0.0;
+  for (int64_t i = 0; i < dim; ++i) nn += abs2(psi[i]);
+  nn = sqrt(nn);
+  for (int64_t i = 0; i < dim; ++i) psi[i] = mk(psi[i].re / nn, psi[i].im / nn);
+  return site;
+}

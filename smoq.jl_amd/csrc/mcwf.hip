@@ -348,6 +348,138 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
   }
 }
 
+// ------------------------------------------------------------------------
+// The dissipative half of one MCWF step for any coherent step (applied
+// before, e.g. by a compiled plan): one workgroup per trajectory (batch
+// item).  Decay (σ⁻: sqrt(1-γdt)^popcount, σ_z: the uniform sqrt(1-nγdt)),
+// norm, decision, jump site, jump and renormalisation in three sweeps over
+// the trajectory's amplitudes.
+constexpr int kJumpMaxQ = 30;
+
+template <typename T>
+__global__ void __launch_bounds__(kMcBlock)
+    k_mcwf_jump(VT_alias<T>* __restrict__ psi, int n, int kind, double gamma_dt,
+                const double* __restrict__ decay_pow, const double* __restrict__ uni,
+                int uni_stride, int32_t* __restrict__ jumps, int jump_stride,
+                unsigned long long* fault, int n_steps, int step) {
+  using V = VT_alias<T>;
+  __shared__ double red[(kJumpMaxQ + 1) * kMcWaves];
+  __shared__ double sums[kJumpMaxQ + 1];
+  __shared__ int s_site;
+  const uint64_t item = blockIdx.x;
+  const uint64_t D = 1ull << n;
+  V* p = psi + (item << n);
+  const bool sz = kind == SMOQ_JUMP_SIGMA_Z;
+
+  double acc = 0.0;
+  for (uint64_t k = threadIdx.x; k < D; k += kMcBlock) {
+    const double d = decay_pow[sz ? 0 : __popcll(k)];
+    const V v = p[k];
+    const V r = make_c<T>((double)v.x * d, (double)v.y * d);
+    p[k] = r;
+    acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+  }
+  double nv[1] = {acc};
+  mc_block_sum<1>(nv, 1, red, sums);
+  const double norm_sq = sums[0];
+  const double dp = 1.0 - norm_sq;
+  const double r = uni[item * uni_stride];
+  const bool jump = (r < dp) && (dp > 0.0);
+  double new_norm_sq = norm_sq;
+  int site = -1;
+  if (jump) {  // uniform over the block
+    if (sz) {
+      const double u2 = uni[item * uni_stride + 1];
+      site = (int)(u2 * n);
+      if (site > n - 1) site = n - 1;
+    } else {
+      // site populations: index k = tid + 256 j, bits 0..7 from tid, 8.. from j
+      double tot = 0.0, hi[kJumpMaxQ - 8];
+#pragma unroll
+      for (int i = 0; i < kJumpMaxQ - 8; ++i) hi[i] = 0.0;
+      for (uint64_t j = 0; (j << 8) < D; ++j) {
+        const uint64_t k = (j << 8) | threadIdx.x;
+        if (k >= D) break;
+        const V v = p[k];
+        const double q = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        tot += q;
+#pragma unroll
+        for (int i = 0; i < kJumpMaxQ - 8; ++i)
+          if ((j >> i) & 1) hi[i] += q;
+      }
+      double v[kJumpMaxQ];
+#pragma unroll
+      for (int i = 0; i < kJumpMaxQ; ++i)
+        v[i] = i < 8 ? (((threadIdx.x >> i) & 1) ? tot : 0.0) : hi[i < 8 ? 0 : i - 8];
+      mc_block_sum<kJumpMaxQ>(v, n, red, sums);
+      if (threadIdx.x == 0) {
+        double cum = 0.0;
+        int sel = 0;
+        for (int k = 0; k < n; ++k) {
+          cum += gamma_dt * sums[k] / norm_sq;
+          if (r < cum) {
+            sel = k;
+            break;
+          }
+        }
+        s_site = sel;
+      }
+      __syncthreads();
+      site = s_site;
+      new_norm_sq = sums[site];
+    }
+  }
+  const double sc = 1.0 / sqrt(new_norm_sq);
+  if (jump && !sz) {
+    const uint64_t m = 1ull << site;
+    for (uint64_t q = threadIdx.x; q < (D >> 1); q += kMcBlock) {
+      const uint64_t i0 = insert_zero(q, site), i1 = i0 | m;
+      const V v = p[i1];
+      p[i0] = make_c<T>((double)v.x * sc, (double)v.y * sc);
+      p[i1] = make_c<T>(0.0, 0.0);
+    }
+  } else {
+    const uint64_t m = (jump && sz) ? (1ull << site) : 0ull;
+    for (uint64_t k = threadIdx.x; k < D; k += kMcBlock) {
+      const double s = (k & m) ? -sc : sc;
+      const V v = p[k];
+      p[k] = make_c<T>((double)v.x * s, (double)v.y * s);
+    }
+  }
+  if (threadIdx.x == 0) {
+    jumps[item * jump_stride] = jump ? site : -1;
+    if (!(new_norm_sq > 0.0) || !isfinite(sc))
+      atomicMin(fault, (unsigned long long)item * n_steps + step);
+  }
+}
+
+int mcwf_jump_launch(smoq_state* s, int kind, double gamma_dt, const double* d_decay,
+                     const double* d_uni, int uni_stride, int32_t* d_jumps, int jump_stride,
+                     unsigned long long* d_fault, int n_steps, int step) {
+  if (s->dtype == SMOQ_C128)
+    hipLaunchKernelGGL(k_mcwf_jump<double>, dim3((unsigned)s->batch), dim3(kMcBlock), 0,
+                       s->ctx->stream, (double2*)s->d, s->n, kind, gamma_dt, d_decay, d_uni,
+                       uni_stride, d_jumps, jump_stride, d_fault, n_steps, step);
+  else
+    hipLaunchKernelGGL(k_mcwf_jump<float>, dim3((unsigned)s->batch), dim3(kMcBlock), 0,
+                       s->ctx->stream, (float2*)s->d, s->n, kind, gamma_dt, d_decay, d_uni,
+                       uni_stride, d_jumps, jump_stride, d_fault, n_steps, step);
+  SMOQ_CHECK_LAUNCH("k_mcwf_jump");
+  return SMOQ_OK;
+}
+
+int mcwf_jump_check(const smoq_state* s, int kind, double gamma_dt) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (kind != SMOQ_JUMP_SIGMA_MINUS && kind != SMOQ_JUMP_SIGMA_Z)
+    return set_error(SMOQ_EINVAL, "mcwf: unknown jump kind %d", kind);
+  if (s->n > kJumpMaxQ)
+    return set_error(SMOQ_ENOTSUP, "mcwf: %d qubits per trajectory (max %d)", s->n, kJumpMaxQ);
+  if (!(gamma_dt >= 0.0 && gamma_dt <= 1.0))
+    return set_error(SMOQ_EINVAL, "mcwf: gamma_dt %g outside [0, 1]", gamma_dt);
+  return SMOQ_OK;
+}
+
 }  // namespace smoq
 
 using namespace smoq;
@@ -543,4 +675,46 @@ extern "C" int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, in
                           groups[t.group].psi->n));
   }
   return done(SMOQ_OK);
+}
+
+extern "C" int smoq_mcwf_jump(smoq_state* s, int kind, double gamma_dt, const double* decay_pow,
+                              const double* uniforms, int32_t* jumps) {
+  int rc = mcwf_jump_check(s, kind, gamma_dt);
+  if (rc) return rc;
+  if (!decay_pow || !uniforms || !jumps)
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_jump: null array");
+  smoq_ctx* ctx = s->ctx;
+  const size_t nb = (size_t)s->batch;
+  const size_t off_u = ((size_t)(s->n + 1) * sizeof(double) + 255) & ~(size_t)255;
+  const size_t off_j = off_u + ((2 * nb * sizeof(double) + 255) & ~(size_t)255);
+  const size_t off_f = off_j + ((nb * sizeof(int32_t) + 255) & ~(size_t)255);
+  unsigned char* dev = nullptr;
+  SMOQ_HIP(hipMalloc(&dev, off_f + 256));
+  const unsigned long long no_fault = ~0ull;
+  unsigned long long fault = no_fault;
+  hipError_t e = hipMemcpyAsync(dev, decay_pow, (size_t)(s->n + 1) * sizeof(double),
+                                hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dev + off_u, uniforms, 2 * nb * sizeof(double), hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dev + off_f, &no_fault, sizeof(no_fault), hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess)
+    rc = mcwf_jump_launch(s, kind, gamma_dt, (const double*)dev, (const double*)(dev + off_u), 2,
+                          (int32_t*)(dev + off_j), 1, (unsigned long long*)(dev + off_f), 1, 0);
+  if (e == hipSuccess && rc == SMOQ_OK)
+    e = hipMemcpyAsync(jumps, dev + off_j, nb * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       ctx->stream);
+  if (e == hipSuccess && rc == SMOQ_OK)
+    e = hipMemcpyAsync(&fault, dev + off_f, sizeof(fault), hipMemcpyDeviceToHost, ctx->stream);
+  const hipError_t se = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(dev);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_error(e, "smoq_mcwf_jump: copy");
+  if (se != hipSuccess) return hip_error(se, "k_mcwf_jump");
+  if (fault != no_fault)
+    return set_error(SMOQ_ENUMERIC, "smoq_mcwf_jump: zero or non-finite norm in trajectory %llu",
+                     fault);
+  return SMOQ_OK;
 }

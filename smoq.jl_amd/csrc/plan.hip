@@ -174,4 +174,70 @@ int smoq_evolve_observe(smoq_plan* p, smoq_state* s, int n_steps, int obs_every,
   return plan_run(p, s, n_steps - records * obs_every);
 }
 
+int smoq_mcwf_trotter_evolve(smoq_plan* p, smoq_state* s, int kind, double gamma_dt,
+                             const double* decay_pow, int n_steps, const double* uniforms,
+                             int32_t* jumps, int obs_every, int n_terms, const uint64_t* xmask,
+                             const uint64_t* zmask, double* out) {
+  int rc = mcwf_jump_check(s, kind, gamma_dt);
+  if (rc) return rc;
+  if (!p) return set_error(SMOQ_EINVAL, "smoq_mcwf_trotter_evolve: null plan");
+  if (s->n != p->n || s->dtype != p->dtype || s->ctx != p->ctx)
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_trotter_evolve: plan is for %d qubits dtype %d",
+                     p->n, p->dtype);
+  if (n_steps < 0 || obs_every < 0)
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_trotter_evolve: n_steps %d obs_every %d", n_steps,
+                     obs_every);
+  if (n_steps > 0 && (!decay_pow || !uniforms || !jumps))
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_trotter_evolve: null array");
+  const bool record = obs_every > 0 && n_terms > 0;
+  if (record && (!xmask || !zmask || !out))
+    return set_error(SMOQ_EINVAL, "smoq_mcwf_trotter_evolve: null observable arrays");
+  if (n_steps == 0) return SMOQ_OK;
+  smoq_ctx* ctx = s->ctx;
+  const size_t nb = (size_t)s->batch, ns = (size_t)n_steps;
+  const size_t off_u = ((size_t)(s->n + 1) * sizeof(double) + 255) & ~(size_t)255;
+  const size_t off_j = off_u + ((2 * nb * ns * sizeof(double) + 255) & ~(size_t)255);
+  const size_t off_f = off_j + ((nb * ns * sizeof(int32_t) + 255) & ~(size_t)255);
+  unsigned char* dev = nullptr;
+  SMOQ_HIP(hipMalloc(&dev, off_f + 256));
+  const unsigned long long no_fault = ~0ull;
+  unsigned long long fault = no_fault;
+  hipError_t e = hipMemcpyAsync(dev, decay_pow, (size_t)(s->n + 1) * sizeof(double),
+                                hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dev + off_u, uniforms, 2 * nb * ns * sizeof(double),
+                       hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dev + off_f, &no_fault, sizeof(no_fault), hipMemcpyHostToDevice,
+                       ctx->stream);
+  for (int step = 0; e == hipSuccess && rc == SMOQ_OK && step < n_steps; ++step) {
+    if ((rc = plan_run(p, s, 1))) break;
+    if ((rc = mcwf_jump_launch(s, kind, gamma_dt, (const double*)dev,
+                               (const double*)(dev + off_u) + 2 * step, 2 * n_steps,
+                               (int32_t*)(dev + off_j) + step, n_steps,
+                               (unsigned long long*)(dev + off_f), n_steps, step)))
+      break;
+    if (record && (step + 1) % obs_every == 0) {
+      const size_t r = (size_t)(step + 1) / obs_every - 1;
+      rc = reduce_pauli(s, n_terms, xmask, zmask, out + r * nb * n_terms, 0);
+    }
+  }
+  if (e == hipSuccess && rc == SMOQ_OK)
+    e = hipMemcpyAsync(jumps, dev + off_j, nb * ns * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       ctx->stream);
+  if (e == hipSuccess && rc == SMOQ_OK)
+    e = hipMemcpyAsync(&fault, dev + off_f, sizeof(fault), hipMemcpyDeviceToHost, ctx->stream);
+  const hipError_t se = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(dev);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_error(e, "smoq_mcwf_trotter_evolve: copy");
+  if (se != hipSuccess) return hip_error(se, "smoq_mcwf_trotter_evolve");
+  if (fault != no_fault)
+    return set_error(SMOQ_ENUMERIC,
+                     "smoq_mcwf_trotter_evolve: zero or non-finite norm at step %llu of "
+                     "trajectory %llu",
+                     fault % ns, fault / ns);
+  return SMOQ_OK;
+}
+
 }  // extern "C"

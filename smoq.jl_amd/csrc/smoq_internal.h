@@ -161,4 +161,12 @@ int lower_tape(const smoq_op* ops, int n_ops, const double* theta, int n_theta, 
                std::vector<IGate>& out);
 int reduce_pauli(smoq_state* s, int n_terms, const uint64_t* xm, const uint64_t* zm,
                  double* out_host, int ref_mode);
+// the dissipative half of an MCWF step for every batch item (mcwf.hip):
+// uniforms of item b at d_uni[b * uni_stride + 0/1], its jump at
+// d_jumps[b * jump_stride]; a zero or non-finite norm records
+// min(item * n_steps + step) in *d_fault
+int mcwf_jump_check(const smoq_state* s, int kind, double gamma_dt);
+int mcwf_jump_launch(smoq_state* s, int kind, double gamma_dt, const double* d_decay,
+                     const double* d_uni, int uni_stride, int32_t* d_jumps, int jump_stride,
+                     unsigned long long* d_fault, int n_steps, int step);
 }  // namespace smoq

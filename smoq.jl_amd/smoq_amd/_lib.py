@@ -109,6 +109,8 @@ SIGNATURES = {
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
     "smoq_set_jit": [_P, _i],
+    "smoq_mcwf_jump": [_P, _i, _d, _P, _P, _P],
+    "smoq_mcwf_trotter_evolve": [_P, _P, _i, _d, _P, _i, _P, _P, _i, _i, _P, _P, _P],
 }
 _RET = {"smoq_last_error": ctypes.c_char_p, "smoq_version": _i}
 

@@ -150,6 +150,84 @@ def lindblad_mcwf_batched_step(psi: GPUState, gates, N: int, n_traj: int, gamma:
                                        seed=seed).jumps[:, 0]
 
 
+# ------------------------------------------- any coherent step (plans) --
+JUMP_KINDS = {"sigma_minus": 0, "sigma_z": 1}
+
+
+def apply_sigma_z(psi: GPUState, k: int, N: int) -> GPUState:
+    """apply_sigma_z! (…MonteCarloWaveFunction.jl:170-180): the phase flip of
+    the σ_z jump, the same operation as apply_pauli_z_psi."""
+    from .gates import apply_pauli_z_psi
+
+    return apply_pauli_z_psi(psi, k, N)
+
+
+def jump_decay_table(kind: str, N: int, gamma: float, dt: float) -> np.ndarray:
+    """σ⁻: sqrt(1-γdt)^m per excitation count m; σ_z: the uniform
+    sqrt(1 - Nγdt) of lindblad_mcwf_step_trotter! (…:286-287)."""
+    if kind == "sigma_z":
+        if not 0.0 <= N * gamma * dt <= 1.0:
+            raise ValueError(f"N*gamma*dt = {N * gamma * dt} outside [0, 1]")
+        return np.full(N + 1, np.sqrt(1.0 - N * gamma * dt))
+    return decay_powers(N, gamma, dt)
+
+
+def mcwf_jump(psi: GPUState, kind: str, gamma: float, dt: float, uniforms) -> np.ndarray:
+    """The dissipative half of one MCWF step for every trajectory (batch
+    item) of psi after its coherent step: decay, decision, jump, renormalise.
+    uniforms: [traj, 2] (decision, site).  Returns 0-based sites or -1."""
+    k = JUMP_KINDS[kind]
+    u = np.ascontiguousarray(np.asarray(uniforms, np.float64).reshape(psi.batch, 2))
+    dec = np.ascontiguousarray(jump_decay_table(kind, psi.n, gamma, dt))
+    jumps = np.empty(psi.batch, np.int32)
+    call("smoq_mcwf_jump", psi.handle, k, gamma * dt, dec.ctypes.data_as(ctypes.c_void_p),
+         u.ctypes.data_as(ctypes.c_void_p), jumps.ctypes.data_as(ctypes.c_void_p))
+    return jumps
+
+
+def lindblad_mcwf_step_trotter(psi: GPUState, gates, N: int, gamma: float, dt: float, *,
+                               uniforms) -> np.ndarray:
+    """lindblad_mcwf_step_trotter! (…MonteCarloWaveFunction.jl:279-306), σ_z
+    dephasing: the gate list (apply_fast_trotter_step_cpu), then
+    mcwf_jump(…, "sigma_z", …); the two rand() draws are supplied as
+    uniforms [traj, 2].  Runs every batch item of psi."""
+    from .trotter import apply_fast_trotter_step_cpu
+
+    apply_fast_trotter_step_cpu(psi, gates, N)
+    return mcwf_jump(psi, "sigma_z", gamma, dt, uniforms)
+
+
+def evolve_mcwf_trotter(psi: GPUState, gates, N: int, n_traj: int, n_steps: int, gamma: float,
+                        dt: float, *, jump: str = "sigma_z", uniforms=None, seed: int = 0,
+                        obs_every: int = 0, xmask=(), zmask=()):
+    """MCWF with the coherent step of any gate list (FastTrotterGate list or
+    Tape), compiled once into a plan: n_steps of [plan, mcwf_jump] on all
+    trajectories, <P_t> per trajectory every obs_every steps.  uniforms:
+    [traj, step, 2] (SplitMix64 stream `seed` when None).  Returns
+    (jumps [traj, step], obs [record, traj, term])."""
+    from .circuits import Tape
+    from .trotter import trotter_tape
+
+    if psi.n != N or psi.batch != n_traj:
+        raise ValueError(f"state holds {psi.batch} x {psi.n} qubits, want {n_traj} x {N}")
+    tape = gates if isinstance(gates, Tape) else trotter_tape(gates, N)
+    plan = tape.compile(psi.dtype, ctx=psi.ctx)
+    if uniforms is None:
+        uniforms = splitmix_uniforms(seed, n_traj * n_steps * 2)
+    u = np.ascontiguousarray(np.asarray(uniforms, np.float64).reshape(n_traj, n_steps, 2))
+    dec = np.ascontiguousarray(jump_decay_table(jump, N, gamma, dt))
+    xm = np.ascontiguousarray(np.asarray(xmask, np.uint64))
+    zm = np.ascontiguousarray(np.asarray(zmask, np.uint64))
+    rec = n_steps // obs_every if obs_every > 0 and xm.size else 0
+    out = np.zeros((rec, n_traj, xm.size), np.float64)
+    jumps = np.empty((n_traj, n_steps), np.int32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    call("smoq_mcwf_trotter_evolve", plan.handle, psi.handle, JUMP_KINDS[jump], gamma * dt,
+         p(dec), int(n_steps), p(u), p(jumps), int(obs_every) if rec else 0, int(xm.size),
+         p(xm), p(zm), p(out))
+    return jumps, out
+
+
 def tfim_mcwf_gates(N: int, J: float, h: float, dt: float) -> MCWFGates:
     """The transverse-field Ising chain H = J Σ Z_i Z_{i+1} + h Σ X_i (open
     boundary; build_hamiltonian_parameters(N, 1, :TFIM_ZZ_X)) as one
@@ -159,6 +237,7 @@ def tfim_mcwf_gates(N: int, J: float, h: float, dt: float) -> MCWFGates:
     return mcwf_gates(precompute_trotter_gates_bitwise_cpu(params, dt), N)
 
 
-__all__ = ["MCWFGates", "MCWFGroup", "MCWFResult", "FastTrotterGate", "decay_powers",
-           "evolve_mcwf_batched_trotter", "evolve_mcwf_groups", "lindblad_mcwf_batched_step",
-           "mcwf_gates", "tfim_mcwf_gates"]
+__all__ = ["MCWFGates", "MCWFGroup", "MCWFResult", "FastTrotterGate", "apply_sigma_z",
+           "decay_powers", "evolve_mcwf_batched_trotter", "evolve_mcwf_groups",
+           "evolve_mcwf_trotter", "jump_decay_table", "lindblad_mcwf_batched_step",
+           "lindblad_mcwf_step_trotter", "mcwf_gates", "mcwf_jump", "tfim_mcwf_gates"]

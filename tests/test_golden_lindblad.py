@@ -74,3 +74,23 @@ def test_fixture_first_record_matches_rk4():
     for key in ("8", "10"):
         zz = np.array(d["z"][key])
         assert np.max(np.abs(zz - zz[:, ::-1])) < 1e-10
+
+
+def test_dephasing_fixture_matches_the_reference_run():
+    """The exact σ_z-dephasing Lindblad values (generator) against the
+    reference's recorded DM+Exact column (output.log, 4 decimals): they
+    differ only by the reference's first-order time stepping."""
+    import importlib.util
+
+    with open(os.path.join(HERE, "golden", "mcwf_dephasing_xxz.json")) as f:
+        d = json.load(f)
+    assert d["rows"] == ["X", "Y", "Z", "XX", "YY", "ZZ"]
+    for N in ("2", "4", "6", "8"):
+        exact = np.array(d["exact"][N])
+        dm = np.array(d["reference"][N]["DM+Exact"])
+        assert np.max(np.abs(exact - dm)) < 3e-3, (N, exact - dm)
+    spec = importlib.util.spec_from_file_location(
+        "make_dephasing_fixture", os.path.join(HERE, "golden", "make_dephasing_fixture.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    assert np.max(np.abs(np.array(m.exact_values(2)) - np.array(d["exact"]["2"]))) < 1e-12

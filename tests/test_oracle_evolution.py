@@ -189,3 +189,31 @@ def test_mcwf_trajectory_matches_dense_step():
             want.append(site - 1)
         np.testing.assert_array_equal(jumps, want)
         assert np.max(np.abs(got - ref)) < 1e-12
+
+
+def test_mcwf_dephasing_step_matches_dense_operators():
+    """lindblad_mcwf_step_trotter!'s dissipative half with σ_z jumps: the
+    uniform decay sqrt(1 - Nγdt), the decision, σ_z on site floor(u2·N) as a
+    dense matrix, renormalisation."""
+    rng = np.random.default_rng(4)
+    N, gdt = 3, 0.2
+    decay = np.sqrt(1.0 - N * gdt)
+    sites = set()
+    for _ in range(100):
+        psi = rng.normal(size=8) + 1j * rng.normal(size=8)
+        psi /= np.linalg.norm(psi)
+        r, u2 = rng.uniform(0, 1), rng.uniform(0, 1)
+        got = psi.copy()
+        site, dp = orc.mcwf_dephasing(got, N, decay, r, u2)
+        want = psi * decay
+        wdp = 1.0 - np.vdot(want, want).real
+        wsite = 0
+        if r < wdp:
+            k = min(N - 1, int(u2 * N))
+            want = op_on({k: Z}, N) @ want
+            wsite = k + 1
+        want /= np.linalg.norm(want)
+        assert site == wsite and abs(dp - wdp) < 1e-15
+        assert np.max(np.abs(got - want)) < 1e-14
+        sites.add(site)
+    assert sites == {0, 1, 2, 3}
