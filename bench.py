@@ -12,8 +12,11 @@ reference's cumsum/searchsortedfirst rule, the complex64 sweep and its error
 against complex128, the per-launch roofline of the fused-pass kernel, the
 adjoint gradient against parameter shift (N=24, 960 parameters; 64 sets at
 N=20), the batched MCWF run (4096 trajectories, N in {12,14,16,18}, 2000
-steps, one launch) and the CPU baseline (the oracle's restatement of the
-reference loop, one core, on a bounded sample of the same circuit).
+steps, one launch), the reference's three published CPU workloads run on
+the device beside its recorded times (shared random circuits + 100 samples,
+the expectation-value benchmark, σ_z-dephasing MCWF with 1000 trajectories)
+and the CPU baseline (the oracle's restatement of the reference loop, one
+core, on a bounded sample of the same circuit).
 """
 from __future__ import annotations
 
@@ -242,12 +245,110 @@ def mcwf_batch(sizes=(12, 14, 16, 18), n_traj=4096, steps=2000, dt=0.005, gamma=
     return out
 
 
+def reference_benchmarks(mcwf_sizes=(10, 12, 14, 16, 18, 20), n_traj=1000, mcwf_steps=200):
+    """The reference's three published CPU workloads, run the same way on the
+    device and set beside its recorded times (tests/golden/reference_timings.json):
+
+    * random-circuit sampling on its shared circuit files (N = 12..20, 1000
+      gates): |0…0>, the circuit (planned and fused per call), 100 Born
+      samples with the cumsum/searchsortedfirst rule; state allocation
+      included, as the reference allocates ψ inside its timed call;
+    * the expectation-value benchmark (N = 12..24): its 4-layer circuit from
+      |0…0>, then the 3N local and 3(N-1) neighbour observables;
+    * MCWF with σ_z dephasing (N = 10..20): 1000 trajectories of the XXZ
+      chain, 200 first-order steps with jumps, and all 6N-3 observables of
+      every trajectory after every step (the reference records their means).
+    Each is run once untimed first (run-time compilation, allocations)."""
+    import smoq_amd
+    from smoq_amd import GPUState
+    from smoq_amd import observables as ob
+    from smoq_amd.circuits import (expectation_benchmark_circuit, load_circuit_file,
+                                   run_circuit_sampling)
+    from smoq_amd.mcwf import evolve_mcwf_trotter, xxz_field_trotter_gates
+    from smoq_amd.streams import uniforms
+
+    ctx = smoq_amd.default_context()
+    golden = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(golden, "reference_timings.json")) as f:
+        ref = json.load(f)
+
+    def entry(kind, N, t, **extra):
+        r = ref[kind]["seconds"].get(str(N))
+        d = {"seconds": t, "reference_seconds": r, "speedup": (r / t) if r else None}
+        d.update(extra)
+        return d
+
+    out = {}
+    rc = {}
+    for N in (12, 14, 16, 18, 20):
+        tape = load_circuit_file(os.path.join(golden, "circuits", f"circuit_N{N:02d}.txt"))
+        r = uniforms(4200 + N, 100)
+        run_circuit_sampling(tape, r)
+        ctx.sync()
+        t0 = time.perf_counter()
+        idx, _ = run_circuit_sampling(tape, r)
+        rc[str(N)] = entry("random_circuit_sampling", N, time.perf_counter() - t0,
+                           first_samples=[int(v) for v in idx[:3]])
+    out["random_circuit_sampling"] = {"workload": ref["random_circuit_sampling"]["workload"],
+                                      "reference_source": ref["random_circuit_sampling"]["source"],
+                                      "by_N": rc}
+    log("reference workloads: random circuits done")
+    ev = {}
+    for N in (12, 14, 16, 18, 20, 22, 24):
+        tape = expectation_benchmark_circuit(N)
+
+        def run():
+            s = GPUState.basis(N, 0)
+            tape.apply(s)
+            v = ob.measure_all_observables_state_vector(s, N, 1)
+            s.free()
+            return v
+
+        run()
+        ctx.sync()
+        t0 = time.perf_counter()
+        v = run()
+        ev[str(N)] = entry("expectation_values", N, time.perf_counter() - t0,
+                           mean_Z=float(np.mean(v[2:3 * N:3])))
+    out["expectation_values"] = {"workload": ref["expectation_values"]["workload"],
+                                 "reference_source": ref["expectation_values"]["source"],
+                                 "by_N": ev}
+    log("reference workloads: expectation values done")
+    mc = {}
+    dt, gamma = 0.05, 0.1
+    for N in mcwf_sizes:
+        gates = xxz_field_trotter_gates(N, dt)
+        xs, zs = ob.qrc_observable_masks(N, 1)
+        s = GPUState.basis(N, 0, batch=n_traj)
+        evolve_mcwf_trotter(s, gates, N, n_traj, 2, gamma, dt, jump="sigma_z", seed=1,
+                            obs_every=1, xmask=xs, zmask=zs)
+        s.set_basis(0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        jumps, obs = evolve_mcwf_trotter(s, gates, N, n_traj, mcwf_steps, gamma, dt,
+                                         jump="sigma_z", seed=100 + N, obs_every=1,
+                                         xmask=xs, zmask=zs)
+        t = time.perf_counter() - t0
+        s.free()
+        final = obs[-1].mean(0)
+        mc[str(N)] = entry("mcwf_dephasing", N, t,
+                           jumps_per_trajectory=float((jumps >= 0).sum() / n_traj),
+                           mean_Z_final=float(np.mean(final[2:3 * N:3])))
+        log(f"reference workloads: MCWF N={N} {t:.2f} s")
+    out["mcwf_dephasing"] = {"workload": ref["mcwf_dephasing"]["workload"],
+                             "reference_source": ref["mcwf_dephasing"]["source"],
+                             "trajectories": n_traj, "steps": mcwf_steps, "by_N": mc}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", choices=["traffic"], default=None)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-mcwf", action="store_true")
     ap.add_argument("--mcwf-only", action="store_true")
+    ap.add_argument("--no-reference", action="store_true")
+    ap.add_argument("--reference-only", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -269,6 +370,9 @@ def main():
         sys.exit(2)
     if args.mcwf_only:
         print(json.dumps({"mcwf": mcwf_batch()}), flush=True)
+        return
+    if args.reference_only:
+        print(json.dumps({"reference_workloads": reference_benchmarks()}), flush=True)
         return
 
     import smoq_amd
@@ -375,6 +479,9 @@ def main():
     if not args.no_mcwf and not args.no_extras:
         log("batched MCWF ...")
         result.setdefault("extras", {})["mcwf"] = mcwf_batch()
+    if not args.no_reference and not args.no_extras:
+        log("the reference's published workloads ...")
+        result.setdefault("extras", {})["reference_workloads"] = reference_benchmarks()
 
     if not args.no_cpu:
         log("CPU baseline ...")

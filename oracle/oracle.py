@@ -319,6 +319,99 @@ def chebyshev_evolve(psi, N, bonds, fields, t, emin, emax, tol=1e-12, max_order=
     return out * np.exp(-1j * b * t), M_used
 
 
+
+# ------------------------------------------------ random-circuit benchmark --
+_R2 = 1.0 / np.sqrt(2.0)
+BENCH_FIXED = {
+    # run_benchmark_smoq_Float64.jl:83-88
+    "H": np.array([[_R2, _R2], [_R2, -_R2]], np.complex128),
+    "X": np.array([[0, 1], [1, 0]], np.complex128),
+    "Y": np.array([[0, -1j], [1j, 0]], np.complex128),
+    "Z": np.array([[1, 0], [0, -1]], np.complex128),
+    "S": np.array([[1, 0], [0, 1j]], np.complex128),
+    "T": np.array([[1, 0], [0, np.exp(1j * np.pi / 4)]], np.complex128),
+}
+
+
+def bench_gate_matrix(name: str, theta: float):
+    """get_gate_matrix (run_benchmark_smoq_Float64.jl:91-93,146-157)."""
+    c, s = np.cos(theta / 2), np.sin(theta / 2)
+    if name == "RX":
+        return np.array([[c, -1j * s], [-1j * s, c]], np.complex128)
+    if name == "RY":
+        return np.array([[c, -s], [s, c]], np.complex128)
+    if name == "RZ":
+        return np.array([[np.exp(-1j * theta / 2), 0], [0, np.exp(1j * theta / 2)]],
+                        np.complex128)
+    return BENCH_FIXED[name]
+
+
+def read_circuit_file(path: str):
+    """load_circuit_from_file (run_benchmark_smoq_Float64.jl:384-404):
+    returns [(name, q1, q2, theta)] with 1-based qubits, q2 = 0 for 1-qubit
+    gates; for CNOT/CZ q1 is the first (control) qubit of the line."""
+    ops = []
+    for line in open(path, encoding="utf-8"):
+        line = line.strip()
+        if not line or line.startswith("#"):
+            continue
+        parts = line.split()
+        name, q1 = parts[0], int(parts[1]) + 1
+        if name in ("CNOT", "CZ"):
+            ops.append((name, q1, int(parts[2]) + 1, 0.0))
+        elif name in ("RX", "RY", "RZ"):
+            ops.append((name, q1, 0, float(parts[2])))
+        else:
+            ops.append((name, q1, 0, 0.0))
+    return ops
+
+
+def fuse_gates(ops):
+    """fuse_gates (run_benchmark_smoq_Float64.jl:167-200): per-qubit pending
+    2x2 products U_new · U_acc, flushed for both qubits of a 2-qubit gate and
+    at the end in ascending qubit order.  Returns [(is_single, qubit,
+    control, U, name)]."""
+    fused, pending = [], {}
+
+    def flush(q):
+        if q in pending:
+            fused.append((True, q, 0, pending.pop(q), "U"))
+
+    for name, q1, q2, th in ops:
+        if q2 == 0:
+            U = bench_gate_matrix(name, th)
+            pending[q1] = U @ pending[q1] if q1 in pending else U.copy()
+        else:
+            flush(q1)
+            flush(q2)
+            fused.append((False, q2, q1, None, name))
+    for q in sorted(pending):
+        flush(q)
+    return fused
+
+
+def run_fused_circuit(psi, fused, N: int):
+    """run_fused_circuit! (run_benchmark_smoq_Float64.jl:356-369)."""
+    for single, q, c, U, name in fused:
+        if single:
+            apply_fused_1q(psi, U, q, N)
+        elif name == "CNOT":
+            lib().or_apply_cnot(_p(_c128(psi)), c, q, N)
+        else:
+            lib().or_apply_cz(_p(_c128(psi)), c, q, N)
+    return psi
+
+
+def run_circuit_sampling(N: int, ops, r):
+    """run_circuit_sampling (run_benchmark_smoq_Float64.jl:414-437) with the
+    uniforms r in place of rand(): fuse, simulate |0…0>, then
+    searchsortedfirst(cumsum(abs2.(ψ)), r) - 1 per shot.  Returns (ψ, 0-based
+    indices; 2^N when r exceeds the last prefix)."""
+    psi = np.zeros(1 << N, np.complex128)
+    psi[0] = 1.0
+    run_fused_circuit(psi, fuse_gates(ops), N)
+    return psi, sample_cumsum_search(psi, N, r)
+
 # ------------------------------------------------------ dense references --
 def dense_1q(U, q: int, N: int):
     """Full 2^N matrix of a 1-qubit gate on 0-based qubit q (little-endian)."""

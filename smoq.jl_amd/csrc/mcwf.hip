@@ -350,121 +350,201 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
 
 // ------------------------------------------------------------------------
 // The dissipative half of one MCWF step for any coherent step (applied
-// before, e.g. by a compiled plan): one workgroup per trajectory (batch
-// item).  Decay (σ⁻: sqrt(1-γdt)^popcount, σ_z: the uniform sqrt(1-nγdt)),
-// norm, decision, jump site, jump and renormalisation in three sweeps over
-// the trajectory's amplitudes.
+// before, e.g. by a compiled plan), on every trajectory (batch item) at
+// once.  Decay (σ⁻: sqrt(1-γdt)^popcount, σ_z: the uniform sqrt(1-nγdt)),
+// norm, decision, jump site, jump and renormalisation in two launches over
+// chunks of each trajectory: k_jump_sums reads the chunk once for the norm
+// of the decayed amplitudes (and, for σ⁻, the site populations);
+// k_jump_apply adds the chunks' partial sums in a fixed order, takes the
+// decision and rewrites the chunk (decay, jump, 1/norm) in one sweep, with
+// the rounding of decaying first and scaling second.
 constexpr int kJumpMaxQ = 30;
+constexpr int kJumpSlots = kJumpMaxQ + 1;  // norm, then one population per site
+
+int mcwf_jump_blocks(const smoq_state* s) {
+  int nb = 1;
+  while ((uint64_t)nb * 2 <= (s->dim >> 12) && (uint64_t)nb * 2 * s->batch <= 2048) nb *= 2;
+  return nb;
+}
+
+size_t mcwf_jump_partial_bytes(const smoq_state* s) {
+  return (size_t)s->batch * mcwf_jump_blocks(s) * kJumpSlots * sizeof(double);
+}
+
+template <typename T, bool POPS>
+__global__ void __launch_bounds__(kMcBlock)
+    k_jump_sums(const VT_alias<T>* __restrict__ psi, int n, int lc, int sz,
+                const double* __restrict__ decay_pow, double* __restrict__ partial) {
+  using V = VT_alias<T>;
+  __shared__ double red[kJumpSlots * kMcWaves];
+  __shared__ double sums[kJumpSlots];
+  const uint64_t item = blockIdx.y, blk = blockIdx.x;
+  const uint64_t chunk = 1ull << lc;
+  const uint64_t g0 = blk << lc;
+  const V* p = psi + (item << n) + g0;
+  const int tid = threadIdx.x;
+  double tot = 0.0;
+  double hi[POPS ? kJumpMaxQ - 8 : 1];
+#pragma unroll
+  for (int i = 0; i < (POPS ? kJumpMaxQ - 8 : 1); ++i) hi[i] = 0.0;
+  for (uint64_t j = 0; (j << 8) < chunk; ++j) {
+    const uint64_t k = (j << 8) | (uint64_t)tid;
+    if (k >= chunk) break;
+    const double d = decay_pow[sz ? 0 : __popcll(g0 | k)];
+    const V v = p[k];
+    const V r = make_c<T>((double)v.x * d, (double)v.y * d);
+    const double q = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+    tot += q;
+    if (POPS) {
+#pragma unroll
+      for (int i = 0; i < kJumpMaxQ - 8; ++i)
+        if (i + 8 < lc && ((j >> i) & 1)) hi[i] += q;
+    }
+  }
+  double v[kJumpSlots];
+  v[0] = tot;
+#pragma unroll
+  for (int b = 0; b < kJumpMaxQ; ++b) {
+    double s;
+    if (b >= lc)
+      s = ((blk >> (b - lc)) & 1) ? tot : 0.0;
+    else if (b < 8)
+      s = ((tid >> b) & 1) ? tot : 0.0;
+    else
+      s = POPS ? hi[POPS ? b - 8 : 0] : 0.0;
+    v[b + 1] = s;
+  }
+  const int m = POPS ? n + 1 : 1;
+  mc_block_sum<kJumpSlots>(v, m, red, sums);
+  if (tid < m) partial[(item * gridDim.x + blk) * kJumpSlots + tid] = sums[tid];
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kMcBlock)
-    k_mcwf_jump(VT_alias<T>* __restrict__ psi, int n, int kind, double gamma_dt,
-                const double* __restrict__ decay_pow, const double* __restrict__ uni,
-                int uni_stride, int32_t* __restrict__ jumps, int jump_stride,
-                unsigned long long* fault, int n_steps, int step) {
+    k_jump_apply(VT_alias<T>* __restrict__ psi, int n, int lc, int sz, double gamma_dt,
+                 const double* __restrict__ decay_pow, const double* __restrict__ partial,
+                 const double* __restrict__ uni, int uni_stride, int32_t* __restrict__ jumps,
+                 int jump_stride, unsigned long long* fault, int n_steps, int step) {
   using V = VT_alias<T>;
-  __shared__ double red[(kJumpMaxQ + 1) * kMcWaves];
-  __shared__ double sums[kJumpMaxQ + 1];
-  __shared__ int s_site;
-  const uint64_t item = blockIdx.x;
-  const uint64_t D = 1ull << n;
+  __shared__ double s_new;
+  __shared__ int s_site, s_jump;
+  const uint64_t item = blockIdx.y, blk = blockIdx.x;
+  const int nbk = gridDim.x;
+  const uint64_t chunk = 1ull << lc;
   V* p = psi + (item << n);
-  const bool sz = kind == SMOQ_JUMP_SIGMA_Z;
-
-  double acc = 0.0;
-  for (uint64_t k = threadIdx.x; k < D; k += kMcBlock) {
-    const double d = decay_pow[sz ? 0 : __popcll(k)];
-    const V v = p[k];
-    const V r = make_c<T>((double)v.x * d, (double)v.y * d);
-    p[k] = r;
-    acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-  }
-  double nv[1] = {acc};
-  mc_block_sum<1>(nv, 1, red, sums);
-  const double norm_sq = sums[0];
-  const double dp = 1.0 - norm_sq;
-  const double r = uni[item * uni_stride];
-  const bool jump = (r < dp) && (dp > 0.0);
-  double new_norm_sq = norm_sq;
-  int site = -1;
-  if (jump) {  // uniform over the block
-    if (sz) {
-      const double u2 = uni[item * uni_stride + 1];
-      site = (int)(u2 * n);
-      if (site > n - 1) site = n - 1;
-    } else {
-      // site populations: index k = tid + 256 j, bits 0..7 from tid, 8.. from j
-      double tot = 0.0, hi[kJumpMaxQ - 8];
-#pragma unroll
-      for (int i = 0; i < kJumpMaxQ - 8; ++i) hi[i] = 0.0;
-      for (uint64_t j = 0; (j << 8) < D; ++j) {
-        const uint64_t k = (j << 8) | threadIdx.x;
-        if (k >= D) break;
-        const V v = p[k];
-        const double q = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-        tot += q;
-#pragma unroll
-        for (int i = 0; i < kJumpMaxQ - 8; ++i)
-          if ((j >> i) & 1) hi[i] += q;
-      }
-      double v[kJumpMaxQ];
-#pragma unroll
-      for (int i = 0; i < kJumpMaxQ; ++i)
-        v[i] = i < 8 ? (((threadIdx.x >> i) & 1) ? tot : 0.0) : hi[i < 8 ? 0 : i - 8];
-      mc_block_sum<kJumpMaxQ>(v, n, red, sums);
-      if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) {
+    const double* pp = partial + item * nbk * kJumpSlots;
+    double norm_sq = 0.0;
+    for (int b = 0; b < nbk; ++b) norm_sq += pp[b * kJumpSlots];
+    const double dp = 1.0 - norm_sq;
+    const double r = uni[item * uni_stride];
+    const bool jump = (r < dp) && (dp > 0.0);
+    double new_norm_sq = norm_sq;
+    int site = -1;
+    if (jump) {
+      if (sz) {
+        site = (int)(uni[item * uni_stride + 1] * n);
+        if (site > n - 1) site = n - 1;
+      } else {
         double cum = 0.0;
         int sel = 0;
         for (int k = 0; k < n; ++k) {
-          cum += gamma_dt * sums[k] / norm_sq;
+          double pop = 0.0;
+          for (int b = 0; b < nbk; ++b) pop += pp[b * kJumpSlots + 1 + k];
+          cum += gamma_dt * pop / norm_sq;
           if (r < cum) {
             sel = k;
             break;
           }
         }
-        s_site = sel;
+        site = sel;
+        new_norm_sq = 0.0;
+        for (int b = 0; b < nbk; ++b) new_norm_sq += pp[b * kJumpSlots + 1 + sel];
       }
-      __syncthreads();
-      site = s_site;
-      new_norm_sq = sums[site];
+    }
+    s_new = new_norm_sq;
+    s_site = site;
+    s_jump = jump;
+    if (blk == 0) {
+      jumps[item * jump_stride] = jump ? site : -1;
+      if (!(new_norm_sq > 0.0) || !isfinite(1.0 / sqrt(new_norm_sq)))
+        atomicMin(fault, (unsigned long long)item * n_steps + step);
     }
   }
-  const double sc = 1.0 / sqrt(new_norm_sq);
-  if (jump && !sz) {
+  __syncthreads();
+  const double sc = 1.0 / sqrt(s_new);
+  const int site = s_site;
+  const uint64_t g0 = blk << lc;
+  if (s_jump && !sz) {
+    // σ⁻ at `site`: ψ[i0] = decayed ψ[i0 | m] / norm, ψ[i0 | m] = 0 for
+    // every i0 with the site bit clear; a chunk holding the site bit
+    // within it handles its own pairs, otherwise the chunk with the bit
+    // clear handles the pairs with its partner chunk
     const uint64_t m = 1ull << site;
-    for (uint64_t q = threadIdx.x; q < (D >> 1); q += kMcBlock) {
-      const uint64_t i0 = insert_zero(q, site), i1 = i0 | m;
-      const V v = p[i1];
-      p[i0] = make_c<T>((double)v.x * sc, (double)v.y * sc);
-      p[i1] = make_c<T>(0.0, 0.0);
+    if (site < lc) {
+      for (uint64_t q = threadIdx.x; q < (chunk >> 1); q += kMcBlock) {
+        const uint64_t i0 = g0 | insert_zero(q, site), i1 = i0 | m;
+        const V v = p[i1];
+        const double d = decay_pow[__popcll(i1)];
+        const V r = make_c<T>((double)v.x * d, (double)v.y * d);
+        p[i0] = make_c<T>((double)r.x * sc, (double)r.y * sc);
+        p[i1] = make_c<T>(0.0, 0.0);
+      }
+    } else if (!(g0 & m)) {
+      for (uint64_t k = threadIdx.x; k < chunk; k += kMcBlock) {
+        const uint64_t i0 = g0 | k, i1 = i0 | m;
+        const V v = p[i1];
+        const double d = decay_pow[__popcll(i1)];
+        const V r = make_c<T>((double)v.x * d, (double)v.y * d);
+        p[i0] = make_c<T>((double)r.x * sc, (double)r.y * sc);
+        p[i1] = make_c<T>(0.0, 0.0);
+      }
     }
-  } else {
-    const uint64_t m = (jump && sz) ? (1ull << site) : 0ull;
-    for (uint64_t k = threadIdx.x; k < D; k += kMcBlock) {
-      const double s = (k & m) ? -sc : sc;
-      const V v = p[k];
-      p[k] = make_c<T>((double)v.x * s, (double)v.y * s);
-    }
+    return;
   }
-  if (threadIdx.x == 0) {
-    jumps[item * jump_stride] = jump ? site : -1;
-    if (!(new_norm_sq > 0.0) || !isfinite(sc))
-      atomicMin(fault, (unsigned long long)item * n_steps + step);
+  const uint64_t mz = (s_jump && sz) ? (1ull << site) : 0ull;
+  for (uint64_t k = threadIdx.x; k < chunk; k += kMcBlock) {
+    const uint64_t gk = g0 | k;
+    const double d = decay_pow[sz ? 0 : __popcll(gk)];
+    const V v = p[gk];
+    const V r = make_c<T>((double)v.x * d, (double)v.y * d);
+    const double s = (gk & mz) ? -sc : sc;
+    p[gk] = make_c<T>((double)r.x * s, (double)r.y * s);
   }
 }
 
 int mcwf_jump_launch(smoq_state* s, int kind, double gamma_dt, const double* d_decay,
                      const double* d_uni, int uni_stride, int32_t* d_jumps, int jump_stride,
-                     unsigned long long* d_fault, int n_steps, int step) {
-  if (s->dtype == SMOQ_C128)
-    hipLaunchKernelGGL(k_mcwf_jump<double>, dim3((unsigned)s->batch), dim3(kMcBlock), 0,
-                       s->ctx->stream, (double2*)s->d, s->n, kind, gamma_dt, d_decay, d_uni,
-                       uni_stride, d_jumps, jump_stride, d_fault, n_steps, step);
-  else
-    hipLaunchKernelGGL(k_mcwf_jump<float>, dim3((unsigned)s->batch), dim3(kMcBlock), 0,
-                       s->ctx->stream, (float2*)s->d, s->n, kind, gamma_dt, d_decay, d_uni,
-                       uni_stride, d_jumps, jump_stride, d_fault, n_steps, step);
-  SMOQ_CHECK_LAUNCH("k_mcwf_jump");
+                     unsigned long long* d_fault, int n_steps, int step, double* d_partial) {
+  const int nbk = mcwf_jump_blocks(s);
+  const int lc = s->n - __builtin_ctz((unsigned)nbk);
+  const int sz = kind == SMOQ_JUMP_SIGMA_Z;
+  const dim3 grid((unsigned)nbk, (unsigned)s->batch);
+  hipStream_t st = s->ctx->stream;
+  if (s->dtype == SMOQ_C128) {
+    if (sz)
+      hipLaunchKernelGGL((k_jump_sums<double, false>), grid, dim3(kMcBlock), 0, st,
+                         (const double2*)s->d, s->n, lc, sz, d_decay, d_partial);
+    else
+      hipLaunchKernelGGL((k_jump_sums<double, true>), grid, dim3(kMcBlock), 0, st,
+                         (const double2*)s->d, s->n, lc, sz, d_decay, d_partial);
+    SMOQ_CHECK_LAUNCH("k_jump_sums");
+    hipLaunchKernelGGL(k_jump_apply<double>, grid, dim3(kMcBlock), 0, st, (double2*)s->d, s->n,
+                       lc, sz, gamma_dt, d_decay, d_partial, d_uni, uni_stride, d_jumps,
+                       jump_stride, d_fault, n_steps, step);
+  } else {
+    if (sz)
+      hipLaunchKernelGGL((k_jump_sums<float, false>), grid, dim3(kMcBlock), 0, st,
+                         (const float2*)s->d, s->n, lc, sz, d_decay, d_partial);
+    else
+      hipLaunchKernelGGL((k_jump_sums<float, true>), grid, dim3(kMcBlock), 0, st,
+                         (const float2*)s->d, s->n, lc, sz, d_decay, d_partial);
+    SMOQ_CHECK_LAUNCH("k_jump_sums");
+    hipLaunchKernelGGL(k_jump_apply<float>, grid, dim3(kMcBlock), 0, st, (float2*)s->d, s->n, lc,
+                       sz, gamma_dt, d_decay, d_partial, d_uni, uni_stride, d_jumps, jump_stride,
+                       d_fault, n_steps, step);
+  }
+  SMOQ_CHECK_LAUNCH("k_jump_apply");
   return SMOQ_OK;
 }
 
@@ -688,8 +768,9 @@ extern "C" int smoq_mcwf_jump(smoq_state* s, int kind, double gamma_dt, const do
   const size_t off_u = ((size_t)(s->n + 1) * sizeof(double) + 255) & ~(size_t)255;
   const size_t off_j = off_u + ((2 * nb * sizeof(double) + 255) & ~(size_t)255);
   const size_t off_f = off_j + ((nb * sizeof(int32_t) + 255) & ~(size_t)255);
+  const size_t off_p = off_f + 256;
   unsigned char* dev = nullptr;
-  SMOQ_HIP(hipMalloc(&dev, off_f + 256));
+  SMOQ_HIP(hipMalloc(&dev, off_p + mcwf_jump_partial_bytes(s)));
   const unsigned long long no_fault = ~0ull;
   unsigned long long fault = no_fault;
   hipError_t e = hipMemcpyAsync(dev, decay_pow, (size_t)(s->n + 1) * sizeof(double),
@@ -702,7 +783,8 @@ extern "C" int smoq_mcwf_jump(smoq_state* s, int kind, double gamma_dt, const do
                        ctx->stream);
   if (e == hipSuccess)
     rc = mcwf_jump_launch(s, kind, gamma_dt, (const double*)dev, (const double*)(dev + off_u), 2,
-                          (int32_t*)(dev + off_j), 1, (unsigned long long*)(dev + off_f), 1, 0);
+                          (int32_t*)(dev + off_j), 1, (unsigned long long*)(dev + off_f), 1, 0,
+                          (double*)(dev + off_p));
   if (e == hipSuccess && rc == SMOQ_OK)
     e = hipMemcpyAsync(jumps, dev + off_j, nb * sizeof(int32_t), hipMemcpyDeviceToHost,
                        ctx->stream);
@@ -712,7 +794,7 @@ extern "C" int smoq_mcwf_jump(smoq_state* s, int kind, double gamma_dt, const do
   (void)hipFree(dev);
   if (rc) return rc;
   if (e != hipSuccess) return hip_error(e, "smoq_mcwf_jump: copy");
-  if (se != hipSuccess) return hip_error(se, "k_mcwf_jump");
+  if (se != hipSuccess) return hip_error(se, "smoq_mcwf_jump");
   if (fault != no_fault)
     return set_error(SMOQ_ENUMERIC, "smoq_mcwf_jump: zero or non-finite norm in trajectory %llu",
                      fault);

@@ -198,8 +198,9 @@ int smoq_mcwf_trotter_evolve(smoq_plan* p, smoq_state* s, int kind, double gamma
   const size_t off_u = ((size_t)(s->n + 1) * sizeof(double) + 255) & ~(size_t)255;
   const size_t off_j = off_u + ((2 * nb * ns * sizeof(double) + 255) & ~(size_t)255);
   const size_t off_f = off_j + ((nb * ns * sizeof(int32_t) + 255) & ~(size_t)255);
+  const size_t off_p = off_f + 256;
   unsigned char* dev = nullptr;
-  SMOQ_HIP(hipMalloc(&dev, off_f + 256));
+  SMOQ_HIP(hipMalloc(&dev, off_p + mcwf_jump_partial_bytes(s)));
   const unsigned long long no_fault = ~0ull;
   unsigned long long fault = no_fault;
   hipError_t e = hipMemcpyAsync(dev, decay_pow, (size_t)(s->n + 1) * sizeof(double),
@@ -215,7 +216,8 @@ int smoq_mcwf_trotter_evolve(smoq_plan* p, smoq_state* s, int kind, double gamma
     if ((rc = mcwf_jump_launch(s, kind, gamma_dt, (const double*)dev,
                                (const double*)(dev + off_u) + 2 * step, 2 * n_steps,
                                (int32_t*)(dev + off_j) + step, n_steps,
-                               (unsigned long long*)(dev + off_f), n_steps, step)))
+                               (unsigned long long*)(dev + off_f), n_steps, step,
+                               (double*)(dev + off_p))))
       break;
     if (record && (step + 1) % obs_every == 0) {
       const size_t r = (size_t)(step + 1) / obs_every - 1;

@@ -123,6 +123,397 @@ __global__ void k_finish(const double* __restrict__ partial, int nblocks, int nt
   out[(uint64_t)item * out_stride + out_col[t]] = scale * v;
 }
 
+// ---------------------------------------------------------------------------
+// Tiled reductions: many Pauli groups per HBM read.
+//
+// A tile is the 2^k amplitudes base | dep(l), l < 2^k, where dep scatters
+// the local index over k tile bit positions (bits 0..3 always among them,
+// so loads run over 256-byte segments) and base enumerates the other bits.
+// Every group whose x mask lies inside the tile bits is evaluated from LDS:
+// the pair (i, i ^ x) is (dep(l), dep(l ^ lx)) with lx the x mask in local
+// bits, and the sign (-1)^{|i & z|} splits into the base's parity and the
+// parity of l & lz.  One pass reads the state once for all its groups; a
+// chain's local and neighbour terms (X, Y, Z, XX, YY, ZZ) need about
+// (N - 12) / 7 + 1 passes instead of one launch per x mask.
+constexpr int kObsTileBits = 12;
+constexpr int kObsLowBits = 4;
+constexpr int kObsMaxTerms = 128;  // terms of one pass
+constexpr int kObsBlockBits = 9;
+constexpr int kObsBlock = 1 << kObsBlockBits;  // 2 workgroups (4 waves per SIMD) per CU
+constexpr int kObsPairIters = (1 << (kObsTileBits - 1)) / kObsBlock;
+
+struct ObsTerm {
+  uint64_t z;     // global z mask (base parity)
+  uint32_t lz;    // z mask in local tile bits
+  int32_t kind;   // -1 diagonal; else |x & z| mod 4 as in k_expect_pair
+};
+// Terms sharing an x mask.  Pair groups (lx != 0) list first the terms whose
+// sign is fixed within a tile (lz has no bit besides hl, so it is the base
+// parity alone: they need only the group's sums of Re w and Im w), then the
+// rest.  Diagonal groups (lx == 0) hold the terms with one value m of the z
+// bits above the low kObsBlockBits tile bits.
+struct ObsGroup {
+  uint32_t lx;    // x mask in local tile bits (0: diagonal group)
+  int32_t hl;     // highest local bit of lx
+  int32_t t0;     // first term (index into the pass-ordered term array)
+  int32_t nt;
+  int32_t nconst; // pair groups: leading terms with a tile-constant sign
+  uint32_t m;     // diagonal groups: lz >> kObsBlockBits shared by the terms
+};
+struct ObsPass {
+  int k, lowb, n_cbits;
+  int group0, n_groups;
+  int term0, n_terms;
+  uint8_t tpos[kObsTileBits];  // tile bit positions, ascending
+  uint8_t cpos[64];            // the other bit positions, ascending
+};
+
+__device__ inline int parity64(uint64_t v) { return __popcll(v) & 1; }
+
+// Sum over the wave by DPP lane moves (quad xor 1, quad xor 2, half-row
+// mirror, row mirror: every lane of a 16-lane row then holds the row sum)
+// and the four row sums read out in a fixed order; the result is uniform.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xf, 0xf,
+                                             false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double read_lane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double wave_total(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+// Terms of a pair group whose sign varies within the tile, CH at a time.
+template <typename T, int CH>
+__device__ inline void obs_pair_varying(const typename CT<T>::V* tile, int k, uint64_t base,
+                                        const ObsGroup& G, const ObsTerm* terms,
+                                        int c, int nc, int local0,
+                                        double (*acc_s)[kObsMaxTerms]) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  uint32_t lz[CH];
+  int sgn0[CH], kd[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    const ObsTerm tm = terms[G.t0 + c + (j < nc ? j : 0)];
+    lz[j] = tm.lz;
+    kd[j] = tm.kind;
+    sgn0[j] = parity64(base & tm.z) ^ ((tm.kind == 1 || tm.kind == 2) ? 1 : 0);
+  }
+  double acc[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j) acc[j] = 0.0;
+  auto add_pair = [&](uint32_t l, const typename CT<T>::V& a, const typename CT<T>::V& b) {
+    const double wr = (double)b.x * (double)a.x + (double)b.y * (double)a.y;
+    const double wi = (double)b.x * (double)a.y - (double)b.y * (double)a.x;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const double v = (kd[j] & 1) ? wi : wr;
+      acc[j] += ((__popc(l & lz[j]) ^ sgn0[j]) & 1) ? -v : v;
+    }
+  };
+  const uint32_t hl = (uint32_t)G.hl;
+#pragma unroll
+  for (int it = 0; it < kObsPairIters; ++it) {
+    const int p = tid + it * kObsBlock;
+    if (p >= (1 << (k - 1))) break;
+    const uint32_t l = ((uint32_t)p >> hl << (hl + 1)) | ((uint32_t)p & ((1u << hl) - 1u));
+    add_pair(l, tile[l], tile[l ^ G.lx]);
+  }
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    if (j >= nc) continue;
+    const double s = wave_total(acc[j]);
+    if (lane == 0) acc_s[wid][G.t0 + c + j - local0] += s;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kObsBlock)
+    k_expect_tiles(const typename CT<T>::V* __restrict__ psi, uint64_t dim, ObsPass P,
+                   const ObsGroup* __restrict__ groups, const ObsTerm* __restrict__ terms,
+                   double* __restrict__ partial, int part_terms) {
+  using V = typename CT<T>::V;
+  constexpr int kJ = (1 << kObsTileBits) / kObsBlock;  // amplitudes per thread (diagonal)
+  __shared__ V tile[1 << kObsTileBits];
+  __shared__ uint64_t dep_hi[1 << (kObsTileBits - kObsLowBits)];
+  __shared__ double acc_s[kObsBlock / 64][kObsMaxTerms];
+  // this pass's descriptors, term indices relative to the pass
+  __shared__ ObsGroup s_groups[kObsMaxTerms];
+  __shared__ ObsTerm s_terms[kObsMaxTerms];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint64_t item = blockIdx.y;
+  const V* p = psi + item * dim;
+  const int k = P.k, lowb = P.lowb, ntile = 1 << k;
+  const int nhi = 1 << (k - lowb);
+  for (int h = tid; h < nhi; h += kObsBlock) {
+    uint64_t off = 0;
+    for (int b = 0; b < k - lowb; ++b)
+      if ((h >> b) & 1) off |= 1ull << P.tpos[lowb + b];
+    dep_hi[h] = off;
+  }
+  for (int t = tid; t < (kObsBlock / 64) * kObsMaxTerms; t += kObsBlock)
+    (&acc_s[0][0])[t] = 0.0;
+  for (int g = tid; g < P.n_groups; g += kObsBlock) {
+    ObsGroup G = groups[P.group0 + g];
+    G.t0 -= P.term0;
+    s_groups[g] = G;
+  }
+  for (int t = tid; t < P.n_terms; t += kObsBlock) s_terms[t] = terms[P.term0 + t];
+  __syncthreads();
+  const uint64_t n_tiles = dim >> k;
+  const int lowmask = (1 << lowb) - 1;
+  const int n_groups = P.n_groups;
+  for (uint64_t tt = blockIdx.x; tt < n_tiles; tt += gridDim.x) {
+    uint64_t base = 0;
+    for (int b = 0; b < P.n_cbits; ++b)
+      if ((tt >> b) & 1) base |= 1ull << P.cpos[b];
+    for (int l = tid; l < ntile; l += kObsBlock)
+      tile[l] = p[base | (uint64_t)(l & lowmask) | dep_hi[l >> lowb]];
+    __syncthreads();
+    // probabilities of this thread's amplitudes l = tid + kObsBlock j
+    double pr[kJ];
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+      const int l = tid + kObsBlock * j;
+      pr[j] = 0.0;
+      if (l < ntile) {
+        const V a = tile[l];
+        pr[j] = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+      }
+    }
+    for (int g = 0; g < n_groups; ++g) {
+      const ObsGroup G = s_groups[g];
+      const uint32_t lx = __builtin_amdgcn_readfirstlane(G.lx);
+      const int hl = __builtin_amdgcn_readfirstlane(G.hl);
+      const int t0 = __builtin_amdgcn_readfirstlane(G.t0);
+      const int nt = __builtin_amdgcn_readfirstlane(G.nt);
+      const int nconst = __builtin_amdgcn_readfirstlane(G.nconst);
+      if (lx == 0) {
+        // sum over j with the sign of the high tile bits, then over the
+        // wave with the sign of the lane and wave bits, term by term
+        const uint32_t m = __builtin_amdgcn_readfirstlane(G.m);
+        double w = 0.0;
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) w += (__popc((uint32_t)j & m) & 1) ? -pr[j] : pr[j];
+        for (int t = 0; t < nt; ++t) {
+          const ObsTerm tm = s_terms[t0 + t];
+          const uint32_t lo = tm.lz & ((1u << kObsBlockBits) - 1u);
+          const int s = (__popc((uint32_t)tid & lo) ^ parity64(base & tm.z)) & 1;
+          const double v = wave_total(s ? -w : w);
+          if (lane == 0) acc_s[wid][t0 + t] += v;
+        }
+        continue;
+      }
+      if (nconst > 0) {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int it = 0; it < kObsPairIters; ++it) {
+          const int q = tid + it * kObsBlock;
+          if (q >= (ntile >> 1)) break;
+          const uint32_t l = ((uint32_t)q >> hl << (hl + 1)) | ((uint32_t)q & ((1u << hl) - 1u));
+          const V a = tile[l], b = tile[l ^ lx];
+          sr += (double)b.x * (double)a.x + (double)b.y * (double)a.y;
+          si += (double)b.x * (double)a.y - (double)b.y * (double)a.x;
+        }
+        sr = wave_total(sr);
+        si = wave_total(si);
+        // one lane per constant-sign term
+        if (lane < nconst) {
+          const ObsTerm tm = s_terms[t0 + lane];
+          const double v = (tm.kind & 1) ? si : sr;
+          const int s = parity64(base & tm.z) ^ ((tm.kind == 1 || tm.kind == 2) ? 1 : 0);
+          acc_s[wid][t0 + lane] += s ? -v : v;
+        }
+        for (int t = t0 + 64; t < t0 + nconst; ++t) {  // more than 64 (rare)
+          if (lane != 0) break;
+          const ObsTerm tm = s_terms[t];
+          const double v = (tm.kind & 1) ? si : sr;
+          const int s = parity64(base & tm.z) ^ ((tm.kind == 1 || tm.kind == 2) ? 1 : 0);
+          acc_s[wid][t] += s ? -v : v;
+        }
+      }
+      ObsGroup Gu;
+      Gu.lx = lx;
+      Gu.hl = hl;
+      Gu.t0 = t0;
+      Gu.nt = nt;
+      Gu.nconst = nconst;
+      Gu.m = 0;
+      for (int c = nconst; c < nt; c += 16) {
+        const int nc = min(16, nt - c);
+        if (nc <= 2)
+          obs_pair_varying<T, 2>(tile, k, base, Gu, s_terms, c, nc, 0, acc_s);
+        else if (nc <= 4)
+          obs_pair_varying<T, 4>(tile, k, base, Gu, s_terms, c, nc, 0, acc_s);
+        else
+          obs_pair_varying<T, 16>(tile, k, base, Gu, s_terms, c, nc, 0, acc_s);
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int t = tid; t < P.n_terms; t += kObsBlock) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kObsBlock / 64; ++w) v += acc_s[w][t];
+    // pair terms count each unordered pair once: 2 Re / 2 Im (exact)
+    if (s_terms[t].kind >= 0) v *= 2.0;
+    partial[((uint64_t)item * gridDim.x + blockIdx.x) * part_terms + P.term0 + t] = v;
+  }
+}
+
+struct ObsPlan {
+  std::vector<ObsPass> passes;
+  std::vector<ObsGroup> groups;
+  std::vector<ObsTerm> terms;
+  std::vector<int> cols;  // pass-ordered term -> caller's term index
+  std::vector<int> rest;  // terms left to the per-mask kernels
+};
+
+static uint32_t pext64(uint64_t v, const uint8_t* pos, int k) {
+  uint32_t r = 0;
+  for (int j = 0; j < k; ++j) r |= (uint32_t)((v >> pos[j]) & 1ull) << j;
+  return r;
+}
+
+// Greedy cover of the distinct x masks (ordered by highest, then lowest
+// bit) by tile bit sets of k bits that always hold bits 0..lowb-1.
+static void plan_obs_tiles(int n, int n_terms, const uint64_t* xm, const uint64_t* zm,
+                           ObsPlan& P) {
+  const int k = std::min(n, kObsTileBits), lowb = std::min(k, kObsLowBits);
+  const uint64_t lowset = (1ull << lowb) - 1ull;
+  std::vector<uint64_t> xs;
+  std::map<uint64_t, std::vector<int>> by_x;
+  for (int t = 0; t < n_terms; ++t) {
+    if (!by_x.count(xm[t])) xs.push_back(xm[t]);
+    by_x[xm[t]].push_back(t);
+  }
+  std::vector<uint64_t> pair_xs;
+  for (uint64_t x : xs)
+    if (x) pair_xs.push_back(x);
+  std::stable_sort(pair_xs.begin(), pair_xs.end(), [](uint64_t a, uint64_t b) {
+    const int ha = 63 - __builtin_clzll(a), hb = 63 - __builtin_clzll(b);
+    if (ha != hb) return ha < hb;
+    return __builtin_ctzll(a) < __builtin_ctzll(b);
+  });
+  struct Draft {
+    uint64_t hi = 0;  // tile bits above the low run
+    std::vector<uint64_t> xs;
+    int n_terms = 0;
+  };
+  std::vector<Draft> drafts(1);
+  std::vector<int> diag;
+  if (by_x.count(0)) diag = by_x[0];
+  if ((int)diag.size() > kObsMaxTerms) {
+    P.rest.insert(P.rest.end(), diag.begin() + kObsMaxTerms, diag.end());
+    diag.resize(kObsMaxTerms);
+  }
+  drafts[0].n_terms = (int)diag.size();
+  for (uint64_t x : pair_xs) {
+    const uint64_t bits = x & ~lowset;
+    const int nt = (int)by_x[x].size();
+    if (__builtin_popcountll(bits) > k - lowb || nt > kObsMaxTerms) {
+      P.rest.insert(P.rest.end(), by_x[x].begin(), by_x[x].end());
+      continue;
+    }
+    Draft& d = drafts.back();
+    if (__builtin_popcountll(d.hi | bits) <= k - lowb && d.n_terms + nt <= kObsMaxTerms) {
+      d.hi |= bits;
+      d.xs.push_back(x);
+      d.n_terms += nt;
+    } else {
+      Draft nd;
+      nd.hi = bits;
+      nd.xs.push_back(x);
+      nd.n_terms = nt;
+      drafts.push_back(nd);
+    }
+  }
+  for (size_t di = 0; di < drafts.size(); ++di) {
+    Draft& d = drafts[di];
+    if (di > 0 && d.xs.empty()) continue;
+    if (di == 0 && d.xs.empty() && diag.empty()) continue;
+    // fill the tile up to k bits with the lowest free positions
+    uint64_t hi = d.hi;
+    for (int b = lowb; b < n && __builtin_popcountll(hi) < k - lowb; ++b) hi |= 1ull << b;
+    ObsPass ps;
+    std::memset(&ps, 0, sizeof(ps));
+    ps.k = k;
+    ps.lowb = lowb;
+    int j = 0, c = 0;
+    for (int b = 0; b < n; ++b) {
+      if (b < lowb || ((hi >> b) & 1))
+        ps.tpos[j++] = (uint8_t)b;
+      else
+        ps.cpos[c++] = (uint8_t)b;
+    }
+    ps.n_cbits = c;
+    ps.group0 = (int)P.groups.size();
+    ps.term0 = (int)P.terms.size();
+    auto push_term = [&](uint64_t x, int t) {
+      ObsTerm tm;
+      tm.z = zm[t];
+      tm.lz = pext64(zm[t], ps.tpos, k);
+      tm.kind = x ? (__builtin_popcountll(x & zm[t]) & 3) : -1;
+      P.terms.push_back(tm);
+      P.cols.push_back(t);
+    };
+    auto add_group = [&](uint64_t x, const std::vector<int>& ts) {
+      if (x == 0) {
+        // one group per value of the z bits above the low kObsBlockBits tile bits
+        std::vector<uint32_t> ms;
+        std::map<uint32_t, std::vector<int>> by_m;
+        for (int t : ts) {
+          const uint32_t m = pext64(zm[t], ps.tpos, k) >> kObsBlockBits;
+          if (!by_m.count(m)) ms.push_back(m);
+          by_m[m].push_back(t);
+        }
+        for (uint32_t m : ms) {
+          ObsGroup g{};
+          g.m = m;
+          g.t0 = (int)P.terms.size();
+          g.nt = (int)by_m[m].size();
+          for (int t : by_m[m]) push_term(0, t);
+          P.groups.push_back(g);
+        }
+        return;
+      }
+      ObsGroup g{};
+      g.lx = pext64(x, ps.tpos, k);
+      g.hl = 31 - __builtin_clz(g.lx);
+      g.t0 = (int)P.terms.size();
+      g.nt = (int)ts.size();
+      const uint32_t keep = ~(1u << g.hl);
+      for (int pass_const = 1; pass_const >= 0; --pass_const)
+        for (int t : ts) {
+          const bool is_const = (pext64(zm[t], ps.tpos, k) & keep) == 0;
+          if (is_const == (bool)pass_const) {
+            push_term(x, t);
+            g.nconst += is_const;
+          }
+        }
+      P.groups.push_back(g);
+    };
+    if (di == 0 && !diag.empty()) add_group(0, diag);
+    for (uint64_t x : d.xs) add_group(x, by_x[x]);
+    ps.n_groups = (int)P.groups.size() - ps.group0;
+    ps.n_terms = (int)P.terms.size() - ps.term0;
+    P.passes.push_back(ps);
+  }
+}
+
 static int red_blocks(uint64_t work) {
   uint64_t b = (work + kRedBlock * 8 - 1) / (kRedBlock * 8);
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(b, 2048));
@@ -133,22 +524,80 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
                     double* out_host, int ref_mode) {
   smoq_ctx* ctx = s->ctx;
   const int B = s->batch;
-  // group term indices by x mask, keeping first-appearance order
+  if (n_terms > 4096) return set_error(SMOQ_EINVAL, "smoq_expect_pauli: more than 4096 terms");
+  // tiled passes when they read the state fewer times than one launch per
+  // x mask would (true values only; the reference-rule path stays per mask)
+  ObsPlan plan;
+  bool tiled = false;
+  if (!ref_mode && ctx->obs_tiles) {
+    plan_obs_tiles(s->n, n_terms, xm, zm, plan);
+    auto launches = [&](const std::vector<int>& ts) {
+      std::map<uint64_t, int> per_x;
+      for (int t : ts) per_x[xm[t]]++;
+      size_t n = 0;
+      for (auto& kv : per_x) n += (kv.second + kRedTerms - 1) / kRedTerms;
+      return n;
+    };
+    std::vector<int> all(n_terms);
+    for (int t = 0; t < n_terms; ++t) all[t] = t;
+    tiled = !plan.passes.empty() && plan.passes.size() + launches(plan.rest) < launches(all);
+  }
+  std::vector<int> plain_terms;
+  if (tiled) {
+    plain_terms = plan.rest;
+  } else {
+    plain_terms.resize(n_terms);
+    for (int t = 0; t < n_terms; ++t) plain_terms[t] = t;
+  }
+  // group the per-mask terms by x mask, keeping first-appearance order
   std::vector<uint64_t> xs;
   std::map<uint64_t, std::vector<int>> groups;
-  for (int t = 0; t < n_terms; ++t) {
+  for (int t : plain_terms) {
     if (!groups.count(xm[t])) xs.push_back(xm[t]);
     groups[xm[t]].push_back(t);
   }
   const int maxblocks = 2048;
+  const int tile_terms = tiled ? (int)plan.terms.size() : 0;
+  const uint64_t n_tiles = tiled ? (s->dim >> plan.passes[0].k) : 0;
+  const int tile_wg = tiled ? (int)std::max<uint64_t>(
+                                  1, std::min<uint64_t>(n_tiles, (4096 + B - 1) / B))
+                            : 0;
   const size_t part_bytes = (size_t)B * maxblocks * kRedTerms * sizeof(double);
+  const size_t tpart_bytes = (size_t)B * tile_wg * tile_terms * sizeof(double);
   const size_t out_bytes = (size_t)B * n_terms * sizeof(double);
-  const size_t col_bytes = 4096 * sizeof(int);
-  int rc = ensure_scratch(ctx, part_bytes + out_bytes + col_bytes + 256);
+  const size_t col_bytes = 2 * 4096 * sizeof(int);
+  const size_t grp_bytes = plan.groups.size() * sizeof(ObsGroup);
+  const size_t trm_bytes = plan.terms.size() * sizeof(ObsTerm);
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  int rc = ensure_scratch(ctx, up(part_bytes) + up(tpart_bytes) + up(out_bytes) + up(col_bytes) +
+                                   up(grp_bytes) + up(trm_bytes));
   if (rc) return rc;
-  double* partial = (double*)ctx->scratch;
-  double* out_dev = partial + (size_t)B * maxblocks * kRedTerms;
-  int* cols = (int*)(out_dev + (size_t)B * n_terms);
+  char* sc = (char*)ctx->scratch;
+  double* partial = (double*)sc;
+  double* tpartial = (double*)(sc += up(part_bytes));
+  double* out_dev = (double*)(sc += up(tpart_bytes));
+  int* cols = (int*)(sc += up(out_bytes));
+  int* tcols = cols + 4096;
+  ObsGroup* grp_dev = (ObsGroup*)(sc += up(col_bytes));
+  ObsTerm* trm_dev = (ObsTerm*)(sc += up(grp_bytes));
+  using V = typename CT<T>::V;
+  if (tiled) {
+    SMOQ_HIP(hipMemcpyAsync(grp_dev, plan.groups.data(), grp_bytes, hipMemcpyHostToDevice,
+                            ctx->stream));
+    SMOQ_HIP(hipMemcpyAsync(trm_dev, plan.terms.data(), trm_bytes, hipMemcpyHostToDevice,
+                            ctx->stream));
+    SMOQ_HIP(hipMemcpyAsync(tcols, plan.cols.data(), plan.cols.size() * sizeof(int),
+                            hipMemcpyHostToDevice, ctx->stream));
+    for (const ObsPass& ps : plan.passes) {
+      hipLaunchKernelGGL(k_expect_tiles<T>, dim3(tile_wg, B), dim3(kObsBlock), 0, ctx->stream,
+                         (const V*)s->d, s->dim, ps, grp_dev, trm_dev, tpartial, tile_terms);
+      SMOQ_CHECK_LAUNCH("k_expect_tiles");
+    }
+    const int nthreads = B * tile_terms;
+    hipLaunchKernelGGL(k_finish, dim3((nthreads + 255) / 256), dim3(256), 0, ctx->stream,
+                       tpartial, tile_wg, tile_terms, 1.0, out_dev, n_terms, tcols, B);
+    SMOQ_CHECK_LAUNCH("k_finish");
+  }
   // the column lists are uploaded per chunk; keep a host mirror
   std::vector<int> colh;
   std::vector<std::pair<size_t, size_t>> chunk_cols;
@@ -170,10 +619,9 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
     chunk_cols.push_back({colh.size(), c.idx.size()});
     colh.insert(colh.end(), c.idx.begin(), c.idx.end());
   }
-  if (colh.size() > 4096) return set_error(SMOQ_EINVAL, "smoq_expect_pauli: more than 4096 terms");
-  SMOQ_HIP(hipMemcpyAsync(cols, colh.data(), colh.size() * sizeof(int), hipMemcpyHostToDevice,
-                          ctx->stream));
-  using V = typename CT<T>::V;
+  if (!colh.empty())
+    SMOQ_HIP(hipMemcpyAsync(cols, colh.data(), colh.size() * sizeof(int), hipMemcpyHostToDevice,
+                            ctx->stream));
   for (size_t ci = 0; ci < chunks.size(); ++ci) {
     const Chunk& c = chunks[ci];
     const int nt = (int)c.idx.size();

@@ -320,6 +320,7 @@ int smoq_init(smoq_ctx** out, int device) {
     const int m = std::atoi(j);
     if (m >= 0 && m <= 2) ctx->jit_mode = m;
   }
+  if (const char* o = std::getenv("SMOQ_OBS_TILES")) ctx->obs_tiles = std::atoi(o) != 0;
   hipError_t se = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (se != hipSuccess) {
     delete ctx;

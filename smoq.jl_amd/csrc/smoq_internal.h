@@ -96,6 +96,8 @@ struct smoq_ctx_s {
   // run-time compiled pass kernels: 0 off, 1 automatic, 2 always
   int jit_mode = 1;
   std::unordered_map<std::string, std::shared_ptr<smoq::JitModule>> jit_cache;
+  // multi-mask Pauli term lists through tiled passes (SMOQ_OBS_TILES=0: off)
+  int obs_tiles = 1;
   // grow-only device scratch, reused across calls on this context
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -166,7 +168,9 @@ int reduce_pauli(smoq_state* s, int n_terms, const uint64_t* xm, const uint64_t*
 // d_jumps[b * jump_stride]; a zero or non-finite norm records
 // min(item * n_steps + step) in *d_fault
 int mcwf_jump_check(const smoq_state* s, int kind, double gamma_dt);
+// d_partial: mcwf_jump_partial_bytes(s) of device memory for the chunk sums
 int mcwf_jump_launch(smoq_state* s, int kind, double gamma_dt, const double* d_decay,
                      const double* d_uni, int uni_stride, int32_t* d_jumps, int jump_stride,
-                     unsigned long long* d_fault, int n_steps, int step);
+                     unsigned long long* d_fault, int n_steps, int step, double* d_partial);
+size_t mcwf_jump_partial_bytes(const smoq_state* s);
 }  // namespace smoq

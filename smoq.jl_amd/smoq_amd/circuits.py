@@ -211,3 +211,90 @@ def gate_matrix(gate: str, theta: float = 0.0) -> np.ndarray:
         "RY": np.array([[c, -s], [s, c]], np.complex128),
         "RZ": np.array([[complex(c, -s), 0], [0, complex(c, s)]], np.complex128),
     }[g]
+
+
+# ------------------------------------------------ shared benchmark circuits --
+BENCH_GATES = ("H", "X", "Y", "Z", "S", "T", "RX", "RY", "RZ", "CNOT", "CZ")
+
+
+def load_circuit_file(path: str) -> Tape:
+    """The shared random-circuit format read by load_circuit_from_file
+    (SmoQ.jl_benchmarks/benchmark_random_circuit_sampling/run_benchmark_smoq_Float64.jl:384-404):
+    a `# N=<n> …` header, then one gate per line, `NAME q [theta]` or
+    `CNOT c t` / `CZ a b`, 0-based qubits.  Gates outside H X Y Z S T RX RY
+    RZ CNOT CZ raise ValueError (the reference's map lookup fails too)."""
+    n, rows = None, []
+    with open(path, encoding="utf-8") as f:
+        for line in f:
+            line = line.strip()
+            if not line:
+                continue
+            if line.startswith("#"):
+                for tok in line[1:].split():
+                    if tok.startswith("N=") and n is None:
+                        n = int(tok[2:])
+                continue
+            rows.append(line.split())
+    if n is None:
+        raise ValueError(f"{path}: no '# N=…' header")
+    t = Tape(n)
+    for parts in rows:
+        name = parts[0].upper()
+        if name not in BENCH_GATES:
+            raise ValueError(f"{path}: unknown gate {parts[0]!r}")
+        if name in ("CNOT", "CZ"):
+            t.add(name, [int(parts[1]), int(parts[2])])
+        elif name in ("RX", "RY", "RZ"):
+            t.add(name, [int(parts[1])], float(parts[2]))
+        else:
+            t.add(name, [int(parts[1])])
+    return t
+
+
+def run_circuit_sampling(tape: Tape, uniforms, dtype: str = "c128", ctx=None):
+    """run_circuit_sampling (run_benchmark_smoq_Float64.jl:414-437) on the
+    device: |0…0>, the tape in fused passes, then one Born draw per uniform
+    with searchsortedfirst(cumsum(abs2.(ψ)), r) - 1.  The uniforms replace
+    the reference's rand() per shot.  Returns (0-based indices, bitstrings
+    [shot][qubit]); the state is freed."""
+    from .measurements import bitstrings, sample_indices
+    from .state import GPUState
+
+    psi = GPUState.basis(tape.n, 0, dtype, ctx=ctx)
+    try:
+        tape.apply(psi)
+        idx = sample_indices(psi, uniforms, "cumsum")
+    finally:
+        psi.free()
+    return idx, bitstrings(idx, tape.n)
+
+
+# angle tables of the expectation-value benchmark (bench_smoq.jl:18-21)
+_EXP_RY = (4, 3, 6, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26)
+_EXP_RX = (5, 7, 4, 9, 3, 11, 6, 13, 8, 10, 12, 14, 15, 17, 16, 19, 18, 20, 21, 22, 23, 24, 25,
+           26)
+
+
+def expectation_benchmark_circuit(N: int, layers: int = 4) -> Tape:
+    """run_circuit of the expectation-value benchmark
+    (SmoQ.jl_benchmarks/benchmark_expectation_values/bench_smoq.jl:24-37), 0-based:
+    per layer H on odd qubits, Ry(π/a_q) on all, CNOT(q, q+1) for even q,
+    S on even qubits, Rx(π/b_q) on all, CZ(q, q+1) for odd q; a and b are the
+    benchmark's angle tables (N ≤ 24)."""
+    if not 1 <= N <= len(_EXP_RY):
+        raise ValueError(f"the benchmark's angle tables cover N = 1..{len(_EXP_RY)}")
+    t = Tape(N)
+    for _ in range(layers):
+        for q in range(1, N, 2):
+            t.add("H", [q])
+        for q in range(N):
+            t.add("RY", [q], math.pi / _EXP_RY[q])
+        for q in range(0, N - 1, 2):
+            t.add("CNOT", [q, q + 1])
+        for q in range(0, N, 2):
+            t.add("S", [q])
+        for q in range(N):
+            t.add("RX", [q], math.pi / _EXP_RX[q])
+        for q in range(1, N - 1, 2):
+            t.add("CZ", [q, q + 1])
+    return t

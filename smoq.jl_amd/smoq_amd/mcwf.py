@@ -228,6 +228,16 @@ def evolve_mcwf_trotter(psi: GPUState, gates, N: int, n_traj: int, n_steps: int,
     return jumps, out
 
 
+def xxz_field_trotter_gates(N: int, dt: float, J=(1.0, 1.0, 0.5), hx: float = 1.0):
+    """The first-order step of the σ_z-dephasing demo
+    (scripts_quantum_simulations/demo_lidbladian_dm_exact_trotter_mcwf_exact_trotter_comparison.jl:64-300):
+    open XXZ chain with couplings J = (Jxx, Jyy, Jzz) and a field hx on X,
+    from precompute_trotter_gates_bitwise_cpu (bonds, then sites)."""
+    params = build_hamiltonian_parameters(N, 1, J_x_direction=tuple(J),
+                                          J_y_direction=(0.0, 0.0, 0.0), h_field=(hx, 0.0, 0.0))
+    return precompute_trotter_gates_bitwise_cpu(params, dt)
+
+
 def tfim_mcwf_gates(N: int, J: float, h: float, dt: float) -> MCWFGates:
     """The transverse-field Ising chain H = J Σ Z_i Z_{i+1} + h Σ X_i (open
     boundary; build_hamiltonian_parameters(N, 1, :TFIM_ZZ_X)) as one
@@ -240,4 +250,5 @@ def tfim_mcwf_gates(N: int, J: float, h: float, dt: float) -> MCWFGates:
 __all__ = ["MCWFGates", "MCWFGroup", "MCWFResult", "FastTrotterGate", "apply_sigma_z",
            "decay_powers", "evolve_mcwf_batched_trotter", "evolve_mcwf_groups",
            "evolve_mcwf_trotter", "jump_decay_table", "lindblad_mcwf_batched_step",
-           "lindblad_mcwf_step_trotter", "mcwf_gates", "mcwf_jump", "tfim_mcwf_gates"]
+           "lindblad_mcwf_step_trotter", "mcwf_gates", "mcwf_jump", "tfim_mcwf_gates",
+           "xxz_field_trotter_gates"]

@@ -138,3 +138,58 @@ def staggered_magnetisation(psi: GPUState) -> float:
     v = expect_pauli(psi, [0] * N, [1 << k for k in range(N)])
     sgn = np.array([(-1) ** k for k in range(N)], np.float64)
     return float(np.dot(sgn, v) / N)
+
+
+def qrc_observable_masks(L: int, n_rails: int):
+    """(xmask, zmask) of the observable list of
+    measure_all_observables_state_vector_cpu! (cpuQuantumStateObservables.jl:1483-1519),
+    in its order: X, Y, Z on every qubit k = 1..N (N = L·n_rails), then XX,
+    YY, ZZ on every intra-rail bond (site, site+1), rail by rail, then on
+    every rung (rail r site s, rail r+1 site s)."""
+    if L < 1 or n_rails < 1:
+        raise ValueError(f"L = {L}, n_rails = {n_rails}: both must be >= 1")
+    N = L * n_rails
+    xs, zs = [], []
+    for k in range(N):
+        b = 1 << k
+        xs += [b, b, 0]
+        zs += [0, b, b]
+    bonds = [(r * L + s, r * L + s + 1) for r in range(n_rails) for s in range(L - 1)]
+    bonds += [(r * L + s, (r + 1) * L + s) for r in range(n_rails - 1) for s in range(L)]
+    for i, j in bonds:
+        b = (1 << i) | (1 << j)
+        xs += [b, b, 0]
+        zs += [0, b, b]
+    return xs, zs
+
+
+def measure_all_observables_state_vector(psi: GPUState, L: int, n_rails: int) -> np.ndarray:
+    """measure_all_observables_state_vector_cpu! (cpuQuantumStateObservables.jl:1483-1519):
+    3N local values, then 3 per intra-rail bond, then 3 per rung, in the
+    reference's order; one device call evaluates all of them (the tiled
+    reduction reads the state once per group of tile bits).  Shape (n_ops,)
+    or (batch, n_ops)."""
+    if psi.n != L * n_rails:
+        raise ValueError(f"state has {psi.n} qubits, L·n_rails = {L * n_rails}")
+    xs, zs = qrc_observable_masks(L, n_rails)
+    return expect_pauli(psi, xs, zs)
+
+
+def measure_all_observables_one_pass(psi: GPUState, L: int, n_rails: int) -> np.ndarray:
+    """measure_all_observables_one_pass (cpuQuantumStateObservables.jl:1532-1638):
+    the same values in the same order as measure_all_observables_state_vector."""
+    return measure_all_observables_state_vector(psi, L, n_rails)
+
+
+def collective_spin_fast(psi: GPUState, N: int):
+    """collective_spin_fast (cpuQuantumStateObservables.jl:1888-1908):
+    (<Jx>, <Jy>, <Jz>) with Jα = ½ Σ_k σα_k, from one term list."""
+    if psi.n != N:
+        raise ValueError(f"state has {psi.n} qubits, N = {N}")
+    xs = [1 << k for k in range(N)] * 2 + [0] * N
+    zs = [0] * N + [1 << k for k in range(N)] * 2
+    v = np.asarray(expect_pauli(psi, xs, zs))
+    sums = [v[..., a * N:(a + 1) * N].sum(-1) / 2.0 for a in range(3)]
+    if psi.batch == 1:
+        return tuple(float(s) for s in sums)
+    return tuple(sums)

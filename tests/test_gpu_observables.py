@@ -131,3 +131,79 @@ def test_sampling_edge_cases(ctx):
     assert bits == [(37 >> k) & 1 for k in range(N)]
     col = st.to_numpy()
     assert col[37] == 1 and np.count_nonzero(col) == 1
+
+
+@pytest.mark.parametrize("dtype,tol", [("c128", 1e-12), ("c64", 1e-5)])
+def test_measure_all_observables_parity(ctx, dtype, tol):
+    """measure_all_observables_* on rails (L, n_rails) through the tiled
+    multi-mask reduction, against the oracle's per-observable loops
+    (_expect_X/Y/Z, _expect_XX/YY/ZZ), in the reference's order."""
+    from smoq_amd import GPUState
+    from smoq_amd import observables as ob
+
+    for L, rails in [(1, 1), (2, 1), (3, 2), (5, 1), (4, 3), (13, 1), (8, 2), (6, 3)]:
+        N = L * rails
+        psi = rand_state(N, 100 + N)
+        s = GPUState.from_numpy(psi.astype(np.complex64) if dtype == "c64" else psi)
+        ref = psi.astype(np.complex64).astype(np.complex128) if dtype == "c64" else psi
+        want = []
+        for k in range(1, N + 1):
+            want += [orc.expect_local(ref, k, N, p) for p in "xyz"]
+        bonds = [(r * L + q + 1, r * L + q + 2) for r in range(rails) for q in range(L - 1)]
+        bonds += [(r * L + q + 1, (r + 1) * L + q + 1) for r in range(rails - 1) for q in range(L)]
+        for i, j in bonds:
+            want += [orc.expect_corr(ref, i, j, N, p) for p in ("xx", "yy", "zz")]
+        got = ob.measure_all_observables_state_vector(s, L, rails)
+        assert got.shape == (3 * N + 3 * len(bonds),)
+        assert np.max(np.abs(got - np.array(want))) < tol, (L, rails)
+        one = ob.measure_all_observables_one_pass(s, L, rails)
+        np.testing.assert_array_equal(one, got)
+        jx, jy, jz = ob.collective_spin_fast(s, N)
+        assert abs(jx - 0.5 * sum(want[0:3 * N:3])) < tol
+        assert abs(jy - 0.5 * sum(want[1:3 * N:3])) < tol
+        assert abs(jz - 0.5 * sum(want[2:3 * N:3])) < tol
+        s.free()
+
+
+def test_tiled_reduction_many_masks_batched(ctx):
+    """Long term lists with many x masks (local, neighbour, random strings of
+    any weight) on batched states at N up to 20: every term against the
+    dense <ψ|P|ψ> (N <= 10) or the oracle's loops, each batch item alone."""
+    from smoq_amd import GPUState
+    from smoq_amd import observables as ob
+
+    rng = np.random.default_rng(7)
+    for N, B in [(4, 3), (10, 2), (14, 3), (17, 2), (20, 1)]:
+        xs, zs = ob.qrc_observable_masks(N, 1)
+        for _ in range(30):
+            xs.append(int(rng.integers(0, 1 << N)))
+            zs.append(int(rng.integers(0, 1 << N)))
+        psis = np.stack([rand_state(N, 1000 * N + b) for b in range(B)])
+        s = GPUState.from_numpy(psis)
+        got = ob.expect_pauli(s, xs, zs)
+        if B == 1:
+            got = got[None, :]
+        for b in range(B):
+            if N <= 10:
+                want = [np.vdot(psis[b], orc.pauli_matrix(x, z, N) @ psis[b]).real
+                        for x, z in zip(xs, zs)]
+                free = [pauli_dense_free(psis[b], x, z) for x, z in zip(xs, zs)]
+                assert np.max(np.abs(np.array(free) - want)) < 1e-13
+            else:
+                want = []
+                for k in range(1, N + 1):
+                    want += [orc.expect_local(psis[b], k, N, p) for p in "xyz"]
+                for i in range(1, N):
+                    want += [orc.expect_corr(psis[b], i, i + 1, N, p) for p in ("xx", "yy", "zz")]
+                want += [pauli_dense_free(psis[b], x, z) for x, z in zip(xs[len(want):],
+                                                                         zs[len(want):])]
+            assert np.max(np.abs(got[b] - np.array(want))) < 1e-12, (N, b)
+        s.free()
+
+
+def pauli_dense_free(psi, x, z):
+    """<ψ| i^{|x&z|} X^x Z^z |ψ> by index arithmetic (no dense matrix)."""
+    idx = np.arange(psi.size, dtype=np.int64)
+    sgn = 1.0 - 2.0 * (np.bitwise_count(idx & z) & 1)
+    ph = 1j ** (bin(x & z).count("1") & 3)
+    return float(np.real(np.vdot(psi[idx ^ x], ph * sgn * psi)))
