@@ -113,18 +113,26 @@ int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, cons
                                : launch_program_t<float>(s, prog, L, dbuf, jit);
 }
 
-int run_program(smoq_state* s, const Program& prog) {
+int run_program(smoq_state* s, const Program& prog, bool fixed_angles) {
   const ProgLayout L = program_layout(prog, s->dtype);
   int rc = ensure_prog(s->ctx, L.total);
   if (rc) return rc;
   std::shared_ptr<JitModule> jit;
+  const Program* use = &prog;
+  Program factored;
   if (jit_wanted(s->ctx, prog, s->n)) {
-    rc = jit_get(s->ctx, prog, s->dtype, jit);
+    if (fixed_angles) {
+      factored = prog;
+      factor_rotations(factored);
+      use = &factored;
+    }
+    rc = jit_get(s->ctx, *use, s->dtype, jit);
     // automatic mode keeps the precompiled kernel when compilation fails
     if (rc && s->ctx->jit_mode >= 2) return rc;
+    if (rc) use = &prog;
   }
-  if ((rc = upload_program(s->ctx, prog, s->dtype, L, s->ctx->prog))) return rc;
-  return launch_program(s, prog, L, s->ctx->prog, jit.get());
+  if ((rc = upload_program(s->ctx, *use, s->dtype, L, s->ctx->prog))) return rc;
+  return launch_program(s, *use, L, s->ctx->prog, jit.get());
 }
 
 }  // namespace smoq

@@ -73,6 +73,42 @@ __device__ __forceinline__ void reg_u1(VT<T> (&x)[kNR], const GateDev& g, bool g
   }
 }
 
+// Factored rotations (GK_U1_ROT, GK_U1_RXR) on slot A without controls:
+// one fused multiply-add per output component; the factor is applied when
+// the pass stores its tile.  FORM 0: unit diagonal, 1: unit anti-diagonal.
+template <typename T, int PT, int A, int FORM>
+__device__ __forceinline__ void reg_u1s(VT<T> (&x)[kNR], const GateDev& g) {
+  if constexpr (A >= kRegQubits) return;
+  const T t = (T)g.m[0];
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) {
+    if (j & (1 << A)) continue;
+    const int h = j | (1 << A);
+    const VT<T> a = x[j], b = x[h];
+    if constexpr (FORM == 0 && PT == GK_U1_ROT) {  // (a - t b, t a + b)
+      x[j].x = fma(-t, b.x, a.x);
+      x[j].y = fma(-t, b.y, a.y);
+      x[h].x = fma(t, a.x, b.x);
+      x[h].y = fma(t, a.y, b.y);
+    } else if constexpr (FORM == 0) {  // (a + i t b, i t a + b)
+      x[j].x = fma(-t, b.y, a.x);
+      x[j].y = fma(t, b.x, a.y);
+      x[h].x = fma(-t, a.y, b.x);
+      x[h].y = fma(t, a.x, b.y);
+    } else if constexpr (PT == GK_U1_ROT) {  // (t a - b, a + t b)
+      x[j].x = fma(t, a.x, -b.x);
+      x[j].y = fma(t, a.y, -b.y);
+      x[h].x = fma(t, b.x, a.x);
+      x[h].y = fma(t, b.y, a.y);
+    } else {  // (t a + i b, i a + t b)
+      x[j].x = fma(t, a.x, -b.y);
+      x[j].y = fma(t, a.y, b.x);
+      x[h].x = fma(t, b.x, -a.y);
+      x[h].y = fma(t, b.y, a.x);
+    }
+  }
+}
+
 template <typename T, int PT, bool CTRL>
 __device__ __forceinline__ void reg_u1_slot(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
   switch (g.a) {
@@ -269,6 +305,23 @@ __device__ __forceinline__ void pass_store(const VT<T>* lds, const PassFrame<T>&
   for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))];
 #pragma unroll
   for (int i = 0; i < kNR; ++i) f.pt[f.off[i]] = x[i];
+}
+
+// LDS -> HBM with the pass's factored-out scalar applied
+template <typename T>
+__device__ __forceinline__ void pass_store_scaled(const VT<T>* lds, const PassFrame<T>& f,
+                                                  double scale) {
+  constexpr int WB = CT<T>::kRowBits;
+  const T sc = (T)scale;
+  VT<T> x[kNR];
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))];
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) {
+    x[i].x *= sc;
+    x[i].y *= sc;
+    f.pt[f.off[i]] = x[i];
+  }
 }
 
 // One register stage: 2^R amplitudes of the thread (its lane part plus the

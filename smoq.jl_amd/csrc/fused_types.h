@@ -33,6 +33,13 @@ enum GKind : int32_t {
   GK_CZSET = 5,    // merged CZs: per-lane 16-bit negate mask
   GK_DIAGREG = 6,  // diagonal over register qubits only: m = 16 phases
   GK_DIAGLANE = 7, // diagonal over non-register qubits only: m = 2^nu phases
+  // Rotations with the real factor taken out (run-time compiled passes of
+  // circuits with fixed angles): the gate is sigma * N with N's unit
+  // entries on the diagonal (b = 0) or the anti-diagonal (b = 1), |t| <= 1
+  // in m[0]; the pass multiplies its amplitudes once by the product of the
+  // sigmas.
+  GK_U1_ROT = 8,   // [[a, -b], [b, a]] real (Ry and products of Ry)
+  GK_U1_RXR = 9,   // [[a, i q], [i q, a]] (Rx and products of Rx)
 };
 constexpr int kMaxTabBits = 3;   // non-register qubits of one phase table
 constexpr int kMaxCzShifts = 4;
@@ -82,6 +89,7 @@ struct PassDev {
   uint64_t tid_mask;      // global bits of local positions 0..k-R-1 (thread index)
   uint64_t hb[kRegQubits];  // global bits of local positions k-R..k-1
   int32_t map[kMaxTileQubits];       // local bit -> global bit
+  double scale;                      // product of the factored-out rotation scalars
 };
 
 }  // namespace smoq

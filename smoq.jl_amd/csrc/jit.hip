@@ -60,6 +60,47 @@ static std::string bit_map_expr(const std::vector<int>& dst, bool wide) {
   return o.str();
 }
 
+// Uncontrolled single-qubit gates with the structure of an Ry or Rx product
+// become factored kinds: sigma * N with |t| <= 1 in N, one fused
+// multiply-add per output component instead of a multiply and a fused
+// multiply-add; the sigmas of a pass multiply into its scale (each
+// |sigma| >= 1/sqrt(2)).  The form (unit diagonal or unit anti-diagonal)
+// is compiled into the kernel, so the source depends on the angles: only
+// circuits whose angles stay fixed across calls are rewritten.
+void factor_rotations(Program& prog) {
+  for (PassDev& pd : prog.passes) {
+    double scale = 1.0;
+    for (int s = pd.s0; s < pd.s1; ++s) {
+      for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g) {
+        GateDev& G = prog.gates[g];
+        if (G.creg != 0 || G.cglob != 0) continue;
+        const double* m = G.m;
+        double a, b;
+        int kind;
+        if (G.kind == GK_U1_REAL && m[0] == m[6] && m[2] == -m[4]) {
+          a = m[0];
+          b = m[4];
+          kind = GK_U1_ROT;
+        } else if (G.kind == GK_U1_RXT && m[0] == m[6] && m[3] == m[5]) {
+          a = m[0];
+          b = m[3];
+          kind = GK_U1_RXR;
+        } else {
+          continue;
+        }
+        const bool diag = std::fabs(a) >= std::fabs(b);
+        const double sigma = diag ? a : b, t = diag ? b / a : a / b;
+        if (!(sigma != 0.0) || !std::isfinite(t)) continue;
+        G.kind = kind;
+        G.b = diag ? 0 : 1;
+        G.m[0] = t;
+        scale *= sigma;
+      }
+    }
+    pd.scale = scale;
+  }
+}
+
 std::string jit_source(const Program& prog, int dtype) {
   std::ostringstream o;
   const char* T = dtype == SMOQ_C128 ? "double" : "float";
@@ -135,12 +176,24 @@ std::string jit_source(const Program& prog, int dtype) {
             break;
           }
           case GK_DIAGREG: o << "    reg_diagreg<double>(x, " << gd << ");\n"; break;
+          case GK_U1_ROT:
+          case GK_U1_RXR:
+            o << "    reg_u1s<double, " << (G.kind == GK_U1_ROT ? "GK_U1_ROT" : "GK_U1_RXR") << ", "
+              << G.a << ", " << G.b << ">(x, " << gd << ");\n";
+            break;
           default: o << "    reg_diaglane<double, " << G.nu << ">(x, " << gd << ", sf.gl);\n"; break;
         }
       }
       o << "    stage_store<T>(lds, sf, x);\n  }\n";
     }
-    o << "  pass_store<T>(lds, f);\n}\n";
+    bool scaled = false;
+    for (int s = pd.s0; s < pd.s1; ++s)
+      for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g)
+        scaled |= prog.gates[g].kind == GK_U1_ROT || prog.gates[g].kind == GK_U1_RXR;
+    if (scaled)
+      o << "  pass_store_scaled<T>(lds, f, pass->scale);\n}\n";
+    else
+      o << "  pass_store<T>(lds, f);\n}\n";
   }
   return o.str();
 }

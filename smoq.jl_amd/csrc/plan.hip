@@ -108,12 +108,15 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
                dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog);
   p->layout = program_layout(p->prog, dtype);
   if (jit_wanted(ctx, p->prog, n_qubits)) {
-    rc = jit_get(ctx, p->prog, dtype, p->jit);
+    Program factored = p->prog;
+    factor_rotations(factored);
+    rc = jit_get(ctx, factored, dtype, p->jit);
     // automatic mode keeps the precompiled kernel when compilation fails
     if (rc && ctx->jit_mode >= 2) {
       delete p;
       return rc;
     }
+    if (!rc) p->prog = std::move(factored);
   }
   hipError_t e = hipMalloc(&p->dev, p->layout.total);
   if (e != hipSuccess) {

@@ -295,6 +295,7 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
     PassDev pd;
     std::memset(&pd, 0, sizeof(pd));
     pd.k = k;
+    pd.scale = 1.0;
     int li = 0, ni = 0;
     int g2l[kMaxQubits];
     for (int b = 0; b < n; ++b) {

@@ -538,7 +538,9 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
       if ((rc = launch_gate(s, g))) return rc;
     return SMOQ_OK;
   }
-  return run_program(s, prog);
+  bool fixed = true;
+  for (int i = 0; i < n_ops; ++i) fixed &= ops[i].param < 0;
+  return run_program(s, prog, fixed);
 }
 
 }  // extern "C"

@@ -141,7 +141,9 @@ int time_mark(smoq_ctx* ctx);
 
 // launchers implemented in the kernel translation units
 int launch_gate(smoq_state* s, const IGate& g);
-int run_program(smoq_state* s, const Program& prog);
+// fixed_angles: the gates' angles do not change between calls with this
+// structure, so run-time compiled passes may factor the rotations
+int run_program(smoq_state* s, const Program& prog, bool fixed_angles = false);
 
 // device image of a Program: passes | stages | gates | phase tables
 struct ProgLayout {
@@ -154,6 +156,8 @@ int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, cons
                    const JitModule* jit);
 // run-time compiled pass kernels (jit.hip)
 std::string jit_source(const Program& prog, int dtype);
+// rewrites rotation gates into the factored kinds of run-time compiled passes
+void factor_rotations(Program& prog);
 bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits);
 // tile qubits of the fused passes for a gate list (automatic unless set)
 int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates);
