@@ -107,8 +107,10 @@ std::string jit_source(const Program& prog, int dtype) {
   o << "#include \"fused_ops.h\"\nusing namespace smoq;\n";
   for (size_t p = 0; p < prog.passes.size(); ++p) {
     const PassDev& pd = prog.passes[p];
+    // two 2^k-amplitude tiles per CU (LDS) need four waves per SIMD: keep the
+    // registers within that budget
     o << "extern \"C\" __global__ void __launch_bounds__(" << (1 << (pd.k - kRegQubits))
-      << ") smoq_pass_" << p << "(VT<" << T
+      << ", 4) smoq_pass_" << p << "(VT<" << T
       << ">* __restrict__ psi, int n, const PassDev* __restrict__ pass,"
          " const StageDev* __restrict__ stages, const GateDev* __restrict__ gates,"
          " const double2* __restrict__ tabs) {\n"
