@@ -123,6 +123,12 @@ int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_sta
 int smoq_state_free(smoq_state* s);
 int smoq_state_info(const smoq_state* s, int* n_qubits, int* dtype, int* batch, void** device_ptr);
 int smoq_state_set_basis(smoq_state* s, uint64_t index);
+/* Product state in every batch item: kets[4q .. 4q+3] = (re, im) of <0|q>
+ * and <1|q> for qubit q (0-based, qubit q is bit q); amplitude i is the
+ * product over qubits multiplied in qubit order, as make_product_ket
+ * (utils/cpu/cpuQuantumStatePreparation.jl:1182-1191) builds it with
+ * tensor_product_ket (:1108-1123).  Not normalised by the call. */
+int smoq_state_set_product(smoq_state* s, const double* kets);
 int smoq_state_upload(smoq_state* s, const void* host, size_t bytes);
 int smoq_state_download(const smoq_state* s, void* host, size_t bytes);
 int smoq_state_copy(smoq_state* dst, const smoq_state* src);
@@ -188,6 +194,16 @@ int smoq_hpsi(const smoq_state* in, smoq_state* out, int n_terms, const uint64_t
 int smoq_chebyshev_evolve(smoq_state* s, int n_terms, const uint64_t* xmask,
                           const uint64_t* zmask, const double* coeff, double t, double emin,
                           double emax, double tol, int max_order, int* orders_used);
+/* estimate_spectral_range_lanczos (…Chebyshev.jl:166-220) with the start
+ * vector supplied (the reference draws an unseeded randn vector): k_iter
+ * Lanczos steps of H on start / ||start|| (start itself is not changed),
+ * stopping early when beta < 1e-12; *e_min, *e_max are the extreme
+ * eigenvalues of the tridiagonal matrix of the alphas and betas,
+ * *iters_used the number of alphas.  One state (batch 1); allocates three
+ * vectors of its size. */
+int smoq_lanczos_range(const smoq_state* start, int n_terms, const uint64_t* xmask,
+                       const uint64_t* zmask, const double* coeff, int k_iter, double* e_min,
+                       double* e_max, int* iters_used);
 
 /* ---- Monte Carlo wave-function trajectories (σ⁻ jumps) -------------------
  * One group = the trajectories of one qubit count: `psi` holds them as batch

@@ -290,6 +290,33 @@ def trotter_step(psi, gates, N):
     return psi
 
 
+def lanczos_range(psi0, N, bonds, fields, k_iter=30):
+    """estimate_spectral_range_lanczos (utils/cpu/cpuQuantumChannelUnitaryEvolutionChebyshev.jl:166-220)
+    from the given start vector (the reference draws randn): H·v by the C
+    pull loop (or_hpsi), the same recurrence and break rule, eigenvalues of
+    SymTridiagonal(alphas, betas) by scipy.  Returns (E_min, E_max, steps)."""
+    from scipy.linalg import eigh_tridiagonal
+
+    v = np.asarray(psi0, np.complex128) / np.linalg.norm(psi0)
+    v_prev = np.zeros_like(v)
+    alphas, betas = [], []
+    for j in range(k_iter):
+        w = hpsi(v, N, bonds, fields)
+        if j > 0:
+            w = w - betas[-1] * v_prev
+        alpha = float(np.vdot(v, w).real)
+        alphas.append(alpha)
+        w = w - alpha * v
+        beta = float(np.linalg.norm(w))
+        if beta < 1e-12:
+            break
+        betas.append(beta)
+        v_prev = v
+        v = w / beta
+    ev = eigh_tridiagonal(np.array(alphas), np.array(betas[:len(alphas) - 1]), eigvals_only=True)
+    return float(ev.min()), float(ev.max()), len(alphas)
+
+
 def chebyshev_evolve(psi, N, bonds, fields, t, emin, emax, tol=1e-12, max_order=1000):
     """chebyshev_evolve_psi! (utils/cpu/cpuQuantumChannelUnitaryEvolutionChebyshev.jl:262-346)
     with supplied bounds; H·ψ by the C pull loop (or_hpsi), Bessel values
@@ -411,6 +438,22 @@ def run_circuit_sampling(N: int, ops, r):
     psi[0] = 1.0
     run_fused_circuit(psi, fuse_gates(ops), N)
     return psi, sample_cumsum_search(psi, N, r)
+
+# ---------------------------------------------------------- preparation --
+def product_ket(kets):
+    """make_product_ket (utils/cpu/cpuQuantumStatePreparation.jl:1182-1191):
+    ψ = ket_1, then tensor_product_ket(ψ, ket_k, k-1, 1) (:1108-1123), i.e.
+    ψ_total[i_A + (i_B << N_A)] = ψ_A[i_A] ψ_B[i_B]."""
+    kets = [np.asarray(k, np.complex128) for k in kets]
+    psi = kets[0].copy()
+    for k in kets[1:]:
+        NA = psi.size
+        out = np.empty(NA * 2, np.complex128)
+        for b in range(2):
+            out[b * NA:(b + 1) * NA] = psi * k[b]
+        psi = out
+    return psi
+
 
 # ------------------------------------------------------ dense references --
 def dense_1q(U, q: int, N: int):

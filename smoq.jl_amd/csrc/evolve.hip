@@ -130,6 +130,206 @@ static int upload_terms(smoq_ctx* ctx, const std::vector<TermDev>& tv, TermDev**
   return SMOQ_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Lanczos (estimate_spectral_range_lanczos, …Chebyshev.jl:166-220): w = H v,
+// w -= beta_prev v_prev, alpha = Re<v|w>, w -= alpha v, beta = ||w||,
+// v_prev = v, v = w / beta.  Block partial sums, added on the host in block
+// order (deterministic).
+constexpr int kLzBlock = 256;
+
+__device__ inline double lz_block_sum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kLzBlock / 64; ++w) t += sh[w];
+  return t;
+}
+
+// w -= beta * vp; partial[block] = sum Re(conj(v) w)
+template <typename T>
+__global__ void __launch_bounds__(kLzBlock)
+    k_lz_orth_prev(VT_alias<T>* __restrict__ w, const VT_alias<T>* __restrict__ v,
+                   const VT_alias<T>* __restrict__ vp, uint64_t dim, double beta,
+                   double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  __shared__ double sh[kLzBlock / 64];
+  double acc = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kLzBlock + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * kLzBlock) {
+    VT_alias<T> x = w[i];
+    double xr = (double)x.x, xi = (double)x.y;
+    if (beta != 0.0) {
+      const VT_alias<T> p = vp[i];
+      x = make_c<T>(xr - beta * (double)p.x, xi - beta * (double)p.y);
+      w[i] = x;
+      xr = (double)x.x;
+      xi = (double)x.y;
+    }
+    const VT_alias<T> a = v[i];
+    acc += (double)a.x * xr + (double)a.y * xi;
+  }
+  const double t = lz_block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// w -= alpha * v; partial[block] = sum |w|^2
+template <typename T>
+__global__ void __launch_bounds__(kLzBlock)
+    k_lz_orth_cur(VT_alias<T>* __restrict__ w, const VT_alias<T>* __restrict__ v, uint64_t dim,
+                  double alpha, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  __shared__ double sh[kLzBlock / 64];
+  double acc = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kLzBlock + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * kLzBlock) {
+    const VT_alias<T> x = w[i], a = v[i];
+    const VT_alias<T> y = make_c<T>((double)x.x - alpha * (double)a.x,
+                                    (double)x.y - alpha * (double)a.y);
+    w[i] = y;
+    acc += (double)y.x * (double)y.x + (double)y.y * (double)y.y;
+  }
+  const double t = lz_block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// partial[block] = sum |v|^2
+template <typename T>
+__global__ void __launch_bounds__(kLzBlock)
+    k_lz_norm2(const VT_alias<T>* __restrict__ v, uint64_t dim, double* __restrict__ partial) {
+  __shared__ double sh[kLzBlock / 64];
+  double acc = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kLzBlock + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * kLzBlock) {
+    const VT_alias<T> a = v[i];
+    acc += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+  }
+  const double t = lz_block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// vp = v; v = w * inv
+template <typename T>
+__global__ void __launch_bounds__(kLzBlock)
+    k_lz_next(const VT_alias<T>* __restrict__ w, VT_alias<T>* __restrict__ v,
+              VT_alias<T>* __restrict__ vp, uint64_t dim, double inv) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kLzBlock + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * kLzBlock) {
+    vp[i] = v[i];
+    const VT_alias<T> x = w[i];
+    v[i] = make_c<T>((double)x.x * inv, (double)x.y * inv);
+  }
+}
+
+// eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm count)
+static int sturm_count(const std::vector<double>& d, const std::vector<double>& e, double x) {
+  int cnt = 0;
+  double q = 1.0;
+  for (size_t i = 0; i < d.size(); ++i) {
+    const double off = i ? e[i - 1] * e[i - 1] : 0.0;
+    q = d[i] - x - (i ? off / q : 0.0);
+    if (q == 0.0) q = -1e-300;
+    if (q < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+// k-th smallest eigenvalue (0-based) by bisection on the Gershgorin interval
+static double tridiag_eig(const std::vector<double>& d, const std::vector<double>& e, int k) {
+  double lo = 1e300, hi = -1e300;
+  for (size_t i = 0; i < d.size(); ++i) {
+    const double r = (i ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < d.size() ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(lo) + std::fabs(hi));
+       ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (sturm_count(d, e, mid) > k)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+template <typename T>
+static int lanczos_t(const smoq_state* start, const TermDev* dterms, int nt, int k_iter,
+                     double* e_min, double* e_max, int* iters) {
+  smoq_ctx* ctx = start->ctx;
+  const uint64_t dim = start->dim;
+  const size_t bytes = dim * sizeof(VT_alias<T>);
+  const int nb = (int)std::max<uint64_t>(1, std::min<uint64_t>((dim + kLzBlock - 1) / kLzBlock,
+                                                               2048));
+  unsigned char* buf = nullptr;
+  SMOQ_HIP(hipMalloc(&buf, 3 * bytes + nb * sizeof(double)));
+  VT_alias<T>* v = (VT_alias<T>*)buf;
+  VT_alias<T>* vp = (VT_alias<T>*)(buf + bytes);
+  VT_alias<T>* w = (VT_alias<T>*)(buf + 2 * bytes);
+  double* partial = (double*)(buf + 3 * bytes);
+  std::vector<double> hp(nb), alphas, betas;
+  smoq_state sv = *start;  // view of v for the H·v launches
+  sv.d = v;
+  int rc = SMOQ_OK;
+  auto sum_partial = [&](double& out) -> int {
+    SMOQ_HIP(hipMemcpyAsync(hp.data(), partial, nb * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+    out = 0.0;
+    for (int b = 0; b < nb; ++b) out += hp[b];
+    return SMOQ_OK;
+  };
+  do {
+    // v = start / ||start||
+    SMOQ_HIP(hipMemcpyAsync(v, start->d, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_lz_norm2<T>, dim3(nb), dim3(kLzBlock), 0, ctx->stream, v, dim, partial);
+    SMOQ_CHECK_LAUNCH("k_lz_norm2");
+    double n2 = 0.0;
+    if ((rc = sum_partial(n2))) break;
+    if (!(n2 > 0.0) || !std::isfinite(n2)) {
+      rc = set_error(SMOQ_EINVAL, "smoq_lanczos_range: start vector has zero or non-finite norm");
+      break;
+    }
+    hipLaunchKernelGGL(k_scale<T>, dim3((unsigned)((dim + 255) / 256)), dim3(256), 0,
+                       ctx->stream, v, dim, 1.0 / std::sqrt(n2), 0.0);
+    SMOQ_CHECK_LAUNCH("k_scale");
+    for (int j = 0; j < k_iter; ++j) {
+      if ((rc = hpsi_any(&sv, v, nullptr, w, nullptr, dterms, nt, HM_PLAIN, ChebArgs{}))) break;
+      const double bprev = j > 0 ? betas.back() : 0.0;
+      hipLaunchKernelGGL(k_lz_orth_prev<T>, dim3(nb), dim3(kLzBlock), 0, ctx->stream, w, v, vp,
+                         dim, bprev, partial);
+      SMOQ_CHECK_LAUNCH("k_lz_orth_prev");
+      double alpha = 0.0;
+      if ((rc = sum_partial(alpha))) break;
+      alphas.push_back(alpha);
+      hipLaunchKernelGGL(k_lz_orth_cur<T>, dim3(nb), dim3(kLzBlock), 0, ctx->stream, w, v, dim,
+                         alpha, partial);
+      SMOQ_CHECK_LAUNCH("k_lz_orth_cur");
+      double b2 = 0.0;
+      if ((rc = sum_partial(b2))) break;
+      const double beta = std::sqrt(b2);
+      if (beta < 1e-12) break;
+      betas.push_back(beta);
+      hipLaunchKernelGGL(k_lz_next<T>, dim3(nb), dim3(kLzBlock), 0, ctx->stream, w, v, vp, dim,
+                         1.0 / beta);
+      SMOQ_CHECK_LAUNCH("k_lz_next");
+    }
+  } while (false);
+  const hipError_t se = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(buf);
+  if (rc) return rc;
+  if (se != hipSuccess) return hip_error(se, "smoq_lanczos_range");
+  // SymTridiagonal(alphas, betas): the off-diagonal is betas[0 .. m-2]
+  std::vector<double> e(betas.begin(), betas.begin() + std::max<size_t>(0, alphas.size() - 1));
+  *e_min = tridiag_eig(alphas, e, 0);
+  *e_max = tridiag_eig(alphas, e, (int)alphas.size() - 1);
+  if (iters) *iters = (int)alphas.size();
+  return SMOQ_OK;
+}
+
 }  // namespace smoq
 
 using namespace smoq;
@@ -210,4 +410,23 @@ extern "C" int smoq_chebyshev_evolve(smoq_state* s, int n_terms, const uint64_t*
     return done(set_error(SMOQ_EDEVICE, "smoq_chebyshev_evolve: k_scale launch failed"));
   if (orders_used) *orders_used = m_used;
   return done(SMOQ_OK);
+}
+
+extern "C" int smoq_lanczos_range(const smoq_state* start, int n_terms, const uint64_t* xmask,
+                                  const uint64_t* zmask, const double* coeff, int k_iter,
+                                  double* e_min, double* e_max, int* iters_used) {
+  int rc = check_state(start);
+  if (rc) return rc;
+  if (start->batch != 1)
+    return set_error(SMOQ_EINVAL, "smoq_lanczos_range: one state (batch 1), got %d",
+                     start->batch);
+  if (k_iter < 1 || !e_min || !e_max)
+    return set_error(SMOQ_EINVAL, "smoq_lanczos_range: k_iter %d or null outputs", k_iter);
+  std::vector<TermDev> tv;
+  if ((rc = check_terms(start, n_terms, xmask, zmask, coeff, tv))) return rc;
+  TermDev* d = nullptr;
+  if ((rc = upload_terms(start->ctx, tv, &d))) return rc;
+  return start->dtype == SMOQ_C128
+             ? lanczos_t<double>(start, d, n_terms, k_iter, e_min, e_max, iters_used)
+             : lanczos_t<float>(start, d, n_terms, k_iter, e_min, e_max, iters_used);
 }

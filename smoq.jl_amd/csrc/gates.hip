@@ -6,6 +6,8 @@
 // pair, so every load and store is a coalesced 16 B (c128) / 8 B (c64)
 // access.  Batch items are contiguous, so the bits above n carry the item.
 // T is the storage type; the arithmetic is complex128 for both dtypes.
+#include <cmath>
+
 #include "device_common.h"
 #include "smoq_internal.h"
 
@@ -107,7 +109,58 @@ __global__ void __launch_bounds__(256)
   psi[i] = make_c<T>((i & dim_mask) == index ? 1.0 : 0.0, 0.0);
 }
 
+// Product state: amplitude i = c_1[b_1] c_2[b_2] ... c_n[b_n] with b_k bit
+// k-1 of i, multiplied left to right as tensor_product_ket builds it
+// (utils/cpu/cpuQuantumStatePreparation.jl:1108-1123,1182-1191).
+struct ProductKets {
+  double2 c[kMaxQubits][2];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_set_product(VT_alias<T>* __restrict__ psi, uint64_t total, int n, ProductKets pk) {
+#pragma clang fp contract(off)
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t k = i & ((1ull << n) - 1ull);
+  double2 p = pk.c[0][k & 1];
+  for (int q = 1; q < n; ++q) {
+    const double2 c = pk.c[q][(k >> q) & 1];
+    p = make_double2(p.x * c.x - p.y * c.y, p.x * c.y + p.y * c.x);
+  }
+  psi[i] = make_c<T>(p.x, p.y);
+}
+
 }  // namespace smoq
+
+extern "C" int smoq_state_set_product(smoq_state* s, const double* kets) {
+  using namespace smoq;
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!kets) return set_error(SMOQ_EINVAL, "smoq_state_set_product: null kets");
+  if (s->n > kMaxQubits)
+    return set_error(SMOQ_ENOTSUP, "smoq_state_set_product: %d qubits (max %d)", s->n,
+                     kMaxQubits);
+  ProductKets pk;
+  std::memset(&pk, 0, sizeof(pk));
+  for (int q = 0; q < s->n; ++q)
+    for (int b = 0; b < 2; ++b) {
+      const double re = kets[4 * q + 2 * b], im = kets[4 * q + 2 * b + 1];
+      if (!std::isfinite(re) || !std::isfinite(im))
+        return set_error(SMOQ_EINVAL, "smoq_state_set_product: non-finite entry for qubit %d", q);
+      pk.c[q][b] = make_double2(re, im);
+    }
+  const uint64_t total = s->dim * (uint64_t)s->batch;
+  const unsigned nb = (unsigned)((total + 255) / 256);
+  if (s->dtype == SMOQ_C128)
+    hipLaunchKernelGGL(k_set_product<double>, dim3(nb), dim3(256), 0, s->ctx->stream,
+                       (double2*)s->d, total, s->n, pk);
+  else
+    hipLaunchKernelGGL(k_set_product<float>, dim3(nb), dim3(256), 0, s->ctx->stream,
+                       (float2*)s->d, total, s->n, pk);
+  SMOQ_CHECK_LAUNCH("k_set_product");
+  return SMOQ_OK;
+}
 
 extern "C" int smoq_state_set_basis(smoq_state* s, uint64_t index) {
   using namespace smoq;

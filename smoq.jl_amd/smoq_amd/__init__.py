@@ -10,7 +10,8 @@ import os
 import subprocess
 
 from ._lib import LIB_PATH, SmoqError, lib  # noqa: F401
-from .state import Context, GPUState, default_context  # noqa: F401
+from .state import (Context, GPUState, default_context, make_initial_psi,  # noqa: F401
+                    make_product_ket)
 
 PKG_ROOT = os.path.normpath(os.path.join(os.path.dirname(os.path.abspath(__file__)), ".."))
 

@@ -87,6 +87,7 @@ SIGNATURES = {
     "smoq_state_info": [_P, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
                         ctypes.POINTER(_P)],
     "smoq_state_set_basis": [_P, _u64],
+    "smoq_state_set_product": [_P, _P],
     "smoq_state_upload": [_P, _P, _sz],
     "smoq_state_download": [_P, _P, _sz],
     "smoq_state_copy": [_P, _P],
@@ -105,6 +106,7 @@ SIGNATURES = {
     "smoq_adjoint_gradient": [_P, _P, _i, _P, _i, _i, _P, _P, _P, _P, _P],
     "smoq_tape_energy": [_P, _P, _i, _P, _i, _i, _P, _P, _P, _i, _P],
     "smoq_chebyshev_evolve": [_P, _i, _P, _P, _P, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
+    "smoq_lanczos_range": [_P, _i, _P, _P, _P, _i, _P, _P, _P],
     "smoq_norm2": [_P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],

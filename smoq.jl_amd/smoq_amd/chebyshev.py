@@ -37,6 +37,34 @@ def get_rescaling_factors(E_min: float, E_max: float, safety_margin: float = 0.0
     return (hi - lo) / 2.0, (hi + lo) / 2.0
 
 
+def estimate_spectral_range_lanczos(params: HamiltonianParams, k_iter: int = 30, *, seed: int = 0,
+                                    start=None, dtype: str = "c128", ctx=None):
+    """estimate_spectral_range_lanczos (…Chebyshev.jl:166-220) on the device:
+    k_iter Lanczos steps of H, then the extreme Ritz values (E_min, E_max).
+
+    The reference starts from an unseeded randn vector; here the start is
+    `start` (a GPUState, left unchanged) or, by default, a complex normal
+    vector drawn from numpy's generator with `seed`."""
+    from .state import GPUState
+
+    own = start is None
+    if own:
+        rng = np.random.default_rng(seed)
+        dim = 1 << params.N
+        v = rng.standard_normal(dim) + 1j * rng.standard_normal(dim)
+        start = GPUState.from_numpy(v.astype(np.complex64 if dtype == "c64" else np.complex128),
+                                    ctx=ctx)
+    xm, zm, c = _arrays(params)
+    lo, hi, used = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    try:
+        call("smoq_lanczos_range", start.handle, xm.size, _p(xm), _p(zm), _p(c), int(k_iter),
+             ctypes.byref(lo), ctypes.byref(hi), ctypes.byref(used))
+    finally:
+        if own:
+            start.free()
+    return lo.value, hi.value
+
+
 def chebyshev_evolve_psi(psi, params: HamiltonianParams, t: float, tol: float = 1e-12,
                          max_order: int = 1000, spectral_bounds=None):
     """chebyshev_evolve_psi! (…Chebyshev.jl:262-346): psi <- exp(-iHt) psi.

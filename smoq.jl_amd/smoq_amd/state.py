@@ -117,6 +117,15 @@ class GPUState:
     def set_basis(self, index: int) -> None:
         call("smoq_state_set_basis", self.handle, ctypes.c_uint64(index))
 
+    def set_product(self, kets) -> None:
+        """Every batch item becomes the product state of the per-qubit kets
+        (kets[q] = (<0|q>, <1|q>), qubit q is bit q): smoq_state_set_product."""
+        k = np.asarray(kets, np.complex128).reshape(-1, 2)
+        if k.shape[0] != self.n:
+            raise ValueError(f"{k.shape[0]} kets for a {self.n}-qubit state")
+        buf = np.ascontiguousarray(k.view(np.float64).reshape(-1))
+        call("smoq_state_set_product", self.handle, buf.ctypes.data_as(ctypes.c_void_p))
+
     def upload(self, psi: np.ndarray) -> None:
         a = np.ascontiguousarray(np.asarray(psi, dtype=self.np_dtype)).reshape(-1)
         if a.nbytes != self.nbytes:
@@ -153,3 +162,37 @@ class GPUState:
             self.free()
         except Exception:
             pass
+
+
+_R2 = 1.0 / np.sqrt(2.0)
+# make_ket(::Symbol) and _char_to_state_symbol
+# (utils/cpu/cpuQuantumStatePreparation.jl:133-147,250-262)
+KETS = {"zero": (1.0, 0.0), "one": (0.0, 1.0), "plus": (_R2, _R2), "minus": (_R2, -_R2)}
+_CHARS = {"0": "zero", "1": "one", "+": "plus", "-": "minus"}
+
+
+def make_product_ket(states, dtype="c128", batch: int = 1, ctx: Context | None = None) -> GPUState:
+    """make_product_ket (cpuQuantumStatePreparation.jl:1182-1191) on the device:
+    states[k] in {"zero", "one", "plus", "minus"} (or the characters 0 1 + -,
+    as make_ket(::String)) for qubit k+1 = bit k."""
+    if isinstance(states, str):
+        clean = states.replace("|", "").replace(">", "").replace("\u27e9", "").strip()
+        states = [_CHARS[c] if c in _CHARS else c for c in clean]
+    names = [str(s).lstrip(":") for s in states]
+    bad = [s for s in names if s not in KETS]
+    if not names or bad:
+        raise ValueError(f"unknown state(s) {bad or names}: use zero, one, plus, minus")
+    psi = GPUState(len(names), dtype, batch, ctx=ctx)
+    psi.set_product([KETS[s] for s in names])
+    return psi
+
+
+def make_initial_psi(N: int, init_bits=(), dtype="c128", batch: int = 1,
+                     ctx: Context | None = None) -> GPUState:
+    """make_initial_psi (cpuQuantumStatePreparation.jl:1211-1229): the basis
+    state with qubit k in init_bits[k-1] (little-endian), |0…0> when empty."""
+    bits = list(init_bits)
+    if bits and len(bits) != N:
+        raise ValueError("init_bits must have length N")
+    idx = sum(1 << k for k, b in enumerate(bits) if b == 1)
+    return GPUState.basis(N, idx, dtype, batch, ctx=ctx)

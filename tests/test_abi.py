@@ -49,6 +49,8 @@ def test_version_and_error_paths_without_gpu():
     assert L.smoq_state_set_basis(None, 0) == _lib.SMOQ_EINVAL
     assert L.smoq_sync(None) == _lib.SMOQ_EINVAL
     assert L.smoq_mcwf_evolve(None, None, 1, 0.1, 1, 0) == _lib.SMOQ_EINVAL
+    assert L.smoq_state_set_product(None, None) == _lib.SMOQ_EINVAL
+    assert L.smoq_lanczos_range(None, 0, None, None, None, 30, None, None, None) == _lib.SMOQ_EINVAL
 
 
 def test_init_reports_missing_device_cleanly():

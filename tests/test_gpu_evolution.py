@@ -149,3 +149,37 @@ def test_trotter_converges_to_chebyshev(ctx):
         errs.append(rel(a.to_numpy(), exact))
     assert errs[1] < 1e-3
     assert 3.5 < errs[0] / errs[1] < 4.5, errs
+
+
+@pytest.mark.parametrize("Nx,Ny,k_iter", [(8, 1, 30), (3, 3, 40), (12, 1, 30), (2, 1, 30)])
+def test_lanczos_range_matches_oracle(ctx, Nx, Ny, k_iter):
+    """estimate_spectral_range_lanczos on the device against the oracle's
+    restatement from the same start vector (2D lattice with all field
+    components, and a 2-qubit case that stops early on beta < 1e-12); at
+    N <= 9 both against the dense spectrum."""
+    from smoq_amd import GPUState
+    from smoq_amd.chebyshev import estimate_spectral_range_lanczos
+    from smoq_amd.hamiltonian import build_hamiltonian_parameters, pauli_sum
+
+    N = Nx * Ny
+    params = build_hamiltonian_parameters(Nx, Ny, J_x_direction=(0.7, 0.4, 1.1),
+                                          J_y_direction=(0.2, 0.9, -0.5),
+                                          h_field=(0.3, -0.8, 0.6))
+    rng = np.random.default_rng(N)
+    v = rng.normal(size=1 << N) + 1j * rng.normal(size=1 << N)
+    bonds = [(b.i, b.j, b.Jxx, b.Jyy, b.Jzz) for b in params.x_bonds + params.y_bonds]
+    fields = np.array([[f.hx, f.hy, f.hz] for f in params.fields])
+    lo_ref, hi_ref, steps = orc.lanczos_range(v, N, bonds, fields, k_iter)
+    start = GPUState.from_numpy(v)
+    lo, hi = estimate_spectral_range_lanczos(params, k_iter, start=start)
+    assert np.max(np.abs(start.to_numpy() - v)) == 0.0  # start left unchanged
+    start.free()
+    assert abs(lo - lo_ref) < 1e-9 * max(1.0, abs(lo_ref)) and abs(hi - hi_ref) < 1e-9 * max(
+        1.0, abs(hi_ref)), (lo, lo_ref, hi, hi_ref)
+    if N <= 9:
+        xs, zs, cs = pauli_sum(params)
+        H = sum(c * orc.pauli_matrix(x, z, N) for x, z, c in zip(xs, zs, cs))
+        ev = np.linalg.eigvalsh(H)
+        assert abs(lo - ev[0]) < 1e-8 and abs(hi - ev[-1]) < 1e-8
+    if N == 2:
+        assert steps < k_iter

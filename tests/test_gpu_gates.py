@@ -274,3 +274,35 @@ def test_invalid_arguments_raise(ctx):
         t.apply(s, theta=[0.0])
     with pytest.raises(SmoqError):
         s.set_basis(16)
+
+
+def test_product_states_and_initial_psi(ctx):
+    """make_product_ket / smoq_state_set_product against the oracle's
+    restatement for random kets (batch 2, complex128 and complex64), the
+    reference's string form, and make_initial_psi."""
+    from smoq_amd import GPUState, make_initial_psi, make_product_ket
+
+    rng = np.random.default_rng(5)
+    for N in (1, 2, 7, 13):
+        kets = [rng.normal(size=2) + 1j * rng.normal(size=2) for _ in range(N)]
+        want = orc.product_ket(kets)
+        for dtype, tol in (("c128", 1e-14), ("c64", 1e-5)):
+            s = GPUState(N, dtype, batch=2)
+            s.set_product(kets)
+            got = s.to_numpy()
+            assert np.max(np.abs(got - want[None, :])) < tol * max(1.0, np.max(np.abs(want)))
+            s.free()
+    r = 1 / np.sqrt(2)
+    s = make_product_ket("0+-1")
+    want = orc.product_ket([(1, 0), (r, r), (r, -r), (0, 1)])
+    assert np.max(np.abs(s.to_numpy() - want)) < 1e-15
+    s.free()
+    s = make_product_ket(["one", "zero", "plus"])
+    assert np.max(np.abs(s.to_numpy() - orc.product_ket([(0, 1), (1, 0), (r, r)]))) < 1e-15
+    s.free()
+    s = make_initial_psi(5, [1, 0, 1, 1, 0])
+    v = s.to_numpy()
+    assert v[0b01101] == 1.0 and np.count_nonzero(v) == 1
+    s.free()
+    with pytest.raises(ValueError):
+        make_product_ket(["zero", "up"])

@@ -304,3 +304,19 @@ def test_hpsi_dense():
             + fields[f, 2] * op_on({f: Z}, N)
     psi = rand_state(N, 8)
     assert np.max(np.abs(orc.hpsi(psi, N, bonds, fields) - H @ psi)) < 1e-13
+
+
+def test_product_ket_matches_kron_chain():
+    """make_product_ket restated: little-endian, ket 1 on the lowest bit;
+    against numpy's Kronecker product taken in the opposite order."""
+    rng = np.random.default_rng(11)
+    kets = [rng.normal(size=2) + 1j * rng.normal(size=2) for _ in range(7)]
+    want = np.array([1.0 + 0j])
+    for k in kets:
+        want = np.kron(k, want)
+    got = orc.product_ket(kets)
+    assert np.max(np.abs(got - want)) < 1e-14
+    r = 1 / np.sqrt(2)
+    # |0>|+>|1> (qubit 1 = |0>): amplitudes at indices with bit0 = 0, bit2 = 1
+    v = orc.product_ket([(1, 0), (r, r), (0, 1)])
+    assert np.allclose(v, [0, 0, 0, 0, r, 0, r, 0])

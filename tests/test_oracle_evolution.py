@@ -217,3 +217,20 @@ def test_mcwf_dephasing_step_matches_dense_operators():
         assert np.max(np.abs(got - want)) < 1e-14
         sites.add(site)
     assert sites == {0, 1, 2, 3}
+
+
+@pytest.mark.parametrize("N,k_iter", [(6, 40), (3, 30)])
+def test_oracle_lanczos_matches_dense_spectrum(N, k_iter):
+    """The oracle's restatement of estimate_spectral_range_lanczos: extreme
+    Ritz values equal the dense extreme eigenvalues (XXZ chain with z fields);
+    with 2^N < k_iter the recurrence stops on beta < 1e-12."""
+    rng = np.random.default_rng(3)
+    h = rng.uniform(-1, 1, N)
+    bonds = [(i, i + 1, 1.0, 1.0, 0.5) for i in range(1, N)]
+    fields = np.array([[0.0, 0.0, hi] for hi in h])
+    v = rng.normal(size=1 << N) + 1j * rng.normal(size=1 << N)
+    lo, hi, steps = orc.lanczos_range(v, N, bonds, fields, k_iter)
+    ev = np.linalg.eigvalsh(xxz_dense(N, 1.0, 0.5, h))
+    assert abs(lo - ev[0]) < 1e-9 and abs(hi - ev[-1]) < 1e-9
+    if (1 << N) < k_iter:
+        assert steps <= (1 << N) + 1
