@@ -1,11 +1,15 @@
-"""Monte Carlo wave-function trajectories with σ⁻ jumps, with the reference's
-names (utils/cpu/cpuQuantumChannelLindbladianEvolutionMonteCarloWaveFunction.jl),
-on the device.
+"""Monte Carlo wave-function trajectories with the reference's names
+(utils/cpu/cpuQuantumChannelLindbladianEvolutionMonteCarloWaveFunction.jl),
+on the device, in two forms:
 
-Every step of every trajectory runs inside one persistent kernel launch
-(smoq_mcwf_evolve): the first-order Trotter step (diagonal bond gates, then
-site gates), the no-jump decay, the jump decision, σ⁻ and the
-renormalisation.  The reference's `rand()` per trajectory and step is
+* σ⁻ jumps with a diagonal-bond Trotter step: every step of every trajectory
+  inside one persistent kernel launch (smoq_mcwf_evolve): the first-order
+  Trotter step (diagonal bond gates, then site gates), the no-jump decay,
+  the jump decision, σ⁻ and the renormalisation;
+* any coherent step (a gate list compiled into a plan) with σ⁻ or σ_z
+  jumps (smoq_mcwf_trotter_evolve / smoq_mcwf_jump): per step the plan's
+  passes over all trajectories, then the dissipative half in two launches,
+  with observables recorded by the tiled Pauli reduction.  The reference's `rand()` per trajectory and step is
 replaced by a caller-supplied uniform (SplitMix64 by default), so a run is
 reproducible and can be replayed on the CPU.
 
