@@ -205,6 +205,25 @@ int smoq_lanczos_range(const smoq_state* start, int n_terms, const uint64_t* xma
                        const uint64_t* zmask, const double* coeff, int k_iter, double* e_min,
                        double* e_max, int* iters_used);
 
+/* ---- density matrices -----------------------------------------------------
+ * rho of N qubits is a state of 2N qubits holding vec(rho): element (i, j)
+ * at index i + 2^N j (column-major, the memory order of the reference's
+ * Matrix{ComplexF64}).  Gates act as U on qubit q and conj(U) on qubit
+ * N + q; the Lindblad dissipators of the reference are 4x4 maps on the pair
+ * (q, N + q); both run through smoq_apply_tape / plans.
+ * smoq_dm_expect_pauli: out[b * n_terms + t] = Re Tr(rho_b P_t) for Pauli
+ *   strings over the N qubits (masks as smoq_expect_pauli), replacing
+ *   _expect_X/Y/Z_dm, _expect_XX/YY/ZZ_dm and the Pauli-string forms
+ *   (utils/cpu/cpuQuantumStateObservables.jl:1138-1420); Tr(rho) is the
+ *   term with both masks 0.
+ * smoq_dm_apply_dense: rho_b <- U rho_b U^H for every item, U a single
+ *   2N-qubit state holding the dense 2^N x 2^N operator column-major, by two
+ *   rocBLAS GEMMs; the coherent half of lindblad_dm_step!
+ *   (utils/cpu/cpuQuantumChannelLindbladianEvolutionDensityMatrix.jl:105-115). */
+int smoq_dm_expect_pauli(const smoq_state* rho, int n_terms, const uint64_t* xmask,
+                         const uint64_t* zmask, double* out);
+int smoq_dm_apply_dense(smoq_state* rho, const smoq_state* u);
+
 /* ---- Monte Carlo wave-function trajectories (σ⁻ jumps) -------------------
  * One group = the trajectories of one qubit count: `psi` holds them as batch
  * items.  Each step of lindblad_mcwf_batched_step!

@@ -439,6 +439,110 @@ def run_circuit_sampling(N: int, ops, r):
     run_fused_circuit(psi, fuse_gates(ops), N)
     return psi, sample_cumsum_search(psi, N, r)
 
+# ------------------------------------------------------- density matrices --
+def dm_apply_gate_list(rho, gates, N):
+    """The coherent half of lindblad_dm_step_trotter!
+    (utils/cpu/cpuQuantumChannelLindbladianEvolutionDensityMatrix.jl:129-238):
+    every gate on every column (U on the row index, 2-qubit gates with
+    sorted qubits and U unpermuted), then conj(U) on every row."""
+    dim = 1 << N
+    for idx, U in gates:
+        for c in range(dim):
+            col = np.ascontiguousarray(rho[:, c])
+            if len(idx) == 1:
+                apply_1q(col, U, idx[0], N)
+            else:
+                apply_2q_trotter(col, U, min(idx), max(idx), N)
+            rho[:, c] = col
+    for idx, U in gates:
+        Uc = np.conj(np.asarray(U))
+        for r in range(dim):
+            row = np.ascontiguousarray(rho[r, :])
+            if len(idx) == 1:
+                apply_1q(row, Uc, idx[0], N)
+            else:
+                apply_2q_trotter(row, Uc, min(idx), max(idx), N)
+            rho[r, :] = row
+    return rho
+
+
+def dm_dissipator(rho, kind, k, gdt, N):
+    """_apply_dissipator_dm! (utils/cpu/cpuQuantumChannelLindbladianEvolution.jl:377-454)
+    for one site k (1-based), first-order step with factor gdt; sigma_z is
+    also apply_sigma_z_dissipator! (…DensityMatrix.jl:53-69) and sigma_minus
+    apply_sigma_minus_dissipator_rho! (cpuQuantumChannelGates.jl:833-854)."""
+    dim = 1 << N
+    m = 1 << (k - 1)
+    idx = np.arange(dim)
+    bi = ((idx >> (k - 1)) & 1)[:, None]
+    bj = ((idx >> (k - 1)) & 1)[None, :]
+    old = rho.copy()
+    if kind == "sigma_z":
+        rho[bi != bj] *= (1 - 2 * gdt)
+    elif kind in ("sigma_minus", "sigma_plus"):
+        src = 1 if kind == "sigma_minus" else 0
+        both = (bi == src) & (bj == src)
+        one = ((bi == src) | (bj == src)) & ~both
+        I, J = np.nonzero(both)
+        rho[I ^ m, J ^ m] += gdt * old[I, J]
+        rho[both] = old[both] * (1 - gdt)
+        rho[one] = old[one] * (1 - 0.5 * gdt)
+    elif kind == "sigma_x":
+        I, J = np.meshgrid(idx, idx, indexing="ij")
+        rho[:, :] = old + gdt * (old[I ^ m, J ^ m] - old)
+    else:
+        raise ValueError(kind)
+    return rho
+
+
+def dm_trotter_step(rho, gates, N, gamma, dt):
+    """lindblad_dm_step_trotter! (…DensityMatrix.jl:129-243): gates, then the
+    sigma_z dissipator on every site with gamma*dt."""
+    dm_apply_gate_list(rho, gates, N)
+    for k in range(1, N + 1):
+        dm_dissipator(rho, "sigma_z", k, gamma * dt, N)
+    return rho
+
+
+def dm_exact_step(rho, U, N, gamma, dt):
+    """lindblad_dm_step! (…DensityMatrix.jl:105-115): rho = U rho U', then
+    the sigma_z dissipator on every site."""
+    rho[:, :] = U @ rho @ np.conj(U).T
+    for k in range(1, N + 1):
+        dm_dissipator(rho, "sigma_z", k, gamma * dt, N)
+    return rho
+
+
+def dm_expect_local(rho, k, N, pauli):
+    """_expect_Z_dm / _expect_X_dm / _expect_Y_dm
+    (utils/cpu/cpuQuantumStateObservables.jl:1271-1340)."""
+    dim = 1 << N
+    idx = np.arange(dim)
+    low = idx[((idx >> (k - 1)) & 1) == 0]
+    if pauli == "z":
+        return float(np.sum(rho[idx, idx].real * (1 - 2 * ((idx >> (k - 1)) & 1))))
+    step = 1 << (k - 1)
+    if pauli == "x":
+        return float(np.sum(2 * rho[low, low + step].real))
+    return float(np.sum(2 * rho[low + step, low].imag))
+
+
+def dm_expect_corr(rho, i, j, N, pair):
+    """_expect_ZZ_dm / _expect_XX_dm / _expect_YY_dm (…Observables.jl:1357-1415)."""
+    dim = 1 << N
+    idx = np.arange(dim)
+    bi, bj = (idx >> (i - 1)) & 1, (idx >> (j - 1)) & 1
+    if pair == "zz":
+        return float(np.sum(rho[idx, idx].real * (1 - 2 * (bi ^ bj))))
+    s = idx[(bi == 0) & (bj == 0)]
+    si, sj = 1 << (i - 1), 1 << (j - 1)
+    s00, s01, s10, s11 = s, s + sj, s + si, s + si + sj
+    if pair == "xx":
+        return float(np.sum(rho[s00, s11].real + rho[s11, s00].real + rho[s01, s10].real
+                            + rho[s10, s01].real))
+    return float(np.sum(-2 * rho[s00, s11].real + 2 * rho[s01, s10].real))
+
+
 # ---------------------------------------------------------- preparation --
 def product_ket(kets):
     """make_product_ket (utils/cpu/cpuQuantumStatePreparation.jl:1182-1191):

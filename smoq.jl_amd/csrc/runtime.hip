@@ -338,6 +338,7 @@ int smoq_fini(smoq_ctx* ctx) {
   if (ctx->prog) (void)hipFree(ctx->prog);
   if (ctx->tree_dev) (void)hipFree(ctx->tree_dev);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  blas_release(ctx);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

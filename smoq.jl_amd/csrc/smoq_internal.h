@@ -98,6 +98,8 @@ struct smoq_ctx_s {
   std::unordered_map<std::string, std::shared_ptr<smoq::JitModule>> jit_cache;
   // multi-mask Pauli term lists through tiled passes (SMOQ_OBS_TILES=0: off)
   int obs_tiles = 1;
+  // rocBLAS handle for dense density-matrix propagation, created on first use
+  void* blas = nullptr;
   // grow-only device scratch, reused across calls on this context
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -177,4 +179,6 @@ int mcwf_jump_launch(smoq_state* s, int kind, double gamma_dt, const double* d_d
                      const double* d_uni, int uni_stride, int32_t* d_jumps, int jump_stride,
                      unsigned long long* d_fault, int n_steps, int step, double* d_partial);
 size_t mcwf_jump_partial_bytes(const smoq_state* s);
+// releases the context's rocBLAS handle (density.hip)
+void blas_release(smoq_ctx* ctx);
 }  // namespace smoq

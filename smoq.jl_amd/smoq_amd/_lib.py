@@ -107,6 +107,8 @@ SIGNATURES = {
     "smoq_tape_energy": [_P, _P, _i, _P, _i, _i, _P, _P, _P, _i, _P],
     "smoq_chebyshev_evolve": [_P, _i, _P, _P, _P, _d, _d, _d, _d, _i, ctypes.POINTER(_i)],
     "smoq_lanczos_range": [_P, _i, _P, _P, _P, _i, _P, _P, _P],
+    "smoq_dm_expect_pauli": [_P, _i, _P, _P, _P],
+    "smoq_dm_apply_dense": [_P, _P],
     "smoq_norm2": [_P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
