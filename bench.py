@@ -14,7 +14,9 @@ adjoint gradient against parameter shift (N=24, 960 parameters; 64 sets at
 N=20), the batched MCWF run (4096 trajectories, N in {12,14,16,18}, 2000
 steps, one launch), the reference's three published CPU workloads run on
 the device beside its recorded times (shared random circuits + 100 samples,
-the expectation-value benchmark, σ_z-dephasing MCWF with 1000 trajectories)
+the expectation-value benchmark, σ_z-dephasing MCWF with 1000 trajectories,
+and the same model as a density matrix with Trotter gates and with the
+dense propagator)
 and the CPU baseline (the oracle's restatement of the reference loop, one
 core, on a bounded sample of the same circuit).
 """
@@ -348,6 +350,60 @@ def reference_benchmarks(mcwf_sizes=(10, 12, 14, 16, 18, 20), n_traj=1000, mcwf_
     out["mcwf_dephasing"] = {"workload": ref["mcwf_dephasing"]["workload"],
                              "reference_source": ref["mcwf_dephasing"]["source"],
                              "trajectories": n_traj, "steps": mcwf_steps, "by_N": mc}
+    # the same model as a density matrix (vec(rho), 2N qubits): Trotter
+    # gates through a compiled plan, and the dense propagator (rocBLAS)
+    from scipy.linalg import expm
+
+    from smoq_amd import density as dm
+    from smoq_amd.hamiltonian import build_hamiltonian_parameters, pauli_sum
+
+    dmt, dme = {}, {}
+    for N in (8, 10, 12, 13):
+        gates = xxz_field_trotter_gates(N, dt)
+        xs, zs = ob.qrc_observable_masks(N, 1)
+        rho = dm.DensityMatrix.basis(N, 0)
+        dm.evolve_lindblad_dm(rho, gates, N, gamma, dt, 2, obs_every=1, xmask=xs, zmask=zs)
+        rho.vec.set_basis(0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        rec = dm.evolve_lindblad_dm(rho, gates, N, gamma, dt, mcwf_steps, obs_every=1,
+                                    xmask=xs, zmask=zs)
+        t = time.perf_counter() - t0
+        rho.free()
+        dmt[str(N)] = entry("dm_dephasing_trotter", N, t,
+                            mean_Z_final=float(np.mean(rec[-1][2:3 * N:3])))
+        if N <= 10:
+            params = build_hamiltonian_parameters(N, 1, J_x_direction=(1.0, 1.0, 0.5),
+                                                  J_y_direction=(0.0, 0.0, 0.0),
+                                                  h_field=(1.0, 0.0, 0.0))
+            px, pz, pc = pauli_sum(params)
+            idx = np.arange(1 << N)
+            H = np.zeros((1 << N, 1 << N), np.complex128)
+            for x, z, c in zip(px, pz, pc):
+                ph = 1j ** (bin(x & z).count("1") & 3)
+                sg = 1.0 - 2.0 * (np.bitwise_count(idx & z) & 1)
+                H[idx ^ x, idx] += c * ph * sg
+            U = dm.dense_operator(expm(-1j * H * dt))
+            rho = dm.DensityMatrix.basis(N, 0)
+            dm.evolve_lindblad_dm(rho, None, N, gamma, dt, 2, obs_every=1, xmask=xs, zmask=zs,
+                                  U=U)
+            rho.vec.set_basis(0)
+            ctx.sync()
+            t0 = time.perf_counter()
+            rec = dm.evolve_lindblad_dm(rho, None, N, gamma, dt, mcwf_steps, obs_every=1,
+                                        xmask=xs, zmask=zs, U=U)
+            t = time.perf_counter() - t0
+            rho.free()
+            U.free()
+            dme[str(N)] = entry("dm_dephasing_exact", N, t,
+                                mean_Z_final=float(np.mean(rec[-1][2:3 * N:3])))
+        log(f"reference workloads: DM N={N} done")
+    out["dm_dephasing_trotter"] = {"workload": ref["dm_dephasing_trotter"]["workload"],
+                                   "reference_source": ref["dm_dephasing_trotter"]["source"],
+                                   "by_N": dmt}
+    out["dm_dephasing_exact"] = {"workload": ref["dm_dephasing_exact"]["workload"],
+                                 "reference_source": ref["dm_dephasing_exact"]["source"],
+                                 "by_N": dme}
     return out
 
 

@@ -9,9 +9,10 @@ into tests/golden/reference_timings.json:
   (benchmark_random_circuit_sampling/timings_smoq_Float64.txt);
 * the 4-layer circuit + 3N local + 3(N-1) neighbour observables
   (benchmark_expectation_values/timing_smoq.txt);
-* MCWF with first-order Trotter and σ_z dephasing, 1000 trajectories
-  (scripts_quantum_simulations/demo_lidbladian_dm_vs_mcwf_exact_vs_trotter/
-  output.log, TIMING SUMMARY "MCWF+Trotter" rows).
+* MCWF with first-order Trotter and σ_z dephasing, 1000 trajectories, and
+  the same model as a density matrix with Trotter gates and with the exact
+  propagator (scripts_quantum_simulations/demo_lidbladian_dm_vs_mcwf_exact_vs_trotter/
+  output.log, TIMING SUMMARY "MCWF+Trotter", "DM+Trotter", "DM+Exact" rows).
 
 The files are data: gate lists and timings.  Run once:
 python tests/golden/copy_benchmark_circuits.py
@@ -39,12 +40,13 @@ def read_two_columns(path: str) -> dict:
     return out
 
 
-def mcwf_timings(path: str) -> dict:
+def demo_timings(path: str, method: str) -> dict:
+    """The TIMING SUMMARY rows of one method, per N (skipped sizes omitted)."""
     text = open(path, encoding="utf-8").read()
     out = {}
     for block in re.split(r"\n={10,}\n\s+N = ", text)[1:]:
         n = block.split()[0]
-        m = re.search(r"MCWF\+Trotter \|\s*([0-9.]+)\s*\|\s*([0-9.]+)", block)
+        m = re.search(re.escape(method) + r"\s*\|\s*([0-9.]+)\s*\|\s*([0-9.]+)", block)
         if m:
             out[n] = float(m.group(1))
     return out
@@ -80,9 +82,28 @@ def main() -> None:
             "workload": "XXZ chain J=(1,1,0.5), h_x=1, sigma_z dephasing gamma=0.1, "
                         "dt=0.05 x 200 first-order Trotter steps, 1000 trajectories from "
                         "|0...0>, 6 mean observables every step; ComplexF64",
-            "seconds": mcwf_timings(os.path.join(
+            "seconds": demo_timings(os.path.join(
                 REF, "scripts_quantum_simulations",
-                "demo_lidbladian_dm_vs_mcwf_exact_vs_trotter", "output.log")),
+                "demo_lidbladian_dm_vs_mcwf_exact_vs_trotter", "output.log"), "MCWF+Trotter"),
+        },
+        "dm_dephasing_trotter": {
+            "source": "scripts_quantum_simulations/demo_lidbladian_dm_vs_mcwf_exact_vs_trotter/"
+                      "output.log",
+            "workload": "the same chain and dissipator as a density matrix: 200 steps of "
+                        "lindblad_dm_step_trotter! from |0...0><0...0|, 6 mean observables "
+                        "every step; ComplexF64",
+            "seconds": demo_timings(os.path.join(
+                REF, "scripts_quantum_simulations",
+                "demo_lidbladian_dm_vs_mcwf_exact_vs_trotter", "output.log"), "DM+Trotter"),
+        },
+        "dm_dephasing_exact": {
+            "source": "scripts_quantum_simulations/demo_lidbladian_dm_vs_mcwf_exact_vs_trotter/"
+                      "output.log",
+            "workload": "200 steps of lindblad_dm_step! (dense U = exp(-iH dt), then the "
+                        "dissipator), 6 mean observables every step; ComplexF64",
+            "seconds": demo_timings(os.path.join(
+                REF, "scripts_quantum_simulations",
+                "demo_lidbladian_dm_vs_mcwf_exact_vs_trotter", "output.log"), "DM+Exact"),
         },
     }
     with open(os.path.join(HERE, "reference_timings.json"), "w") as f:
