@@ -205,6 +205,40 @@ int smoq_lanczos_range(const smoq_state* start, int n_terms, const uint64_t* xma
                        const uint64_t* zmask, const double* coeff, int k_iter, double* e_min,
                        double* e_max, int* iters_used);
 
+/* ---- single-qubit stochastic operations ------------------------------------
+ * For k = 0 .. n_qubits-1 in order and every batch item b, with the caller's
+ * uniform u[k * batch + b] in place of the reference's rand() and the record
+ * out[k * batch + b] (out may be NULL); qubits are 0-based:
+ *   SMOQ_QC_MEASURE_Z: _measure_z_single! (utils/cpu/cpuQuantumStateMeasurements.jl:224-256):
+ *     outcome 0 if u < P(0) else 1, collapse, multiply by 1/sqrt(kept
+ *     probability); record = outcome.  A zero-probability outcome (only
+ *     possible for an unnormalised state) returns SMOQ_ENUMERIC.
+ *   SMOQ_QC_DEPHASING / DEPOLARIZING / BIT_FLIP / PHASE_FLIP /
+ *   AMPLITUDE_DAMPING: the state-vector (trajectory) branches of
+ *     apply_channel_*! (utils/cpu/cpuQuantumChannelKrausOperators.jl:262-385):
+ *     Z when u < (1 - sqrt(1-p))/2 (none at p == 0); X, Y, Z when u < p/3,
+ *     2p/3, p; X when u < p; Z when u < p; a jump when u < p P(1) and
+ *     P(1) > 1e-12, which adds the |1> amplitudes onto the |0> ones as the
+ *     reference does, else the |1> amplitudes times sqrt(1-p), then division
+ *     by the norm when it exceeds 1e-12.  record = 0 none, 1 X, 2 Y, 3 Z,
+ *     4 jump.  p must lie in [0, 1].
+ *   SMOQ_QC_FORCE_ZERO / FORCE_ONE: _force_qubit_state! for :zero / :one
+ *     (…Measurements.jl:584-603, 629-635): the other half's amplitudes are
+ *     added onto the target half, then division by the norm when norm^2
+ *     exceeds 1e-15; uniforms may be NULL, record = the forced bit. */
+enum smoq_qubit_channel_kind {
+  SMOQ_QC_MEASURE_Z = 0,
+  SMOQ_QC_DEPHASING = 1,
+  SMOQ_QC_DEPOLARIZING = 2,
+  SMOQ_QC_BIT_FLIP = 3,
+  SMOQ_QC_PHASE_FLIP = 4,
+  SMOQ_QC_AMPLITUDE_DAMPING = 5,
+  SMOQ_QC_FORCE_ZERO = 6,
+  SMOQ_QC_FORCE_ONE = 7
+};
+int smoq_qubit_channel(smoq_state* s, int n_qubits, const int* qubits, int kind, double p,
+                       const double* uniforms, int32_t* out);
+
 /* ---- density matrices -----------------------------------------------------
  * rho of N qubits is a state of 2N qubits holding vec(rho): element (i, j)
  * at index i + 2^N j (column-major, the memory order of the reference's
@@ -223,6 +257,17 @@ int smoq_lanczos_range(const smoq_state* start, int n_terms, const uint64_t* xma
 int smoq_dm_expect_pauli(const smoq_state* rho, int n_terms, const uint64_t* xmask,
                          const uint64_t* zmask, double* out);
 int smoq_dm_apply_dense(smoq_state* rho, const smoq_state* u);
+/* Z measurement / forced projection of a density matrix (vec(rho) with
+ * 2N vec qubits; qubits 0-based below N), same list and uniform layout as
+ * smoq_qubit_channel:
+ *   SMOQ_QC_MEASURE_Z: _measure_z_single!(rho) (…Measurements.jl:259-295):
+ *     b = u < P(0) ? 0 : 1 with P(0) the sum of Re rho_ii over bit k = 0,
+ *     rho -> P_b rho P_b / prob with prob = P(0) or 1 - P(0), divided only
+ *     when prob > 1e-15; record = b.
+ *   SMOQ_QC_FORCE_ZERO / FORCE_ONE: _force_qubit_rho! (…Measurements.jl:686-708):
+ *     P_b rho P_b divided by its trace when that exceeds 1e-15. */
+int smoq_dm_qubit_channel(smoq_state* rho, int n_qubits, const int* qubits, int kind,
+                          const double* uniforms, int32_t* out);
 
 /* ---- Monte Carlo wave-function trajectories (σ⁻ jumps) -------------------
  * One group = the trajectories of one qubit count: `psi` holds them as batch
@@ -336,7 +381,11 @@ enum smoq_sample_semantics {
   /* first i with r <= sequential prefix, else the last index:
    * sample_from_probabilities, utils/cpu/cpuQuantumStateMeasurements.jl:203-213
    * (0-based result). */
-  SMOQ_SAMPLE_SEQUENTIAL = 1
+  SMOQ_SAMPLE_SEQUENTIAL = 1,
+  /* the same rule over the populations real(rho_ii) of a density matrix
+   * stored as vec(rho) (2N vec qubits): projective_measurement_all!(rho) and
+   * sample_state(rho) (…Measurements.jl:435-446, 505-520); out_index in [0, 2^N). */
+  SMOQ_SAMPLE_SEQUENTIAL_DIAG = 2
 };
 /* One draw per uniform r[d] from batch item 0; out_index[d] 0-based. */
 int smoq_sample(smoq_state* s, const double* uniforms, int64_t n_draws,

@@ -543,6 +543,196 @@ def dm_expect_corr(rho, i, j, N, pair):
     return float(np.sum(-2 * rho[s00, s11].real + 2 * rho[s01, s10].real))
 
 
+# --------------------------------------- noise channels and measurement --
+def _halves(N, k):
+    idx = np.arange(1 << N)
+    lo = idx[(idx & (1 << (k - 1))) == 0]
+    return lo, lo | (1 << (k - 1))
+
+
+def _abs2(v):
+    return v.real * v.real + v.imag * v.imag
+
+
+def psi_channel(psi, kind, p, k, N, u):
+    """The state-vector (trajectory) branch of one channel on qubit k (1-based)
+    with rand() = u; returns the record (0 none, 1 X, 2 Y, 3 Z, 4 jump).
+    utils/cpu/cpuQuantumChannelKrausOperators.jl: _apply_dephasing_single!
+    :262-282, _apply_depolarizing_single! :285-318, _apply_bit_flip_single!
+    :321-334, _apply_phase_flip_single! :337-347,
+    _apply_amplitude_damping_single! :350-385."""
+    lo, hi = _halves(N, k)
+
+    def x():
+        psi[lo], psi[hi] = psi[hi].copy(), psi[lo].copy()
+
+    def y():
+        a, b = psi[lo].copy(), psi[hi].copy()
+        psi[lo], psi[hi] = -1j * b, 1j * a
+
+    def z():
+        psi[hi] = -psi[hi]
+
+    if kind == "dephasing":
+        if p == 0.0:
+            return 0
+        if u < (1.0 - np.sqrt(1.0 - p)) / 2.0:
+            z()
+            return 3
+        return 0
+    if kind == "depolarizing":
+        if u < p / 3:
+            x()
+            return 1
+        if u < 2 * p / 3:
+            y()
+            return 2
+        if u < p:
+            z()
+            return 3
+        return 0
+    if kind == "bit_flip":
+        if u < p:
+            x()
+            return 1
+        return 0
+    if kind == "phase_flip":
+        if u < p:
+            z()
+            return 3
+        return 0
+    if kind == "amplitude_damping":
+        pop1 = float(np.sum(_abs2(psi[hi])))
+        rec = 0
+        if u < p * pop1 and pop1 > 1e-12:
+            psi[lo] += psi[hi]
+            psi[hi] = 0
+            rec = 4
+        else:
+            psi[hi] *= np.sqrt(1 - p)
+        nrm = np.sqrt(np.sum(_abs2(psi)))
+        if nrm > 1e-12:
+            psi /= nrm
+        return rec
+    raise ValueError(kind)
+
+
+def dm_channel(rho, kind, p, k, N):
+    """The density-matrix update of one channel on qubit k (1-based),
+    …KrausOperators.jl: dephasing :86-103, depolarizing :107-141, amplitude
+    damping :145-181, bit flip :184-215, phase flip :218-235."""
+    if p == 0.0:
+        return rho
+    dim = 1 << N
+    m = 1 << (k - 1)
+    idx = np.arange(dim)
+    bi = ((idx >> (k - 1)) & 1)[:, None]
+    bj = ((idx >> (k - 1)) & 1)[None, :]
+    diff = np.broadcast_to(bi != bj, rho.shape)
+    I, J = np.meshgrid(idx, idx, indexing="ij")
+    old = rho.copy()
+    flip = old[I ^ m, J ^ m]
+    if kind == "dephasing":
+        rho[diff] *= np.sqrt(1 - p)
+    elif kind == "phase_flip":
+        rho[diff] *= 1 - 2 * p
+    elif kind == "depolarizing":
+        p3 = p / 3.0
+        same = ~diff
+        rho[same] = (1.0 - p) * old[same] + p3 * (2 * flip[same] + old[same])
+        rho[diff] = (1.0 - p) * old[diff] - p3 * old[diff]
+    elif kind == "bit_flip":
+        rho[:, :] = (1 - p) * old + p * flip
+    elif kind == "amplitude_damping":
+        rho[diff] *= np.sqrt(1 - p)
+        both = (bi == 1) & (bj == 1)
+        Ib, Jb = np.nonzero(both)
+        rho[Ib ^ m, Jb ^ m] += p * old[Ib, Jb]
+        rho[Ib, Jb] = old[Ib, Jb] * (1 - p)
+    else:
+        raise ValueError(kind)
+    return rho
+
+
+def measure_z(psi, k, N, u):
+    """_measure_z_single!(ψ) (utils/cpu/cpuQuantumStateMeasurements.jl:224-256):
+    outcome = u < P(0) ? 0 : 1, collapse, ψ *= 1/sqrt(kept norm^2)."""
+    lo, hi = _halves(N, k)
+    p0 = float(np.sum(_abs2(psi[lo])))
+    o = 0 if u < p0 else 1
+    kill, keep = (hi, lo) if o == 0 else (lo, hi)
+    psi[kill] = 0
+    psi *= 1.0 / np.sqrt(float(np.sum(_abs2(psi[keep]))))
+    return o
+
+
+def force_qubit(psi, k, target, N):
+    """_force_qubit_state! (…Measurements.jl:584-626) with _normalize!
+    (:629-635): the other half's amplitudes added onto the target half, ψ
+    divided by its norm when norm^2 > 1e-15; :plus / :minus then H."""
+    lo, hi = _halves(N, k)
+    b = 0 if target in ("zero", "plus") else 1
+    src, dst = (hi, lo) if b == 0 else (lo, hi)
+    psi[dst] += psi[src]
+    psi[src] = 0
+    n2 = float(np.sum(_abs2(psi)))
+    if n2 > 1e-15:
+        psi /= np.sqrt(n2)
+    if target in ("plus", "minus"):
+        a, c = psi[lo].copy(), psi[hi].copy()
+        psi[lo], psi[hi] = (a + c) / np.sqrt(2), (a - c) / np.sqrt(2)
+    return psi
+
+
+def reset_state(psi, qubits, targets, N, uniforms):
+    """reset_state!(ψ) (…Measurements.jl:555-567): per qubit a Z measurement
+    (one uniform) then the forced target."""
+    for k, t, u in zip(qubits, targets, uniforms):
+        measure_z(psi, k, N, u)
+        force_qubit(psi, k, t, N)
+    return psi
+
+
+def dm_measure_z(rho, k, N, u):
+    """_measure_z_single!(ρ) (…Measurements.jl:259-295): P(0) from the diagonal,
+    outcome = u < P(0) ? 0 : 1, rows and columns with the other bit zeroed,
+    ρ /= prob (prob = P(0) or 1 - P(0)) when prob > 1e-15."""
+    idx = np.arange(1 << N)
+    bit = (idx >> (k - 1)) & 1
+    p0 = float(np.sum(rho[idx, idx].real[bit == 0]))
+    o = 0 if u < p0 else 1
+    prob = p0 if o == 0 else 1.0 - p0
+    rho[bit != o, :] = 0
+    rho[:, bit != o] = 0
+    if prob > 1e-15:
+        rho /= prob
+    return o
+
+
+def dm_force(rho, k, b, N):
+    """_force_qubit_rho! (…Measurements.jl:686-708): P_b ρ P_b / Tr when the
+    trace exceeds 1e-15."""
+    idx = np.arange(1 << N)
+    bit = (idx >> (k - 1)) & 1
+    rho[bit != b, :] = 0
+    rho[:, bit != b] = 0
+    tr = float(np.sum(rho[idx, idx].real))
+    if tr > 1e-15:
+        rho /= tr
+    return rho
+
+
+def sample_from_probabilities(probs, r):
+    """sample_from_probabilities (…Measurements.jl:203-213), 0-based: the first
+    i with r <= running sum, else the last index."""
+    s = 0.0
+    for i, v in enumerate(probs):
+        s = s + float(v)
+        if r <= s:
+            return i
+    return len(probs) - 1
+
+
 # ---------------------------------------------------------- preparation --
 def product_ket(kets):
     """make_product_ket (utils/cpu/cpuQuantumStatePreparation.jl:1182-1191):

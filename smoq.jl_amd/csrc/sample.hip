@@ -139,14 +139,15 @@ __global__ void k_search(const double* __restrict__ c, uint64_t n, const double*
   out[d] = lo - 1;  // 0-based; equals n when r exceeds every prefix
 }
 
-// Sequential running sum, one thread (exact reproduction of the loop).
+// Sequential running sum, one thread (exact reproduction of the loop).  With
+// diag_step > 0 the terms are real(rho_ii) of vec(rho), element i * diag_step.
 template <typename T>
 __global__ void k_seq_prefix(const typename CT<T>::V* __restrict__ psi, uint64_t n,
-                             double* __restrict__ c) {
+                             double* __restrict__ c, uint64_t diag_step) {
 #pragma clang fp contract(off)
   double s = 0.0;
   for (uint64_t i = 0; i < n; ++i) {
-    s = s + prob_of<T>(psi[i]);
+    s = s + (diag_step ? (double)psi[i * diag_step].x : prob_of<T>(psi[i]));
     c[i] = s;
   }
 }
@@ -174,7 +175,9 @@ template <typename T>
 static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, uint64_t* out_host) {
   using V = typename CT<T>::V;
   smoq_ctx* ctx = s->ctx;
-  const uint64_t n = s->dim;
+  const bool diag = sem == SMOQ_SAMPLE_SEQUENTIAL_DIAG;
+  const uint64_t n = diag ? (1ull << (s->n / 2)) : s->dim;
+  const uint64_t diag_step = diag ? n + 1 : 0;
   const V* psi = (const V*)s->d;
   // scratch: c[n] | r[nd] | out[nd] | tree
   size_t off_r = n * sizeof(double);
@@ -205,10 +208,10 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   uint64_t* od = (uint64_t*)(base + off_o);
   SMOQ_HIP(hipMemcpyAsync(rd, r_host, nd * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
 
-  if (sem == SMOQ_SAMPLE_SEQUENTIAL || n == 1) {
-    hipLaunchKernelGGL(k_seq_prefix<T>, dim3(1), dim3(1), 0, ctx->stream, psi, n, c);
+  if (sem != SMOQ_SAMPLE_CUMSUM_SEARCH || n == 1) {
+    hipLaunchKernelGGL(k_seq_prefix<T>, dim3(1), dim3(1), 0, ctx->stream, psi, n, c, diag_step);
     SMOQ_CHECK_LAUNCH("k_seq_prefix");
-    if (sem == SMOQ_SAMPLE_SEQUENTIAL) {
+    if (sem != SMOQ_SAMPLE_CUMSUM_SEARCH) {
       hipLaunchKernelGGL(k_search_le, dim3(nblk(nd)), dim3(256), 0, ctx->stream, c, n, rd, nd, od);
     } else {
       hipLaunchKernelGGL(k_search, dim3(nblk(nd)), dim3(256), 0, ctx->stream, c, n, rd, nd, od);
@@ -279,8 +282,12 @@ extern "C" int smoq_sample(smoq_state* s, const double* uniforms, int64_t n_draw
   if (rc) return rc;
   if (n_draws < 0 || (n_draws > 0 && (!uniforms || !out_index)))
     return set_error(SMOQ_EINVAL, "smoq_sample: bad draw arrays");
-  if (semantics != SMOQ_SAMPLE_CUMSUM_SEARCH && semantics != SMOQ_SAMPLE_SEQUENTIAL)
+  if (semantics != SMOQ_SAMPLE_CUMSUM_SEARCH && semantics != SMOQ_SAMPLE_SEQUENTIAL &&
+      semantics != SMOQ_SAMPLE_SEQUENTIAL_DIAG)
     return set_error(SMOQ_EINVAL, "smoq_sample: unknown semantics %d", semantics);
+  if (semantics == SMOQ_SAMPLE_SEQUENTIAL_DIAG && s->n % 2)
+    return set_error(SMOQ_EINVAL, "smoq_sample: a density matrix has an even number of vec "
+                                  "qubits, got %d", s->n);
   if (n_draws == 0) return SMOQ_OK;
   if (s->dim > (1ull << 32))
     return set_error(SMOQ_EINVAL, "smoq_sample: more than 2^32 amplitudes");

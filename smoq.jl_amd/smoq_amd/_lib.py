@@ -33,6 +33,11 @@ ROTATIONS = {G["RX"], G["RY"], G["RZ"], G["CRY"], G["RXX"], G["RYY"], G["RZZ"]}
 
 SAMPLE_CUMSUM_SEARCH = 0
 SAMPLE_SEQUENTIAL = 1
+SAMPLE_SEQUENTIAL_DIAG = 2
+
+# smoq_qubit_channel_kind
+QC = dict(measure_z=0, dephasing=1, depolarizing=2, bit_flip=3, phase_flip=4,
+          amplitude_damping=5, force_zero=6, force_one=7)
 
 
 class SmoqError(RuntimeError):
@@ -109,6 +114,8 @@ SIGNATURES = {
     "smoq_lanczos_range": [_P, _i, _P, _P, _P, _i, _P, _P, _P],
     "smoq_dm_expect_pauli": [_P, _i, _P, _P, _P],
     "smoq_dm_apply_dense": [_P, _P],
+    "smoq_qubit_channel": [_P, _i, _P, _i, _d, _P, _P],
+    "smoq_dm_qubit_channel": [_P, _i, _P, _i, _P, _P],
     "smoq_norm2": [_P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
