@@ -15,8 +15,9 @@ N=20), the batched MCWF run (4096 trajectories, N in {12,14,16,18}, 2000
 steps, one launch), the reference's three published CPU workloads run on
 the device beside its recorded times (shared random circuits + 100 samples,
 the expectation-value benchmark, σ_z-dephasing MCWF with 1000 trajectories,
-and the same model as a density matrix with Trotter gates and with the
-dense propagator)
+the same model as a density matrix with Trotter gates and with the
+dense propagator, and the Kraus-channel demo's trajectory scaling table up
+to N=30)
 and the CPU baseline (the oracle's restatement of the reference loop, one
 core, on a bounded sample of the same circuit).
 """
@@ -404,7 +405,75 @@ def reference_benchmarks(mcwf_sizes=(10, 12, 14, 16, 18, 20), n_traj=1000, mcwf_
     out["dm_dephasing_exact"] = {"workload": ref["dm_dephasing_exact"]["workload"],
                                  "reference_source": ref["dm_dephasing_exact"]["source"],
                                  "by_N": dme}
+    out["channel_scaling"] = channel_scaling()
     return out
+
+
+def channel_scaling(sizes=(12, 16, 20, 24, 26, 28, 30), n_traj=100, p=0.2, seed=686,
+                    chunk_bytes=64 << 30):
+    """The scaling table of the Kraus-channel demo
+    (scripts_quantum_simulations/demo_quantum_channels.jl:686-790): per N,
+    100 trajectories of |+>^N, the dephasing channel with p = 0.2 on qubits 1
+    and N (one uniform per qubit and trajectory), then <X>, <Y>, <Z> of
+    qubits 1 and N of every trajectory; trajectories run as batches of at
+    most 64 GiB of amplitudes in one state allocated per N (allocation
+    included, its release not) and refilled with |+>^N for every batch:
+    freeing and re-allocating tens of GiB costs 0.4-1.3 s a time on the
+    device, far more than the batch's work.  The density matrix of the same channel gives the
+    exact values up to N = 12, as in the demo.  The reference prints no times
+    for this table."""
+    from smoq_amd import GPUState
+    from smoq_amd.channels import apply_channel_dephasing
+    from smoq_amd.density import DensityMatrix
+    from smoq_amd.observables import expect_pauli
+    from smoq_amd.state import KETS
+    from smoq_amd.streams import uniforms
+
+    res = {}
+    plus = KETS["plus"]
+    for N in sizes:
+        hi = 1 << (N - 1)
+        xm, zm = [1, 1, 0, hi, hi, 0], [0, 1, 1, 0, hi, hi]
+        per = int(max(1, min(n_traj, chunk_bytes // (16 << N))))
+        u = uniforms(seed + N, 2 * n_traj).reshape(n_traj, 2)
+
+        def run():
+            vals, done, states = [], 0, {}
+            while done < n_traj:
+                b = min(per, n_traj - done)
+                if b not in states:
+                    states[b] = GPUState(N, "c128", b)
+                s = states[b]
+                s.set_product([plus] * N)
+                apply_channel_dephasing(s, p, [1, N], N, u[done:done + b])
+                vals.append(np.atleast_2d(expect_pauli(s, xm, zm)))
+                done += b
+            return np.concatenate(vals), states
+
+        def release(states):
+            for s in states.values():
+                s.free()
+
+        if N == sizes[0]:
+            release(run()[1])
+        t0 = time.perf_counter()
+        v, states = run()
+        t = time.perf_counter() - t0
+        release(states)
+        e = {"seconds": t, "trajectories_per_batch": per,
+             "mean": [float(x) for x in v.mean(0)],
+             "standard_error": [float(x) for x in v.std(0) / np.sqrt(n_traj)]}
+        if N <= 12:
+            d = DensityMatrix.product(["plus"] * N)
+            apply_channel_dephasing(d, p, [1, N], N)
+            e["density_matrix"] = [float(x) for x in d.expect_pauli(xm, zm)]
+            d.free()
+        res[str(N)] = e
+        log(f"channel scaling N={N}: {t:.3f} s for {n_traj} trajectories")
+    return {"workload": "demo_quantum_channels.jl scaling table: |+>^N, dephasing p=0.2 on "
+                        "qubits 1 and N, <X,Y,Z> of qubits 1 and N per trajectory",
+            "observables": ["X1", "Y1", "Z1", "XN", "YN", "ZN"], "trajectories": n_traj,
+            "by_N": res}
 
 
 def main():

@@ -8,9 +8,11 @@
 //
 // Per qubit, all on the context stream with one synchronisation per call:
 //   k_qc_sums     P(bit 0), P(bit 1) per chunk (measurement, damping only)
-//   k_qc_apply    the decision from u and the sums (every block of an item
-//                 takes the same one), the pair rewrite, per-chunk norms
-//   k_qc_scale    division by the new norm (damping, forced reset)
+//   k_qc_decide   per item: the chunk sums in a fixed order, the decision
+//                 from u and the populations, the record
+//   k_qc_apply    the pair rewrite of every chunk, per-chunk norms
+//   k_qc_norm     per item: the norm (damping, forced reset)
+//   k_qc_scale    division by it
 // Items whose decision is "nothing" and need no renormalisation read nothing.
 #include <cmath>
 #include <vector>
@@ -76,14 +78,15 @@ __global__ void __launch_bounds__(kQcBlock)
   }
 }
 
+// the operation chosen for one item; every block of the item reads it
 struct QcDecision {
   int op;
-  double scale;  // multiplies the kept half (measurement); 1 otherwise
   int record;    // written to out[item]
+  double scale;  // multiplies the kept half (measurement); 1 otherwise
 };
 
 __device__ inline QcDecision qc_decide(int kind, double p, double u, double p0, double p1) {
-  QcDecision d{kOpNone, 1.0, 0};
+  QcDecision d{kOpNone, 0, 1.0};
   switch (kind) {
     case SMOQ_QC_MEASURE_Z: {
       // outcome = rand() < prob_0 ? 0 : 1; psi *= 1 / sqrt(kept norm)
@@ -128,42 +131,52 @@ __device__ inline QcDecision qc_decide(int kind, double p, double u, double p0, 
   return d;
 }
 
-__device__ inline bool qc_renormalises(int kind) {
+__host__ __device__ inline bool qc_renormalises(int kind) {
   return kind == SMOQ_QC_AMPLITUDE_DAMPING || kind == SMOQ_QC_FORCE_ZERO ||
          kind == SMOQ_QC_FORCE_ONE;
 }
 
-// sums == nullptr when the decision does not depend on the populations
+// Sum of the nb chunk values of one item, nb fixed per call: strided per
+// thread, then the block sum, so the order does not depend on scheduling.
+__device__ inline double qc_item_sum(const double* __restrict__ v, int nb, int stride,
+                                     double* sh) {
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kQcBlock) a += v[(size_t)b * stride];
+  return qc_block_sum(a, sh);
+}
+
+// one block per item: populations (when sums != nullptr), then the decision
+__global__ void __launch_bounds__(kQcBlock)
+    k_qc_decide(int kind, double prm, const double* __restrict__ uni,
+                const double* __restrict__ sums, int nb, QcDecision* __restrict__ dec,
+                int32_t* __restrict__ out, unsigned long long* __restrict__ fault) {
+  __shared__ double sh[kQcBlock / 64];
+  const uint64_t item = blockIdx.x;
+  double p0 = 0.0, p1 = 0.0;
+  if (sums) {
+    p0 = qc_item_sum(sums + item * nb * 2, nb, 2, sh);
+    p1 = qc_item_sum(sums + item * nb * 2 + 1, nb, 2, sh);
+  }
+  if (threadIdx.x == 0) {
+    const QcDecision d = qc_decide(kind, prm, uni[item], p0, p1);
+    dec[item] = d;
+    out[item] = d.record;
+    if (!isfinite(d.scale)) atomicMin(fault, (unsigned long long)item);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kQcBlock)
     k_qc_apply(VT_alias<T>* __restrict__ psi, int n, int lc, int q, int kind, double prm,
-               const double* __restrict__ uni, const double* __restrict__ sums,
-               double* __restrict__ normp, int32_t* __restrict__ out,
-               unsigned long long* __restrict__ fault) {
+               const QcDecision* __restrict__ dec, double* __restrict__ normp) {
   using V = VT_alias<T>;
   __shared__ double sh[kQcBlock / 64];
-  __shared__ QcDecision s_d;
   const uint64_t item = blockIdx.y, blk = blockIdx.x;
   const int nb = gridDim.x;
-  V* p = psi + (item << n);
-  if (threadIdx.x == 0) {
-    double p0 = 0.0, p1 = 0.0;
-    if (sums) {
-      for (int b = 0; b < nb; ++b) {
-        p0 += sums[(item * nb + b) * 2];
-        p1 += sums[(item * nb + b) * 2 + 1];
-      }
-    }
-    s_d = qc_decide(kind, prm, uni[item], p0, p1);
-    if (blk == 0) {
-      out[item] = s_d.record;
-      if (!isfinite(s_d.scale)) atomicMin(fault, (unsigned long long)item);
-    }
-  }
-  __syncthreads();
-  const QcDecision d = s_d;
+  const QcDecision d = dec[item];
   const bool renorm = qc_renormalises(kind);
   if (d.op == kOpNone) return;  // no renormalising kind decides "nothing"
+  V* p = psi + (item << n);
   const uint64_t m = 1ull << q;
   const uint64_t g0 = blk << lc, chunk = 1ull << lc;
   const double sq = d.op == kOpNoJump ? sqrt(1.0 - prm) : 1.0;
@@ -209,24 +222,26 @@ __global__ void __launch_bounds__(kQcBlock)
   }
 }
 
-// psi ./= norm: amplitude damping divides by sqrt(norm2) when that exceeds
-// 1e-12, the forced reset (_normalize!) when norm2 exceeds 1e-15
-template <typename T>
+// one block per item: the divisor of psi ./= norm — amplitude damping divides
+// by sqrt(norm2) when that exceeds 1e-12, the forced reset (_normalize!) when
+// norm2 exceeds 1e-15; 1 means no division
 __global__ void __launch_bounds__(kQcBlock)
-    k_qc_scale(VT_alias<T>* __restrict__ psi, int n, int lc, int kind,
-               const double* __restrict__ normp) {
-  __shared__ double s_div;
-  const uint64_t item = blockIdx.y, blk = blockIdx.x;
-  const int nb = gridDim.x;
+    k_qc_norm(int kind, const double* __restrict__ normp, int nb, double* __restrict__ div) {
+  __shared__ double sh[kQcBlock / 64];
+  const uint64_t item = blockIdx.x;
+  const double nrm = qc_item_sum(normp + item * nb, nb, 1, sh);
   if (threadIdx.x == 0) {
-    double nrm = 0.0;
-    for (int b = 0; b < nb; ++b) nrm += normp[item * nb + b];
     const double r = sqrt(nrm);
     const bool go = kind == SMOQ_QC_AMPLITUDE_DAMPING ? r > 1e-12 : nrm > 1e-15;
-    s_div = go ? r : 1.0;
+    div[item] = go ? r : 1.0;
   }
-  __syncthreads();
-  const double dv = s_div;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kQcBlock)
+    k_qc_scale(VT_alias<T>* __restrict__ psi, int n, int lc, const double* __restrict__ div) {
+  const uint64_t item = blockIdx.y, blk = blockIdx.x;
+  const double dv = div[item];
   if (dv == 1.0) return;
   VT_alias<T>* p = psi + (item << n);
   const uint64_t g0 = blk << lc, chunk = 1ull << lc;
@@ -263,23 +278,17 @@ __global__ void __launch_bounds__(kQcBlock)
   }
 }
 
-// P_b rho P_b / prob with b the outcome (measurement: b = rand() < P(0) ? 0 : 1,
-// prob = P(0) or 1 - P(0)) or the forced bit (prob = the kept trace); the
-// division happens only when prob > 1e-15
-template <typename T>
+// one block per item: the kept bit b (measurement: b = rand() < P(0) ? 0 : 1,
+// prob = P(0) or 1 - P(0); forced: prob = the kept trace) in op, the divisor
+// (prob when prob > 1e-15, else 1) in scale
 __global__ void __launch_bounds__(kQcBlock)
-    k_qc_dm_project(VT_alias<T>* __restrict__ rho, int N, int lc, int q, int kind,
-                    const double* __restrict__ uni, const double* __restrict__ sums, int nbd,
-                    int32_t* __restrict__ out) {
-  __shared__ int s_b;
-  __shared__ double s_div;
-  const uint64_t item = blockIdx.y, blk = blockIdx.x;
+    k_qc_dm_decide(int kind, const double* __restrict__ uni, const double* __restrict__ sums,
+                   int nbd, QcDecision* __restrict__ dec, int32_t* __restrict__ out) {
+  __shared__ double sh[kQcBlock / 64];
+  const uint64_t item = blockIdx.x;
+  const double p0 = qc_item_sum(sums + item * nbd * 2, nbd, 2, sh);
+  const double p1 = qc_item_sum(sums + item * nbd * 2 + 1, nbd, 2, sh);
   if (threadIdx.x == 0) {
-    double p0 = 0.0, p1 = 0.0;
-    for (int b = 0; b < nbd; ++b) {
-      p0 += sums[(item * nbd + b) * 2];
-      p1 += sums[(item * nbd + b) * 2 + 1];
-    }
     int b;
     double prob;
     if (kind == SMOQ_QC_MEASURE_Z) {
@@ -289,13 +298,19 @@ __global__ void __launch_bounds__(kQcBlock)
       b = kind == SMOQ_QC_FORCE_ONE ? 1 : 0;
       prob = b ? p1 : p0;
     }
-    s_b = b;
-    s_div = prob > 1e-15 ? prob : 1.0;
-    if (blk == 0) out[item] = b;
+    dec[item] = QcDecision{b, b, prob > 1e-15 ? prob : 1.0};
+    out[item] = b;
   }
-  __syncthreads();
-  const uint64_t b = (uint64_t)s_b;
-  const double dv = s_div;
+}
+
+// P_b rho P_b / divisor
+template <typename T>
+__global__ void __launch_bounds__(kQcBlock)
+    k_qc_dm_project(VT_alias<T>* __restrict__ rho, int N, int lc, int q,
+                    const QcDecision* __restrict__ dec) {
+  const uint64_t item = blockIdx.y, blk = blockIdx.x;
+  const uint64_t b = (uint64_t)dec[item].op;
+  const double dv = dec[item].scale;
   VT_alias<T>* p = rho + (item << (2 * N));
   const uint64_t g0 = blk << lc, chunk = 1ull << lc;
   const uint64_t low = (1ull << N) - 1;
@@ -311,12 +326,17 @@ __global__ void __launch_bounds__(kQcBlock)
   }
 }
 
-// chunks per item: at least 4096 elements each, at most 2048 blocks in all
+// Chunks per item, a power of two: items are cut into as many chunks of at
+// least 4096 elements as keep the grid within 2^16 blocks, so that even one
+// item's rewrite spreads over the whole chip; blocks of items whose decision
+// is "nothing" return at once.
 static int qc_chunks(uint64_t len, size_t batch) {
   int nb = 1;
-  while ((uint64_t)nb * 2 * 4096 <= len && (uint64_t)nb * 2 * batch <= 2048) nb *= 2;
+  while ((uint64_t)nb * 2 * 4096 <= len && (uint64_t)nb * 2 * batch <= 65536) nb *= 2;
   return nb;
 }
+
+static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 template <typename T>
 static int qc_run(smoq_state* s, int nq, const int* qubits, int kind, double prm,
@@ -326,16 +346,18 @@ static int qc_run(smoq_state* s, int nq, const int* qubits, int kind, double prm
   const size_t B = (size_t)s->batch;
   const int nb = qc_chunks(s->dim, B);
   const int lc = s->n - __builtin_ctz((unsigned)nb);
-  // scratch: fault | u[nq B] | out[nq B] | sums[B nb 2] | norms[B nb]
-  const size_t o_u = 256, o_out = o_u + ((nq * B * 8 + 255) & ~(size_t)255);
-  const size_t o_sums = o_out + ((nq * B * 4 + 255) & ~(size_t)255);
-  const size_t o_norm = o_sums + B * nb * 16;
+  // scratch: fault | u[nq B] | out[nq B] | dec[B] | div[B] | sums[B nb 2] | norms[B nb]
+  const size_t o_u = 256, o_out = o_u + up256(nq * B * 8);
+  const size_t o_dec = o_out + up256(nq * B * 4), o_div = o_dec + up256(B * sizeof(QcDecision));
+  const size_t o_sums = o_div + up256(B * 8), o_norm = o_sums + up256(B * nb * 16);
   int rc = ensure_scratch(ctx, o_norm + B * nb * 8);
   if (rc) return rc;
   unsigned char* base = (unsigned char*)ctx->scratch;
   unsigned long long* fault = (unsigned long long*)base;
   double* uni = (double*)(base + o_u);
   int32_t* out = (int32_t*)(base + o_out);
+  QcDecision* dec = (QcDecision*)(base + o_dec);
+  double* div = (double*)(base + o_div);
   double* sums = (double*)(base + o_sums);
   double* normp = (double*)(base + o_norm);
   const unsigned long long no_fault = ~0ull;
@@ -344,19 +366,22 @@ static int qc_run(smoq_state* s, int nq, const int* qubits, int kind, double prm
   SMOQ_HIP(hipMemcpyAsync(uni, uni_host, nq * B * 8, hipMemcpyHostToDevice, ctx->stream));
   const dim3 grid((unsigned)nb, (unsigned)B);
   const bool need_sums = kind == SMOQ_QC_MEASURE_Z || kind == SMOQ_QC_AMPLITUDE_DAMPING;
-  const bool renorm = kind == SMOQ_QC_AMPLITUDE_DAMPING || kind == SMOQ_QC_FORCE_ZERO ||
-                      kind == SMOQ_QC_FORCE_ONE;
+  const bool renorm = qc_renormalises(kind);
   for (int k = 0; k < nq; ++k) {
     const int q = qubits[k];
     if (need_sums)
       hipLaunchKernelGGL(k_qc_sums<T>, grid, dim3(kQcBlock), 0, ctx->stream, (const V*)s->d, s->n,
                          lc, q, sums);
+    hipLaunchKernelGGL(k_qc_decide, dim3((unsigned)B), dim3(kQcBlock), 0, ctx->stream, kind, prm,
+                       uni + k * B, need_sums ? sums : nullptr, nb, dec, out + k * B, fault);
     hipLaunchKernelGGL(k_qc_apply<T>, grid, dim3(kQcBlock), 0, ctx->stream, (V*)s->d, s->n, lc, q,
-                       kind, prm, uni + k * B, need_sums ? sums : nullptr, normp, out + k * B,
-                       fault);
-    if (renorm)
+                       kind, prm, dec, normp);
+    if (renorm) {
+      hipLaunchKernelGGL(k_qc_norm, dim3((unsigned)B), dim3(kQcBlock), 0, ctx->stream, kind, normp,
+                         nb, div);
       hipLaunchKernelGGL(k_qc_scale<T>, grid, dim3(kQcBlock), 0, ctx->stream, (V*)s->d, s->n, lc,
-                         kind, normp);
+                         div);
+    }
   }
   SMOQ_CHECK_LAUNCH("qubit channel kernels");
   if (out_host)
@@ -381,22 +406,24 @@ static int qc_dm_run(smoq_state* rho, int nq, const int* qubits, int kind, const
   const int lc = rho->n - __builtin_ctz((unsigned)nb);
   const int nbd = qc_chunks(1ull << N, B);
   const int lcd = N - __builtin_ctz((unsigned)nbd);
-  const size_t o_out = (nq * B * 8 + 255) & ~(size_t)255;
-  const size_t o_sums = o_out + ((nq * B * 4 + 255) & ~(size_t)255);
+  const size_t o_out = up256(nq * B * 8), o_dec = o_out + up256(nq * B * 4);
+  const size_t o_sums = o_dec + up256(B * sizeof(QcDecision));
   int rc = ensure_scratch(ctx, o_sums + B * nbd * 16);
   if (rc) return rc;
   unsigned char* base = (unsigned char*)ctx->scratch;
   double* uni = (double*)base;
   int32_t* out = (int32_t*)(base + o_out);
+  QcDecision* dec = (QcDecision*)(base + o_dec);
   double* sums = (double*)(base + o_sums);
   SMOQ_HIP(hipMemcpyAsync(uni, uni_host, nq * B * 8, hipMemcpyHostToDevice, ctx->stream));
   for (int k = 0; k < nq; ++k) {
     const int q = qubits[k];
     hipLaunchKernelGGL(k_qc_dm_sums<T>, dim3((unsigned)nbd, (unsigned)B), dim3(kQcBlock), 0,
                        ctx->stream, (const V*)rho->d, N, lcd, q, sums);
+    hipLaunchKernelGGL(k_qc_dm_decide, dim3((unsigned)B), dim3(kQcBlock), 0, ctx->stream, kind,
+                       uni + k * B, sums, nbd, dec, out + k * B);
     hipLaunchKernelGGL(k_qc_dm_project<T>, dim3((unsigned)nb, (unsigned)B), dim3(kQcBlock), 0,
-                       ctx->stream, (V*)rho->d, N, lc, q, kind, uni + k * B, sums, nbd,
-                       out + k * B);
+                       ctx->stream, (V*)rho->d, N, lc, q, dec);
   }
   SMOQ_CHECK_LAUNCH("density-matrix projection kernels");
   if (out_host)
@@ -407,6 +434,8 @@ static int qc_dm_run(smoq_state* rho, int nq, const int* qubits, int kind, const
 
 static int qc_check(const smoq_state* s, int n_logical, int nq, const int* qubits, const char* fn) {
   if (nq < 0 || (nq > 0 && !qubits)) return set_error(SMOQ_EINVAL, "%s: bad qubit list", fn);
+  if (s->batch > 65535)
+    return set_error(SMOQ_ENOTSUP, "%s: batch %d above 65535 items", fn, s->batch);
   for (int k = 0; k < nq; ++k)
     if (qubits[k] < 0 || qubits[k] >= n_logical)
       return set_error(SMOQ_EINVAL, "%s: qubit %d outside 0..%d", fn, qubits[k], n_logical - 1);

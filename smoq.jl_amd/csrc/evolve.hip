@@ -34,8 +34,8 @@ __global__ void __launch_bounds__(256)
            VT_alias<T>* __restrict__ acc, uint64_t total, uint64_t dim, int nt,
            const TermDev* __restrict__ terms, int mode, ChebArgs ca) {
 #pragma clang fp contract(off)
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
   const uint64_t k = i & (dim - 1);
   const uint64_t base = i - k;
   double hr = 0.0, hi = 0.0;
@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256)
   }
   if (mode == HM_PLAIN) {
     out[i] = make_c<T>(hr, hi);
-    return;
+    continue;
   }
   const VT_alias<T> x = in[i];
   const double xr = (double)x.x, xi = (double)x.y;
@@ -73,15 +73,17 @@ __global__ void __launch_bounds__(256)
     const VT_alias<T> o = acc[i];
     acc[i] = make_c<T>((double)o.x + ar, (double)o.y + ai);
   }
+  }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_scale(VT_alias<T>* __restrict__ psi, uint64_t total, double re, double im) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const VT_alias<T> v = psi[i];
-  psi[i] = make_c<T>(re * (double)v.x - im * (double)v.y, re * (double)v.y + im * (double)v.x);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const VT_alias<T> v = psi[i];
+    psi[i] = make_c<T>(re * (double)v.x - im * (double)v.y, re * (double)v.y + im * (double)v.x);
+  }
 }
 
 static int check_terms(const smoq_state* s, int n_terms, const uint64_t* x, const uint64_t* z,
@@ -104,7 +106,7 @@ template <typename T>
 static int hpsi_launch(smoq_state* s, const void* in, const void* prev, void* out, void* acc,
                        const TermDev* dterms, int nt, int mode, ChebArgs ca) {
   const uint64_t total = s->dim * (uint64_t)s->batch;
-  hipLaunchKernelGGL(k_hpsi<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(k_hpsi<T>, dim3(stride_blocks(total, 256)), dim3(256), 0,
                      s->ctx->stream, (const VT_alias<T>*)in, (const VT_alias<T>*)prev,
                      (VT_alias<T>*)out, (VT_alias<T>*)acc, total, s->dim, nt, dterms, mode, ca);
   SMOQ_CHECK_LAUNCH("k_hpsi");
@@ -293,7 +295,7 @@ static int lanczos_t(const smoq_state* start, const TermDev* dterms, int nt, int
       rc = set_error(SMOQ_EINVAL, "smoq_lanczos_range: start vector has zero or non-finite norm");
       break;
     }
-    hipLaunchKernelGGL(k_scale<T>, dim3((unsigned)((dim + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_scale<T>, dim3(stride_blocks(dim, 256)), dim3(256), 0,
                        ctx->stream, v, dim, 1.0 / std::sqrt(n2), 0.0);
     SMOQ_CHECK_LAUNCH("k_scale");
     for (int j = 0; j < k_iter; ++j) {
@@ -401,10 +403,10 @@ extern "C" int smoq_chebyshev_evolve(smoq_state* s, int n_terms, const uint64_t*
   // global phase exp(-i b t)
   const uint64_t total = s->dim * (uint64_t)s->batch;
   if (s->dtype == SMOQ_C128)
-    hipLaunchKernelGGL(k_scale<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_scale<double>, dim3(stride_blocks(total, 256)), dim3(256), 0,
                        ctx->stream, (double2*)s->d, total, std::cos(b * t), -std::sin(b * t));
   else
-    hipLaunchKernelGGL(k_scale<float>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_scale<float>, dim3(stride_blocks(total, 256)), dim3(256), 0,
                        ctx->stream, (float2*)s->d, total, std::cos(b * t), -std::sin(b * t));
   if (hipGetLastError() != hipSuccess)
     return done(set_error(SMOQ_EDEVICE, "smoq_chebyshev_evolve: k_scale launch failed"));

@@ -91,15 +91,30 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
       return set_error(SMOQ_EINVAL, "fused pass: tile of 2^%d needs %d threads (max %d)", pd.k,
                        nt, kMaxPassThreads);
     const size_t shm = sizeof(VT<T>) << pd.k;
+    // rows of 2^16 tiles in blockIdx.y once a pass has 16 or more non-tile
+    // qubits (the dispatch order is the same as one row's), so that a pass
+    // never needs 2^32 work-items in x; one row otherwise
+    uint64_t gx = tiles, gy = 1;
+    if (pd.n_nonlocal >= 16) {
+      gx = 1ull << 16;
+      gy = tiles >> 16;
+    }
+    if (gy > 65535) {
+      gx = tiles;
+      gy = 1;
+    }
+    if (gx * (uint64_t)nt >= (1ull << 32))
+      return set_error(SMOQ_ENOTSUP, "fused pass: %llu tiles of %d threads exceed the grid",
+                       (unsigned long long)tiles, nt);
     if (ctx->timing && (rc = time_mark(ctx))) return rc;
     if (jit) {
       const PassDev* pp = dp + p;
       void* args[] = {&psi, &n, &pp, (void*)&ds, (void*)&dg, (void*)&dt};
-      SMOQ_HIP(hipModuleLaunchKernel(jit->fn[p], (unsigned)tiles, 1, 1, nt, 1, 1, (unsigned)shm,
-                                     ctx->stream, args, nullptr));
+      SMOQ_HIP(hipModuleLaunchKernel(jit->fn[p], (unsigned)gx, (unsigned)gy, 1, nt, 1, 1,
+                                     (unsigned)shm, ctx->stream, args, nullptr));
     } else {
-      hipLaunchKernelGGL(k_fused_pass<T>, dim3((unsigned)tiles), dim3(nt), shm, ctx->stream, psi,
-                         n, dp + p, ds, dg, dt);
+      hipLaunchKernelGGL(k_fused_pass<T>, dim3((unsigned)gx, (unsigned)gy), dim3(nt), shm,
+                         ctx->stream, psi, n, dp + p, ds, dg, dt);
       SMOQ_CHECK_LAUNCH("k_fused_pass");
     }
     if (ctx->timing && (rc = time_mark(ctx))) return rc;

@@ -272,7 +272,9 @@ __device__ __forceinline__ void pass_load(VT<T>* lds, VT<T>* __restrict__ psi, i
   constexpr int R = kRegQubits;
   constexpr int WB = CT<T>::kRowBits;
   const int nn = pass->n_nonlocal;
-  const uint64_t tile = blockIdx.x;
+  // tiles beyond one grid row continue in blockIdx.y (states of 2^32 threads
+  // and more per pass)
+  const uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
   const uint64_t item = tile >> nn;
   const uint64_t tt = tile & ((1ull << nn) - 1ull);
   f.nt = 1 << (pass->k - R);

@@ -16,38 +16,40 @@ namespace smoq {
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_u1(VT_alias<T>* __restrict__ psi, uint64_t npairs, int q, uint64_t ctrl, Mat2<double> u) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npairs) return;
-  const uint64_t lo = insert_zero(t, q);
-  if ((lo & ctrl) != ctrl) return;
-  const uint64_t hi = lo | (1ull << q);
-  const double2 a = to_d2(psi[lo]), b = to_d2(psi[hi]);
-  const double2 na = cfma2(u.m[0], a, u.m[1], b);
-  const double2 nb = cfma2(u.m[2], a, u.m[3], b);
-  psi[lo] = make_c<T>(na.x, na.y);
-  psi[hi] = make_c<T>(nb.x, nb.y);
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < npairs;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t lo = insert_zero(t, q);
+    if ((lo & ctrl) != ctrl) continue;
+    const uint64_t hi = lo | (1ull << q);
+    const double2 a = to_d2(psi[lo]), b = to_d2(psi[hi]);
+    const double2 na = cfma2(u.m[0], a, u.m[1], b);
+    const double2 nb = cfma2(u.m[2], a, u.m[3], b);
+    psi[lo] = make_c<T>(na.x, na.y);
+    psi[hi] = make_c<T>(nb.x, nb.y);
+  }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_u2(VT_alias<T>* __restrict__ psi, uint64_t nquads, int qa, int qb, uint64_t ctrl,
          Mat4<double> u) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nquads) return;
   const int lo_b = qa < qb ? qa : qb, hi_b = qa < qb ? qb : qa;
-  const uint64_t i0 = insert_zero(insert_zero(t, lo_b), hi_b);
-  if ((i0 & ctrl) != ctrl) return;
   const uint64_t ma = 1ull << qa, mb = 1ull << qb;
-  const uint64_t idx[4] = {i0, i0 | ma, i0 | mb, i0 | ma | mb};
-  double2 x[4];
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nquads;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i0 = insert_zero(insert_zero(t, lo_b), hi_b);
+    if ((i0 & ctrl) != ctrl) continue;
+    const uint64_t idx[4] = {i0, i0 | ma, i0 | mb, i0 | ma | mb};
+    double2 x[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) x[e] = to_d2(psi[idx[e]]);
+    for (int e = 0; e < 4; ++e) x[e] = to_d2(psi[idx[e]]);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    double2 acc = cmul(u.m[4 * r], x[0]);
+    for (int r = 0; r < 4; ++r) {
+      double2 acc = cmul(u.m[4 * r], x[0]);
 #pragma unroll
-    for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(u.m[4 * r + c], x[c]));
-    psi[idx[r]] = make_c<T>(acc.x, acc.y);
+      for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(u.m[4 * r + c], x[c]));
+      psi[idx[r]] = make_c<T>(acc.x, acc.y);
+    }
   }
 }
 
@@ -55,14 +57,15 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     k_diag(VT_alias<T>* __restrict__ psi, uint64_t total, uint64_t dim_mask, int nd, int q0,
            int q1, int q2, Diag8<double> ph) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const uint64_t g = i & dim_mask;
-  int e = (int)((g >> q0) & 1);
-  if (nd > 1) e |= (int)((g >> q1) & 1) << 1;
-  if (nd > 2) e |= (int)((g >> q2) & 1) << 2;
-  const double2 v = cmul(ph.p[e], to_d2(psi[i]));
-  psi[i] = make_c<T>(v.x, v.y);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t g = i & dim_mask;
+    int e = (int)((g >> q0) & 1);
+    if (nd > 1) e |= (int)((g >> q1) & 1) << 1;
+    if (nd > 2) e |= (int)((g >> q2) & 1) << 2;
+    const double2 v = cmul(ph.p[e], to_d2(psi[i]));
+    psi[i] = make_c<T>(v.x, v.y);
+  }
 }
 
 template <typename T>
@@ -75,18 +78,18 @@ static int launch_gate_t(smoq_state* s, const IGate& g) {
     Mat2<double> u;
     for (int i = 0; i < 4; ++i) u.m[i] = make_c<double>(g.m[2 * i], g.m[2 * i + 1]);
     const uint64_t np = total >> 1;
-    hipLaunchKernelGGL(k_u1<T>, dim3((unsigned)((np + B - 1) / B)), dim3(B), 0, st, psi, np, g.t0,
+    hipLaunchKernelGGL(k_u1<T>, dim3(stride_blocks(np, B)), dim3(B), 0, st, psi, np, g.t0,
                        g.ctrl, u);
   } else if (g.kind == IK_U2) {
     Mat4<double> u;
     for (int i = 0; i < 16; ++i) u.m[i] = make_c<double>(g.m[2 * i], g.m[2 * i + 1]);
     const uint64_t nq = total >> 2;
-    hipLaunchKernelGGL(k_u2<T>, dim3((unsigned)((nq + B - 1) / B)), dim3(B), 0, st, psi, nq, g.t0,
+    hipLaunchKernelGGL(k_u2<T>, dim3(stride_blocks(nq, B)), dim3(B), 0, st, psi, nq, g.t0,
                        g.t1, g.ctrl, u);
   } else {
     Diag8<double> ph;
     for (int i = 0; i < (1 << g.nd); ++i) ph.p[i] = make_c<double>(g.m[2 * i], g.m[2 * i + 1]);
-    hipLaunchKernelGGL(k_diag<T>, dim3((unsigned)((total + B - 1) / B)), dim3(B), 0, st, psi, total,
+    hipLaunchKernelGGL(k_diag<T>, dim3(stride_blocks(total, B)), dim3(B), 0, st, psi, total,
                        s->dim - 1, g.nd, g.dq[0], g.dq[1] < 0 ? 0 : g.dq[1],
                        g.dq[2] < 0 ? 0 : g.dq[2], ph);
   }
@@ -104,9 +107,9 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     k_set_basis(VT_alias<T>* __restrict__ psi, uint64_t total, uint64_t dim_mask,
                 uint64_t index) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  psi[i] = make_c<T>((i & dim_mask) == index ? 1.0 : 0.0, 0.0);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    psi[i] = make_c<T>((i & dim_mask) == index ? 1.0 : 0.0, 0.0);
 }
 
 // Product state: amplitude i = c_1[b_1] c_2[b_2] ... c_n[b_n] with b_k bit
@@ -120,15 +123,16 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     k_set_product(VT_alias<T>* __restrict__ psi, uint64_t total, int n, ProductKets pk) {
 #pragma clang fp contract(off)
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const uint64_t k = i & ((1ull << n) - 1ull);
-  double2 p = pk.c[0][k & 1];
-  for (int q = 1; q < n; ++q) {
-    const double2 c = pk.c[q][(k >> q) & 1];
-    p = make_double2(p.x * c.x - p.y * c.y, p.x * c.y + p.y * c.x);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = i & ((1ull << n) - 1ull);
+    double2 p = pk.c[0][k & 1];
+    for (int q = 1; q < n; ++q) {
+      const double2 c = pk.c[q][(k >> q) & 1];
+      p = make_double2(p.x * c.x - p.y * c.y, p.x * c.y + p.y * c.x);
+    }
+    psi[i] = make_c<T>(p.x, p.y);
   }
-  psi[i] = make_c<T>(p.x, p.y);
 }
 
 }  // namespace smoq
@@ -151,7 +155,7 @@ extern "C" int smoq_state_set_product(smoq_state* s, const double* kets) {
       pk.c[q][b] = make_double2(re, im);
     }
   const uint64_t total = s->dim * (uint64_t)s->batch;
-  const unsigned nb = (unsigned)((total + 255) / 256);
+  const unsigned nb = stride_blocks(total, 256);
   if (s->dtype == SMOQ_C128)
     hipLaunchKernelGGL(k_set_product<double>, dim3(nb), dim3(256), 0, s->ctx->stream,
                        (double2*)s->d, total, s->n, pk);
@@ -170,7 +174,7 @@ extern "C" int smoq_state_set_basis(smoq_state* s, uint64_t index) {
     return set_error(SMOQ_EINVAL, "smoq_state_set_basis: index %llu >= 2^%d",
                      (unsigned long long)index, s->n);
   const uint64_t total = s->dim * (uint64_t)s->batch;
-  const unsigned nb = (unsigned)((total + 255) / 256);
+  const unsigned nb = stride_blocks(total, 256);
   if (s->dtype == SMOQ_C128)
     hipLaunchKernelGGL(k_set_basis<double>, dim3(nb), dim3(256), 0, s->ctx->stream,
                        (double2*)s->d, total, s->dim - 1, index);

@@ -134,6 +134,14 @@ struct smoq_state_s {
 
 namespace smoq {
 int ensure_scratch(smoq_ctx* ctx, size_t bytes);
+
+// Blocks of a grid-stride launch over n items with `threads` per block: one
+// item per thread up to 2^20 blocks (far more than the chip holds at once),
+// so the launch stays valid however large batch x 2^n grows.
+inline unsigned stride_blocks(uint64_t n, unsigned threads) {
+  const uint64_t want = (n + threads - 1) / threads;
+  return (unsigned)(want == 0 ? 1 : (want < (1ull << 20) ? want : (1ull << 20)));
+}
 int ensure_prog(smoq_ctx* ctx, size_t bytes);
 int ensure_pinned(smoq_ctx* ctx, size_t bytes);
 int check_state(const smoq_state* s);
