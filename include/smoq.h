@@ -205,6 +205,18 @@ int smoq_lanczos_range(const smoq_state* start, int n_terms, const uint64_t* xma
                        const uint64_t* zmask, const double* coeff, int k_iter, double* e_min,
                        double* e_max, int* iters_used);
 
+/* ---- reduced density matrices ---------------------------------------------
+ * partial_trace(psi, trace_qubits, N) (utils/cpu/cpuQuantumStatePartialTrace.jl:68-127)
+ * for every batch item: the qubits set in keep_mask (bit q = qubit q, 0-based,
+ * at least one) are kept and the others traced out.  out is a separate state
+ * of 2K qubits (K = popcount(keep_mask)), same batch, dtype and context: it
+ * receives vec(rho_A), element (i, j) at i + 2^K j with i, j over the kept
+ * qubits in increasing order (the reference's keep_indices). */
+int smoq_partial_trace(const smoq_state* psi, uint64_t keep_mask, smoq_state* out);
+/* partial_trace(rho, trace_qubits, N) (…PartialTrace.jl:141-188) of a density
+ * matrix stored as vec(rho) (2N vec qubits); keep_mask over its N qubits. */
+int smoq_dm_partial_trace(const smoq_state* rho, uint64_t keep_mask, smoq_state* out);
+
 /* ---- single-qubit stochastic operations ------------------------------------
  * For k = 0 .. n_qubits-1 in order and every batch item b, with the caller's
  * uniform u[k * batch + b] in place of the reference's rand() and the record

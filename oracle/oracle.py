@@ -733,6 +733,46 @@ def sample_from_probabilities(probs, r):
     return len(probs) - 1
 
 
+# --------------------------------------------------------- partial trace --
+def _bit_map(bits, count):
+    """index -> full index with the bits of the index deposited at `bits`
+    (0-based positions, in increasing order): the reference's keep_to_full /
+    trace_to_full tables."""
+    out = np.zeros(1 << count, np.int64)
+    for b_idx, full_bit in enumerate(bits):
+        out |= ((np.arange(1 << count) >> b_idx) & 1) << full_bit
+    return out
+
+
+def partial_trace_psi(psi, trace_qubits, N):
+    """partial_trace(ψ, trace_qubits, N) (utils/cpu/cpuQuantumStatePartialTrace.jl:68-127):
+    rho[i, j] = sum_t ψ[keep_to_full[i] | trace_to_full[t]] conj(ψ[... j ...])."""
+    tr = sorted(set(int(q) for q in trace_qubits))
+    keep = [q for q in range(1, N + 1) if q not in tr]
+    k2f = _bit_map([q - 1 for q in keep], len(keep))
+    t2f = _bit_map([q - 1 for q in tr], len(tr))
+    rho = np.zeros((len(k2f), len(k2f)), np.complex128)
+    for i, ib in enumerate(k2f):
+        a = psi[ib | t2f]
+        for j, jb in enumerate(k2f):
+            rho[i, j] = np.sum(a * np.conj(psi[jb | t2f]))
+    return rho
+
+
+def partial_trace_rho(rho, trace_qubits, N):
+    """partial_trace(ρ, trace_qubits, N) (…PartialTrace.jl:141-188):
+    rho_A[i, j] = sum_t ρ[i|t, j|t]."""
+    tr = sorted(set(int(q) for q in trace_qubits))
+    keep = [q for q in range(1, N + 1) if q not in tr]
+    k2f = _bit_map([q - 1 for q in keep], len(keep))
+    t2f = _bit_map([q - 1 for q in tr], len(tr))
+    out = np.zeros((len(k2f), len(k2f)), np.complex128)
+    for i, ib in enumerate(k2f):
+        for j, jb in enumerate(k2f):
+            out[i, j] = np.sum(rho[ib | t2f, jb | t2f])
+    return out
+
+
 # ---------------------------------------------------------- preparation --
 def product_ket(kets):
     """make_product_ket (utils/cpu/cpuQuantumStatePreparation.jl:1182-1191):

@@ -53,6 +53,19 @@ __global__ void __launch_bounds__(kDmBlock)
   }
 }
 
+int blas_handle(smoq_ctx* ctx, const char* who, void** out) {
+  if (!ctx->blas) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      return set_error(SMOQ_EDEVICE, "%s: rocblas_create_handle failed", who);
+    ctx->blas = h;
+  }
+  if (rocblas_set_stream((rocblas_handle)ctx->blas, ctx->stream) != rocblas_status_success)
+    return set_error(SMOQ_EDEVICE, "%s: rocblas_set_stream failed", who);
+  *out = ctx->blas;
+  return SMOQ_OK;
+}
+
 void blas_release(smoq_ctx* ctx) {
   if (ctx->blas) (void)rocblas_destroy_handle((rocblas_handle)ctx->blas);
   ctx->blas = nullptr;
@@ -123,15 +136,9 @@ extern "C" int smoq_dm_apply_dense(smoq_state* rho, const smoq_state* u) {
                      "smoq_dm_apply_dense: U must be a separate single 2N-qubit state of the "
                      "same dtype and context");
   smoq_ctx* ctx = rho->ctx;
-  if (!ctx->blas) {
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success)
-      return set_error(SMOQ_EDEVICE, "smoq_dm_apply_dense: rocblas_create_handle failed");
-    ctx->blas = h;
-  }
-  rocblas_handle h = (rocblas_handle)ctx->blas;
-  if (rocblas_set_stream(h, ctx->stream) != rocblas_status_success)
-    return set_error(SMOQ_EDEVICE, "smoq_dm_apply_dense: rocblas_set_stream failed");
+  void* hv = nullptr;
+  if ((rc = blas_handle(ctx, "smoq_dm_apply_dense", &hv))) return rc;
+  rocblas_handle h = (rocblas_handle)hv;
   const rocblas_int d = (rocblas_int)(1u << N);
   const size_t mb = (size_t)d * d * (rho->dtype == SMOQ_C128 ? 16 : 8);
   void* tmp = nullptr;

@@ -189,4 +189,6 @@ int mcwf_jump_launch(smoq_state* s, int kind, double gamma_dt, const double* d_d
 size_t mcwf_jump_partial_bytes(const smoq_state* s);
 // releases the context's rocBLAS handle (density.hip)
 void blas_release(smoq_ctx* ctx);
+// the context's rocBLAS handle (created on first use) bound to its stream
+int blas_handle(smoq_ctx* ctx, const char* who, void** out);
 }  // namespace smoq

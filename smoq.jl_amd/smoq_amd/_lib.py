@@ -116,6 +116,8 @@ SIGNATURES = {
     "smoq_dm_apply_dense": [_P, _P],
     "smoq_qubit_channel": [_P, _i, _P, _i, _d, _P, _P],
     "smoq_dm_qubit_channel": [_P, _i, _P, _i, _P, _P],
+    "smoq_partial_trace": [_P, ctypes.c_uint64, _P],
+    "smoq_dm_partial_trace": [_P, ctypes.c_uint64, _P],
     "smoq_norm2": [_P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
