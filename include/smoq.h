@@ -217,6 +217,26 @@ int smoq_partial_trace(const smoq_state* psi, uint64_t keep_mask, smoq_state* ou
  * matrix stored as vec(rho) (2N vec qubits); keep_mask over its N qubits. */
 int smoq_dm_partial_trace(const smoq_state* rho, uint64_t keep_mask, smoq_state* out);
 
+/* ---- classical-shadow snapshots -------------------------------------------
+ * For snapshot m (0 <= m < n_snap) of batch item 0: a copy of psi rotated
+ * qubit by qubit in increasing order by codes[m * n + q], then one draw with
+ * sample_from_probabilities' rule (utils/cpu/cpuQuantumStateMeasurements.jl:203-213)
+ * on uniforms[m]; out_index[m] is the 0-based outcome (bit q = the outcome of
+ * qubit q).  psi is not changed.
+ *   SMOQ_SHADOW_PAULI: codes 1 = X (H), 2 = Y (S† then H), 3 = Z (none), with
+ *     the reference's arithmetic: classical_shadow_snapshot
+ *     (utils/cpu/cpuClassicalShadows.jl:1001-1037), bit-exact outcomes.
+ *   SMOQ_SHADOW_LOCAL_CLIFFORD: codes 0..23 index clifford_table (24 row-major
+ *     2x2 complex matrices as re, im pairs, 192 doubles): clifford_shadow_snapshot
+ *     (…ClassicalShadows.jl:580-603), which applies the same product as one
+ *     dense matrix.
+ * Snapshots with equal code strings share one rotated state.  At most 32
+ * qubits. */
+enum smoq_shadow_kind { SMOQ_SHADOW_PAULI = 0, SMOQ_SHADOW_LOCAL_CLIFFORD = 1 };
+int smoq_shadow_snapshots(const smoq_state* psi, int64_t n_snap, int kind, const uint8_t* codes,
+                          const double* clifford_table, const double* uniforms,
+                          uint64_t* out_index);
+
 /* ---- single-qubit stochastic operations ------------------------------------
  * For k = 0 .. n_qubits-1 in order and every batch item b, with the caller's
  * uniform u[k * batch + b] in place of the reference's rand() and the record

@@ -773,6 +773,32 @@ def partial_trace_rho(rho, trace_qubits, N):
     return out
 
 
+# ---------------------------------------------------- classical shadows --
+def pauli_shadow_snapshot(psi, N, bases, r):
+    """classical_shadow_snapshot(ψ, N) (utils/cpu/cpuClassicalShadows.jl:1001-1037)
+    with the bases (1 = X, 2 = Y, 3 = Z) and rand() given: a copy rotated
+    qubit by qubit (H; S† then H), then sample_from_probabilities
+    (…Measurements.jl:203-213) on |ψ|^2; returns the 0-based outcome index."""
+    w = np.array(psi, np.complex128)
+    for k in range(1, N + 1):
+        if bases[k - 1] == 1:
+            apply_named(w, "h", [k], 0.0, N)
+        elif bases[k - 1] == 2:
+            apply_named(w, "sdg", [k], 0.0, N)
+            apply_named(w, "h", [k], 0.0, N)
+    return int(sample_sequential(w, N, [r])[0])
+
+
+def clifford_shadow_snapshot(psi, N, indices, r, table):
+    """clifford_shadow_snapshot(ψ, N) (…ClassicalShadows.jl:580-603): ψ times
+    the dense kron(C_N, ..., C_1) (build_local_clifford_unitary, :539-546) of
+    the 1-based `indices` into `table`, then the same draw."""
+    C = np.asarray(table[indices[N - 1] - 1], np.complex128)
+    for j in range(N - 2, -1, -1):
+        C = np.kron(C, table[indices[j] - 1])
+    return int(sample_sequential(C @ np.asarray(psi, np.complex128), N, [r])[0])
+
+
 # ---------------------------------------------------------- preparation --
 def product_ket(kets):
     """make_product_ket (utils/cpu/cpuQuantumStatePreparation.jl:1182-1191):

@@ -1,0 +1,216 @@
+"""Classical-shadow snapshot collection on the device, with the reference's
+names (utils/cpu/cpuClassicalShadows.jl: ShadowConfig, PauliSnapshot,
+CliffordSnapshot, sample_pauli_bases, classical_shadow_snapshot,
+collect_shadows, SINGLE_QUBIT_CLIFFORDS, clifford_shadow_snapshot,
+collect_clifford_shadows).
+
+The reference draws the bases and the outcome uniform with `rand()` inside
+each snapshot.  Here both are arrays (given, or drawn from `rng`), and all
+snapshots go to the device in one call (smoq_shadow_snapshots): snapshots
+with the same rotation string share one rotated state, and each draw is a
+search of that state's sequential prefix.  Estimators built on the snapshots
+(reconstruction, expectation values, purity) are host post-processing of
+these integer arrays and are not part of this module.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import call
+from .state import GPUState
+
+SHADOW_PAULI = 0
+SHADOW_LOCAL_CLIFFORD = 1
+
+
+@dataclass
+class PauliSnapshot:
+    """PauliSnapshot (…ClassicalShadows.jl:416-419): bases 1 = X, 2 = Y,
+    3 = Z and outcomes 0/1, one per qubit (qubit 1 first)."""
+    bases: np.ndarray
+    outcomes: np.ndarray
+
+
+Snapshot = PauliSnapshot
+
+
+@dataclass
+class CliffordSnapshot:
+    """CliffordSnapshot (…ClassicalShadows.jl:566-569): 1-based indices into
+    SINGLE_QUBIT_CLIFFORDS and outcomes, one per qubit."""
+    clifford_indices: np.ndarray
+    outcomes: np.ndarray
+
+
+@dataclass
+class ShadowConfig:
+    """ShadowConfig (…ClassicalShadows.jl:366-392) with its group aliases."""
+    n_qubits: int
+    n_shots: int
+    measurement_group: str = "pauli"
+
+    def __post_init__(self):
+        if self.n_qubits <= 0 or self.n_shots <= 0:
+            raise ValueError("n_qubits and n_shots must be positive")
+        g = str(self.measurement_group).lstrip(":")
+        g = {"pauli_group": "pauli", "clifford_group": "local_clifford",
+             "clifford": "local_clifford"}.get(g, g)
+        if g not in ("pauli", "local_clifford", "global_clifford"):
+            raise ValueError("measurement_group must be :pauli, :local_clifford, or "
+                             ":global_clifford")
+        self.measurement_group = g
+
+
+def _single_qubit_cliffords() -> np.ndarray:
+    """SINGLE_QUBIT_CLIFFORDS (…ClassicalShadows.jl:457-505), built the same
+    way: breadth-first products G·C with G in (H, S), global phase fixed by the
+    first entry of magnitude > 0.1 in column order, entries rounded to 10
+    digits, duplicates (Frobenius distance <= 1e-8) dropped."""
+    H = np.array([[1, 1], [1, -1]], np.complex128) / np.sqrt(2)
+    S = np.array([[1, 0], [0, 1j]], np.complex128)
+
+    def normalize(M):
+        for v in M.T.reshape(-1):  # column-major order
+            if abs(v) > 0.1:
+                return M / (v / abs(v))
+        return M
+
+    def rnd(M):
+        return np.round(M.real, 10) + 1j * np.round(M.imag, 10)
+
+    cliffs = [np.eye(2, dtype=np.complex128)]
+    explore = [cliffs[0]]
+    while True:
+        new = []
+        for C in explore:
+            for G in (H, S):
+                N = rnd(normalize(G @ C))
+                if not any(np.linalg.norm(N - E) <= 1e-8 for E in cliffs):
+                    new.append(N)
+                    cliffs.append(N)
+        if not new:
+            break
+        explore = new
+    if len(cliffs) != 24:
+        raise RuntimeError(f"Expected 24 Cliffords, got {len(cliffs)}")
+    return np.array(cliffs)
+
+
+SINGLE_QUBIT_CLIFFORDS = _single_qubit_cliffords()
+
+
+def get_clifford_gate(idx: int) -> np.ndarray:
+    """get_clifford_gate (…ClassicalShadows.jl:528-530), 1-based."""
+    return SINGLE_QUBIT_CLIFFORDS[int(idx) - 1]
+
+
+def sample_pauli_bases(N: int, rng=None) -> np.ndarray:
+    """sample_pauli_bases (…ClassicalShadows.jl:988-990): N draws from 1:3."""
+    g = rng if rng is not None else np.random.default_rng()
+    return g.integers(1, 4, N).astype(np.uint8)
+
+
+def sample_local_clifford_indices(N: int, rng=None) -> np.ndarray:
+    """sample_local_clifford_indices (…ClassicalShadows.jl:519-521): N draws from 1:24."""
+    g = rng if rng is not None else np.random.default_rng()
+    return g.integers(1, 25, N).astype(np.uint8)
+
+
+def _outcomes(idx: np.ndarray, N: int) -> np.ndarray:
+    return ((idx[:, None] >> np.arange(N, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(
+        np.uint8)
+
+
+def shadow_outcomes(psi: GPUState, codes, uniforms, kind: int = SHADOW_PAULI) -> np.ndarray:
+    """smoq_shadow_snapshots: the 0-based outcome index of every snapshot.
+    codes: (M, N) array (Pauli 1..3, or Clifford 0-based 0..23)."""
+    c = np.ascontiguousarray(np.asarray(codes, np.uint8).reshape(-1, psi.n))
+    u = np.ascontiguousarray(np.asarray(uniforms, np.float64).reshape(-1))
+    if u.size != c.shape[0]:
+        raise ValueError(f"{c.shape[0]} snapshots but {u.size} uniforms")
+    out = np.empty(u.size, np.uint64)
+    tab = None
+    if kind == SHADOW_LOCAL_CLIFFORD:
+        tab = np.ascontiguousarray(SINGLE_QUBIT_CLIFFORDS.view(np.float64).reshape(-1))
+    call("smoq_shadow_snapshots", psi.handle, u.size, kind, c.ctypes.data_as(ctypes.c_void_p),
+         None if tab is None else tab.ctypes.data_as(ctypes.c_void_p),
+         u.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+class ShadowSet:
+    """M snapshots as two (M, N) uint8 arrays; indexing gives PauliSnapshot /
+    CliffordSnapshot objects like the reference's Vector of snapshots."""
+
+    def __init__(self, codes: np.ndarray, outcomes: np.ndarray, kind: str):
+        self.codes, self.outcomes, self.kind = codes, outcomes, kind
+
+    def __len__(self):
+        return self.codes.shape[0]
+
+    def __getitem__(self, m):
+        if self.kind == "pauli":
+            return PauliSnapshot(self.codes[m], self.outcomes[m])
+        return CliffordSnapshot(self.codes[m], self.outcomes[m])
+
+
+def collect_shadows(state, N, M: int | None = None, *, bases=None, uniforms=None,
+                    rng=None) -> ShadowSet:
+    """collect_shadows(ψ, N, M) (…ClassicalShadows.jl:1098-1106) and
+    collect_shadows(ψ, config) (:1135-1160, :pauli and :local_clifford):
+    M snapshots of batch item 0.  `bases` (M, N) holds 1..3 for Pauli or
+    1..24 for local Clifford; drawn from `rng` when absent, as are the
+    outcome uniforms."""
+    group = "pauli"
+    if isinstance(N, ShadowConfig):
+        cfg = N
+        N, M, group = cfg.n_qubits, cfg.n_shots, cfg.measurement_group
+        if group == "global_clifford":
+            raise NotImplementedError("global Clifford snapshots need a dense 2^N unitary per "
+                                      "snapshot and are not on the device path")
+    if not isinstance(state, GPUState):
+        raise TypeError("snapshots are taken of a GPUState")
+    if state.n != N:
+        raise ValueError(f"state has {state.n} qubits, N = {N}")
+    g = rng if rng is not None else np.random.default_rng()
+    hi = 4 if group == "pauli" else 25
+    b = (g.integers(1, hi, (M, N)) if bases is None else np.asarray(bases)).astype(np.uint8)
+    b = b.reshape(M, N)
+    u = g.random(M) if uniforms is None else np.asarray(uniforms, np.float64)
+    if group == "pauli":
+        idx = shadow_outcomes(state, b, u, SHADOW_PAULI)
+    else:
+        idx = shadow_outcomes(state, b - 1, u, SHADOW_LOCAL_CLIFFORD)
+    return ShadowSet(b, _outcomes(idx, N), group)
+
+
+def classical_shadow_snapshot(state, N: int, bases=None, r=None, *, rng=None) -> PauliSnapshot:
+    """classical_shadow_snapshot(ψ, N) (…ClassicalShadows.jl:1001-1037)."""
+    b = None if bases is None else np.asarray(bases).reshape(1, N)
+    u = None if r is None else [float(r)]
+    return collect_shadows(state, N, 1, bases=b, uniforms=u, rng=rng)[0]
+
+
+def collect_clifford_shadows(state, N: int, M: int, *, indices=None, uniforms=None,
+                             rng=None) -> ShadowSet:
+    """collect_clifford_shadows(ψ, N, M) (…ClassicalShadows.jl:636-641)."""
+    return collect_shadows(state, ShadowConfig(N, M, "local_clifford"), bases=indices,
+                           uniforms=uniforms, rng=rng)
+
+
+def clifford_shadow_snapshot(state, N: int, indices=None, r=None, *, rng=None):
+    """clifford_shadow_snapshot(ψ, N) (…ClassicalShadows.jl:580-603)."""
+    i = None if indices is None else np.asarray(indices).reshape(1, N)
+    u = None if r is None else [float(r)]
+    return collect_clifford_shadows(state, N, 1, indices=i, uniforms=u, rng=rng)[0]
+
+
+__all__ = [
+    "CliffordSnapshot", "PauliSnapshot", "SINGLE_QUBIT_CLIFFORDS", "ShadowConfig", "ShadowSet",
+    "Snapshot", "classical_shadow_snapshot", "clifford_shadow_snapshot",
+    "collect_clifford_shadows", "collect_shadows", "get_clifford_gate",
+    "sample_local_clifford_indices", "sample_pauli_bases", "shadow_outcomes",
+]
