@@ -17,7 +17,8 @@ the device beside its recorded times (shared random circuits + 100 samples,
 the expectation-value benchmark, σ_z-dephasing MCWF with 1000 trajectories,
 the same model as a density matrix with Trotter gates and with the
 dense propagator, and the Kraus-channel demo's trajectory scaling table up
-to N=30)
+to N=30), partial traces of the N=30 state against the HBM and FP64
+rooflines, classical-shadow snapshot collection (the shadows demo's counts)
 and the CPU baseline (the oracle's restatement of the reference loop, one
 core, on a bounded sample of the same circuit).
 """
@@ -476,6 +477,67 @@ def channel_scaling(sizes=(12, 16, 20, 24, 26, 28, 30), n_traj=100, p=0.2, seed=
             "by_N": res}
 
 
+def reduced_states(N=30, reps=3):
+    """Partial trace of a 30-qubit complex128 state (16 GiB) for several kept
+    sets: ms per call, the state's bytes over that time (HBM: the state is
+    read once) and 8·D²·2^(N-K) flops over it (FP64: rho_A = M M^H)."""
+    import smoq_amd
+    from smoq_amd import GPUState
+    from smoq_amd._lib import call
+
+    ctx = smoq_amd.default_context()
+    s = GPUState(N, "c128", ctx=ctx)
+    s.set_product([(0.6, 0.8)] * N)
+    rows = []
+    for keep in (0b1, 0b11, 0b111, 0b1111, 0b11111, (1 << 29) | 0b111, 0x3f, 0xff, 0x3ff):
+        k = bin(keep).count("1")
+        out = GPUState(2 * k, "c128", ctx=ctx)
+        call("smoq_partial_trace", s.handle, keep, out.handle)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call("smoq_partial_trace", s.handle, keep, out.handle)
+        t = (time.perf_counter() - t0) / reps
+        out.free()
+        D = 1 << k
+        path = "register" if k <= 2 else ("tile" if k <= 5 else "gemm")
+        rows.append({"kept_mask": hex(keep), "kept": k, "path": path, "ms": t * 1e3,
+                     "GB_per_s": s.nbytes / t / 1e9,
+                     "TFLOP_per_s": 8.0 * D * D * (1 << (N - k)) / t / 1e12})
+        log(f"partial trace N={N} K={k}: {t * 1e3:.2f} ms")
+    s.free()
+    return {"workload": f"partial_trace of a random-phase product state, N={N} c128 (16 GiB)",
+            "hbm_peak_GB_per_s": HBM_PEAK / 1e9, "fp64_peak_TFLOP_per_s": 78.6, "by_kept": rows}
+
+
+def shadow_snapshots():
+    """Pauli classical-shadow snapshots: the shadows demo's N = 2..6 with all
+    of its snapshot counts at once (100 + ... + 10^6 = 1,111,100 per N;
+    scripts_quantum_information/demo_classical_shadows.jl:77-78), and larger
+    states with fewer snapshots.  Bases and uniforms from a seeded stream."""
+    import smoq_amd
+    from smoq_amd import GPUState
+    from smoq_amd.shadows import collect_shadows
+
+    smoq_amd.default_context()
+    rows = []
+    for N, M in ((2, 1111100), (3, 1111100), (4, 1111100), (5, 1111100), (6, 1111100),
+                 (12, 100000), (20, 1000), (24, 100)):
+        rng = np.random.default_rng(N)
+        v = rng.normal(size=1 << N) + 1j * rng.normal(size=1 << N)
+        s = GPUState.from_numpy(v / np.linalg.norm(v))
+        b = rng.integers(1, 4, (M, N)).astype(np.uint8)
+        u = rng.random(M)
+        collect_shadows(s, N, min(M, 100), bases=b[:100], uniforms=u[:100])
+        t0 = time.perf_counter()
+        collect_shadows(s, N, M, bases=b, uniforms=u)
+        t = time.perf_counter() - t0
+        s.free()
+        rows.append({"N": N, "snapshots": M, "seconds": t, "snapshots_per_s": M / t})
+        log(f"shadows N={N} M={M}: {t:.3f} s")
+    return {"workload": "collect_shadows (Pauli) of a random state, complex128",
+            "by_N": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", choices=["traffic"], default=None)
@@ -495,6 +557,8 @@ def main():
     ap.add_argument("--shots", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-analytics", action="store_true")
+    ap.add_argument("--analytics-only", action="store_true")
     args = ap.parse_args()
 
     if args.probe == "traffic":
@@ -508,6 +572,10 @@ def main():
         return
     if args.reference_only:
         print(json.dumps({"reference_workloads": reference_benchmarks()}), flush=True)
+        return
+    if args.analytics_only:
+        print(json.dumps({"partial_trace": reduced_states(),
+                          "classical_shadows": shadow_snapshots()}), flush=True)
         return
 
     import smoq_amd
@@ -617,6 +685,10 @@ def main():
     if not args.no_reference and not args.no_extras:
         log("the reference's published workloads ...")
         result.setdefault("extras", {})["reference_workloads"] = reference_benchmarks()
+    if not args.no_analytics and not args.no_extras:
+        log("partial trace and classical shadows ...")
+        result.setdefault("extras", {})["partial_trace"] = reduced_states()
+        result["extras"]["classical_shadows"] = shadow_snapshots()
 
     if not args.no_cpu:
         log("CPU baseline ...")

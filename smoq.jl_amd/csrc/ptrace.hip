@@ -235,6 +235,41 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same sum with one block per output entry (few entries, many partials):
+// each thread sums a strided share in order, then the block sums the threads'
+// results in a fixed order.
+template <typename T, typename P>
+__global__ void __launch_bounds__(256)
+    k_ptrace_finish_wide(const P* __restrict__ part, int nb, uint64_t dd,
+                         uint64_t part_item_stride, VT_alias<T>* __restrict__ out) {
+  __shared__ double sh[kPtBlock / 64];
+  const uint64_t o = blockIdx.x, item = o / dd, e = o % dd;
+  const P* q = part + item * part_item_stride + e;
+  double x = 0.0, y = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kPtBlock) {
+    x += (double)q[(uint64_t)b * dd].x;
+    y += (double)q[(uint64_t)b * dd].y;
+  }
+  x = block_sum_ptrace(x, sh);
+  y = block_sum_ptrace(y, sh);
+  if (threadIdx.x == 0) out[o] = make_c<T>(x, y);
+}
+
+// partials -> out: a block per entry when entries are few, a thread per
+// entry otherwise
+template <typename T, typename P>
+static int ptrace_finish(hipStream_t st, const P* part, int nb, uint64_t dd, uint64_t stride,
+                         VT_alias<T>* out, uint64_t total) {
+  if (total <= 4096 && nb >= 64)
+    hipLaunchKernelGGL((k_ptrace_finish_wide<T, P>), dim3((unsigned)total), dim3(kPtBlock), 0, st,
+                       part, nb, dd, stride, out);
+  else
+    hipLaunchKernelGGL((k_ptrace_finish<T, P>), dim3(stride_blocks(total, 256)), dim3(256), 0, st,
+                       part, nb, dd, stride, out, total);
+  SMOQ_CHECK_LAUNCH("k_ptrace_finish");
+  return SMOQ_OK;
+}
+
 // M[i + D t] = psi[pdep(i, keep) | pdep(t, trace)] for one item; the bit
 // deposits come from byte tables in LDS (i: 3 bytes, t: 4 bytes; the state
 // has at most 36 qubits and the output at most 2^34 entries, so i < 2^17 and
@@ -358,9 +393,8 @@ static int ptrace_tile(const smoq_state* s, uint64_t keep, int k, smoq_state* ou
   }
   SMOQ_CHECK_LAUNCH("k_ptrace_tile");
   const uint64_t total = B * dd;
-  hipLaunchKernelGGL((k_ptrace_finish<T, double2>), dim3(stride_blocks(total, 256)), dim3(256), 0,
-                     ctx->stream, part, (int)nb, dd, nb * dd, (V*)out->d, total);
-  SMOQ_CHECK_LAUNCH("k_ptrace_finish");
+  if ((rc = ptrace_finish<T, double2>(ctx->stream, part, (int)nb, dd, nb * dd, (V*)out->d, total)))
+    return rc;
   SMOQ_HIP(hipStreamSynchronize(ctx->stream));
   return SMOQ_OK;
 }
@@ -386,9 +420,8 @@ static int ptrace_pairs(const smoq_state* s, uint64_t keep, int k, smoq_state* o
                        (const V*)s->d, s->n, lo, hi, part);
   SMOQ_CHECK_LAUNCH("k_ptrace_pairs");
   const uint64_t total = B * dd;
-  hipLaunchKernelGGL((k_ptrace_finish<T, double2>), dim3(stride_blocks(total, 256)), dim3(256), 0,
-                     ctx->stream, part, (int)nb, dd, nb * dd, (V*)out->d, total);
-  SMOQ_CHECK_LAUNCH("k_ptrace_finish");
+  if ((rc = ptrace_finish<T, double2>(ctx->stream, part, (int)nb, dd, nb * dd, (V*)out->d, total)))
+    return rc;
   SMOQ_HIP(hipStreamSynchronize(ctx->stream));
   return SMOQ_OK;
 }
@@ -446,11 +479,8 @@ static int ptrace_gemm(const smoq_state* s, uint64_t keep, int k, smoq_state* ou
     }
     if (st != rocblas_status_success)
       return set_error(SMOQ_EDEVICE, "smoq_partial_trace: rocBLAS gemm failed (%d)", (int)st);
-    if (P > 1) {
-      hipLaunchKernelGGL((k_ptrace_finish<T, V>), dim3(stride_blocks(dd, 256)), dim3(256), 0,
-                         ctx->stream, part, (int)P, dd, P * dd, res, dd);
-      SMOQ_CHECK_LAUNCH("k_ptrace_finish");
-    }
+    if (P > 1 && (rc = ptrace_finish<T, V>(ctx->stream, part, (int)P, dd, P * dd, res, dd)))
+      return rc;
   }
   SMOQ_HIP(hipStreamSynchronize(ctx->stream));
   return SMOQ_OK;
@@ -476,9 +506,9 @@ static int ptrace_rho(const smoq_state* rho, uint64_t keep, int k, smoq_state* o
     SMOQ_CHECK_LAUNCH("k_ptrace_rho");
   }
   const uint64_t total = B * dd;
-  hipLaunchKernelGGL((k_ptrace_finish<T, double2>), dim3(stride_blocks(total, 256)), dim3(256), 0,
-                     ctx->stream, part, (int)P, dd, P * dd, (VT_alias<T>*)out->d, total);
-  SMOQ_CHECK_LAUNCH("k_ptrace_finish");
+  if ((rc = ptrace_finish<T, double2>(ctx->stream, part, (int)P, dd, P * dd,
+                                      (VT_alias<T>*)out->d, total)))
+    return rc;
   SMOQ_HIP(hipStreamSynchronize(ctx->stream));
   return SMOQ_OK;
 }
