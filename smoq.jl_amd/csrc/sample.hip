@@ -44,9 +44,9 @@ struct Level {
   double* carry;
 };
 
-__global__ void k_tree_level(Level parent, Level child, uint64_t nslots) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nslots) return;
+// One step of each tree phase for slot j of a level; the per-level kernels
+// below and the single-workgroup k_cumsum_block run the same steps.
+__device__ inline void tree_level_at(Level parent, Level child, uint64_t j) {
   const uint64_t pj = j >> 1;
   const bool right = j & 1;
   const uint8_t pf = parent.flag[pj];
@@ -67,10 +67,10 @@ __global__ void k_tree_level(Level parent, Level child, uint64_t nslots) {
 }
 
 template <typename T>
-__global__ void k_leaf_sum(const typename CT<T>::V* __restrict__ psi, Level lv, uint64_t nslots) {
+__device__ inline void leaf_sum_at(const typename CT<T>::V* __restrict__ psi, Level lv,
+                                   uint64_t j) {
 #pragma clang fp contract(off)
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nslots || lv.flag[j] != 1) return;
+  if (lv.flag[j] != 1) return;
   const uint64_t st = lv.start[j];
   const uint32_t ln = lv.len[j];
   double s = prob_of<T>(psi[st]);
@@ -78,28 +78,25 @@ __global__ void k_leaf_sum(const typename CT<T>::V* __restrict__ psi, Level lv, 
   lv.sum[j] = s;
 }
 
-__global__ void k_up(Level lv, Level child, uint64_t nslots) {
+__device__ inline void up_at(Level lv, Level child, uint64_t j) {
 #pragma clang fp contract(off)
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nslots || lv.flag[j] != 0) return;
+  if (lv.flag[j] != 0) return;
   lv.sum[j] = child.sum[2 * j] + child.sum[2 * j + 1];
 }
 
-__global__ void k_down(Level lv, Level child, uint64_t nslots) {
+__device__ inline void down_at(Level lv, Level child, uint64_t j) {
 #pragma clang fp contract(off)
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nslots || lv.flag[j] != 0) return;
+  if (lv.flag[j] != 0) return;
   const double s = lv.carry[j];
   child.carry[2 * j] = s;
   child.carry[2 * j + 1] = s + child.sum[2 * j];
 }
 
 template <typename T>
-__global__ void k_leaf_write(const typename CT<T>::V* __restrict__ psi, Level lv, uint64_t nslots,
-                             double* __restrict__ c) {
+__device__ inline void leaf_write_at(const typename CT<T>::V* __restrict__ psi, Level lv,
+                                     uint64_t j, double* __restrict__ c) {
 #pragma clang fp contract(off)
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nslots || lv.flag[j] != 1) return;
+  if (lv.flag[j] != 1) return;
   const uint64_t st = lv.start[j];
   const uint32_t ln = lv.len[j];
   const double s = lv.carry[j];
@@ -111,20 +108,9 @@ __global__ void k_leaf_write(const typename CT<T>::V* __restrict__ psi, Level lv
   }
 }
 
-template <typename T>
-__global__ void k_first(const typename CT<T>::V* __restrict__ psi, double* c, Level root) {
-  const double p0 = prob_of<T>(psi[0]);
-  c[0] = p0;
-  root.carry[0] = p0;
-}
-
 // searchsortedfirst(v, x) over 1-based v[1..n]: Julia Base's halving loop
 // (lo, len) with lt = isless.  Returns the 1-based index, n+1 if none.
-__global__ void k_search(const double* __restrict__ c, uint64_t n, const double* __restrict__ r,
-                         int64_t nd, uint64_t* __restrict__ out) {
-  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= nd) return;
-  const double x = r[d];
+__device__ inline uint64_t search_first(const double* __restrict__ c, uint64_t n, double x) {
   uint64_t lo = 1, len = n;
   while (len != 0) {
     const uint64_t half = len >> 1;
@@ -136,7 +122,97 @@ __global__ void k_search(const double* __restrict__ c, uint64_t n, const double*
       len = half;
     }
   }
-  out[d] = lo - 1;  // 0-based; equals n when r exceeds every prefix
+  return lo - 1;  // 0-based; equals n when r exceeds every prefix
+}
+
+__global__ void k_tree_level(Level parent, Level child, uint64_t nslots) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nslots) tree_level_at(parent, child, j);
+}
+
+template <typename T>
+__global__ void k_leaf_sum(const typename CT<T>::V* __restrict__ psi, Level lv, uint64_t nslots) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nslots) leaf_sum_at<T>(psi, lv, j);
+}
+
+__global__ void k_up(Level lv, Level child, uint64_t nslots) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nslots) up_at(lv, child, j);
+}
+
+__global__ void k_down(Level lv, Level child, uint64_t nslots) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nslots) down_at(lv, child, j);
+}
+
+template <typename T>
+__global__ void k_leaf_write(const typename CT<T>::V* __restrict__ psi, Level lv, uint64_t nslots,
+                             double* __restrict__ c) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nslots) leaf_write_at<T>(psi, lv, j, c);
+}
+
+template <typename T>
+__global__ void k_first(const typename CT<T>::V* __restrict__ psi, double* c, Level root) {
+  const double p0 = prob_of<T>(psi[0]);
+  c[0] = p0;
+  root.carry[0] = p0;
+}
+
+__global__ void k_search(const double* __restrict__ c, uint64_t n, const double* __restrict__ r,
+                         int64_t nd, uint64_t* __restrict__ out) {
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < nd) out[d] = search_first(c, n, r[d]);
+}
+
+// The whole pairwise cumsum and the draws in one workgroup (short vectors:
+// the per-level launches would cost more than the work).  Barriers separate
+// the phases; the level arrays live in global memory, which the workgroup's
+// barrier makes visible to all its threads.
+constexpr int kMaxLevels = 24;
+constexpr uint64_t kBlockCumsumMax = 1ull << 16;
+struct LevelSet {
+  Level lv[kMaxLevels];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+    k_cumsum_block(const typename CT<T>::V* __restrict__ psi, uint64_t n, LevelSet ls, int D,
+                   double* __restrict__ c, const double* __restrict__ r, int64_t nd,
+                   uint64_t* __restrict__ out) {
+  const uint64_t t = threadIdx.x, B = blockDim.x;
+  if (t == 0) {  // root: elements 2..n (1-based)
+    ls.lv[0].start[0] = 1;
+    ls.lv[0].len[0] = (uint32_t)(n - 1);
+    ls.lv[0].flag[0] = n - 1 < kLeaf ? 1 : 0;
+  }
+  __syncthreads();
+  for (int d = 1; d <= D; ++d) {
+    for (uint64_t j = t; j < (1ull << d); j += B) tree_level_at(ls.lv[d - 1], ls.lv[d], j);
+    __syncthreads();
+  }
+  for (int d = 0; d <= D; ++d)
+    for (uint64_t j = t; j < (1ull << d); j += B) leaf_sum_at<T>(psi, ls.lv[d], j);
+  __syncthreads();
+  for (int d = D - 1; d >= 0; --d) {
+    for (uint64_t j = t; j < (1ull << d); j += B) up_at(ls.lv[d], ls.lv[d + 1], j);
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double p0 = prob_of<T>(psi[0]);
+    c[0] = p0;
+    ls.lv[0].carry[0] = p0;
+  }
+  __syncthreads();
+  for (int d = 0; d < D; ++d) {
+    for (uint64_t j = t; j < (1ull << d); j += B) down_at(ls.lv[d], ls.lv[d + 1], j);
+    __syncthreads();
+  }
+  for (int d = 0; d <= D; ++d)
+    for (uint64_t j = t; j < (1ull << d); j += B) leaf_write_at<T>(psi, ls.lv[d], j, c);
+  __syncthreads();
+  for (int64_t dd = (int64_t)t; dd < nd; dd += (int64_t)B) out[dd] = search_first(c, n, r[dd]);
 }
 
 // Sequential running sum, one thread (exact reproduction of the loop).  With
@@ -232,6 +308,17 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
       L[d].flag = (uint8_t*)take(ns);
       L[d].sum = (double*)take(ns * sizeof(double));
       L[d].carry = (double*)take(ns * sizeof(double));
+    }
+    if (n <= kBlockCumsumMax && D < kMaxLevels) {
+      LevelSet ls;
+      for (int d = 0; d <= D; ++d) ls.lv[d] = L[d];
+      hipLaunchKernelGGL(k_cumsum_block<T>, dim3(1), dim3(1024), 0, ctx->stream, psi, n, ls, D,
+                         c, rd, nd, od);
+      SMOQ_CHECK_LAUNCH("k_cumsum_block");
+      SMOQ_HIP(hipMemcpyAsync(out_host, od, nd * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+      SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+      return SMOQ_OK;
     }
     // root: elements 2..n (1-based) = [1, n) 0-based
     uint64_t rs = 1;

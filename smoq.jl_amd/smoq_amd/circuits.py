@@ -22,10 +22,15 @@ _NQ = {"CNOT": 2, "CZ": 2, "CRY": 2, "RXX": 2, "RYY": 2, "RZZ": 2, "U2": 2, "DIA
 
 
 class Tape:
+    """A gate list for smoq_apply_tape / smoq_plan_create.  Change it through
+    add(): the C array handed to the library is built once and reused until
+    the next add()."""
+
     def __init__(self, n_qubits: int):
         self.n = n_qubits
         self.ops: list[tuple] = []  # (gate, qubits, theta, param, matrix)
         self.n_params = 0
+        self._ct = None
 
     def add(self, gate: str, qubits, theta: float = 0.0, param: int = -1, m=None) -> "Tape":
         gate = gate.upper()
@@ -37,12 +42,21 @@ class Tape:
         if param >= 0:
             self.n_params = max(self.n_params, param + 1)
         self.ops.append((gate, qubits, float(theta), int(param), m))
+        self._ct = None
         return self
 
     def __len__(self):
         return len(self.ops)
 
     def to_ctypes(self):
+        """The smoq_op array of the tape (cached; read-only for callers)."""
+        if self._ct is not None and self._ct[0] == len(self.ops):
+            return self._ct[1]
+        arr = self._build_ctypes()
+        self._ct = (len(self.ops), arr)
+        return arr
+
+    def _build_ctypes(self):
         arr = (smoq_op * max(1, len(self.ops)))()
         for i, (gate, qs, th, par, m) in enumerate(self.ops):
             o = arr[i]
