@@ -799,6 +799,19 @@ def clifford_shadow_snapshot(psi, N, indices, r, table):
     return int(sample_sequential(C @ np.asarray(psi, np.complex128), N, [r])[0])
 
 
+def rho_shadow_snapshot(rho, N, mats, r):
+    """classical_shadow_snapshot(ρ, N) / clifford_shadow_snapshot(ρ, N)
+    (…ClassicalShadows.jl:608-632, 1039-1070): U ρ U' with U the Kronecker
+    product of the per-qubit `mats` (qubit 1 lowest), populations clamped at
+    0 and divided by their sum, then sample_from_probabilities."""
+    U = np.asarray(mats[N - 1], np.complex128)
+    for j in range(N - 2, -1, -1):
+        U = np.kron(U, mats[j])
+    p = np.maximum(np.real(np.diag(U @ rho @ U.conj().T)), 0.0)
+    p = p / np.sum(p)
+    return sample_from_probabilities(p, r)
+
+
 # ---------------------------------------------------------- preparation --
 def product_ket(kets):
     """make_product_ket (utils/cpu/cpuQuantumStatePreparation.jl:1182-1191):

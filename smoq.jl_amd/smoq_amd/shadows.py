@@ -171,20 +171,65 @@ def collect_shadows(state, N, M: int | None = None, *, bases=None, uniforms=None
         if group == "global_clifford":
             raise NotImplementedError("global Clifford snapshots need a dense 2^N unitary per "
                                       "snapshot and are not on the device path")
-    if not isinstance(state, GPUState):
-        raise TypeError("snapshots are taken of a GPUState")
-    if state.n != N:
-        raise ValueError(f"state has {state.n} qubits, N = {N}")
+    from .density import DensityMatrix
+
+    if not isinstance(state, (GPUState, DensityMatrix)):
+        raise TypeError("snapshots are taken of a GPUState or a DensityMatrix")
+    if (state.N if isinstance(state, DensityMatrix) else state.n) != N:
+        raise ValueError(f"state has the wrong number of qubits for N = {N}")
     g = rng if rng is not None else np.random.default_rng()
     hi = 4 if group == "pauli" else 25
     b = (g.integers(1, hi, (M, N)) if bases is None else np.asarray(bases)).astype(np.uint8)
     b = b.reshape(M, N)
     u = g.random(M) if uniforms is None else np.asarray(uniforms, np.float64)
+    if isinstance(state, DensityMatrix):
+        return ShadowSet(b, _outcomes(_rho_outcomes(state, b, u, group), N), group)
     if group == "pauli":
         idx = shadow_outcomes(state, b, u, SHADOW_PAULI)
     else:
         idx = shadow_outcomes(state, b - 1, u, SHADOW_LOCAL_CLIFFORD)
     return ShadowSet(b, _outcomes(idx, N), group)
+
+
+def _rho_outcomes(rho, codes: np.ndarray, u: np.ndarray, group: str) -> np.ndarray:
+    """Snapshots of a density matrix (classical_shadow_snapshot(ρ, N),
+    …ClassicalShadows.jl:1039-1070; clifford_shadow_snapshot(ρ, N), :608-632):
+    for each distinct rotation string, U ρ U† on a copy of vec(ρ) (V on row
+    qubit q, conj(V) on column qubit N + q, through the fused passes), then
+    every draw of that string from the populations real(ρ_ii) with
+    sample_from_probabilities' rule.  The reference clamps the populations at
+    0 and divides by their sum first; both change them only at rounding
+    level, so a draw can differ only within rounding of a cumulative sum."""
+    from .circuits import Tape
+    from .density import dm_tape
+    from .measurements import sample_indices
+
+    if rho.vec.batch != 1:
+        raise ValueError("snapshots are taken of one density matrix (batch 1)")
+    N = rho.N
+    keys, inv = np.unique(codes, axis=0, return_inverse=True)
+    out = np.empty(codes.shape[0], np.uint64)
+    work = rho.vec.clone()
+    try:
+        for k, key in enumerate(keys):
+            t = Tape(N)
+            for q in range(N):
+                c = int(key[q])
+                if group == "pauli":
+                    if c == 1:
+                        t.add("H", [q])
+                    elif c == 2:
+                        t.add("SDG", [q]).add("H", [q])
+                else:
+                    t.add("U1", [q], m=SINGLE_QUBIT_CLIFFORDS[c - 1])
+            work.copy_from(rho.vec)
+            if len(t):
+                dm_tape(t).apply(work)
+            sel = np.nonzero(inv.reshape(-1) == k)[0]
+            out[sel] = sample_indices(work, u[sel], "sequential_diag")
+    finally:
+        work.free()
+    return out
 
 
 def classical_shadow_snapshot(state, N: int, bases=None, r=None, *, rng=None) -> PauliSnapshot:

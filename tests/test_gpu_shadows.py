@@ -123,3 +123,28 @@ def test_arguments_are_checked(ctx):
     sh = collect_shadows(s, ShadowConfig(3, 10, "clifford_group"), rng=np.random.default_rng(1))
     assert sh.kind == "local_clifford" and len(sh) == 10
     s.free()
+
+
+@pytest.mark.parametrize("group", ["pauli", "local_clifford"])
+def test_density_matrix_snapshots(ctx, group):
+    from smoq_amd.density import DensityMatrix
+    from smoq_amd.shadows import SINGLE_QUBIT_CLIFFORDS, ShadowConfig, collect_shadows
+
+    N, M = 3, 600
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    rho = A @ A.conj().T
+    rho /= np.trace(rho).real
+    d = DensityMatrix.from_numpy(rho)
+    hi = 4 if group == "pauli" else 25
+    bases = rng.integers(1, hi, (M, N))
+    u = rng.random(M)
+    sh = collect_shadows(d, ShadowConfig(N, M, group), bases=bases, uniforms=u)
+    H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    rot = {1: H, 2: H @ np.diag([1, -1j]), 3: np.eye(2)}
+    for m in range(M):
+        mats = [rot[int(c)] if group == "pauli" else SINGLE_QUBIT_CLIFFORDS[int(c) - 1]
+                for c in bases[m]]
+        i = orc.rho_shadow_snapshot(rho, N, mats, u[m])
+        assert list(sh.outcomes[m]) == [(i >> k) & 1 for k in range(N)], m
+    d.free()

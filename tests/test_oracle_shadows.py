@@ -90,3 +90,17 @@ def test_clifford_snapshot_inverse_cdf():
             if p[i] < 1e-9:
                 continue
             assert orc.clifford_shadow_snapshot(psi, N, idx, r, CL) == i
+
+
+def test_rho_snapshot_matches_pure_state_form():
+    """On |ψ><ψ| the ρ form draws what the ψ form draws (away from ties)."""
+    N = 3
+    psi = rand_psi(N, 11)
+    rho = np.outer(psi, psi.conj())
+    rng = np.random.default_rng(3)
+    for _ in range(40):
+        bases = rng.integers(1, 4, N)
+        r = rng.random()
+        a = orc.pauli_shadow_snapshot(psi, N, bases, r)
+        b = orc.rho_shadow_snapshot(rho, N, [ROT[int(x)] for x in bases], r)
+        assert a == b
