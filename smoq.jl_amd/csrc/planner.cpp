@@ -16,6 +16,7 @@
 // H, X) and Rx-type matrices use the cheaper arithmetic.
 #include <algorithm>
 #include <complex>
+#include <cstdlib>
 #include <map>
 
 #include "smoq_internal.h"
@@ -267,10 +268,17 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
   const int m = (k == n) ? k : std::min(low_qubits, k - 2);
   std::vector<int> remaining(gates.size());
   for (size_t i = 0; i < gates.size(); ++i) remaining[i] = (int)i;
+  // optional cap on the non-diagonal gates one pass takes (0: none), read
+  // once from SMOQ_PASS_MAX_GATES: spreads deep passes over more passes
+  static const int max_gates = [] {
+    const char* e = std::getenv("SMOQ_PASS_MAX_GATES");
+    return e ? std::atoi(e) : 0;
+  }();
 
   while (!remaining.empty()) {
     uint64_t L = (m >= 64) ? ~0ull : ((1ull << m) - 1);
     uint64_t blocked = 0;
+    int nondiag = 0;
     std::vector<int> taken, deferred;
     for (int gi : remaining) {
       const IGate& g = gates[gi];
@@ -280,10 +288,16 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
         blocked |= touch;
         continue;
       }
+      if (max_gates > 0 && g.kind != IK_DIAG && nondiag >= max_gates && !taken.empty()) {
+        deferred.push_back(gi);
+        blocked |= touch;
+        continue;
+      }
       uint64_t nl = L | target_mask(g);
       if (popc(nl) <= k) {
         L = nl;
         taken.push_back(gi);
+        nondiag += g.kind != IK_DIAG;
       } else {
         deferred.push_back(gi);
         blocked |= touch;
