@@ -15,6 +15,7 @@
 
 #include <cstdlib>
 #include <sstream>
+#include <type_traits>
 
 #include "device_common.h"
 #include "smoq_internal.h"
@@ -40,10 +41,50 @@ static const char* u1_kind_name(int kind) {
   }
 }
 
+// The kernel cache key: the generator's output in binary — every literal it
+// writes by address, every value by its bytes, every stream manipulator by
+// address — so two programs share a key exactly when the generator would
+// write the same source for them, without formatting that source.
+struct KeyOut {
+  std::string k;
+  void put(const void* p, size_t n) { k.append(static_cast<const char*>(p), n); }
+  KeyOut& operator<<(const char* s) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(s);
+    put(&a, sizeof a);
+    return *this;
+  }
+  KeyOut& operator<<(const std::string& s) {
+    const uint64_t n = s.size();
+    put(&n, sizeof n);
+    put(s.data(), s.size());
+    return *this;
+  }
+  template <typename I, typename std::enable_if<std::is_integral<I>::value, int>::type = 0>
+  KeyOut& operator<<(I v) {
+    const uint64_t w = (uint64_t)v;
+    const uint8_t sz = sizeof(I);
+    put(&sz, 1);
+    put(&w, sizeof w);
+    return *this;
+  }
+  KeyOut& operator<<(std::ios_base& (*f)(std::ios_base&)) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(f);
+    put(&a, sizeof a);
+    return *this;
+  }
+};
+
+// "gates[g]" in the source, g in the key
+struct GateRef {
+  int g;
+};
+static std::ostream& operator<<(std::ostream& o, GateRef r) { return o << "gates[" << r.g << "]"; }
+static KeyOut& operator<<(KeyOut& o, GateRef r) { return o << "gates[" << r.g; }
+
 // Bit-field expression for a linear map of the thread index: thread bit i
 // goes to bit dst[i]; runs of consecutive bits move as one field.
-static std::string bit_map_expr(const std::vector<int>& dst, bool wide) {
-  std::ostringstream o;
+template <typename O>
+static void bit_map_emit(O& o, const std::vector<int>& dst, bool wide) {
   bool first = true;
   size_t i = 0;
   while (i < dst.size()) {
@@ -57,7 +98,6 @@ static std::string bit_map_expr(const std::vector<int>& dst, bool wide) {
     i = j;
   }
   if (first) o << (wide ? "(uint64_t)0" : "0u");
-  return o.str();
 }
 
 // Uncontrolled single-qubit gates with the structure of an Ry or Rx product
@@ -101,8 +141,8 @@ void factor_rotations(Program& prog) {
   }
 }
 
-std::string jit_source(const Program& prog, int dtype) {
-  std::ostringstream o;
+template <typename O>
+static void jit_emit(const Program& prog, int dtype, O& o) {
   const char* T = dtype == SMOQ_C128 ? "double" : "float";
   o << "#include \"fused_ops.h\"\nusing namespace smoq;\n";
   for (size_t p = 0; p < prog.passes.size(); ++p) {
@@ -140,11 +180,14 @@ std::string jit_source(const Program& prog, int dtype) {
           if ((i >> j) & 1) r ^= st.sw_reg[j];
         o << (i ? ", " : "") << r << "u";
       }
-      o << "};\n    stage_load_c<T>(lds, " << bit_map_expr(lpos, false) << ", f.gbase | "
-        << bit_map_expr(gpos, true) << ", sr, x, sf);\n";
+      o << "};\n    stage_load_c<T>(lds, ";
+      bit_map_emit(o, lpos, false);
+      o << ", f.gbase | ";
+      bit_map_emit(o, gpos, true);
+      o << ", sr, x, sf);\n";
       for (int g = st.g0; g < st.g1; ++g) {
         const GateDev& G = prog.gates[g];
-        const std::string gd = "gates[" + std::to_string(g) + "]";
+        const GateRef gd{g};
         switch (G.kind) {
           case GK_U1_REAL:
           case GK_U1_RXT:
@@ -197,7 +240,19 @@ std::string jit_source(const Program& prog, int dtype) {
     else
       o << "  pass_store<T>(lds, f);\n}\n";
   }
+}
+
+std::string jit_source(const Program& prog, int dtype) {
+  std::ostringstream o;
+  jit_emit(prog, dtype, o);
   return o.str();
+}
+
+static std::string jit_key(const Program& prog, int dtype) {
+  KeyOut o;
+  o.k.reserve(64 * prog.gates.size() + 4096);
+  jit_emit(prog, dtype, o);
+  return std::move(o.k);
 }
 
 JitModule::~JitModule() {
@@ -272,16 +327,16 @@ bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits) {
 
 int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out) {
   out.reset();
-  std::string src = jit_source(prog, dtype);
-  auto it = ctx->jit_cache.find(src);
+  std::string key = jit_key(prog, dtype);
+  auto it = ctx->jit_cache.find(key);
   if (it != ctx->jit_cache.end()) {
     out = it->second;
     return SMOQ_OK;
   }
   std::shared_ptr<JitModule> m;
-  const int rc = compile(src, prog, m);
+  const int rc = compile(jit_source(prog, dtype), prog, m);
   if (rc) return rc;
-  ctx->jit_cache.emplace(std::move(src), m);
+  ctx->jit_cache.emplace(std::move(key), m);
   out = m;
   return SMOQ_OK;
 }
