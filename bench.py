@@ -534,8 +534,22 @@ def shadow_snapshots():
         s.free()
         rows.append({"N": N, "snapshots": M, "seconds": t, "snapshots_per_s": M / t})
         log(f"shadows N={N} M={M}: {t:.3f} s")
+    gc = []
+    from smoq_amd.shadows import collect_global_clifford_shadows
+    for N, M in ((2, 100000), (4, 100000), (6, 100000), (10, 10000)):
+        rng = np.random.default_rng(N)
+        v = rng.normal(size=1 << N) + 1j * rng.normal(size=1 << N)
+        s = GPUState.from_numpy(v / np.linalg.norm(v))
+        collect_global_clifford_shadows(s, N, 100, rng=rng)
+        t0 = time.perf_counter()
+        collect_global_clifford_shadows(s, N, M, rng=rng)
+        t = time.perf_counter() - t0
+        s.free()
+        gc.append({"N": N, "snapshots": M, "seconds_including_circuit_draws": t,
+                   "snapshots_per_s": M / t})
+        log(f"global Clifford shadows N={N} M={M}: {t:.3f} s")
     return {"workload": "collect_shadows (Pauli) of a random state, complex128",
-            "by_N": rows}
+            "by_N": rows, "global_clifford": gc}
 
 
 def main():

@@ -236,6 +236,15 @@ enum smoq_shadow_kind { SMOQ_SHADOW_PAULI = 0, SMOQ_SHADOW_LOCAL_CLIFFORD = 1 };
 int smoq_shadow_snapshots(const smoq_state* psi, int64_t n_snap, int kind, const uint8_t* codes,
                           const double* clifford_table, const double* uniforms,
                           uint64_t* out_index);
+/* Global-Clifford snapshots of batch item 0 (global_clifford_shadow_snapshot,
+ * utils/cpu/cpuClassicalShadows.jl:884-903): snapshot m applies gates
+ * offsets[m] .. offsets[m+1]-1 of `gates` (triples type, a, b: 0 = H on bit a,
+ * 1 = S on bit a, 2 = CNOT control bit a, target bit b) to a copy of psi and
+ * draws with sample_from_probabilities' rule on uniforms[m].  The reference
+ * multiplies by the circuit's dense unitary instead.  At most 12 qubits. */
+int smoq_global_clifford_snapshots(const smoq_state* psi, int64_t n_snap, const uint8_t* gates,
+                                   const int64_t* offsets, const double* uniforms,
+                                   uint64_t* out_index);
 
 /* ---- single-qubit stochastic operations ------------------------------------
  * For k = 0 .. n_qubits-1 in order and every batch item b, with the caller's

@@ -812,6 +812,43 @@ def rho_shadow_snapshot(rho, N, mats, r):
     return sample_from_probabilities(p, r)
 
 
+def global_clifford_unitary(N, gates):
+    """build_global_clifford_unitary (utils/cpu/cpuClassicalShadows.jl:835-870):
+    U = G_L ... G_1; ("H", q) and ("S", q) as kron(G_1, ..., G_N) with the
+    gate at factor q (bit N - q), ("CNOT", c, t) on bits c-1 -> t-1."""
+    dim = 1 << N
+    H = np.array([[1, 1], [1, -1]], np.complex128) / np.sqrt(2)
+    S = np.array([[1, 0], [0, 1j]], np.complex128)
+    U = np.eye(dim, dtype=np.complex128)
+    for gt in gates:
+        if gt[0] in ("H", "S"):
+            P = H if gt[0] == "H" else S
+            G = np.ones((1, 1), np.complex128)
+            for j in range(1, N + 1):
+                G = np.kron(G, P if j == gt[1] else np.eye(2))
+        else:
+            G = np.zeros((dim, dim), np.complex128)
+            for k in range(dim):
+                if (k >> (gt[1] - 1)) & 1:
+                    G[k ^ (1 << (gt[2] - 1)), k] = 1
+                else:
+                    G[k, k] = 1
+        U = G @ U
+    return U
+
+
+def global_clifford_snapshot(state, N, gates, r):
+    """global_clifford_shadow_snapshot (…ClassicalShadows.jl:884-921): U ψ (or
+    the diagonal of U ρ U'), then sample_from_probabilities; 0-based index."""
+    U = global_clifford_unitary(N, gates)
+    st = np.asarray(state, np.complex128)
+    if st.ndim == 1:
+        p = np.abs(U @ st) ** 2
+    else:
+        p = np.real(np.diag(U @ st @ U.conj().T))
+    return sample_from_probabilities(p, r)
+
+
 # ---------------------------------------------------------- preparation --
 def product_ket(kets):
     """make_product_ket (utils/cpu/cpuQuantumStatePreparation.jl:1182-1191):

@@ -210,6 +210,65 @@ __global__ void __launch_bounds__(64)
     for (; k < d1; ++k) out[pos[k]] = dim - 1;
 }
 
+// Global Clifford snapshots (global_clifford_shadow_snapshot,
+// …ClassicalShadows.jl:884-903): one workgroup per snapshot holds psi in LDS,
+// applies the snapshot's gate list — 0 = H on bit a, 1 = S on bit a,
+// 2 = CNOT with control bit a and target bit b — and draws with
+// sample_from_probabilities' rule (thread 0 walks the prefix in index order).
+constexpr int kGcMaxQubits = 12;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_global_clifford(const VT_alias<T>* __restrict__ psi, int n, const uint8_t* __restrict__ g,
+                      const int64_t* __restrict__ off, const double* __restrict__ r,
+                      uint64_t* __restrict__ out, double s) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char gc_smem[];
+  double2* x = reinterpret_cast<double2*>(gc_smem);  // 2^n amplitudes
+  const uint64_t m = blockIdx.x;
+  const int dim = 1 << n, t = threadIdx.x, nt = blockDim.x;
+  for (int i = t; i < dim; i += nt) {
+    const VT_alias<T> v = psi[i];
+    x[i] = make_double2((double)v.x, (double)v.y);
+  }
+  __syncthreads();
+  for (int64_t k = off[m]; k < off[m + 1]; ++k) {
+    const int type = g[3 * k], a = g[3 * k + 1], b = g[3 * k + 2];
+    const int ma = 1 << a, mb = 1 << b;
+    for (int p = t; p < dim / 2; p += nt) {
+      const int i0 = ((p >> a) << (a + 1)) | (p & (ma - 1)), i1 = i0 | ma;
+      if (type == 0) {
+        const double2 u = x[i0], w = x[i1];
+        x[i0] = make_double2(s * (u.x + w.x), s * (u.y + w.y));
+        x[i1] = make_double2(s * (u.x - w.x), s * (u.y - w.y));
+      } else if (type == 1) {
+        const double2 w = x[i1];
+        x[i1] = make_double2(-w.y, w.x);  // times i
+      } else if (!(i1 & mb)) {            // control set, target clear: swap with target set
+        const double2 u = x[i1];
+        x[i1] = x[i1 | mb];
+        x[i1 | mb] = u;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double rr = r[m];
+    double acc = 0.0;
+    int idx = dim - 1;
+    for (int i = 0; i < dim; ++i) {
+      const double xx = x[i].x * x[i].x;
+      const double yy = x[i].y * x[i].y;
+      acc = acc + (xx + yy);
+      if (rr <= acc) {
+        idx = i;
+        break;
+      }
+    }
+    out[m] = (uint64_t)idx;
+  }
+}
+
 template <typename T>
 static int shadows_t(const smoq_state* s, int64_t n_snap, const uint8_t* codes, int clifford,
                      const double* cliff_host, const double* uniforms, uint64_t* out_index) {
@@ -354,6 +413,67 @@ static int shadows_t(const smoq_state* s, int64_t n_snap, const uint8_t* codes, 
 }  // namespace smoq
 
 using namespace smoq;
+
+extern "C" int smoq_global_clifford_snapshots(const smoq_state* psi, int64_t n_snap,
+                                              const uint8_t* gates, const int64_t* offsets,
+                                              const double* uniforms, uint64_t* out_index) {
+  int rc = check_state(psi);
+  if (rc) return rc;
+  if (n_snap < 0 || (n_snap > 0 && (!gates || !offsets || !uniforms || !out_index)))
+    return set_error(SMOQ_EINVAL, "smoq_global_clifford_snapshots: bad snapshot arrays");
+  if (psi->n > kGcMaxQubits)
+    return set_error(SMOQ_ENOTSUP, "smoq_global_clifford_snapshots: more than %d qubits",
+                     kGcMaxQubits);
+  if (n_snap == 0) return SMOQ_OK;
+  if (n_snap > 0x7fffffff)
+    return set_error(SMOQ_ENOTSUP, "smoq_global_clifford_snapshots: more than 2^31 snapshots");
+  if (offsets[0] != 0) return set_error(SMOQ_EINVAL, "smoq_global_clifford_snapshots: offsets[0] != 0");
+  for (int64_t m = 0; m < n_snap; ++m)
+    if (offsets[m + 1] < offsets[m])
+      return set_error(SMOQ_EINVAL, "smoq_global_clifford_snapshots: offsets decrease at %lld",
+                       (long long)m);
+  const int64_t ng = offsets[n_snap];
+  for (int64_t k = 0; k < ng; ++k) {
+    const int type = gates[3 * k], a = gates[3 * k + 1], b = gates[3 * k + 2];
+    if (type > 2 || a >= psi->n || (type == 2 && (b >= psi->n || b == a)))
+      return set_error(SMOQ_EINVAL, "smoq_global_clifford_snapshots: bad gate %lld", (long long)k);
+  }
+  smoq_ctx* ctx = psi->ctx;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t gb = up((size_t)ng * 3 + 1), ob = up((size_t)(n_snap + 1) * 8),
+               rb = up((size_t)n_snap * 8);
+  unsigned char* dev = nullptr;
+  SMOQ_HIP(hipMalloc(&dev, gb + ob + rb + (size_t)n_snap * 8));
+  uint8_t* dg = dev;
+  int64_t* doff = (int64_t*)(dev + gb);
+  double* dr = (double*)(dev + gb + ob);
+  uint64_t* dout = (uint64_t*)(dev + gb + ob + rb);
+  hipError_t e = hipMemcpyAsync(dg, gates, (size_t)ng * 3, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(doff, offsets, (size_t)(n_snap + 1) * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dr, uniforms, (size_t)n_snap * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    const double sq = 1 / std::sqrt(2.0);
+    // one wave for small states (its barriers are cheap), up to 256 threads
+    const unsigned nt = (unsigned)std::min<uint64_t>(256, std::max<uint64_t>(64, psi->dim / 2));
+    const size_t shm = (size_t)psi->dim * sizeof(double2);
+    if (psi->dtype == SMOQ_C128)
+      hipLaunchKernelGGL(k_global_clifford<double>, dim3((unsigned)n_snap), dim3(nt), shm,
+                         ctx->stream, (const double2*)psi->d, psi->n, dg, doff, dr, dout, sq);
+    else
+      hipLaunchKernelGGL(k_global_clifford<float>, dim3((unsigned)n_snap), dim3(nt), shm,
+                         ctx->stream, (const float2*)psi->d, psi->n, dg, doff, dr, dout, sq);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out_index, dout, (size_t)n_snap * 8, hipMemcpyDeviceToHost, ctx->stream);
+  const hipError_t se = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return hip_error(e, "smoq_global_clifford_snapshots");
+  if (se != hipSuccess) return hip_error(se, "smoq_global_clifford_snapshots");
+  return SMOQ_OK;
+}
 
 extern "C" int smoq_shadow_snapshots(const smoq_state* psi, int64_t n_snap, int kind,
                                      const uint8_t* codes, const double* clifford_table,

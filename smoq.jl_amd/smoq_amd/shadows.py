@@ -169,8 +169,7 @@ def collect_shadows(state, N, M: int | None = None, *, bases=None, uniforms=None
         cfg = N
         N, M, group = cfg.n_qubits, cfg.n_shots, cfg.measurement_group
         if group == "global_clifford":
-            raise NotImplementedError("global Clifford snapshots need a dense 2^N unitary per "
-                                      "snapshot and are not on the device path")
+            return collect_global_clifford_shadows(state, N, M, uniforms=uniforms, rng=rng)
     from .density import DensityMatrix
 
     if not isinstance(state, (GPUState, DensityMatrix)):
@@ -232,6 +231,211 @@ def _rho_outcomes(rho, codes: np.ndarray, u: np.ndarray, group: str) -> np.ndarr
     return out
 
 
+@dataclass
+class GlobalCliffordCircuit:
+    """GlobalCliffordCircuit (…ClassicalShadows.jl:771-779): gates as
+    ("H", q), ("S", q), ("CNOT", c, t), 1-based qubits."""
+    n_qubits: int
+    gates: list
+
+
+@dataclass
+class GlobalCliffordSnapshot:
+    """GlobalCliffordSnapshot (…ClassicalShadows.jl:786-789)."""
+    circuit: GlobalCliffordCircuit
+    outcomes: np.ndarray
+
+
+def sample_random_global_clifford(N: int, depth: int | None = None,
+                                  rng=None) -> GlobalCliffordCircuit:
+    """sample_random_global_clifford (…ClassicalShadows.jl:802-826): per layer
+    and qubit one of H, S, H then S, or nothing (1:4 uniform); between layers
+    CNOTs on consecutive pairs of a shuffled qubit order."""
+    g = rng if rng is not None else np.random.default_rng()
+    depth = 2 * N if depth is None else int(depth)
+    gates = []
+    for layer in range(1, depth + 1):
+        for q in range(1, N + 1):
+            r = int(g.integers(1, 5))
+            if r == 1:
+                gates.append(("H", q))
+            elif r == 2:
+                gates.append(("S", q))
+            elif r == 3:
+                gates += [("H", q), ("S", q)]
+        if layer < depth and N >= 2:
+            order = g.permutation(np.arange(1, N + 1))
+            for i in range(0, N - 1, 2):
+                gates.append(("CNOT", int(order[i]), int(order[i + 1])))
+    return GlobalCliffordCircuit(N, gates)
+
+
+def _gc_bits(circuit: GlobalCliffordCircuit):
+    """Device triples of a circuit.  The reference's unitary places H and S
+    on qubit q as the q-th Kronecker factor from the left (bit N - q) and
+    CNOT on bits c - 1, t - 1 (…ClassicalShadows.jl:835-870); kept."""
+    N = circuit.n_qubits
+    out = []
+    for gt in circuit.gates:
+        if gt[0] == "H":
+            out.append((0, N - gt[1], 0))
+        elif gt[0] == "S":
+            out.append((1, N - gt[1], 0))
+        else:
+            out.append((2, gt[1] - 1, gt[2] - 1))
+    return out
+
+
+def build_global_clifford_unitary(circuit: GlobalCliffordCircuit) -> np.ndarray:
+    """build_global_clifford_unitary (…ClassicalShadows.jl:835-870), dense
+    (small N): U = G_L ... G_1 with H/S as kron(G_1, ..., G_N) and CNOT on
+    bits c - 1 -> t - 1."""
+    N = circuit.n_qubits
+    dim = 1 << N
+    H = np.array([[1, 1], [1, -1]], np.complex128) / np.sqrt(2)
+    S = np.array([[1, 0], [0, 1j]], np.complex128)
+    I2 = np.eye(2, dtype=np.complex128)
+    U = np.eye(dim, dtype=np.complex128)
+    for gt in circuit.gates:
+        if gt[0] in ("H", "S"):
+            P = H if gt[0] == "H" else S
+            G = P if gt[1] == 1 else I2
+            for j in range(2, N + 1):
+                G = np.kron(G, P if j == gt[1] else I2)
+        else:
+            c, t = gt[1], gt[2]
+            G = np.zeros((dim, dim), np.complex128)
+            k = np.arange(dim)
+            dst = np.where((k >> (c - 1)) & 1, k ^ (1 << (t - 1)), k)
+            G[dst, k] = 1
+        U = G @ U
+    return U
+
+
+def _random_gc_triples(N: int, M: int, depth: int, g):
+    """M circuits of sample_random_global_clifford's structure drawn at once,
+    as the device's flat triples (type, bit a, bit b) and per-circuit offsets."""
+    R = g.integers(1, 5, (M, depth, N))
+    q = np.arange(1, N + 1)
+    # per (circuit, layer): H and S slots of qubits 1..N in order, then CNOTs
+    qt = np.broadcast_to(np.array([0, 1], np.uint8), (M, depth, N, 2))
+    qa = np.broadcast_to((N - q).astype(np.uint8)[:, None], (M, depth, N, 2))
+    qv = np.stack([(R == 1) | (R == 3), (R == 2) | (R == 3)], axis=-1)
+    P = N // 2 if N >= 2 else 0
+    perm = np.argsort(g.random((M, depth, N)), axis=-1).astype(np.uint8)  # 0-based qubits
+    ct = np.full((M, depth, P), 2, np.uint8)
+    ca, cb = perm[..., 0:2 * P:2], perm[..., 1:2 * P:2]
+    cv = np.ones((M, depth, P), bool)
+    cv[:, depth - 1, :] = False  # no CNOT layer after the last
+    T = np.concatenate([qt.reshape(M, depth, 2 * N), ct], axis=2).reshape(M, -1)
+    A = np.concatenate([qa.reshape(M, depth, 2 * N), ca], axis=2).reshape(M, -1)
+    B = np.concatenate([np.zeros((M, depth, 2 * N), np.uint8), cb], axis=2).reshape(M, -1)
+    V = np.concatenate([qv.reshape(M, depth, 2 * N), cv], axis=2).reshape(M, -1)
+    off = np.zeros(M + 1, np.int64)
+    off[1:] = np.cumsum(V.sum(axis=1))
+    flat = np.stack([T[V], A[V], B[V]], axis=1).astype(np.uint8)
+    return flat, off
+
+
+def _circuit_of(N: int, trip: np.ndarray) -> GlobalCliffordCircuit:
+    gates = []
+    for typ, a, b in trip:
+        if typ == 0:
+            gates.append(("H", N - int(a)))
+        elif typ == 1:
+            gates.append(("S", N - int(a)))
+        else:
+            gates.append(("CNOT", int(a) + 1, int(b) + 1))
+    return GlobalCliffordCircuit(N, gates)
+
+
+class GlobalCliffordShadowSet:
+    """M global-Clifford snapshots: the circuits as flat device triples with
+    offsets, the outcomes as an (M, N) array; indexing gives a
+    GlobalCliffordSnapshot."""
+
+    def __init__(self, N, triples, offsets, outcomes):
+        self.N, self.triples, self.offsets, self.outcomes = N, triples, offsets, outcomes
+
+    def __len__(self):
+        return self.outcomes.shape[0]
+
+    def circuit(self, m) -> GlobalCliffordCircuit:
+        return _circuit_of(self.N, self.triples[self.offsets[m]:self.offsets[m + 1]])
+
+    def __getitem__(self, m):
+        return GlobalCliffordSnapshot(self.circuit(m), self.outcomes[m])
+
+    def __iter__(self):
+        return (self[m] for m in range(len(self)))
+
+
+def collect_global_clifford_shadows(state, N: int, M: int, *, circuits=None, uniforms=None,
+                                    rng=None, depth: int | None = None):
+    """collect_global_clifford_shadows(ψ, N, M) (…ClassicalShadows.jl:926-933)
+    and (ρ, N, M) (:939-946): one random Clifford circuit per snapshot
+    (given, or drawn from `rng` with sample_random_global_clifford's
+    structure).  ψ: every snapshot runs in one workgroup on the device
+    (smoq_global_clifford_snapshots, N <= 12); ρ: U ρ U† through the fused
+    passes per snapshot, the draw from the populations."""
+    from .density import DensityMatrix
+
+    g = rng if rng is not None else np.random.default_rng()
+    if circuits is not None:
+        circs = list(circuits)
+        trips = [np.array(_gc_bits(c) or np.zeros((0, 3)), np.uint8).reshape(-1, 3) for c in circs]
+        off = np.zeros(len(circs) + 1, np.int64)
+        off[1:] = np.cumsum([len(t) for t in trips])
+        flat = np.concatenate(trips) if trips else np.zeros((0, 3), np.uint8)
+    else:
+        flat, off = _random_gc_triples(N, M, 2 * N if depth is None else int(depth), g)
+    u = g.random(M) if uniforms is None else np.asarray(uniforms, np.float64).reshape(-1)
+    if len(off) != M + 1 or u.size != M:
+        raise ValueError(f"need {M} circuits and {M} uniforms")
+    if isinstance(state, DensityMatrix):
+        from .circuits import Tape
+        from .density import dm_tape
+        from .measurements import sample_indices
+
+        if state.N != N or state.vec.batch != 1:
+            raise ValueError("one density matrix of N qubits")
+        idx = np.empty(M, np.uint64)
+        work = state.vec.clone()
+        try:
+            for m in range(M):
+                t = Tape(N)
+                for typ, a, b in flat[off[m]:off[m + 1]]:
+                    if typ == 0:
+                        t.add("H", [int(a)])
+                    elif typ == 1:
+                        t.add("S", [int(a)])
+                    else:
+                        t.add("CNOT", [int(a), int(b)])
+                work.copy_from(state.vec)
+                if len(t):
+                    dm_tape(t).apply(work)
+                idx[m] = sample_indices(work, [u[m]], "sequential_diag")[0]
+        finally:
+            work.free()
+    else:
+        if state.n != N:
+            raise ValueError(f"state has {state.n} qubits, N = {N}")
+        fl = np.ascontiguousarray(flat if flat.size else np.zeros((1, 3), np.uint8))
+        u = np.ascontiguousarray(u)
+        idx = np.empty(M, np.uint64)
+        call("smoq_global_clifford_snapshots", state.handle, M,
+             fl.ctypes.data_as(ctypes.c_void_p), off.ctypes.data_as(ctypes.c_void_p),
+             u.ctypes.data_as(ctypes.c_void_p), idx.ctypes.data_as(ctypes.c_void_p))
+    return GlobalCliffordShadowSet(N, flat, off, _outcomes(idx, N))
+
+
+def global_clifford_shadow_snapshot(state, N: int, circuit=None, r=None, *, rng=None):
+    """global_clifford_shadow_snapshot(ψ or ρ, N) (…ClassicalShadows.jl:884-921)."""
+    return collect_global_clifford_shadows(
+        state, N, 1, circuits=None if circuit is None else [circuit],
+        uniforms=None if r is None else [float(r)], rng=rng)[0]
+
+
 def classical_shadow_snapshot(state, N: int, bases=None, r=None, *, rng=None) -> PauliSnapshot:
     """classical_shadow_snapshot(ψ, N) (…ClassicalShadows.jl:1001-1037)."""
     b = None if bases is None else np.asarray(bases).reshape(1, N)
@@ -254,6 +458,10 @@ def clifford_shadow_snapshot(state, N: int, indices=None, r=None, *, rng=None):
 
 
 __all__ = [
+    "GlobalCliffordCircuit", "GlobalCliffordShadowSet", "GlobalCliffordSnapshot",
+    "build_global_clifford_unitary",
+    "collect_global_clifford_shadows", "global_clifford_shadow_snapshot",
+    "sample_random_global_clifford",
     "CliffordSnapshot", "PauliSnapshot", "SINGLE_QUBIT_CLIFFORDS", "ShadowConfig", "ShadowSet",
     "Snapshot", "classical_shadow_snapshot", "clifford_shadow_snapshot",
     "collect_clifford_shadows", "collect_shadows", "get_clifford_gate",

@@ -116,8 +116,8 @@ def test_arguments_are_checked(ctx):
     with pytest.raises(SmoqError) as e:
         shadow_outcomes(s, np.full((2, 3), 4), [0.1, 0.2], SHADOW_PAULI)
     assert e.value.code == SMOQ_EINVAL
-    with pytest.raises(NotImplementedError):
-        collect_shadows(s, ShadowConfig(3, 10, "global_clifford"))
+    assert len(collect_shadows(s, ShadowConfig(3, 10, "global_clifford"),
+                               rng=np.random.default_rng(2))) == 10
     with pytest.raises(ValueError):
         ShadowConfig(3, 10, "bogus")
     sh = collect_shadows(s, ShadowConfig(3, 10, "clifford_group"), rng=np.random.default_rng(1))
@@ -147,4 +147,43 @@ def test_density_matrix_snapshots(ctx, group):
                 for c in bases[m]]
         i = orc.rho_shadow_snapshot(rho, N, mats, u[m])
         assert list(sh.outcomes[m]) == [(i >> k) & 1 for k in range(N)], m
+    d.free()
+
+
+@pytest.mark.parametrize("N,M", [(1, 200), (3, 400), (6, 300), (8, 10)])
+def test_global_clifford_snapshots(ctx, N, M):
+    from smoq_amd import GPUState
+    from smoq_amd.shadows import collect_global_clifford_shadows
+
+    psi = rand_psi(N, 5 * N)
+    s = GPUState.from_numpy(psi)
+    rng = np.random.default_rng(N)
+    drawn = collect_global_clifford_shadows(s, N, M, rng=rng)  # the vectorised generator
+    circuits = [sn.circuit for sn in drawn]
+    for c in circuits[:5]:  # its circuits have sample_random_global_clifford's shape
+        assert sum(g[0] == "CNOT" for g in c.gates) == (2 * N - 1) * (N // 2)
+    u = np.random.default_rng(99).random(M)
+    snaps = collect_global_clifford_shadows(s, N, M, circuits=circuits, uniforms=u)
+    for m in range(M):
+        i = orc.global_clifford_snapshot(psi, N, circuits[m].gates, u[m])
+        assert list(snaps[m].outcomes) == [(i >> k) & 1 for k in range(N)], m
+    s.free()
+
+
+def test_global_clifford_density_matrix(ctx):
+    from smoq_amd.density import DensityMatrix
+    from smoq_amd.shadows import ShadowConfig, collect_shadows
+
+    N, M = 3, 60
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    rho = A @ A.conj().T
+    rho /= np.trace(rho).real
+    d = DensityMatrix.from_numpy(rho)
+    u = rng.random(M)
+    snaps = collect_shadows(d, ShadowConfig(N, M, "global_clifford"), uniforms=u,
+                            rng=np.random.default_rng(9))
+    for m, sn in enumerate(snaps):
+        i = orc.global_clifford_snapshot(rho, N, sn.circuit.gates, u[m])
+        assert list(sn.outcomes) == [(i >> k) & 1 for k in range(N)], m
     d.free()

@@ -104,3 +104,30 @@ def test_rho_snapshot_matches_pure_state_form():
         a = orc.pauli_shadow_snapshot(psi, N, bases, r)
         b = orc.rho_shadow_snapshot(rho, N, [ROT[int(x)] for x in bases], r)
         assert a == b
+
+
+def test_global_clifford_unitary():
+    """The reference's unitary: H/S on qubit q is the q-th Kronecker factor
+    from the left (bit N - q), CNOT acts on bits c-1 -> t-1; the product maps
+    Paulis to signed Paulis (a Clifford)."""
+    H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+    U = orc.global_clifford_unitary(2, [("H", 1)])
+    assert np.allclose(U, np.kron(H, np.eye(2)))  # acts on bit 1 = qubit 2's position
+    U = orc.global_clifford_unitary(2, [("CNOT", 1, 2)])
+    assert np.allclose(U @ np.array([0, 1, 0, 0]), [0, 0, 0, 1])  # bit 0 set flips bit 1
+    rng = np.random.default_rng(4)
+    gates = []
+    for _ in range(12):
+        r = rng.integers(0, 3)
+        if r == 2:
+            c, t = rng.choice(3, 2, replace=False) + 1
+            gates.append(("CNOT", int(c), int(t)))
+        else:
+            gates.append(("H" if r == 0 else "S", int(rng.integers(1, 4))))
+    U = orc.global_clifford_unitary(3, gates)
+    assert np.abs(U.conj().T @ U - np.eye(8)).max() < 1e-12
+    for P in (X, Y, Z):
+        Q = U @ np.kron(np.eye(4), P) @ U.conj().T
+        # a signed Pauli string has one nonzero of modulus 1 per row
+        assert np.allclose(np.sort(np.abs(Q), axis=1)[:, -1], 1.0)
+        assert np.allclose(np.sum(np.abs(Q) > 1e-9, axis=1), 1)
