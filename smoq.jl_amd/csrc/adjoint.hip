@@ -263,9 +263,10 @@ int forward(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta, i
   if (rc) return rc;
   if (!one_gate && s->batch == 1 && s->n >= kRegQubits) {
     // one item: angles can be baked into a fused program
-    std::vector<IGate> gs;
+    std::vector<IGate>& gs = s->ctx->tape_gates;
+    gs.clear();
     if ((rc = lower_tape(ops, n_ops, theta, n_params, s->n, gs))) return rc;
-    Program prog;
+    Program& prog = s->ctx->tape_prog;
     plan_program(gs, s->n, pass_tile(s->ctx, s->n, s->batch, gs.size()), s->ctx->low_qubits,
                  s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog);
     return run_program(s, prog);

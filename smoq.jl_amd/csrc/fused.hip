@@ -128,26 +128,22 @@ int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, cons
                                : launch_program_t<float>(s, prog, L, dbuf, jit);
 }
 
-int run_program(smoq_state* s, const Program& prog, bool fixed_angles) {
+int run_program(smoq_state* s, Program& prog, bool fixed_angles) {
   const ProgLayout L = program_layout(prog, s->dtype);
   int rc = ensure_prog(s->ctx, L.total);
   if (rc) return rc;
   std::shared_ptr<JitModule> jit;
-  const Program* use = &prog;
-  Program factored;
   if (jit_wanted(s->ctx, prog, s->n)) {
-    if (fixed_angles) {
-      factored = prog;
-      factor_rotations(factored);
-      use = &factored;
-    }
-    rc = jit_get(s->ctx, *use, s->dtype, jit);
-    // automatic mode keeps the precompiled kernel when compilation fails
+    std::vector<FactorUndo> undo;
+    if (fixed_angles) factor_rotations(prog, &undo);
+    rc = jit_get(s->ctx, prog, s->dtype, jit);
+    // automatic mode keeps the precompiled kernel (unfactored gates) when
+    // compilation fails
     if (rc && s->ctx->jit_mode >= 2) return rc;
-    if (rc) use = &prog;
+    if (rc) unfactor_rotations(prog, undo);
   }
-  if ((rc = upload_program(s->ctx, *use, s->dtype, L, s->ctx->prog))) return rc;
-  return launch_program(s, *use, L, s->ctx->prog, jit.get());
+  if ((rc = upload_program(s->ctx, prog, s->dtype, L, s->ctx->prog))) return rc;
+  return launch_program(s, prog, L, s->ctx->prog, jit.get());
 }
 
 }  // namespace smoq

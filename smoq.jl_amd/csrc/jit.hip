@@ -107,7 +107,7 @@ static void bit_map_emit(O& o, const std::vector<int>& dst, bool wide) {
 // |sigma| >= 1/sqrt(2)).  The form (unit diagonal or unit anti-diagonal)
 // is compiled into the kernel, so the source depends on the angles: only
 // circuits whose angles stay fixed across calls are rewritten.
-void factor_rotations(Program& prog) {
+void factor_rotations(Program& prog, std::vector<FactorUndo>* undo) {
   for (PassDev& pd : prog.passes) {
     double scale = 1.0;
     for (int s = pd.s0; s < pd.s1; ++s) {
@@ -131,6 +131,7 @@ void factor_rotations(Program& prog) {
         const bool diag = std::fabs(a) >= std::fabs(b);
         const double sigma = diag ? a : b, t = diag ? b / a : a / b;
         if (!(sigma != 0.0) || !std::isfinite(t)) continue;
+        if (undo) undo->push_back({g, G.kind, G.b, G.m[0]});
         G.kind = kind;
         G.b = diag ? 0 : 1;
         G.m[0] = t;
@@ -139,6 +140,16 @@ void factor_rotations(Program& prog) {
     }
     pd.scale = scale;
   }
+}
+
+void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo) {
+  for (const FactorUndo& u : undo) {
+    GateDev& G = prog.gates[u.g];
+    G.kind = u.kind;
+    G.b = u.b;
+    G.m[0] = u.m0;
+  }
+  for (PassDev& pd : prog.passes) pd.scale = 1.0;
 }
 
 template <typename O>
@@ -248,11 +259,14 @@ std::string jit_source(const Program& prog, int dtype) {
   return o.str();
 }
 
-static std::string jit_key(const Program& prog, int dtype) {
+// the key of prog into buf (whose capacity is reused across calls)
+static void jit_key(const Program& prog, int dtype, std::string& buf) {
   KeyOut o;
-  o.k.reserve(64 * prog.gates.size() + 4096);
+  o.k.swap(buf);
+  o.k.clear();
+  o.k.reserve(256 * prog.gates.size() + 4096);
   jit_emit(prog, dtype, o);
-  return std::move(o.k);
+  o.k.swap(buf);
 }
 
 JitModule::~JitModule() {
@@ -327,7 +341,8 @@ bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits) {
 
 int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out) {
   out.reset();
-  std::string key = jit_key(prog, dtype);
+  std::string& key = ctx->jit_key_buf;
+  jit_key(prog, dtype, key);
   auto it = ctx->jit_cache.find(key);
   if (it != ctx->jit_cache.end()) {
     out = it->second;
@@ -336,7 +351,7 @@ int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitMo
   std::shared_ptr<JitModule> m;
   const int rc = compile(jit_source(prog, dtype), prog, m);
   if (rc) return rc;
-  ctx->jit_cache.emplace(std::move(key), m);
+  ctx->jit_cache.emplace(key, m);
   out = m;
   return SMOQ_OK;
 }

@@ -525,9 +525,10 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
                     int n_theta, int* n_passes) {
   int rc = check_state(s);
   if (rc) return rc;
-  std::vector<IGate> gs;
+  std::vector<IGate>& gs = s->ctx->tape_gates;
+  gs.clear();
   if ((rc = lower_tape(ops, n_ops, theta, n_theta, s->n, gs))) return rc;
-  Program prog;
+  Program& prog = s->ctx->tape_prog;
   plan_program(gs, s->n, pass_tile(s->ctx, s->n, s->batch, gs.size()), s->ctx->low_qubits,
                s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog);
   if (n_passes) *n_passes = (int)prog.passes.size();

@@ -96,6 +96,7 @@ struct smoq_ctx_s {
   // run-time compiled pass kernels: 0 off, 1 automatic, 2 always
   int jit_mode = 1;
   std::unordered_map<std::string, std::shared_ptr<smoq::JitModule>> jit_cache;
+  std::string jit_key_buf;  // the last key (capacity reused across calls)
   // multi-mask Pauli term lists through tiled passes (SMOQ_OBS_TILES=0: off)
   int obs_tiles = 1;
   // rocBLAS handle for dense density-matrix propagation, created on first use
@@ -119,6 +120,11 @@ struct smoq_ctx_s {
   size_t tree_dev_bytes = 0;
   std::vector<int64_t> tree_level_off;  // node offsets per level
   int64_t tree_nodes = 0, tree_leaves = 0;
+  // host buffers of tape applications (lowered gates, planned program),
+  // kept so that their capacity is reused: fresh ones of a 1000-gate tape
+  // (~0.6 MB) cost page faults on every call
+  std::vector<smoq::IGate> tape_gates;
+  smoq::Program tape_prog;
 };
 
 struct smoq_state_s {
@@ -152,8 +158,9 @@ int time_mark(smoq_ctx* ctx);
 // launchers implemented in the kernel translation units
 int launch_gate(smoq_state* s, const IGate& g);
 // fixed_angles: the gates' angles do not change between calls with this
-// structure, so run-time compiled passes may factor the rotations
-int run_program(smoq_state* s, const Program& prog, bool fixed_angles = false);
+// structure, so run-time compiled passes may factor the rotations (in place:
+// prog may come back factored)
+int run_program(smoq_state* s, Program& prog, bool fixed_angles = false);
 
 // device image of a Program: passes | stages | gates | phase tables
 struct ProgLayout {
@@ -166,8 +173,15 @@ int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, cons
                    const JitModule* jit);
 // run-time compiled pass kernels (jit.hip)
 std::string jit_source(const Program& prog, int dtype);
-// rewrites rotation gates into the factored kinds of run-time compiled passes
-void factor_rotations(Program& prog);
+// rewrites rotation gates into the factored kinds of run-time compiled
+// passes; with undo, records what unfactor_rotations needs to restore them
+struct FactorUndo {
+  int g;
+  int32_t kind, b;
+  double m0;
+};
+void factor_rotations(Program& prog, std::vector<FactorUndo>* undo = nullptr);
+void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo);
 bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits);
 // tile qubits of the fused passes for a gate list (automatic unless set)
 int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates);
