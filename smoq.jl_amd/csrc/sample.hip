@@ -66,6 +66,11 @@ __device__ inline void tree_level_at(Level parent, Level child, uint64_t j) {
   child.flag[j] = ln < kLeaf ? 1 : 0;
 }
 
+// Leaves are summed left to right; the loads of each group of kBatch
+// elements are issued before its additions, so one leaf costs a few memory
+// latencies instead of one per element (the sums are the same).
+constexpr uint32_t kBatch = 16;
+
 template <typename T>
 __device__ inline void leaf_sum_at(const typename CT<T>::V* __restrict__ psi, Level lv,
                                    uint64_t j) {
@@ -74,7 +79,15 @@ __device__ inline void leaf_sum_at(const typename CT<T>::V* __restrict__ psi, Le
   const uint64_t st = lv.start[j];
   const uint32_t ln = lv.len[j];
   double s = prob_of<T>(psi[st]);
-  for (uint32_t i = 1; i < ln; ++i) s = s + prob_of<T>(psi[st + i]);
+  uint32_t i = 1;
+  for (; i + kBatch <= ln; i += kBatch) {
+    double p[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) p[u] = prob_of<T>(psi[st + i + u]);
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) s = s + p[u];
+  }
+  for (; i < ln; ++i) s = s + prob_of<T>(psi[st + i]);
   lv.sum[j] = s;
 }
 
@@ -102,7 +115,18 @@ __device__ inline void leaf_write_at(const typename CT<T>::V* __restrict__ psi, 
   const double s = lv.carry[j];
   double loc = prob_of<T>(psi[st]);
   c[st] = s + loc;
-  for (uint32_t i = 1; i < ln; ++i) {
+  uint32_t i = 1;
+  for (; i + kBatch <= ln; i += kBatch) {
+    double p[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) p[u] = prob_of<T>(psi[st + i + u]);
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) {
+      loc = loc + p[u];
+      c[st + i + u] = s + loc;
+    }
+  }
+  for (; i < ln; ++i) {
     loc = loc + prob_of<T>(psi[st + i]);
     c[st + i] = s + loc;
   }
@@ -222,7 +246,19 @@ __global__ void k_seq_prefix(const typename CT<T>::V* __restrict__ psi, uint64_t
                              double* __restrict__ c, uint64_t diag_step) {
 #pragma clang fp contract(off)
   double s = 0.0;
-  for (uint64_t i = 0; i < n; ++i) {
+  uint64_t i = 0;
+  for (; i + kBatch <= n; i += kBatch) {  // loads of a group ahead of its sums
+    double p[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u)
+      p[u] = diag_step ? (double)psi[(i + u) * diag_step].x : prob_of<T>(psi[i + u]);
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) {
+      s = s + p[u];
+      c[i + u] = s;
+    }
+  }
+  for (; i < n; ++i) {
     s = s + (diag_step ? (double)psi[i * diag_step].x : prob_of<T>(psi[i]));
     c[i] = s;
   }
