@@ -152,6 +152,10 @@ void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo) {
   for (PassDev& pd : prog.passes) pd.scale = 1.0;
 }
 
+// passes with at most 2^kWarmMaxNonlocal tiles per item warm their gate
+// records (small states: 1000-gate circuits at 12-16 qubits run 25-30% faster)
+constexpr int kWarmMaxNonlocal = 10;
+
 template <typename O>
 static void jit_emit(const Program& prog, int dtype, O& o) {
   const char* T = dtype == SMOQ_C128 ? "double" : "float";
@@ -168,8 +172,21 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
       << "  typedef " << T << " T;\n"
       << "  extern __shared__ __align__(16) unsigned char smem_raw[];\n"
       << "  VT<T>* lds = reinterpret_cast<VT<T>*>(smem_raw);\n"
-      << "  PassFrame<T> f;\n"
-      << "  pass_load<T>(lds, psi, n, pass, f);\n";
+      << "  PassFrame<T> f;\n";
+    // Passes of few tiles (small states) run one workgroup per CU, and each
+    // stage would wait a full memory latency for its gate records on their
+    // first scalar load.  Every thread first loads one dword per 128 B line
+    // of the pass's gate records, so that the stages' scalar loads find them
+    // in L2; the loaded values are consumed after the store.
+    const int g0 = prog.stages[pd.s0].g0, g1 = prog.stages[pd.s1 - 1].g1;
+    const bool warm = pd.n_nonlocal <= kWarmMaxNonlocal;
+    if (warm) {
+      const size_t nw = (size_t)(g1 - g0) * sizeof(GateDev) / 4;
+      o << "  uint32_t warm = 0;\n  {\n    const uint32_t* gw = reinterpret_cast<const uint32_t*>(gates + "
+        << g0 << ");\n    for (uint32_t w = threadIdx.x * 32u; w < " << nw
+        << "u; w += blockDim.x * 32u) warm ^= gw[w];\n  }\n";
+    }
+    o << "  pass_load<T>(lds, psi, n, pass, f);\n";
     for (int s = pd.s0; s < pd.s1; ++s) {
       const StageDev& st = prog.stages[s];
       // thread bit i -> the i-th local position outside the register slots
@@ -247,9 +264,11 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
       for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g)
         scaled |= prog.gates[g].kind == GK_U1_ROT || prog.gates[g].kind == GK_U1_RXR;
     if (scaled)
-      o << "  pass_store_scaled<T>(lds, f, pass->scale);\n}\n";
+      o << "  pass_store_scaled<T>(lds, f, pass->scale);\n";
     else
-      o << "  pass_store<T>(lds, f);\n}\n";
+      o << "  pass_store<T>(lds, f);\n";
+    if (warm) o << "  asm volatile(\"\" :: \"v\"(warm));\n";
+    o << "}\n";
   }
 }
 
