@@ -154,7 +154,10 @@ void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo) {
 
 // passes with at most 2^kWarmMaxNonlocal tiles per item warm their gate
 // records (small states: 1000-gate circuits at 12-16 qubits run 25-30% faster)
+// when that takes at most kWarmLoads loads per thread (a state that is one
+// tile runs all its gates in one pass of 2..64 threads)
 constexpr int kWarmMaxNonlocal = 10;
+constexpr size_t kWarmLoads = 4;
 
 template <typename O>
 static void jit_emit(const Program& prog, int dtype, O& o) {
@@ -179,9 +182,10 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
     // of the pass's gate records, so that the stages' scalar loads find them
     // in L2; the loaded values are consumed after the store.
     const int g0 = prog.stages[pd.s0].g0, g1 = prog.stages[pd.s1 - 1].g1;
-    const bool warm = pd.n_nonlocal <= kWarmMaxNonlocal;
+    const size_t nw = (size_t)(g1 - g0) * sizeof(GateDev) / 4;
+    const size_t nt = (size_t)1 << (pd.k - kRegQubits);
+    const bool warm = pd.n_nonlocal <= kWarmMaxNonlocal && nw <= nt * 32 * kWarmLoads;
     if (warm) {
-      const size_t nw = (size_t)(g1 - g0) * sizeof(GateDev) / 4;
       o << "  uint32_t warm = 0;\n  {\n    const uint32_t* gw = reinterpret_cast<const uint32_t*>(gates + "
         << g0 << ");\n    for (uint32_t w = threadIdx.x * 32u; w < " << nw
         << "u; w += blockDim.x * 32u) warm ^= gw[w];\n  }\n";
