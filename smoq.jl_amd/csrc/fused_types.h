@@ -22,6 +22,13 @@ constexpr int kMaxTileQubits = 12;     // LDS tile of up to 4096 amplitudes
 // a tile of 2^k amplitudes runs 2^(k - kRegQubits) threads
 constexpr int kMaxPassThreads = 1 << (kMaxTileQubits - kRegQubits);
 constexpr int kMaxQubits = 40;
+// Passes with at most 2^kWarmMaxNonlocal tiles per item warm their stage and
+// gate records into L2 before the stages read them (small states: 1000-gate
+// circuits at 12-16 qubits run 25-30% faster), when that takes at most
+// kWarmLoads loads per thread (a state that is one tile runs all its gates
+// in one pass of 2..64 threads).
+constexpr int kWarmMaxNonlocal = 10;
+constexpr uint32_t kWarmLoads = 4;
 
 // Gate kinds inside a register stage.
 enum GKind : int32_t {

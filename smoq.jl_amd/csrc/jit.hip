@@ -152,12 +152,6 @@ void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo) {
   for (PassDev& pd : prog.passes) pd.scale = 1.0;
 }
 
-// passes with at most 2^kWarmMaxNonlocal tiles per item warm their gate
-// records (small states: 1000-gate circuits at 12-16 qubits run 25-30% faster)
-// when that takes at most kWarmLoads loads per thread (a state that is one
-// tile runs all its gates in one pass of 2..64 threads)
-constexpr int kWarmMaxNonlocal = 10;
-constexpr size_t kWarmLoads = 4;
 
 template <typename O>
 static void jit_emit(const Program& prog, int dtype, O& o) {
