@@ -336,6 +336,7 @@ int smoq_fini(smoq_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->prog) (void)hipFree(ctx->prog);
+  for (auto& b : ctx->state_pool) (void)hipFree(b.second);
   if (ctx->tree_dev) (void)hipFree(ctx->tree_dev);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   blas_release(ctx);
@@ -400,6 +401,16 @@ int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_sta
   s->batch = batch;
   s->dim = 1ull << n_qubits;
   s->bytes = s->dim * s->amp_bytes() * (size_t)batch;
+  // a small buffer freed on this context is reused: its earlier work is on
+  // the same stream, so the zero fill below runs after it
+  for (size_t i = 0; i < ctx->state_pool.size(); ++i) {
+    if (ctx->state_pool[i].first != s->bytes) continue;
+    s->d = ctx->state_pool[i].second;
+    ctx->state_pool.erase(ctx->state_pool.begin() + (ptrdiff_t)i);
+    SMOQ_HIP(hipMemsetAsync(s->d, 0, s->bytes, ctx->stream));
+    *out = s;
+    return SMOQ_OK;
+  }
   size_t free_b = 0, total_b = 0;
   SMOQ_HIP(hipSetDevice(ctx->device));
   SMOQ_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -421,8 +432,11 @@ int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_sta
 
 int smoq_state_free(smoq_state* s) {
   if (!s) return SMOQ_OK;
-  if (s->d) {
-    (void)hipStreamSynchronize(s->ctx->stream);
+  smoq_ctx* ctx = s->ctx;
+  if (s->d && s->bytes <= kStatePoolMaxBytes && ctx->state_pool.size() < kStatePoolMax) {
+    ctx->state_pool.emplace_back(s->bytes, s->d);  // no sync, no hipFree
+  } else if (s->d) {
+    (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(s->d);
   }
   s->d = nullptr;

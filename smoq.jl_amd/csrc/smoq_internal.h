@@ -83,6 +83,8 @@ struct JitModule {
 };
 // automatic mode compiles for states of at least this many qubits and
 // programs of at least this many stage gates
+constexpr size_t kStatePoolMaxBytes = 64ull << 20;
+constexpr size_t kStatePoolMax = 8;
 constexpr int kJitMinQubits = 16;
 constexpr int kJitMinGates = 16;
 }  // namespace smoq
@@ -125,6 +127,9 @@ struct smoq_ctx_s {
   // (~0.6 MB) cost page faults on every call
   std::vector<smoq::IGate> tape_gates;
   smoq::Program tape_prog;
+  // freed state buffers of at most kStatePoolMaxBytes (size, pointer), kept
+  // for reuse: hipFree synchronises and short calls would pay for it
+  std::vector<std::pair<size_t, void*>> state_pool;
 };
 
 struct smoq_state_s {
