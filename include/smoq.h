@@ -120,6 +120,10 @@ int smoq_pass_timing(smoq_ctx* ctx, int enable, double* mean_ms, int64_t* launch
  * Replaces: zeros(ComplexF64, 2^N); ψ[1] = 1 and make_ket / make_initial_psi
  * (utils/cpu/cpuQuantumStatePreparation.jl:208-243,1211-1229) for basis kets. */
 int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_state** out);
+/* States of at most 64 MiB go back to a pool of eight buffers on their
+ * context (reused, stream-ordered, by a later allocation of the same size;
+ * released by smoq_fini); larger ones are freed after the context's stream
+ * drains.  Free every state before smoq_fini of its context. */
 int smoq_state_free(smoq_state* s);
 int smoq_state_info(const smoq_state* s, int* n_qubits, int* dtype, int* batch, void** device_ptr);
 int smoq_state_set_basis(smoq_state* s, uint64_t index);
