@@ -8,11 +8,16 @@ alternating brick pairs: 890 gate applications, fused into HBM passes over
 LDS tiles.  value = gate applications per second.
 
 Also reported (outside the timed steps): 1e6 Born samples with the
-reference's cumsum/searchsortedfirst rule, the complex64 sweep and its error
+reference's cumsum/searchsortedfirst rule and their mismatch count against
+the oracle on the same state and uniforms, the complex64 sweep and its error
 against complex128, the per-launch roofline of the fused-pass kernel, the
-adjoint gradient against parameter shift (N=24, 960 parameters; 64 sets at
-N=20), the batched MCWF run (4096 trajectories, N in {12,14,16,18}, 2000
-steps, one launch), the reference's three published CPU workloads run on
+adjoint gradient against parameter shift (N=24, 960 parameters, every
+component checked against parameter shift with fused energies and a subset
+against central finite differences; 64 sets at N=20), the second-order
+Trotter XXZ chain at N=28 (1000 steps in complex128 and complex64, the
+staggered magnetisation every 10 steps, the Chebyshev propagator at t=10 as
+the physics check), the batched MCWF run (4096 trajectories, N in
+{12,14,16,18}, 2000 steps, one launch), the reference's three published CPU workloads run on
 the device beside its recorded times (shared random circuits + 100 samples,
 the expectation-value benchmark, σ_z-dephasing MCWF with 1000 trajectories,
 the same model as a density matrix with Trotter gates and with the
@@ -47,22 +52,48 @@ def log(msg):
 
 
 def cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    name = ""
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
         for line in out.splitlines():
             if line.startswith("Model name"):
-                return line.split(":", 1)[1].strip()
+                name = line.split(":", 1)[1].strip()
+                break
     except Exception:
         pass
-    return platform.processor()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    full = line.split(":", 1)[1].strip()
+                    if len(full) > len(name):
+                        name = full
+                    break
+    except OSError:
+        pass
+    return name or platform.processor()
 
 
-def cpu_baseline(tape, N, budget_s=20.0):
+def cpu_counts():
+    """(logical CPUs of the host, CPUs this process may run on)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return os.cpu_count(), usable
+
+
+def cpu_baseline(tape, N, budget_s=30.0, min_gates=None):
     """Times the oracle's restatement of apply_*_psi! (one full 2^N loop per
-    gate, single thread, as the reference's gate loops) on the first gates of
-    the same circuit until ~budget_s of CPU work has run."""
+    gate, single thread, as the reference's gate loops) on the leading gates
+    of the same circuit: at least its first layer (N rotations and the CZ
+    bricks), then on until ~budget_s of CPU work has run."""
     from oracle import oracle as orc
 
+    if min_gates is None:
+        min_gates = N + (N - 1) // 2 + (1 if N % 2 == 0 else 0)
+        min_gates = min(min_gates, len(tape.ops))
     psi = np.zeros(1 << N, np.complex128)
     psi[0] = 1.0
     done = 0
@@ -70,13 +101,17 @@ def cpu_baseline(tape, N, budget_s=20.0):
     for gate, qs, th, par, m in tape.ops:
         orc.apply_named(psi, gate.lower(), [q + 1 for q in qs], th, N)
         done += 1
-        if time.perf_counter() - t0 > budget_s:
+        if done >= min_gates and time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
     del psi
+    logical, usable = cpu_counts()
     return {"value": done / dt, "unit": "gate-applications/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} gates of the same N={N} c128 circuit, oracle loop "
-                      f"(reference apply_*_psi! restated), {dt:.1f} s, {cpu_model()}"}
+            "host_logical_cpus": logical, "host_usable_cpus": usable,
+            "cpu_model": cpu_model(),
+            "sample": f"first {done} gates (the whole first layer and more) of the same N={N} "
+                      f"c128 circuit, oracle loop (reference apply_*_psi! restated), one core, "
+                      f"{dt:.1f} s"}
 
 
 def traffic_probe(N=28):
@@ -558,6 +593,142 @@ def shadow_snapshots():
             "by_N": rows, "global_clifford": gc}
 
 
+def trotter_xxz(N=28, steps=1000, dt=0.01, obs_every=10, J=1.0, delta=0.5, seed=28,
+                t_cheb=10.0, timed_steps=20):
+    """Second-order Trotter evolution of the open XXZ chain (J = 1, Δ = 0.5,
+    h_i uniform in [-1, 1] from the seeded stream) from the Néel state:
+    `steps` steps of dt in complex128 and in complex64, the staggered
+    magnetisation m_s = Σ_k (-1)^(k-1) <Z_k> / N (k 1-based) after every
+    obs_every-th step; a short run with per-launch events for the pass
+    roofline; the Chebyshev propagator (complex128, the rigorous spectral
+    bound ±Σ|c|, tol 1e-12) to t = steps·dt as the physics check."""
+    import smoq_amd
+    from smoq_amd import GPUState
+    from smoq_amd import observables as ob
+    from smoq_amd import streams
+    from smoq_amd.chebyshev import chebyshev_evolve_psi
+    from smoq_amd.hamiltonian import xxz_chain
+    from smoq_amd.trotter import second_order_gates, trotter_tape
+
+    ctx = smoq_amd.default_context()
+    h = 2.0 * streams.uniforms(seed, N) - 1.0
+    params = xxz_chain(N, J, delta, h)
+    seq = second_order_gates(params, dt)
+    tape = trotter_tape(seq, N)
+    neel = sum(1 << (k - 1) for k in range(2, N + 1, 2))
+    zs = [1 << k for k in range(N)]
+    sign = np.array([1.0 if k % 2 == 0 else -1.0 for k in range(N)])
+    out = {"workload": f"open XXZ chain N={N}, J={J}, delta={delta}, h_i~U[-1,1] (seed {seed}), "
+                       f"Neel start, second order dt={dt} x {steps}, m_s every {obs_every} steps",
+           "gates_per_step": len(seq)}
+    recs, finals = {}, {}
+    for dtype in ("c128", "c64"):
+        plan = tape.compile(dtype)
+        psi = GPUState.basis(N, neel, dtype)
+        S = psi.nbytes
+        # per-launch roofline of the step's passes (events, no graph)
+        ctx.pass_timing(True)
+        plan.run(psi, timed_steps)
+        pass_ms, launches = ctx.pass_timing(False)
+        psi.set_basis(neel)
+        plan.run(psi, 1)  # captures the step's graph on this state
+        psi.set_basis(neel)
+        ctx.sync()
+        t0 = time.perf_counter()
+        rec = plan.evolve_observe(psi, steps, obs_every, [0] * N, zs)
+        ctx.sync()
+        t = time.perf_counter() - t0
+        recs[dtype] = rec
+        finals[dtype] = ob.expect_pauli(psi, [0] * N, zs)
+        passes = plan.passes
+        out[dtype] = {
+            "seconds": t, "ms_per_step": t * 1e3 / steps, "passes_per_step": passes,
+            "gate_ops_per_s": len(seq) * steps / t,
+            "hbm_GBps_whole_run": passes * 2.0 * S * steps / t / 1e9,
+            "pass_roofline": {"mean_launch_ms": pass_ms, "launches": launches,
+                              "achieved_GBps": 2.0 * S / (pass_ms * 1e-3) / 1e9,
+                              "frac": 2.0 * S / (pass_ms * 1e-3) / HBM_PEAK},
+            "m_s_first_last": [float(rec[0] @ sign / N), float(rec[-1] @ sign / N)],
+        }
+        log(f"C3 XXZ N={N} {dtype}: {t:.2f} s for {steps} steps "
+            f"({t * 1e3 / steps:.2f} ms/step, {passes} passes/step, pass frac "
+            f"{out[dtype]['pass_roofline']['frac']:.3f})")
+        plan.free()
+        psi.free()
+    ms128, ms64 = recs["c128"] @ sign / N, recs["c64"] @ sign / N
+    out["c64_vs_c128"] = {"max_abs_diff_m_s": float(np.max(np.abs(ms64 - ms128))),
+                          "max_abs_diff_Z_i": float(np.max(np.abs(recs["c64"] - recs["c128"])))}
+    out["m_s_c128"] = [float(v) for v in ms128]
+    # Chebyshev reference at t = steps * dt
+    psi = GPUState.basis(N, neel, "c128")
+    ctx.sync()
+    t0 = time.perf_counter()
+    used, a, b = chebyshev_evolve_psi(psi, params, steps * dt, tol=1e-12, max_order=4000)
+    ctx.sync()
+    tc = time.perf_counter() - t0
+    zc = ob.expect_pauli(psi, [0] * N, zs)
+    psi.free()
+    out["chebyshev"] = {"t": steps * dt, "orders": used, "seconds": tc,
+                        "m_s": float(zc @ sign / N),
+                        "trotter_m_s": float(finals["c128"] @ sign / N),
+                        "abs_diff_m_s_vs_trotter": float(abs((zc - finals["c128"]) @ sign / N)),
+                        "max_abs_diff_Z_i_vs_trotter": float(np.max(np.abs(zc - finals["c128"]))),
+                        "note": "second-order Trotter error O(dt^2 t), a physics check, "
+                                "not a parity bound"}
+    log(f"C3 Chebyshev t={steps * dt}: {used} orders, {tc:.1f} s, |dm_s| vs Trotter "
+        f"{out['chebyshev']['abs_diff_m_s_vs_trotter']:.2e}; c64 max |dm_s| "
+        f"{out['c64_vs_c128']['max_abs_diff_m_s']:.2e}")
+    return out
+
+
+def adjoint_full_check(N=24, layers=20, seed=24, fd_params=16, fd_eps=1e-6):
+    """Every component of the N-qubit, 2·N·layers-parameter adjoint gradient
+    against parameter shift evaluated with fused-pass energies on the
+    device (two energies per parameter), and a subset against central
+    finite differences (eps 1e-6, about 1e-9 accurate)."""
+    import smoq_amd
+    from smoq_amd import streams
+    from smoq_amd.circuits import hardware_efficient_ansatz
+    from smoq_amd.gradients import VQEEnergy, calculate_gradients_parameter_shift
+    from smoq_amd.observables import xxz_terms
+
+    ctx = smoq_amd.default_context()
+    tape = hardware_efficient_ansatz(N, layers)
+    P = tape.n_params
+    theta = 2 * np.pi * streams.uniforms(seed, P)
+    xs, zs, cs = xxz_terms(N, 0.0, 0.0, 1.0, 1.0)
+    v = VQEEnergy(tape, xs, zs, cs)
+    e, g = v.energy_and_grad(theta)
+    cost = v.cost_fn()
+    cost(theta)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ps = calculate_gradients_parameter_shift(cost, theta)
+    t_ps = time.perf_counter() - t0
+    sub = np.linspace(0, P - 1, fd_params).astype(int)
+    fd = np.empty(len(sub))
+    w = theta.copy()
+    for i, k in enumerate(sub):
+        w[k] = theta[k] + fd_eps
+        cp = cost(w)
+        w[k] = theta[k] - fd_eps
+        cm = cost(w)
+        w[k] = theta[k]
+        fd[i] = (cp - cm) / (2 * fd_eps)
+    v.free()
+    d_ps = float(np.max(np.abs(g[0] - ps)))
+    d_fd = float(np.max(np.abs(g[0][sub] - fd)))
+    log(f"C5 check N={N}: all {P} adjoint components vs parameter shift max |d| = {d_ps:.2e} "
+        f"({t_ps:.1f} s of fused energies); {len(sub)} vs FD max |d| = {d_fd:.2e}")
+    return {"workload": f"TFIM energy gradient, N={N}, {layers} layers, {P} parameters, c128",
+            "components_checked_vs_param_shift": P,
+            "max_abs_diff_vs_param_shift": d_ps,
+            "param_shift_seconds_fused_energies": t_ps,
+            "fd_components": int(len(sub)), "fd_eps": fd_eps,
+            "max_abs_diff_vs_central_fd": d_fd,
+            "tolerances": {"param_shift": 1e-10, "fd": 1e-7}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", choices=["traffic"], default=None)
@@ -579,6 +750,10 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-analytics", action="store_true")
     ap.add_argument("--analytics-only", action="store_true")
+    ap.add_argument("--no-trotter", action="store_true")
+    ap.add_argument("--trotter-only", action="store_true")
+    ap.add_argument("--no-checks", action="store_true",
+                    help="skip the oracle / parameter-shift checks at the full sizes")
     args = ap.parse_args()
 
     if args.probe == "traffic":
@@ -592,6 +767,9 @@ def main():
         return
     if args.reference_only:
         print(json.dumps({"reference_workloads": reference_benchmarks()}), flush=True)
+        return
+    if args.trotter_only:
+        print(json.dumps({"trotter_xxz": trotter_xxz()}), flush=True)
         return
     if args.analytics_only:
         print(json.dumps({"partial_trace": reduced_states(),
@@ -671,8 +849,23 @@ def main():
         t0 = time.perf_counter()
         idx = me.sample_indices(psi, r, "cumsum")
         ts = time.perf_counter() - t0
+        log(f"sampling: {args.shots} draws in {ts * 1e3:.2f} ms")
         zs = [1 << i for i in range(N)] + [3 << i for i in range(N - 1)]
         obs128 = ob.expect_pauli(psi, [0] * len(zs), zs)
+        samp_check = None
+        if not args.no_checks:
+            # the oracle's cumsum + searchsortedfirst on the downloaded state
+            # and the same uniforms (outside the timed draws)
+            from oracle import oracle as orc
+
+            t0 = time.perf_counter()
+            host = psi.to_numpy()
+            ref = orc.sample_cumsum_search(host, N, r)
+            del host
+            mism = int(np.count_nonzero(ref != idx.astype(np.int64)))
+            samp_check = {"oracle_mismatches": mism, "draws_compared": int(r.size),
+                          "seconds": time.perf_counter() - t0}
+            log(f"sampling check vs oracle at N={N}: {mism} mismatches in {r.size} draws")
         # complex64 sweep of the same circuit
         psi.free()
         p64 = GPUState(N, "c64")
@@ -687,9 +880,12 @@ def main():
         t64 = (time.perf_counter() - t0) / 3
         obs64 = ob.expect_pauli(p64, [0] * len(zs), zs)
         p64.free()
+        log(f"c64 sweep: {t64 * 1e3:.1f} ms, max |d<Z>,<ZZ>| vs c128 "
+            f"{float(np.max(np.abs(obs64 - obs128))):.1e}")
         result["extras"] = {
             "sampling": {"shots": args.shots, "ms": ts * 1e3, "rule": "searchsortedfirst(cumsum)",
-                         "first_indices": [int(v) for v in idx[:4]]},
+                         "first_indices": [int(v) for v in idx[:4]],
+                         "oracle_check": samp_check},
             "c64_sweep": {"ms_per_circuit": t64 * 1e3, "gate_ops_per_s": n_gates / t64,
                           "max_abs_err_Z_ZZ_vs_c128": float(np.max(np.abs(obs64 - obs128)))},
         }
@@ -698,10 +894,22 @@ def main():
 
     if not args.no_adjoint and not args.no_extras:
         log("adjoint gradient vs parameter shift ...")
-        result.setdefault("extras", {})["adjoint"] = adjoint_vs_parameter_shift()
+        adj = adjoint_vs_parameter_shift()
+        result.setdefault("extras", {})["adjoint"] = adj
+        log(f"adjoint N=24: {adj['adjoint_s_per_gradient'] * 1e3:.1f} ms/gradient, "
+            f"{adj['speedup_vs_param_shift']:.0f}x parameter shift; batch 64 x N=20: "
+            f"{adj['batched']['gradients_per_s']:.0f} gradients/s")
+        if not args.no_checks:
+            adj["full_check"] = adjoint_full_check()
+    if not args.no_trotter and not args.no_extras:
+        log("second-order Trotter XXZ N=28 ...")
+        result.setdefault("extras", {})["trotter_xxz"] = trotter_xxz()
     if not args.no_mcwf and not args.no_extras:
         log("batched MCWF ...")
-        result.setdefault("extras", {})["mcwf"] = mcwf_batch()
+        mc = mcwf_batch()
+        result.setdefault("extras", {})["mcwf"] = mc
+        log(f"batched MCWF: {mc['seconds']:.2f} s, {mc['hbm_GBps_two_pass_sizes']:.0f} GB/s "
+            f"on the two-pass sizes")
     if not args.no_reference and not args.no_extras:
         log("the reference's published workloads ...")
         result.setdefault("extras", {})["reference_workloads"] = reference_benchmarks()
@@ -713,7 +921,45 @@ def main():
     if not args.no_cpu:
         log("CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(tape, N)
+        log(f"CPU baseline: {result['cpu_baseline']['value']:.3f} gate-ops/s, "
+            f"{result['cpu_baseline']['sample']}")
     print(json.dumps(result), flush=True)
+    log("summary " + json.dumps(summary(result)))
+
+
+def summary(result):
+    """The headline figures of every extra, short enough for a log tail."""
+    x = result.get("extras", {})
+    out = {"value": result["value"], "ms_per_step": result["ms_per_step"],
+           "roofline_frac": result["roofline"]["frac"]}
+    if "sampling" in x:
+        out["sampling_ms"] = x["sampling"]["ms"]
+        chk = x["sampling"].get("oracle_check")
+        if chk:
+            out["sampling_oracle_mismatches"] = chk["oracle_mismatches"]
+    if "c64_sweep" in x:
+        out["c64_ms"] = x["c64_sweep"]["ms_per_circuit"]
+        out["c64_err"] = x["c64_sweep"]["max_abs_err_Z_ZZ_vs_c128"]
+    if "adjoint" in x:
+        a = x["adjoint"]
+        out["adjoint_ms"] = a["adjoint_s_per_gradient"] * 1e3
+        out["adjoint_speedup_vs_ps"] = a["speedup_vs_param_shift"]
+        out["adjoint_batch_grads_per_s"] = a["batched"]["gradients_per_s"]
+        if "full_check" in a:
+            out["adjoint_max_diff_ps_all"] = a["full_check"]["max_abs_diff_vs_param_shift"]
+            out["adjoint_max_diff_fd"] = a["full_check"]["max_abs_diff_vs_central_fd"]
+    if "trotter_xxz" in x:
+        t = x["trotter_xxz"]
+        out["c3_ms_per_step"] = {d: t[d]["ms_per_step"] for d in ("c128", "c64")}
+        out["c3_pass_frac"] = {d: t[d]["pass_roofline"]["frac"] for d in ("c128", "c64")}
+        out["c3_c64_dm_s"] = t["c64_vs_c128"]["max_abs_diff_m_s"]
+        out["c3_cheb_dm_s"] = t["chebyshev"]["abs_diff_m_s_vs_trotter"]
+    if "mcwf" in x:
+        out["mcwf_s"] = x["mcwf"]["seconds"]
+        out["mcwf_GBps"] = x["mcwf"]["hbm_GBps_two_pass_sizes"]
+    if "cpu_baseline" in result:
+        out["cpu_gate_ops_per_s"] = result["cpu_baseline"]["value"]
+    return out
 
 
 if __name__ == "__main__":

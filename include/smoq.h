@@ -121,9 +121,11 @@ int smoq_pass_timing(smoq_ctx* ctx, int enable, double* mean_ms, int64_t* launch
  * (utils/cpu/cpuQuantumStatePreparation.jl:208-243,1211-1229) for basis kets. */
 int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_state** out);
 /* States of at most 64 MiB go back to a pool of eight buffers on their
- * context (reused, stream-ordered, by a later allocation of the same size;
- * released by smoq_fini); larger ones are freed after the context's stream
- * drains.  Free every state before smoq_fini of its context. */
+ * context (reused, stream-ordered and zero-filled, by a later allocation of
+ * the same byte size; released by smoq_fini, or earlier when an allocation
+ * would not fit otherwise); larger ones are freed after the context's stream
+ * drains.  Free every state and plan before smoq_fini of its context:
+ * smoq_fini returns SMOQ_EINVAL, and keeps the context, while any remain. */
 int smoq_state_free(smoq_state* s);
 int smoq_state_info(const smoq_state* s, int* n_qubits, int* dtype, int* batch, void** device_ptr);
 int smoq_state_set_basis(smoq_state* s, uint64_t index);
@@ -222,7 +224,8 @@ int smoq_partial_trace(const smoq_state* psi, uint64_t keep_mask, smoq_state* ou
 int smoq_dm_partial_trace(const smoq_state* rho, uint64_t keep_mask, smoq_state* out);
 
 /* ---- classical-shadow snapshots -------------------------------------------
- * For snapshot m (0 <= m < n_snap) of batch item 0: a copy of psi rotated
+ * psi must hold one vector (batch 1; a batched state is rejected with
+ * SMOQ_EINVAL).  For snapshot m (0 <= m < n_snap): a copy of psi rotated
  * qubit by qubit in increasing order by codes[m * n + q], then one draw with
  * sample_from_probabilities' rule (utils/cpu/cpuQuantumStateMeasurements.jl:203-213)
  * on uniforms[m]; out_index[m] is the 0-based outcome (bit q = the outcome of
@@ -240,7 +243,7 @@ enum smoq_shadow_kind { SMOQ_SHADOW_PAULI = 0, SMOQ_SHADOW_LOCAL_CLIFFORD = 1 };
 int smoq_shadow_snapshots(const smoq_state* psi, int64_t n_snap, int kind, const uint8_t* codes,
                           const double* clifford_table, const double* uniforms,
                           uint64_t* out_index);
-/* Global-Clifford snapshots of batch item 0 (global_clifford_shadow_snapshot,
+/* Global-Clifford snapshots of psi (batch 1) (global_clifford_shadow_snapshot,
  * utils/cpu/cpuClassicalShadows.jl:884-903): snapshot m applies gates
  * offsets[m] .. offsets[m+1]-1 of `gates` (triples type, a, b: 0 = H on bit a,
  * 1 = S on bit a, 2 = CNOT control bit a, target bit b) to a copy of psi and
@@ -415,7 +418,6 @@ int smoq_pauli_string_reference(smoq_state* s, int n_terms, const uint64_t* xmas
                                 const uint64_t* zmask, double* out);
 /* sum_i |psi_i|^2 per batch item (FP64 accumulation). */
 int smoq_norm2(smoq_state* s, double* out);
-
 /* ---- measurement sampling ---------------------------------------------- */
 enum smoq_sample_semantics {
   /* idx = searchsortedfirst(cumsum(abs2.(ψ)), r) - 1 with Julia's pairwise

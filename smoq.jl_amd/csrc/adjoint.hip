@@ -242,18 +242,17 @@ int prot(smoq_state* s, void* lam, const PauliRot& g, const double2* cs, int n_p
                                : launch_prot<float>(s, lam, g, cs, n_params, pidx, sign, partial, bpi);
 }
 
-// device memory owned by one adjoint / energy call
+// Device buffers of one call, all grow-only work buffers of the context
+// (kept across calls, so repeated evaluations allocate nothing):
+//   slot 0: per-item (cos, sin) of every parameter's half angle
+//   slot 1: λ = Hψ
+//   slot 2: partial sums and their row sums
+//   slot 3: the reverse sweep's working copy of ψ
 struct Work {
   void* lam = nullptr;
   double2* cs = nullptr;
   double* partial = nullptr;
   double* sums = nullptr;
-  ~Work() {
-    if (lam) (void)hipFree(lam);
-    if (cs) (void)hipFree(cs);
-    if (partial) (void)hipFree(partial);
-    if (sums) (void)hipFree(sums);
-  }
 };
 
 // forward sweep from |0> on every item; per-item angles from cs
@@ -288,53 +287,45 @@ int setup(smoq_state* s, const double* theta, int n_params, Work& w, std::vector
     const double th = n_params > 0 ? theta[i] : 0.0;
     hcs[i] = make_c<double>(std::cos(0.5 * th), std::sin(0.5 * th));
   }
-  SMOQ_HIP(hipMalloc(&w.cs, ncs * sizeof(double2)));
+  void* p = nullptr;
+  int rc = ensure_work(s->ctx, 0, ncs * sizeof(double2), &p);
+  if (rc) return rc;
+  w.cs = (double2*)p;
   SMOQ_HIP(hipMemcpyAsync(w.cs, hcs.data(), ncs * sizeof(double2), hipMemcpyHostToDevice,
                           s->ctx->stream));
+  // hcs is the caller's host vector: the copy must finish before it goes
+  SMOQ_HIP(hipStreamSynchronize(s->ctx->stream));
   return SMOQ_OK;
 }
 
 // λ = H ψ into w.lam and E_b = Re<ψ|λ>
 int energy_of(smoq_state* s, int n_terms, const uint64_t* x, const uint64_t* z, const double* c,
               Work& w, double* energy) {
-  SMOQ_HIP(hipMalloc(&w.lam, s->bytes));
-  smoq_state lam;
-  lam.ctx = s->ctx;
-  lam.n = s->n;
-  lam.dtype = s->dtype;
-  lam.batch = s->batch;
-  lam.dim = s->dim;
+  int rc = ensure_work(s->ctx, 1, s->bytes, &w.lam);
+  if (rc) return rc;
+  smoq_state lam = *s;
   lam.d = w.lam;
-  lam.bytes = s->bytes;
-  int rc = smoq_hpsi(s, &lam, n_terms, x, z, c);
-  lam.d = nullptr;
+  rc = smoq_hpsi(s, &lam, n_terms, x, z, c);
   if (rc) return rc;
   const int bpi = blocks_per_item(s);
-  double* part = nullptr;
-  SMOQ_HIP(hipMalloc(&part, (size_t)bpi * s->batch * sizeof(double) + 16));
-  double* esum = nullptr;
-  hipError_t e = hipMalloc(&esum, (size_t)s->batch * sizeof(double) + 16);
-  if (e != hipSuccess) {
-    (void)hipFree(part);
-    return hip_error(e, "hipMalloc(energy)");
-  }
+  void* p = nullptr;
+  const size_t part_n = (size_t)bpi * s->batch;
+  if ((rc = ensure_work(s->ctx, 2, (part_n + s->batch + 2) * sizeof(double), &p))) return rc;
+  double* part = (double*)p;
+  double* esum = part + part_n;
   if (s->dtype == SMOQ_C128)
     hipLaunchKernelGGL(k_dot_re<double>, dim3(bpi, s->batch), dim3(kAdjBlock), 0, s->ctx->stream,
                        (const double2*)s->d, (const double2*)w.lam, s->n, part);
   else
     hipLaunchKernelGGL(k_dot_re<float>, dim3(bpi, s->batch), dim3(kAdjBlock), 0, s->ctx->stream,
                        (const float2*)s->d, (const float2*)w.lam, s->n, part);
+  SMOQ_CHECK_LAUNCH("k_dot_re");
   hipLaunchKernelGGL(k_rowsum, dim3((s->batch + 255) / 256), dim3(256), 0, s->ctx->stream, part,
                      bpi, s->batch, esum);
-  hipError_t le = hipGetLastError();
-  hipError_t ce = hipMemcpyAsync(energy, esum, s->batch * sizeof(double), hipMemcpyDeviceToHost,
-                                 s->ctx->stream);
-  hipError_t se = hipStreamSynchronize(s->ctx->stream);
-  (void)hipFree(part);
-  (void)hipFree(esum);
-  if (le != hipSuccess) return hip_error(le, "energy kernels");
-  if (ce != hipSuccess) return hip_error(ce, "energy copy");
-  if (se != hipSuccess) return hip_error(se, "energy sync");
+  SMOQ_CHECK_LAUNCH("k_rowsum");
+  SMOQ_HIP(hipMemcpyAsync(energy, esum, s->batch * sizeof(double), hipMemcpyDeviceToHost,
+                          s->ctx->stream));
+  SMOQ_HIP(hipStreamSynchronize(s->ctx->stream));
   return SMOQ_OK;
 }
 
@@ -351,15 +342,28 @@ extern "C" int smoq_tape_energy(smoq_state* s, const smoq_op* ops, int n_ops, co
   if (rc) return rc;
   if (!energy || n_params < 0 || (n_params > 0 && !theta))
     return set_error(SMOQ_EINVAL, "smoq_tape_energy: bad arguments");
+  if (n_terms < 0 || (n_terms > 0 && (!xmask || !zmask || !coeff)))
+    return set_error(SMOQ_EINVAL, "smoq_tape_energy: bad term arrays");
   std::vector<Step> steps;
   if ((rc = build_steps(ops, n_ops, n_params, s->n, steps))) return rc;
+  const bool one_gate = (flags & SMOQ_ENERGY_ONE_GATE_KERNELS) != 0;
   Work w;
   std::vector<double2> hcs;
-  if ((rc = setup(s, theta, n_params, w, hcs))) return rc;
-  if ((rc = forward(s, ops, n_ops, theta, n_params, steps, w.cs,
-                    (flags & SMOQ_ENERGY_ONE_GATE_KERNELS) != 0)))
+  // the angle table feeds the per-gate rotations only (batches, one-gate mode)
+  if ((one_gate || s->batch > 1 || s->n < kRegQubits) &&
+      (rc = setup(s, theta, n_params, w, hcs)))
     return rc;
-  return energy_of(s, n_terms, xmask, zmask, coeff, w, energy);
+  if ((rc = forward(s, ops, n_ops, theta, n_params, steps, w.cs, one_gate))) return rc;
+  // E_b = Σ_t c_t <P_t>_b from the tiled Pauli reduction (one read of ψ per
+  // tile pass), summed on the host in term order
+  std::vector<double> vals((size_t)s->batch * std::max(1, n_terms));
+  if (n_terms > 0 && (rc = reduce_pauli(s, n_terms, xmask, zmask, vals.data(), 0))) return rc;
+  for (int b = 0; b < s->batch; ++b) {
+    double e = 0.0;
+    for (int t = 0; t < n_terms; ++t) e += coeff[t] * vals[(size_t)b * n_terms + t];
+    energy[b] = e;
+  }
+  return SMOQ_OK;
 }
 
 extern "C" int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_ops,
@@ -379,10 +383,10 @@ extern "C" int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_op
   if ((rc = energy_of(s, n_terms, xmask, zmask, coeff, w, energy))) return rc;
 
   // reverse sweep on a working copy of ψ, so that s keeps U(θ)|0>
-  smoq_state ps = *s;
   void* pcopy = nullptr;
-  SMOQ_HIP(hipMalloc(&pcopy, s->bytes));
+  if ((rc = ensure_work(s->ctx, 3, s->bytes, &pcopy))) return rc;
   SMOQ_HIP(hipMemcpyAsync(pcopy, s->d, s->bytes, hipMemcpyDeviceToDevice, s->ctx->stream));
+  smoq_state ps = *s;
   ps.d = pcopy;
   smoq_state ls = *s;
   ls.d = w.lam;
@@ -390,17 +394,10 @@ extern "C" int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_op
   for (const Step& st : steps) n_slots += st.param;
   const int bpi = blocks_per_item(s);
   const size_t rows = (size_t)n_slots * s->batch;
-  auto fail = [&](int code) {
-    (void)hipStreamSynchronize(s->ctx->stream);
-    (void)hipFree(pcopy);
-    ps.d = nullptr;
-    ls.d = nullptr;
-    return code;
-  };
-  if (hipMalloc(&w.partial, rows * bpi * sizeof(double) + 16) != hipSuccess ||
-      hipMalloc(&w.sums, rows * sizeof(double) + 16) != hipSuccess)
-    return fail(set_error(SMOQ_ENOMEM, "smoq_adjoint_gradient: partial buffers"));
-  std::vector<int> slot_param;
+  void* pp = nullptr;
+  if ((rc = ensure_work(s->ctx, 2, (rows * bpi + rows + 2) * sizeof(double), &pp))) return rc;
+  w.partial = (double*)pp;
+  w.sums = w.partial + rows * bpi;
   int slot = n_slots;
   for (int i = (int)steps.size() - 1; i >= 0; --i) {
     const Step& st = steps[i];
@@ -412,19 +409,17 @@ extern "C" int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_op
       rc = launch_gate(&ps, st.inv);
       if (!rc) rc = launch_gate(&ls, st.inv);
     }
-    if (rc) return fail(rc);
+    if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s->ctx->stream,
-                     w.partial, bpi, (int)rows, w.sums);
-  if (hipGetLastError() != hipSuccess)
-    return fail(set_error(SMOQ_EDEVICE, "smoq_adjoint_gradient: reduction launch failed"));
   std::vector<double> sums(rows);
-  if (rows &&
-      hipMemcpyAsync(sums.data(), w.sums, rows * sizeof(double), hipMemcpyDeviceToHost,
-                     s->ctx->stream) != hipSuccess)
-    return fail(set_error(SMOQ_EDEVICE, "smoq_adjoint_gradient: copy failed"));
-  if (hipStreamSynchronize(s->ctx->stream) != hipSuccess)
-    return fail(set_error(SMOQ_EDEVICE, "smoq_adjoint_gradient: sync failed"));
+  if (rows) {
+    hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0,
+                       s->ctx->stream, w.partial, bpi, (int)rows, w.sums);
+    SMOQ_CHECK_LAUNCH("k_rowsum");
+    SMOQ_HIP(hipMemcpyAsync(sums.data(), w.sums, rows * sizeof(double), hipMemcpyDeviceToHost,
+                            s->ctx->stream));
+  }
+  SMOQ_HIP(hipStreamSynchronize(s->ctx->stream));
   // sum the slots of each parameter in gate order (deterministic)
   for (int b = 0; b < s->batch; ++b)
     for (int p = 0; p < n_params; ++p) grad[(size_t)b * n_params + p] = 0.0;
@@ -435,5 +430,5 @@ extern "C" int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_op
       grad[(size_t)b * n_params + st.pidx] += sums[(size_t)slot * s->batch + b];
     ++slot;
   }
-  return fail(SMOQ_OK);
+  return SMOQ_OK;
 }

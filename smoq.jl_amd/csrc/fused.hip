@@ -154,7 +154,7 @@ int run_program(smoq_state* s, Program& prog, bool fixed_angles) {
   std::shared_ptr<JitModule> jit;
   if (jit_wanted(s->ctx, prog, s->n)) {
     std::vector<FactorUndo> undo;
-    if (fixed_angles) factor_rotations(prog, &undo);
+    if (fixed_angles) factor_rotations(prog, s->dtype, &undo);
     rc = jit_get(s->ctx, prog, s->dtype, jit);
     // automatic mode keeps the precompiled kernel (unfactored gates) when
     // compilation fails

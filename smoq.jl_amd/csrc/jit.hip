@@ -107,9 +107,17 @@ static void bit_map_emit(O& o, const std::vector<int>& dst, bool wide) {
 // |sigma| >= 1/sqrt(2)).  The form (unit diagonal or unit anti-diagonal)
 // is compiled into the kernel, so the source depends on the angles: only
 // circuits whose angles stay fixed across calls are rewritten.
-void factor_rotations(Program& prog, std::vector<FactorUndo>* undo) {
+//
+// A pass's tile holds the unnormalised amplitudes until its store, which
+// grow by 1/|sigma| (up to sqrt 2) per factored gate, and complex64 tiles
+// keep them in float.  Factoring therefore stops once the product of a
+// pass's sigmas would fall below 2^-40 (complex64) or 2^-500 (complex128);
+// later rotations of that pass stay unfactored.
+void factor_rotations(Program& prog, int dtype, std::vector<FactorUndo>* undo) {
+  const int max_log2 = dtype == SMOQ_C64 ? 40 : 500;
   for (PassDev& pd : prog.passes) {
     double scale = 1.0;
+    int scale_exp = 0;  // scale = frexp mantissa * 2^scale_exp, tracked exactly
     for (int s = pd.s0; s < pd.s1; ++s) {
       for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g) {
         GateDev& G = prog.gates[g];
@@ -131,14 +139,18 @@ void factor_rotations(Program& prog, std::vector<FactorUndo>* undo) {
         const bool diag = std::fabs(a) >= std::fabs(b);
         const double sigma = diag ? a : b, t = diag ? b / a : a / b;
         if (!(sigma != 0.0) || !std::isfinite(t)) continue;
+        int e = 0;
+        const double mant = std::frexp(scale * sigma, &e);
+        if (-(scale_exp + e) >= max_log2) continue;
         if (undo) undo->push_back({g, G.kind, G.b, G.m[0]});
         G.kind = kind;
         G.b = diag ? 0 : 1;
         G.m[0] = t;
-        scale *= sigma;
+        scale = mant;
+        scale_exp += e;
       }
     }
-    pd.scale = scale;
+    pd.scale = std::ldexp(scale, scale_exp);
   }
 }
 

@@ -101,6 +101,7 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
   p->dtype = dtype;
   if (n_qubits < kRegQubits) {
     p->small = gs;
+    ctx->live_objects++;
     *out = p;
     return SMOQ_OK;
   }
@@ -109,7 +110,7 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
   p->layout = program_layout(p->prog, dtype);
   if (jit_wanted(ctx, p->prog, n_qubits)) {
     Program factored = p->prog;
-    factor_rotations(factored);
+    factor_rotations(factored, dtype);
     rc = jit_get(ctx, factored, dtype, p->jit);
     // automatic mode keeps the precompiled kernel when compilation fails
     if (rc && ctx->jit_mode >= 2) {
@@ -128,6 +129,7 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
     delete p;
     return rc;
   }
+  ctx->live_objects++;
   *out = p;
   return SMOQ_OK;
 }
@@ -150,6 +152,7 @@ int smoq_plan_run(smoq_plan* p, smoq_state* s, int repeat) {
 
 int smoq_plan_free(smoq_plan* p) {
   if (!p) return SMOQ_OK;
+  p->ctx->live_objects--;
   (void)hipStreamSynchronize(p->ctx->stream);
   drop_graph(p);
   if (p->dev) (void)hipFree(p->dev);
@@ -165,8 +168,9 @@ int smoq_evolve_observe(smoq_plan* p, smoq_state* s, int n_steps, int obs_every,
   if (n_steps < 0 || obs_every < 1)
     return set_error(SMOQ_EINVAL, "smoq_evolve_observe: n_steps %d, obs_every %d", n_steps,
                      obs_every);
-  if (s->n != p->n || s->dtype != p->dtype)
-    return set_error(SMOQ_EINVAL, "smoq_evolve_observe: plan/state shape mismatch");
+  if (s->n != p->n || s->dtype != p->dtype || s->ctx != p->ctx)
+    return set_error(SMOQ_EINVAL,
+                     "smoq_evolve_observe: plan/state shape or context mismatch");
   const int records = n_steps / obs_every;
   for (int r = 0; r < records; ++r) {
     if ((rc = plan_run(p, s, obs_every))) return rc;

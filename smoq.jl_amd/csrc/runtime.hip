@@ -46,6 +46,40 @@ int ensure_scratch(smoq_ctx* ctx, size_t bytes) {
   return SMOQ_OK;
 }
 
+int ensure_work(smoq_ctx* ctx, int slot, size_t bytes, void** out) {
+  if (slot < 0 || slot >= smoq_ctx::kWorkSlots)
+    return set_error(SMOQ_EINVAL, "work buffer slot %d", slot);
+  if (bytes > ctx->work_bytes[slot]) {
+    if (ctx->work[slot]) {
+      SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+      SMOQ_HIP(hipFree(ctx->work[slot]));
+      ctx->work[slot] = nullptr;
+      ctx->work_bytes[slot] = 0;
+    }
+    const size_t want = bytes + 4096;
+    hipError_t e = hipMalloc(&ctx->work[slot], want);
+    if (e != hipSuccess && !ctx->state_pool.empty()) {
+      (void)hipGetLastError();
+      release_state_pool(ctx);
+      e = hipMalloc(&ctx->work[slot], want);
+    }
+    if (e != hipSuccess) {
+      ctx->work[slot] = nullptr;
+      return set_error(SMOQ_ENOMEM, "work buffer of %zu bytes: %s", want, hipGetErrorString(e));
+    }
+    ctx->work_bytes[slot] = want;
+  }
+  *out = ctx->work[slot];
+  return SMOQ_OK;
+}
+
+void release_state_pool(smoq_ctx* ctx) {
+  if (ctx->state_pool.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& b : ctx->state_pool) (void)hipFree(b.second);
+  ctx->state_pool.clear();
+}
+
 int ensure_prog(smoq_ctx* ctx, size_t bytes) {
   if (bytes <= ctx->prog_bytes) return SMOQ_OK;
   if (ctx->prog) {
@@ -332,12 +366,19 @@ int smoq_init(smoq_ctx** out, int device) {
 
 int smoq_fini(smoq_ctx* ctx) {
   if (!ctx) return SMOQ_OK;
+  if (ctx->live_objects > 0)
+    return set_error(SMOQ_EINVAL,
+                     "smoq_fini: %lld states or plans of this context are still allocated; "
+                     "free them first",
+                     (long long)ctx->live_objects);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->prog) (void)hipFree(ctx->prog);
   for (auto& b : ctx->state_pool) (void)hipFree(b.second);
   if (ctx->tree_dev) (void)hipFree(ctx->tree_dev);
+  for (int i = 0; i < smoq_ctx::kWorkSlots; ++i)
+    if (ctx->work[i]) (void)hipFree(ctx->work[i]);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   blas_release(ctx);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -407,13 +448,25 @@ int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_sta
     if (ctx->state_pool[i].first != s->bytes) continue;
     s->d = ctx->state_pool[i].second;
     ctx->state_pool.erase(ctx->state_pool.begin() + (ptrdiff_t)i);
-    SMOQ_HIP(hipMemsetAsync(s->d, 0, s->bytes, ctx->stream));
+    const hipError_t me = hipMemsetAsync(s->d, 0, s->bytes, ctx->stream);
+    if (me != hipSuccess) {
+      ctx->state_pool.emplace_back(s->bytes, s->d);
+      delete s;
+      return hip_error(me, "hipMemsetAsync(state)");
+    }
+    ctx->live_objects++;
     *out = s;
     return SMOQ_OK;
   }
   size_t free_b = 0, total_b = 0;
   SMOQ_HIP(hipSetDevice(ctx->device));
   SMOQ_HIP(hipMemGetInfo(&free_b, &total_b));
+  // pooled buffers are allocated memory that hipMemGetInfo does not count
+  // as free: give them back before refusing a request
+  if (s->bytes > free_b && !ctx->state_pool.empty()) {
+    release_state_pool(ctx);
+    SMOQ_HIP(hipMemGetInfo(&free_b, &total_b));
+  }
   if (s->bytes > free_b) {
     size_t want = s->bytes;
     delete s;
@@ -421,11 +474,22 @@ int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_sta
                      want, free_b);
   }
   hipError_t e = hipMalloc(&s->d, s->bytes);
+  if (e != hipSuccess && !ctx->state_pool.empty()) {
+    (void)hipGetLastError();
+    release_state_pool(ctx);
+    e = hipMalloc(&s->d, s->bytes);
+  }
   if (e != hipSuccess) {
     delete s;
     return set_error(SMOQ_ENOMEM, "smoq_state_alloc: hipMalloc failed: %s", hipGetErrorString(e));
   }
-  SMOQ_HIP(hipMemsetAsync(s->d, 0, s->bytes, ctx->stream));
+  const hipError_t me = hipMemsetAsync(s->d, 0, s->bytes, ctx->stream);
+  if (me != hipSuccess) {
+    (void)hipFree(s->d);
+    delete s;
+    return hip_error(me, "hipMemsetAsync(state)");
+  }
+  ctx->live_objects++;
   *out = s;
   return SMOQ_OK;
 }
@@ -433,6 +497,7 @@ int smoq_state_alloc(smoq_ctx* ctx, int n_qubits, int dtype, int batch, smoq_sta
 int smoq_state_free(smoq_state* s) {
   if (!s) return SMOQ_OK;
   smoq_ctx* ctx = s->ctx;
+  ctx->live_objects--;
   if (s->d && s->bytes <= kStatePoolMaxBytes && ctx->state_pool.size() < kStatePoolMax) {
     ctx->state_pool.emplace_back(s->bytes, s->d);  // no sync, no hipFree
   } else if (s->d) {

@@ -421,6 +421,10 @@ extern "C" int smoq_global_clifford_snapshots(const smoq_state* psi, int64_t n_s
   if (rc) return rc;
   if (n_snap < 0 || (n_snap > 0 && (!gates || !offsets || !uniforms || !out_index)))
     return set_error(SMOQ_EINVAL, "smoq_global_clifford_snapshots: bad snapshot arrays");
+  if (psi->batch != 1)
+    return set_error(SMOQ_EINVAL,
+                     "smoq_global_clifford_snapshots: state has %d items; snapshots take one",
+                     psi->batch);
   if (psi->n > kGcMaxQubits)
     return set_error(SMOQ_ENOTSUP, "smoq_global_clifford_snapshots: more than %d qubits",
                      kGcMaxQubits);
@@ -482,6 +486,9 @@ extern "C" int smoq_shadow_snapshots(const smoq_state* psi, int64_t n_snap, int 
   if (rc) return rc;
   if (n_snap < 0 || (n_snap > 0 && (!codes || !uniforms || !out_index)))
     return set_error(SMOQ_EINVAL, "smoq_shadow_snapshots: bad snapshot arrays");
+  if (psi->batch != 1)
+    return set_error(SMOQ_EINVAL, "smoq_shadow_snapshots: state has %d items; snapshots take one",
+                     psi->batch);
   if (kind != SMOQ_SHADOW_PAULI && kind != SMOQ_SHADOW_LOCAL_CLIFFORD)
     return set_error(SMOQ_EINVAL, "smoq_shadow_snapshots: unknown kind %d", kind);
   const bool cl = kind == SMOQ_SHADOW_LOCAL_CLIFFORD;

@@ -130,6 +130,14 @@ struct smoq_ctx_s {
   // freed state buffers of at most kStatePoolMaxBytes (size, pointer), kept
   // for reuse: hipFree synchronises and short calls would pay for it
   std::vector<std::pair<size_t, void*>> state_pool;
+  // grow-only device buffers of the gradient calls (angle tables, λ, the
+  // reverse sweep's copy of ψ, partial sums), reused across calls
+  static constexpr int kWorkSlots = 4;
+  void* work[kWorkSlots] = {};
+  size_t work_bytes[kWorkSlots] = {};
+  // states and plans allocated on this context and not yet freed:
+  // smoq_fini refuses to release the context while any remain
+  int64_t live_objects = 0;
 };
 
 struct smoq_state_s {
@@ -154,6 +162,10 @@ inline unsigned stride_blocks(uint64_t n, unsigned threads) {
   return (unsigned)(want == 0 ? 1 : (want < (1ull << 20) ? want : (1ull << 20)));
 }
 int ensure_prog(smoq_ctx* ctx, size_t bytes);
+// grow-only work buffer `slot` (0 .. kWorkSlots-1) of at least `bytes`
+int ensure_work(smoq_ctx* ctx, int slot, size_t bytes, void** out);
+// hipFree of every pooled state buffer (after the stream drains)
+void release_state_pool(smoq_ctx* ctx);
 int ensure_pinned(smoq_ctx* ctx, size_t bytes);
 int check_state(const smoq_state* s);
 int check_qubit(const smoq_state* s, int q, const char* what);
@@ -185,7 +197,7 @@ struct FactorUndo {
   int32_t kind, b;
   double m0;
 };
-void factor_rotations(Program& prog, std::vector<FactorUndo>* undo = nullptr);
+void factor_rotations(Program& prog, int dtype, std::vector<FactorUndo>* undo = nullptr);
 void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo);
 bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits);
 // tile qubits of the fused passes for a gate list (automatic unless set)

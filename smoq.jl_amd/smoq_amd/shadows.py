@@ -161,7 +161,8 @@ def collect_shadows(state, N, M: int | None = None, *, bases=None, uniforms=None
                     rng=None) -> ShadowSet:
     """collect_shadows(ψ, N, M) (…ClassicalShadows.jl:1098-1106) and
     collect_shadows(ψ, config) (:1135-1160, :pauli and :local_clifford):
-    M snapshots of batch item 0.  `bases` (M, N) holds 1..3 for Pauli or
+    M snapshots of a state holding one vector (a batched state is rejected).
+    `bases` (M, N) holds 1..3 for Pauli or
     1..24 for local Clifford; drawn from `rng` when absent, as are the
     outcome uniforms."""
     group = "pauli"
