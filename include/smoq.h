@@ -418,6 +418,16 @@ int smoq_pauli_string_reference(smoq_state* s, int n_terms, const uint64_t* xmas
                                 const uint64_t* zmask, double* out);
 /* sum_i |psi_i|^2 per batch item (FP64 accumulation). */
 int smoq_norm2(smoq_state* s, double* out);
+/* smoq_expect_pauli without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and results as smoq_expect_pauli; the partial sums live in a
+ * fixed 16 MiB region that each device holds for the life of the process
+ * (about batch * 256 KiB + batch * tiles * terms * 8 B are needed; larger
+ * requests return SMOQ_ENOMEM without running).  Calls that overlap on one
+ * device share that region and race: serialise them. */
+int smoq_expect_pauli_noalloc(smoq_state* s, int n_terms, const uint64_t* xmask,
+                              const uint64_t* zmask, double* out);
 /* ---- measurement sampling ---------------------------------------------- */
 enum smoq_sample_semantics {
   /* idx = searchsortedfirst(cumsum(abs2.(ψ)), r) - 1 with Julia's pairwise

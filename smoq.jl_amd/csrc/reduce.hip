@@ -2,6 +2,7 @@
 // reductions (per-block partials, then a fixed-order second stage).
 #include <algorithm>
 #include <map>
+#include <mutex>
 
 #include "device_common.h"
 #include "smoq_internal.h"
@@ -519,9 +520,16 @@ static int red_blocks(uint64_t work) {
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(b, 2048));
 }
 
+// Device memory for the partial sums of one reduction: the context's
+// grow-only scratch, or a fixed region that is never grown (ptr != null).
+struct RedScratch {
+  void* ptr = nullptr;
+  size_t cap = 0;
+};
+
 template <typename T>
 static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64_t* zm,
-                    double* out_host, int ref_mode) {
+                    double* out_host, int ref_mode, RedScratch fixed) {
   smoq_ctx* ctx = s->ctx;
   const int B = s->batch;
   if (n_terms > 4096) return set_error(SMOQ_EINVAL, "smoq_expect_pauli: more than 4096 terms");
@@ -569,10 +577,21 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
   const size_t grp_bytes = plan.groups.size() * sizeof(ObsGroup);
   const size_t trm_bytes = plan.terms.size() * sizeof(ObsTerm);
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  int rc = ensure_scratch(ctx, up(part_bytes) + up(tpart_bytes) + up(out_bytes) + up(col_bytes) +
-                                   up(grp_bytes) + up(trm_bytes));
-  if (rc) return rc;
-  char* sc = (char*)ctx->scratch;
+  const size_t need = up(part_bytes) + up(tpart_bytes) + up(out_bytes) + up(col_bytes) +
+                      up(grp_bytes) + up(trm_bytes);
+  char* sc = nullptr;
+  if (fixed.ptr) {
+    if (need > fixed.cap)
+      return set_error(SMOQ_ENOMEM,
+                       "smoq_expect_pauli_noalloc: %zu bytes of partial sums needed (%d items, "
+                       "%d terms), the fixed region holds %zu",
+                       need, B, n_terms, fixed.cap);
+    sc = (char*)fixed.ptr;
+  } else {
+    int rc = ensure_scratch(ctx, need);
+    if (rc) return rc;
+    sc = (char*)ctx->scratch;
+  }
   double* partial = (double*)sc;
   double* tpartial = (double*)(sc += up(part_bytes));
   double* out_dev = (double*)(sc += up(tpart_bytes));
@@ -662,7 +681,8 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
 using namespace smoq;
 
 static int expect_entry(smoq_state* s, int n_terms, const uint64_t* xmask,
-                        const uint64_t* zmask, double* out, int ref_mode) {
+                        const uint64_t* zmask, double* out, int ref_mode,
+                        RedScratch fixed = RedScratch()) {
   int rc = check_state(s);
   if (rc) return rc;
   if (n_terms < 0 || (n_terms > 0 && (!xmask || !zmask || !out)))
@@ -673,8 +693,63 @@ static int expect_entry(smoq_state* s, int n_terms, const uint64_t* xmask,
     if ((xmask[t] | zmask[t]) & ~lim)
       return set_error(SMOQ_EINVAL, "smoq_expect_pauli: term %d acts outside the %d qubits", t,
                        s->n);
-  return s->dtype == SMOQ_C128 ? expect_t<double>(s, n_terms, xmask, zmask, out, ref_mode)
-                               : expect_t<float>(s, n_terms, xmask, zmask, out, ref_mode);
+  return s->dtype == SMOQ_C128 ? expect_t<double>(s, n_terms, xmask, zmask, out, ref_mode, fixed)
+                               : expect_t<float>(s, n_terms, xmask, zmask, out, ref_mode, fixed);
+}
+
+// ------------------------------------------------- no-allocation variant --
+// smoq_expect_pauli_noalloc keeps its partial sums in `scratch`, a fixed
+// device array of kNoAllocScratchBytes that every device holds from module
+// load to process exit, so the call never allocates device memory.  Host
+// code only ever uses the per-device address that hipGetSymbolAddress
+// returns for it (taken in smoq_init through noalloc_scratch_init); the
+// array's own name is a host-side stand-in, not a device address.  Every
+// partial-sum slot a reduction reads is written earlier in the same call,
+// so the region needs no clearing between calls; calls that overlap on one
+// device share the region and race.
+namespace smoq {
+constexpr size_t kNoAllocScratchBytes = 16ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices)
+    return set_error(SMOQ_ENOTSUP, "device %d beyond the %d the library tracks", device,
+                     kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return SMOQ_OK;
+  void* ptr = nullptr;
+  // the caller has made `device` current
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)");
+  if (!ptr) return set_error(SMOQ_EDEVICE, "hipGetSymbolAddress(scratch) returned null");
+  g_noalloc_ptr[device] = ptr;
+  return SMOQ_OK;
+}
+
+static void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  return g_noalloc_ptr[device];
+}
+}  // namespace smoq
+
+extern "C" int smoq_expect_pauli_noalloc(smoq_state* s, int n_terms, const uint64_t* xmask,
+                                         const uint64_t* zmask, double* out) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  RedScratch fixed;
+  fixed.ptr = noalloc_scratch(s->ctx->device);
+  fixed.cap = kNoAllocScratchBytes;
+  if (!fixed.ptr)
+    return set_error(SMOQ_EDEVICE,
+                     "smoq_expect_pauli_noalloc: no fixed region on device %d (smoq_init sets "
+                     "it up)",
+                     s->ctx->device);
+  return expect_entry(s, n_terms, xmask, zmask, out, 0, fixed);
 }
 
 namespace smoq {

@@ -348,6 +348,7 @@ int smoq_init(smoq_ctx** out, int device) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return set_error(SMOQ_ENOTSUP, "smoq_init: device %d is %s; this library is built for gfx950",
                      device, prop.gcnArchName);
+  if (int rc = noalloc_scratch_init(device)) return rc;
   smoq_ctx* ctx = new smoq_ctx();
   ctx->device = device;
   if (const char* j = std::getenv("SMOQ_JIT")) {

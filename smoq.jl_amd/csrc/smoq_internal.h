@@ -208,6 +208,9 @@ int lower_tape(const smoq_op* ops, int n_ops, const double* theta, int n_theta, 
                std::vector<IGate>& out);
 int reduce_pauli(smoq_state* s, int n_terms, const uint64_t* xm, const uint64_t* zm,
                  double* out_host, int ref_mode);
+// takes the device address of the fixed partial-sum region of
+// smoq_expect_pauli_noalloc on `device` (current); once per device
+int noalloc_scratch_init(int device);
 // the dissipative half of an MCWF step for every batch item (mcwf.hip):
 // uniforms of item b at d_uni[b * uni_stride + 0/1], its jump at
 // d_jumps[b * jump_stride]; a zero or non-finite norm records

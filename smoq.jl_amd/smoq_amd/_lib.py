@@ -121,6 +121,7 @@ SIGNATURES = {
     "smoq_shadow_snapshots": [_P, _i64, _i, _P, _P, _P, _P],
     "smoq_global_clifford_snapshots": [_P, _i64, _P, _P, _P, _P],
     "smoq_norm2": [_P, _P],
+    "smoq_expect_pauli_noalloc": [_P, _i, _P, _P, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
     "smoq_set_jit": [_P, _i],
