@@ -13,10 +13,27 @@
 // with exactly the same floating-point operations, so every c[i] is bit-equal
 // to the reference's.  The search is Julia's searchsortedfirst loop.
 //
+// Long vectors never store the 2^N prefixes.  A leaf's prefixes rise
+// monotonically inside it, so one coalesced pass sums every leaf (staged
+// through LDS), the tree sweeps give the carries, and only each leaf's end
+// value E = carry + leaf sum (its last prefix) is kept.  A draw searches
+// the leaf ends and recomputes the prefixes of the one leaf it lands in.
+// That equals Julia's binary search over the full prefix array whenever
+// "prefix >= r" is false-then-true along the array.  Across a leaf boundary
+// the prefixes can step down by a rounding difference (the carry into a
+// subtree is rounded differently from the running chain before it): a draw
+// r inside such a step's interval (first prefix of the leaf, last prefix
+// before it] could make the reference's search path-dependent, so those
+// draws replay Julia's halving search itself with every probed prefix
+// recomputed from its leaf.  The intervals are collected on the device and
+// merged on the host, so the check per draw is one binary search.
+//
 // SMOQ_SAMPLE_SEQUENTIAL follows sample_from_probabilities
 // (utils/cpu/cpuQuantumStateMeasurements.jl:203-213): a strictly sequential
 // running sum, first index with r <= C_i, else the last index.
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "device_common.h"
 #include "smoq_internal.h"
@@ -154,12 +171,6 @@ __global__ void k_tree_level(Level parent, Level child, uint64_t nslots) {
   if (j < nslots) tree_level_at(parent, child, j);
 }
 
-template <typename T>
-__global__ void k_leaf_sum(const typename CT<T>::V* __restrict__ psi, Level lv, uint64_t nslots) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nslots) leaf_sum_at<T>(psi, lv, j);
-}
-
 __global__ void k_up(Level lv, Level child, uint64_t nslots) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < nslots) up_at(lv, child, j);
@@ -168,20 +179,6 @@ __global__ void k_up(Level lv, Level child, uint64_t nslots) {
 __global__ void k_down(Level lv, Level child, uint64_t nslots) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < nslots) down_at(lv, child, j);
-}
-
-template <typename T>
-__global__ void k_leaf_write(const typename CT<T>::V* __restrict__ psi, Level lv, uint64_t nslots,
-                             double* __restrict__ c) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nslots) leaf_write_at<T>(psi, lv, j, c);
-}
-
-template <typename T>
-__global__ void k_first(const typename CT<T>::V* __restrict__ psi, double* c, Level root) {
-  const double p0 = prob_of<T>(psi[0]);
-  c[0] = p0;
-  root.carry[0] = p0;
 }
 
 __global__ void k_search(const double* __restrict__ c, uint64_t n, const double* __restrict__ r,
@@ -194,7 +191,7 @@ __global__ void k_search(const double* __restrict__ c, uint64_t n, const double*
 // the per-level launches would cost more than the work).  Barriers separate
 // the phases; the level arrays live in global memory, which the workgroup's
 // barrier makes visible to all its threads.
-constexpr int kMaxLevels = 24;
+constexpr int kMaxLevels = 28;  // 2^32 elements need 27 levels
 constexpr uint64_t kBlockCumsumMax = 1ull << 16;
 struct LevelSet {
   Level lv[kMaxLevels];
@@ -281,7 +278,296 @@ __global__ void k_search_le(const double* __restrict__ c, uint64_t n, const doub
   out[d] = lo < n ? lo : n - 1;
 }
 
+// ------------------------------------------------ leaf-end path (long) --
+// Leaves of the split tree sit at depths D-1 and D only (lengths at one
+// depth differ by at most one, so a depth mixes leaves of 127 elements with
+// nodes of 128 that split into two leaves of 64).  Virtual slot v of depth
+// D names leaf (D, v), or, when its parent p = v/2 is itself a leaf,
+// (D-1, p) for even v and nothing ("duplicate") for odd v.
+struct LeafRef {
+  int d;
+  uint64_t j;
+  bool dup;
+};
+
+__device__ inline LeafRef vleaf(const Level& lp, int D, uint64_t v) {
+  LeafRef r;
+  const uint64_t p = v >> 1;
+  if (lp.flag[p] == 1) {
+    r.d = D - 1;
+    r.j = p;
+    r.dup = (v & 1) != 0;
+  } else {
+    r.d = D;
+    r.j = v;
+    r.dup = false;
+  }
+  return r;
+}
+
+constexpr int kStageBlock = 256;
+// depth D-1 nodes per workgroup; such a node holds at most 254 elements
+// (two leaves of at most 127)
+constexpr int kStageParents = 16;
+constexpr int kStageSlots = 2 * kStageParents;
+constexpr int kStageMax = kStageParents * 2 * kLeaf;  // elements staged (4096)
+constexpr int kStageLoads = kStageMax / kStageBlock;
+__device__ inline int stage_pad(int l) { return l + (l >> 6); }  // spreads the leaf starts over banks
+
+// Every leaf's sum (left to right, as the reference's leaf loop) into its
+// level slot and its first element's probability into first[v]: a
+// workgroup stages the elements of 16 consecutive depth D-1 nodes through
+// LDS with coalesced loads, then one lane per virtual slot adds its leaf.
+template <typename T>
+__global__ void __launch_bounds__(kStageBlock)
+    k_leaf_sums_staged(const typename CT<T>::V* __restrict__ psi, LevelSet ls, int D,
+                       uint64_t n_parents, double* __restrict__ first) {
+#pragma clang fp contract(off)
+  __shared__ double p_s[kStageMax + kStageMax / 64];
+  const Level lp = ls.lv[D - 1];
+  const uint64_t p0 = (uint64_t)blockIdx.x * kStageParents;
+  const uint64_t p_last = min(p0 + kStageParents, n_parents) - 1;
+  const uint64_t lo = lp.start[p0];
+  const int cnt = (int)(lp.start[p_last] + lp.len[p_last] - lo);
+  typename CT<T>::V a[kStageLoads];
+#pragma unroll
+  for (int it = 0; it < kStageLoads; ++it) {
+    const int l = threadIdx.x + it * kStageBlock;
+    if (l < cnt) a[it] = psi[lo + l];
+  }
+#pragma unroll
+  for (int it = 0; it < kStageLoads; ++it) {
+    const int l = threadIdx.x + it * kStageBlock;
+    if (l < cnt) p_s[stage_pad(l)] = prob_of<T>(a[it]);
+  }
+  __syncthreads();
+  if (threadIdx.x >= kStageSlots) return;
+  const uint64_t v = 2 * p0 + threadIdx.x;
+  if ((v >> 1) > p_last) return;
+  const LeafRef lf = vleaf(lp, D, v);
+  if (lf.dup) return;
+  const Level lv = ls.lv[lf.d];
+  const int o = (int)(lv.start[lf.j] - lo);
+  const int ln = (int)lv.len[lf.j];
+  double sum = p_s[stage_pad(o)];
+  first[v] = sum;
+  for (int i = 1; i < ln; ++i) sum = sum + p_s[stage_pad(o + i)];
+  lv.sum[lf.j] = sum;
+}
+
+// E[v] = carry + sum of v's leaf (its last prefix; a duplicate slot repeats
+// the leaf before it).  A leaf whose first prefix lies below the previous
+// leaf's last one appends the interval (first, previous last] to dips.
+__global__ void k_leaf_ends(LevelSet ls, int D, uint64_t nv, const double* __restrict__ first,
+                            double* __restrict__ E, double* __restrict__ dips,
+                            unsigned long long* __restrict__ n_dips, unsigned long long cap) {
+#pragma clang fp contract(off)
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const Level lp = ls.lv[D - 1];
+  const LeafRef lf = vleaf(lp, D, v);
+  const LeafRef own = lf.dup ? vleaf(lp, D, v - 1) : lf;
+  const Level lv = ls.lv[own.d];
+  const double c = lv.carry[own.j];
+  const double e = c + lv.sum[own.j];
+  E[v] = e;
+  if (lf.dup || v == 0) return;
+  LeafRef pr = vleaf(lp, D, v - 1);
+  if (pr.dup) pr = vleaf(lp, D, v - 2);
+  const Level lq = ls.lv[pr.d];
+  const double e_prev = lq.carry[pr.j] + lq.sum[pr.j];
+  const double f = c + first[v];
+  if (f < e_prev) {
+    const unsigned long long k = atomicAdd(n_dips, 1ull);
+    if (k < cap) {
+      dips[2 * k] = f;
+      dips[2 * k + 1] = e_prev;
+    }
+  }
+}
+
+// The up-sweep of levels top-1 .. 0 and the down-sweep of levels 0 ..
+// top-1 (every level of at most 2^top slots) in one workgroup, with the
+// root's carry c[1] = v[1] set between them; the per-level kernels handle
+// the wider levels below.
+template <typename T>
+__global__ void __launch_bounds__(1024)
+    k_sweep_top(const typename CT<T>::V* __restrict__ psi, LevelSet ls, int top, bool up,
+                bool down) {
+#pragma clang fp contract(off)
+  const uint64_t t = threadIdx.x, B = blockDim.x;
+  if (up) {
+    for (int d = top - 1; d >= 0; --d) {
+      for (uint64_t j = t; j < (1ull << d); j += B) up_at(ls.lv[d], ls.lv[d + 1], j);
+      __syncthreads();
+    }
+  }
+  if (!down) return;
+  if (t == 0) ls.lv[0].carry[0] = prob_of<T>(psi[0]);
+  __syncthreads();
+  for (int d = 0; d < top; ++d) {
+    for (uint64_t j = t; j < (1ull << d); j += B) down_at(ls.lv[d], ls.lv[d + 1], j);
+    __syncthreads();
+  }
+}
+
+// the prefix c[i] (0-based) of the reference's pairwise cumsum, recomputed
+// from the leaf holding element i
+template <typename T>
+__device__ inline double prefix_at(const typename CT<T>::V* __restrict__ psi, const LevelSet& ls,
+                                   double c0, uint64_t i) {
+#pragma clang fp contract(off)
+  if (i == 0) return c0;
+  int d = 0;
+  uint64_t j = 0;
+  while (ls.lv[d].flag[j] == 0) {
+    const uint64_t half = ls.lv[d].len[j] >> 1;
+    j = 2 * j + (i >= ls.lv[d].start[j] + half ? 1 : 0);
+    ++d;
+  }
+  const uint64_t st = ls.lv[d].start[j];
+  double loc = prob_of<T>(psi[st]);
+  for (uint64_t k = st + 1; k <= i; ++k) loc = loc + prob_of<T>(psi[k]);
+  return ls.lv[d].carry[j] + loc;
+}
+
+// One draw per thread: the leaf-end search and the in-leaf recomputation,
+// or, for a draw inside a merged step interval (ulo[k], uhi[k]], Julia's
+// searchsortedfirst loop over the recomputed prefixes.
+template <typename T>
+__global__ void k_search_leaves(const typename CT<T>::V* __restrict__ psi, LevelSet ls, int D,
+                                uint64_t n, const double* __restrict__ E, uint64_t nv,
+                                const double* __restrict__ ulo, const double* __restrict__ uhi,
+                                int n_union, const double* __restrict__ r, int64_t nd,
+                                uint64_t* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nd) return;
+  const double x = r[d];
+  const double c0 = ls.lv[0].carry[0];
+  if (n_union > 0) {
+    // the last interval whose lower end lies below x
+    int a = 0, b = n_union;
+    while (a < b) {
+      const int m = (a + b) >> 1;
+      if (ulo[m] < x)
+        a = m + 1;
+      else
+        b = m;
+    }
+    if (a > 0 && x <= uhi[a - 1]) {
+      uint64_t lo = 1, len = n;
+      while (len != 0) {
+        const uint64_t half = len >> 1;
+        const uint64_t m = lo + half;
+        if (prefix_at<T>(psi, ls, c0, m - 1) < x) {
+          lo = m + 1;
+          len -= half + 1;
+        } else {
+          len = half;
+        }
+      }
+      out[d] = lo - 1;
+      return;
+    }
+  }
+  if (!(c0 < x)) {
+    out[d] = 0;
+    return;
+  }
+  uint64_t lo = 0, hi = nv;  // first slot with !(E < x)
+  while (lo < hi) {
+    const uint64_t m = lo + ((hi - lo) >> 1);
+    if (E[m] < x)
+      lo = m + 1;
+    else
+      hi = m;
+  }
+  if (lo == nv) {
+    out[d] = n;
+    return;
+  }
+  const LeafRef lf = vleaf(ls.lv[D - 1], D, lo);
+  const Level lv = ls.lv[lf.d];
+  const uint64_t st = lv.start[lf.j];
+  const uint32_t ln = lv.len[lf.j];
+  const double c = lv.carry[lf.j];
+  double loc = prob_of<T>(psi[st]);
+  uint64_t hit = st + ln - 1;
+  if (!(c + loc < x)) {
+    hit = st;
+  } else {
+    uint32_t i = 1;
+    bool found = false;
+    for (; i + kBatch <= ln && !found; i += kBatch) {
+      double pb[kBatch];
+#pragma unroll
+      for (uint32_t u = 0; u < kBatch; ++u) pb[u] = prob_of<T>(psi[st + i + u]);
+#pragma unroll
+      for (uint32_t u = 0; u < kBatch; ++u) {
+        loc = loc + pb[u];
+        if (!found && !(c + loc < x)) {
+          hit = st + i + u;
+          found = true;
+        }
+      }
+    }
+    for (; i < ln && !found; ++i) {
+      loc = loc + prob_of<T>(psi[st + i]);
+      if (!(c + loc < x)) {
+        hit = st + i;
+        found = true;
+      }
+    }
+  }
+  out[d] = hit;
+}
+
 static unsigned nblk(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// merges the step intervals (lo, hi] of the dip list into sorted disjoint ones
+static void merge_intervals(std::vector<double>& d, std::vector<double>& lo,
+                            std::vector<double>& hi) {
+  const size_t k = d.size() / 2;
+  std::vector<std::pair<double, double>> iv(k);
+  for (size_t i = 0; i < k; ++i) iv[i] = {d[2 * i], d[2 * i + 1]};
+  std::sort(iv.begin(), iv.end());
+  lo.clear();
+  hi.clear();
+  for (const auto& a : iv) {
+    if (!lo.empty() && a.first <= hi.back()) {
+      hi.back() = std::max(hi.back(), a.second);
+    } else {
+      lo.push_back(a.first);
+      hi.push_back(a.second);
+    }
+  }
+}
+
+constexpr unsigned long long kDipCap = 1ull << 20;
+
+// The draws' uniforms go to the device through the context's pinned
+// staging buffer (one host copy, then a DMA at full link speed); the
+// indices come back the same way.
+static int stage_draws(smoq_ctx* ctx, const double* r_host, int64_t nd, double* rd) {
+  const size_t bytes = (size_t)nd * sizeof(uint64_t);
+  int rc = ensure_pinned(ctx, bytes);
+  if (rc) return rc;
+  // the staging buffer may still feed an earlier copy on the stream
+  SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(ctx->host_pinned, r_host, (size_t)nd * sizeof(double));
+  SMOQ_HIP(hipMemcpyAsync(rd, ctx->host_pinned, (size_t)nd * sizeof(double),
+                          hipMemcpyHostToDevice, ctx->stream));
+  return SMOQ_OK;
+}
+
+static int finish_draws(smoq_ctx* ctx, const uint64_t* od, int64_t nd, uint64_t* out_host) {
+  SMOQ_HIP(hipMemcpyAsync(ctx->host_pinned, od, (size_t)nd * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+  SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(out_host, ctx->host_pinned, (size_t)nd * sizeof(uint64_t));
+  return SMOQ_OK;
+}
 
 template <typename T>
 static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, uint64_t* out_host) {
@@ -291,36 +577,41 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   const uint64_t n = diag ? (1ull << (s->n / 2)) : s->dim;
   const uint64_t diag_step = diag ? n + 1 : 0;
   const V* psi = (const V*)s->d;
-  // scratch: c[n] | r[nd] | out[nd] | tree
-  size_t off_r = n * sizeof(double);
-  size_t off_o = off_r + nd * sizeof(double);
-  size_t need = off_o + nd * sizeof(uint64_t);
-  need = (need + 255) & ~(size_t)255;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const bool tree = sem == SMOQ_SAMPLE_CUMSUM_SEARCH && n > 1;
+  const bool leaf_ends = tree && n > kBlockCumsumMax;
 
   // tree geometry: depth D with every slot at depth D a leaf or absent
-  std::vector<uint64_t> lvl_off;
   int D = 0;
-  if (sem == SMOQ_SAMPLE_CUMSUM_SEARCH && n > 1) {
+  if (tree) {
     uint64_t hi_len = n - 1;  // the largest node length at a depth
     while (hi_len >= kLeaf) {
       hi_len = hi_len - (hi_len >> 1);
       ++D;
     }
   }
-  const size_t per_slot = sizeof(uint64_t) + sizeof(uint32_t) + sizeof(uint8_t) + 2 * sizeof(double);
+  if (tree && D >= kMaxLevels) return set_error(SMOQ_ENOTSUP, "smoq_sample: tree too deep");
+  const uint64_t nv = leaf_ends ? (1ull << D) : 0;
+  const size_t per_slot =
+      sizeof(uint64_t) + sizeof(uint32_t) + sizeof(uint8_t) + 2 * sizeof(double);
   size_t tree_bytes = 0;
-  if (sem == SMOQ_SAMPLE_CUMSUM_SEARCH && n > 1) {
+  if (tree)
     for (int d = 0; d <= D; ++d) tree_bytes += ((uint64_t)1 << d) * per_slot + 256 * 5;
-  }
-  int rc = ensure_scratch(ctx, need + tree_bytes);
+  // scratch: r[nd] | out[nd] | c (the whole prefix array: short vectors) |
+  // tree levels (short vectors; long ones keep theirs in the tree buffer)
+  const size_t off_r = 0;
+  const size_t off_o = off_r + up(nd * sizeof(double));
+  const size_t off_c = off_o + up(nd * sizeof(uint64_t));
+  const size_t off_levels = off_c + up((leaf_ends ? 1 : n) * sizeof(double));
+  int rc = ensure_scratch(ctx, off_levels + (leaf_ends ? 0 : tree_bytes));
   if (rc) return rc;
   unsigned char* base = (unsigned char*)ctx->scratch;
-  double* c = (double*)base;
   double* rd = (double*)(base + off_r);
   uint64_t* od = (uint64_t*)(base + off_o);
-  SMOQ_HIP(hipMemcpyAsync(rd, r_host, nd * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  double* c = (double*)(base + off_c);
+  if ((rc = stage_draws(ctx, r_host, nd, rd))) return rc;
 
-  if (sem != SMOQ_SAMPLE_CUMSUM_SEARCH || n == 1) {
+  if (!tree) {
     hipLaunchKernelGGL(k_seq_prefix<T>, dim3(1), dim3(1), 0, ctx->stream, psi, n, c, diag_step);
     SMOQ_CHECK_LAUNCH("k_seq_prefix");
     if (sem != SMOQ_SAMPLE_CUMSUM_SEARCH) {
@@ -329,9 +620,11 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
       hipLaunchKernelGGL(k_search, dim3(nblk(nd)), dim3(256), 0, ctx->stream, c, n, rd, nd, od);
     }
     SMOQ_CHECK_LAUNCH("search kernel");
-  } else {
-    std::vector<Level> L(D + 1);
-    unsigned char* t = base + need;
+    return finish_draws(ctx, od, nd, out_host);
+  }
+
+  LevelSet ls;
+  auto lay_levels = [&](unsigned char* t) {
     for (int d = 0; d <= D; ++d) {
       const uint64_t ns = (uint64_t)1 << d;
       auto take = [&](size_t bytes) {
@@ -339,60 +632,117 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
         t += (bytes + 255) & ~(size_t)255;
         return p;
       };
-      L[d].start = (uint64_t*)take(ns * sizeof(uint64_t));
-      L[d].len = (uint32_t*)take(ns * sizeof(uint32_t));
-      L[d].flag = (uint8_t*)take(ns);
-      L[d].sum = (double*)take(ns * sizeof(double));
-      L[d].carry = (double*)take(ns * sizeof(double));
+      ls.lv[d].start = (uint64_t*)take(ns * sizeof(uint64_t));
+      ls.lv[d].len = (uint32_t*)take(ns * sizeof(uint32_t));
+      ls.lv[d].flag = (uint8_t*)take(ns);
+      ls.lv[d].sum = (double*)take(ns * sizeof(double));
+      ls.lv[d].carry = (double*)take(ns * sizeof(double));
     }
-    if (n <= kBlockCumsumMax && D < kMaxLevels) {
-      LevelSet ls;
-      for (int d = 0; d <= D; ++d) ls.lv[d] = L[d];
-      hipLaunchKernelGGL(k_cumsum_block<T>, dim3(1), dim3(1024), 0, ctx->stream, psi, n, ls, D,
-                         c, rd, nd, od);
-      SMOQ_CHECK_LAUNCH("k_cumsum_block");
-      SMOQ_HIP(hipMemcpyAsync(out_host, od, nd * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
+    return t;
+  };
+  if (!leaf_ends) {
+    lay_levels(base + off_levels);
+    hipLaunchKernelGGL(k_cumsum_block<T>, dim3(1), dim3(1024), 0, ctx->stream, psi, n, ls, D, c,
+                       rd, nd, od);
+    SMOQ_CHECK_LAUNCH("k_cumsum_block");
+    return finish_draws(ctx, od, nd, out_host);
+  }
+  // long vectors: the tree and the leaf-end buffers live in the context's
+  // sampling buffer, whose node geometry is kept for the next call of the
+  // same length
+  const size_t leaf_bytes = up(nv * sizeof(double)) * 2 + up(2 * kDipCap * sizeof(double)) + 256;
+  if (ctx->tree_dev_bytes < tree_bytes + leaf_bytes) {
+    if (ctx->tree_dev) {
       SMOQ_HIP(hipStreamSynchronize(ctx->stream));
-      return SMOQ_OK;
+      SMOQ_HIP(hipFree(ctx->tree_dev));
+      ctx->tree_dev = nullptr;
+      ctx->tree_dev_bytes = 0;
     }
+    ctx->tree_n = 0;
+    const size_t want = tree_bytes + leaf_bytes + 4096;
+    hipError_t e = hipMalloc(&ctx->tree_dev, want);
+    if (e != hipSuccess) {
+      ctx->tree_dev = nullptr;
+      return set_error(SMOQ_ENOMEM, "smoq_sample: %zu bytes for the cumsum tree: %s", want,
+                       hipGetErrorString(e));
+    }
+    ctx->tree_dev_bytes = want;
+  }
+  unsigned char* tb = (unsigned char*)ctx->tree_dev;
+  unsigned char* tl = lay_levels(tb);
+  double* first = (double*)tl;
+  double* E = (double*)(tl + up(nv * sizeof(double)));
+  double* dips = (double*)(tl + 2 * up(nv * sizeof(double)));
+  unsigned long long* ndips =
+      (unsigned long long*)(tl + 2 * up(nv * sizeof(double)) + up(2 * kDipCap * sizeof(double)));
+  if (ctx->tree_n != n) {
     // root: elements 2..n (1-based) = [1, n) 0-based
-    uint64_t rs = 1;
-    uint32_t rl = (uint32_t)(n - 1);
-    uint8_t rf = rl < kLeaf ? 1 : 0;
-    SMOQ_HIP(hipMemcpyAsync(L[0].start, &rs, sizeof(rs), hipMemcpyHostToDevice, ctx->stream));
-    SMOQ_HIP(hipMemcpyAsync(L[0].len, &rl, sizeof(rl), hipMemcpyHostToDevice, ctx->stream));
-    SMOQ_HIP(hipMemcpyAsync(L[0].flag, &rf, sizeof(rf), hipMemcpyHostToDevice, ctx->stream));
+    const uint64_t rs = 1;
+    const uint32_t rl = (uint32_t)(n - 1);
+    const uint8_t rf = rl < kLeaf ? 1 : 0;
+    SMOQ_HIP(hipMemcpyAsync(ls.lv[0].start, &rs, sizeof(rs), hipMemcpyHostToDevice, ctx->stream));
+    SMOQ_HIP(hipMemcpyAsync(ls.lv[0].len, &rl, sizeof(rl), hipMemcpyHostToDevice, ctx->stream));
+    SMOQ_HIP(hipMemcpyAsync(ls.lv[0].flag, &rf, sizeof(rf), hipMemcpyHostToDevice, ctx->stream));
     SMOQ_HIP(hipStreamSynchronize(ctx->stream));  // host scalars above live on the stack
     for (int d = 1; d <= D; ++d) {
       const uint64_t ns = (uint64_t)1 << d;
-      hipLaunchKernelGGL(k_tree_level, dim3(nblk(ns)), dim3(256), 0, ctx->stream, L[d - 1], L[d], ns);
+      hipLaunchKernelGGL(k_tree_level, dim3(nblk(ns)), dim3(256), 0, ctx->stream, ls.lv[d - 1],
+                         ls.lv[d], ns);
     }
-    for (int d = 0; d <= D; ++d) {
-      const uint64_t ns = (uint64_t)1 << d;
-      hipLaunchKernelGGL(k_leaf_sum<T>, dim3(nblk(ns, 64)), dim3(64), 0, ctx->stream, psi, L[d], ns);
-    }
-    for (int d = D - 1; d >= 0; --d) {
-      const uint64_t ns = (uint64_t)1 << d;
-      hipLaunchKernelGGL(k_up, dim3(nblk(ns)), dim3(256), 0, ctx->stream, L[d], L[d + 1], ns);
-    }
-    hipLaunchKernelGGL(k_first<T>, dim3(1), dim3(1), 0, ctx->stream, psi, c, L[0]);
-    for (int d = 0; d < D; ++d) {
-      const uint64_t ns = (uint64_t)1 << d;
-      hipLaunchKernelGGL(k_down, dim3(nblk(ns)), dim3(256), 0, ctx->stream, L[d], L[d + 1], ns);
-    }
-    for (int d = 0; d <= D; ++d) {
-      const uint64_t ns = (uint64_t)1 << d;
-      hipLaunchKernelGGL(k_leaf_write<T>, dim3(nblk(ns, 64)), dim3(64), 0, ctx->stream, psi, L[d],
-                         ns, c);
-    }
-    SMOQ_CHECK_LAUNCH("pairwise cumsum kernels");
-    hipLaunchKernelGGL(k_search, dim3(nblk(nd)), dim3(256), 0, ctx->stream, c, n, rd, nd, od);
-    SMOQ_CHECK_LAUNCH("k_search");
+    SMOQ_CHECK_LAUNCH("k_tree_level");
+    ctx->tree_n = n;
   }
-  SMOQ_HIP(hipMemcpyAsync(out_host, od, nd * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  const uint64_t n_parents = nv >> 1;
+  hipLaunchKernelGGL(k_leaf_sums_staged<T>,
+                     dim3((unsigned)((n_parents + kStageParents - 1) / kStageParents)),
+                     dim3(kStageBlock), 0, ctx->stream, psi, ls, D, n_parents, first);
+  // levels of at most 2^kTopLevels slots sweep in one workgroup
+  constexpr int kTopLevels = 12;
+  const int top = std::min(D, kTopLevels);
+  for (int d = D - 1; d >= top; --d) {
+    const uint64_t ns = (uint64_t)1 << d;
+    hipLaunchKernelGGL(k_up, dim3(nblk(ns)), dim3(256), 0, ctx->stream, ls.lv[d], ls.lv[d + 1],
+                       ns);
+  }
+  hipLaunchKernelGGL(k_sweep_top<T>, dim3(1), dim3(1024), 0, ctx->stream, psi, ls, top, true,
+                     true);
+  for (int d = top; d < D; ++d) {
+    const uint64_t ns = (uint64_t)1 << d;
+    hipLaunchKernelGGL(k_down, dim3(nblk(ns)), dim3(256), 0, ctx->stream, ls.lv[d], ls.lv[d + 1],
+                       ns);
+  }
+  SMOQ_HIP(hipMemsetAsync(ndips, 0, sizeof(unsigned long long), ctx->stream));
+  hipLaunchKernelGGL(k_leaf_ends, dim3(nblk(nv)), dim3(256), 0, ctx->stream, ls, D, nv, first, E,
+                     dips, ndips, kDipCap);
+  SMOQ_CHECK_LAUNCH("pairwise cumsum kernels");
+  unsigned long long n_dips = 0;
+  SMOQ_HIP(hipMemcpyAsync(&n_dips, ndips, sizeof(n_dips), hipMemcpyDeviceToHost, ctx->stream));
   SMOQ_HIP(hipStreamSynchronize(ctx->stream));
-  return SMOQ_OK;
+  // the step intervals, merged; more than the list holds: every draw takes
+  // the reference's search itself (one interval covering every value)
+  std::vector<double> ulo, uhi;
+  if (n_dips > kDipCap) {
+    ulo.push_back(-HUGE_VAL);
+    uhi.push_back(HUGE_VAL);
+  } else if (n_dips > 0) {
+    std::vector<double> hd(2 * n_dips);
+    SMOQ_HIP(hipMemcpyAsync(hd.data(), dips, hd.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+    merge_intervals(hd, ulo, uhi);
+  }
+  const int n_union = (int)ulo.size();
+  if (n_union > 0) {
+    // the dip list is read: its space holds the merged intervals now
+    SMOQ_HIP(hipMemcpyAsync(dips, ulo.data(), n_union * sizeof(double), hipMemcpyHostToDevice,
+                            ctx->stream));
+    SMOQ_HIP(hipMemcpyAsync(dips + kDipCap, uhi.data(), n_union * sizeof(double),
+                            hipMemcpyHostToDevice, ctx->stream));
+  }
+  hipLaunchKernelGGL(k_search_leaves<T>, dim3(nblk(nd)), dim3(256), 0, ctx->stream, psi, ls, D, n,
+                     E, nv, dips, dips + kDipCap, n_union, rd, nd, od);
+  SMOQ_CHECK_LAUNCH("k_search_leaves");
+  return finish_draws(ctx, od, nd, out_host);
 }
 
 }  // namespace smoq

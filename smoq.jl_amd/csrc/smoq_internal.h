@@ -116,12 +116,12 @@ struct smoq_ctx_s {
   size_t ev_used = 0;
   double timed_ms = 0.0;
   int64_t timed_launches = 0;
-  // cached pairwise-cumsum tree (sampling), keyed by vector length
+  // pairwise-cumsum tree of long-vector sampling: level arrays and leaf-end
+  // buffers, grow-only; the node geometry in it is valid for tree_n
+  // elements (0: none)
   uint64_t tree_n = 0;
   void* tree_dev = nullptr;
   size_t tree_dev_bytes = 0;
-  std::vector<int64_t> tree_level_off;  // node offsets per level
-  int64_t tree_nodes = 0, tree_leaves = 0;
   // host buffers of tape applications (lowered gates, planned program),
   // kept so that their capacity is reused: fresh ones of a 1000-gate tape
   // (~0.6 MB) cost page faults on every call
