@@ -146,6 +146,79 @@ __global__ void k_rowsum(const double* __restrict__ in, int bpi, int rows, doubl
   out[r] = v;
 }
 
+// A Pauli term of H with its phase folded in: c * i^{|x&z|}.
+struct AdjTerm {
+  uint64_t x, z;
+  double fre, fim;
+};
+
+// Start of the fused reverse sweep: W (n+1 qubits per item) interleaves ψ
+// and λ = Hψ — W[2k] = ψ_k, W[2k+1] = λ_k, so W's qubit 0 selects the
+// vector and its qubit q+1 is ψ's qubit q — and the block partials of
+// E = Re<ψ|λ> (λ rounded to the state's type first, as a stored λ is).
+// λ is the pull form of apply_hamiltonian_bitwise!, one write per index.
+template <typename T>
+__global__ void __launch_bounds__(kAdjBlock)
+    k_adj_init(const VT_alias<T>* __restrict__ psi, VT_alias<T>* __restrict__ W, int n, int nt,
+               const AdjTerm* __restrict__ terms, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  __shared__ double sh[kAdjBlock / 64];
+  const uint64_t item = blockIdx.y;
+  const uint64_t dim = 1ull << n;
+  const VT_alias<T>* p = psi + item * dim;
+  VT_alias<T>* w = W + 2 * item * dim;
+  double acc = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < dim; k += stride) {
+    double hr = 0.0, hi = 0.0;
+    for (int t = 0; t < nt; ++t) {
+      const AdjTerm tm = terms[t];
+      const uint64_t j = k ^ tm.x;
+      const VT_alias<T> v = p[j];
+      const double vr = (double)v.x, vi = (double)v.y;
+      const bool neg = __popcll(j & tm.z) & 1;
+      const double sr = neg ? -vr : vr, si = neg ? -vi : vi;
+      hr += tm.fre * sr - tm.fim * si;
+      hi += tm.fre * si + tm.fim * sr;
+    }
+    const VT_alias<T> a = p[k];
+    const VT_alias<T> l = make_c<T>(hr, hi);
+    w[2 * k] = a;
+    w[2 * k + 1] = l;
+    acc += (double)a.x * (double)l.x + (double)a.y * (double)l.y;
+  }
+  acc = wave_sum_d(acc);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) v += sh[ww];
+    partial[item * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// out[slot * batch + item] = the sum of the item's tile partials of one
+// overlap slot (tiles_per_item consecutive tiles), in a fixed order: strided
+// per-thread sums, then a fixed tree over the threads.
+constexpr int kOvlRedBlock = 256;
+__global__ void __launch_bounds__(kOvlRedBlock)
+    k_ovl_reduce(const double* __restrict__ opart, uint64_t ntiles, uint64_t tiles_per_item,
+                 int batch, double* __restrict__ out) {
+  __shared__ double sh[kOvlRedBlock];
+  const uint64_t slot = blockIdx.x / batch, item = blockIdx.x % batch;
+  const double* src = opart + slot * ntiles + item * tiles_per_item;
+  double v = 0.0;
+  for (uint64_t t = threadIdx.x; t < tiles_per_item; t += kOvlRedBlock) v += src[t];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = kOvlRedBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
 namespace {
 
 // one step of a sweep: a fixed gate (and its inverse) or a Pauli rotation
@@ -153,6 +226,7 @@ struct Step {
   bool param = false;
   int pidx = -1;
   PauliRot rot{};
+  smoq_op op{};  // the tape op of a parametrised rotation
   IGate fwd, inv;
 };
 
@@ -189,6 +263,7 @@ int build_steps(const smoq_op* ops, int n_ops, int n_params, int n, std::vector<
       Step st;
       st.param = true;
       st.pidx = op.param;
+      st.op = op;
       PauliRot& g = st.rot;
       const uint64_t a = 1ull << op.q[0], b = op.q[1] >= 0 ? 1ull << op.q[1] : 0;
       g.ctrl = 0;
@@ -329,6 +404,143 @@ int energy_of(smoq_state* s, int n_terms, const uint64_t* x, const uint64_t* z, 
   return SMOQ_OK;
 }
 
+// g with every qubit moved up by one: W's qubit 0 is the ψ/λ selector
+IGate shifted_up(const IGate& g) {
+  IGate r = g;
+  if (r.t0 >= 0) r.t0 += 1;
+  if (r.t1 >= 0) r.t1 += 1;
+  for (int i = 0; i < r.nd; ++i) r.dq[i] += 1;
+  r.ctrl <<= 1;
+  r.oz <<= 1;
+  return r;
+}
+
+// The reverse sweep as fused passes over W (ψ and λ interleaved, one state
+// of n+1 qubits): walking the gates backwards, a parametrised rotation
+// becomes an overlap gate (Im<λ|G|ψ> into its slot) followed by R(-θ), any
+// other gate its inverse; every gate acts on both halves at once.  Batches
+// of one parameter set with compiled passes only; *done stays false when
+// the caller should take the gate-by-gate sweep instead.
+int fused_adjoint(smoq_state* s, const std::vector<Step>& steps, const double* theta,
+                  int n_terms, const uint64_t* xm, const uint64_t* zm, const double* coeff,
+                  int n_params, double* energy, double* grad, bool* done) {
+  *done = false;
+  smoq_ctx* ctx = s->ctx;
+  const int nW = s->n + 1;
+  if (s->batch != 1 || ctx->jit_mode == 0 || nW < 6 || nW > kMaxQubits) return SMOQ_OK;
+  if (n_terms < 0 || (n_terms > 0 && (!xm || !zm || !coeff)))
+    return set_error(SMOQ_EINVAL, "smoq_adjoint_gradient: bad term arrays");
+  std::vector<AdjTerm> tv;
+  for (int t = 0; t < n_terms; ++t) {
+    if ((xm[t] | zm[t]) & ~(s->dim - 1))
+      return set_error(SMOQ_EINVAL, "Pauli sum: term %d acts outside the %d qubits", t, s->n);
+    const int cnt = __builtin_popcountll(xm[t] & zm[t]) & 3;
+    const double re[4] = {1, 0, -1, 0}, im[4] = {0, 1, 0, -1};
+    tv.push_back({xm[t], zm[t], coeff[t] * re[cnt], coeff[t] * im[cnt]});
+  }
+  // the reverse program
+  std::vector<IGate>& rg = ctx->tape_gates;
+  rg.clear();
+  int n_slots = 0;
+  for (const Step& st : steps) n_slots += st.param;
+  int slot = 0;
+  std::vector<IGate> tmp;
+  for (int i = (int)steps.size() - 1; i >= 0; --i) {
+    const Step& st = steps[i];
+    if (!st.param) {
+      rg.push_back(shifted_up(st.inv));
+      continue;
+    }
+    IGate o;
+    o.kind = IK_OVL;
+    const uint64_t x = st.rot.x;
+    o.t0 = x ? __builtin_ctzll(x) : -1;
+    o.t1 = __builtin_popcountll(x) == 2 ? 63 - __builtin_clzll(x) : -1;
+    o.ctrl = st.rot.ctrl;
+    o.oz = st.rot.z;
+    o.ocnt = st.rot.cnt;
+    o.oslot = slot++;
+    IGate os = shifted_up(o);
+    os.sel = 0;
+    rg.push_back(os);
+    tmp.clear();
+    int rc = lower_op(st.op, -theta[st.pidx], s->n, tmp);
+    if (rc) return rc;
+    for (const IGate& g : tmp) rg.push_back(shifted_up(g));
+  }
+  Program& prog = ctx->tape_prog;
+  const int rowbits = s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits;
+  plan_program(rg, nW, pass_tile(ctx, nW, 1, rg.size()), ctx->low_qubits, rowbits, prog);
+  std::shared_ptr<JitModule> jit;
+  if (!prog.passes.empty()) {
+    const int jrc = jit_get(ctx, prog, s->dtype, jit);
+    if (jrc) {
+      if (ctx->jit_mode >= 2) return jrc;
+      return SMOQ_OK;  // automatic mode: the gate-by-gate sweep
+    }
+  }
+  // buffers: W (2S) | overlap partials, slot sums, energy partials | terms
+  void* wbuf = nullptr;
+  int rc = ensure_work(ctx, 1, 2 * s->bytes, &wbuf);
+  if (rc) return rc;
+  const uint64_t ntiles = prog.passes.empty() ? 1 : (1ull << prog.passes[0].n_nonlocal);
+  const int bpi = blocks_per_item(s);
+  const size_t n_op = (size_t)n_slots * ntiles;
+  void* pb = nullptr;
+  if ((rc = ensure_work(ctx, 2, (n_op + n_slots + bpi + 2) * sizeof(double), &pb))) return rc;
+  double* opart = (double*)pb;
+  double* sums = opart + n_op;
+  double* epart = sums + n_slots;
+  double* esum = epart + bpi;
+  void* tb = nullptr;
+  if ((rc = ensure_work(ctx, 3, (tv.size() + 1) * sizeof(AdjTerm), &tb))) return rc;
+  AdjTerm* dterms = (AdjTerm*)tb;
+  if (!tv.empty())
+    SMOQ_HIP(hipMemcpyAsync(dterms, tv.data(), tv.size() * sizeof(AdjTerm), hipMemcpyHostToDevice,
+                            ctx->stream));
+  if (s->dtype == SMOQ_C128)
+    hipLaunchKernelGGL(k_adj_init<double>, dim3(bpi, 1), dim3(kAdjBlock), 0, ctx->stream,
+                       (const double2*)s->d, (double2*)wbuf, s->n, (int)tv.size(), dterms, epart);
+  else
+    hipLaunchKernelGGL(k_adj_init<float>, dim3(bpi, 1), dim3(kAdjBlock), 0, ctx->stream,
+                       (const float2*)s->d, (float2*)wbuf, s->n, (int)tv.size(), dterms, epart);
+  SMOQ_CHECK_LAUNCH("k_adj_init");
+  hipLaunchKernelGGL(k_rowsum, dim3(1), dim3(256), 0, ctx->stream, epart, bpi, 1, esum);
+  SMOQ_CHECK_LAUNCH("k_rowsum");
+  if (!prog.passes.empty()) {
+    smoq_state ws = *s;
+    ws.n = nW;
+    ws.dim = 2 * s->dim;
+    ws.bytes = 2 * s->bytes;
+    ws.d = wbuf;
+    const ProgLayout L = program_layout(prog, s->dtype);
+    if ((rc = ensure_prog(ctx, L.total))) return rc;
+    if ((rc = upload_program(ctx, prog, s->dtype, L, ctx->prog))) return rc;
+    if ((rc = launch_program(&ws, prog, L, ctx->prog, jit.get(), opart))) return rc;
+    if (n_slots > 0) {
+      hipLaunchKernelGGL(k_ovl_reduce, dim3((unsigned)n_slots), dim3(kOvlRedBlock), 0,
+                         ctx->stream, opart, ntiles, ntiles, 1, sums);
+      SMOQ_CHECK_LAUNCH("k_ovl_reduce");
+    }
+  }
+  std::vector<double> hs(n_slots + 1);
+  if (n_slots > 0)
+    SMOQ_HIP(hipMemcpyAsync(hs.data(), sums, n_slots * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+  SMOQ_HIP(hipMemcpyAsync(energy, esum, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+  // each parameter's slots in gate order (slot numbers run backwards)
+  for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
+  int fwd = 0;
+  for (const Step& st : steps) {
+    if (!st.param) continue;
+    grad[st.pidx] += hs[n_slots - 1 - fwd];
+    ++fwd;
+  }
+  *done = true;
+  return SMOQ_OK;
+}
+
 }  // namespace
 }  // namespace smoq
 
@@ -378,8 +590,19 @@ extern "C" int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_op
   if ((rc = build_steps(ops, n_ops, n_params, s->n, steps))) return rc;
   Work w;
   std::vector<double2> hcs;
-  if ((rc = setup(s, theta, n_params, w, hcs))) return rc;
-  if ((rc = forward(s, ops, n_ops, theta, n_params, steps, w.cs, false))) return rc;
+  // one parameter set: fused forward passes, then the fused reverse sweep
+  if (s->batch == 1 && s->n >= kRegQubits) {
+    if ((rc = forward(s, ops, n_ops, theta, n_params, steps, nullptr, false))) return rc;
+    bool done = false;
+    if ((rc = fused_adjoint(s, steps, theta, n_terms, xmask, zmask, coeff, n_params, energy, grad,
+                            &done)))
+      return rc;
+    if (done) return SMOQ_OK;
+    if ((rc = setup(s, theta, n_params, w, hcs))) return rc;
+  } else {
+    if ((rc = setup(s, theta, n_params, w, hcs))) return rc;
+    if ((rc = forward(s, ops, n_ops, theta, n_params, steps, w.cs, false))) return rc;
+  }
   if ((rc = energy_of(s, n_terms, xmask, zmask, coeff, w, energy))) return rc;
 
   // reverse sweep on a working copy of ψ, so that s keeps U(θ)|0>

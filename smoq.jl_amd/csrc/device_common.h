@@ -125,6 +125,32 @@ __device__ inline uint32_t swz(uint32_t l) {
 }  // namespace smoq
 
 namespace smoq {
+// Sum over the wave by DPP lane moves (quad xor 1, quad xor 2, half-row
+// mirror, row mirror: every lane of a 16-lane row then holds the row sum)
+// and the four row sums read out in a fixed order; the result is uniform
+// and the same on every run.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xf, 0xf,
+                                             false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double read_lane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double wave_total(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
 // component-wise select: a struct-level `c ? a : b` becomes a pointer select
 // plus copy, which keeps register arrays from being promoted out of scratch
 template <typename V>

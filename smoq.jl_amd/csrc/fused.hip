@@ -89,7 +89,7 @@ int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayo
 
 template <typename T>
 static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout& L,
-                            const void* dbuf, const JitModule* jit) {
+                            const void* dbuf, const JitModule* jit, double* opart) {
   smoq_ctx* ctx = s->ctx;
   int rc = SMOQ_OK;
   const unsigned char* d = (const unsigned char*)dbuf;
@@ -100,6 +100,10 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
   if (jit && jit->fn.size() != prog.passes.size())
     return set_error(SMOQ_EINVAL, "fused pass: compiled module has %zu passes, program %zu",
                      jit->fn.size(), prog.passes.size());
+  if (!jit)
+    for (const GateDev& g : prog.gates)
+      if (g.kind == GK_OVL)
+        return set_error(SMOQ_EINVAL, "fused pass: overlap gates need compiled passes");
   VT<T>* psi = (VT<T>*)s->d;
   int n = s->n;
   for (size_t p = 0; p < prog.passes.size(); ++p) {
@@ -128,7 +132,7 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
     if (ctx->timing && (rc = time_mark(ctx))) return rc;
     if (jit) {
       const PassDev* pp = dp + p;
-      void* args[] = {&psi, &n, &pp, (void*)&ds, (void*)&dg, (void*)&dt};
+      void* args[] = {&psi, &n, &pp, (void*)&ds, (void*)&dg, (void*)&dt, &opart};
       SMOQ_HIP(hipModuleLaunchKernel(jit->fn[p], (unsigned)gx, (unsigned)gy, 1, nt, 1, 1,
                                      (unsigned)shm, ctx->stream, args, nullptr));
     } else {
@@ -142,9 +146,9 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
 }
 
 int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf,
-                   const JitModule* jit) {
-  return s->dtype == SMOQ_C128 ? launch_program_t<double>(s, prog, L, dbuf, jit)
-                               : launch_program_t<float>(s, prog, L, dbuf, jit);
+                   const JitModule* jit, double* opart) {
+  return s->dtype == SMOQ_C128 ? launch_program_t<double>(s, prog, L, dbuf, jit, opart)
+                               : launch_program_t<float>(s, prog, L, dbuf, jit, opart);
 }
 
 int run_program(smoq_state* s, Program& prog, bool fixed_angles) {

@@ -251,6 +251,57 @@ __device__ __forceinline__ void apply_gate(VT<T> (&x)[kNR], const GateDev& g, ui
   }
 }
 
+// Adjoint overlap (GK_OVL): the register patterns with the selector slot
+// SEL clear hold ψ, those with it set λ at the same index.  For every ψ
+// pattern j (index k) the thread adds Im[conj(λ_k) (Gψ)_k] with
+// (Gψ)_k = i^CNT (-1)^{|(k ^ x) & z|} ψ_{k ^ x}, x on slots A, B (-1: none),
+// for k inside the controls.  The wave's sum goes to acc[wave] (written once
+// per tile; the pass adds the waves in order at its end).
+// NT: threads per tile; below a wave, the sum is a butterfly over the
+// tile's lanes.
+template <int A, int B, int SEL, int CNT, bool CTRL, int NT>
+__device__ __forceinline__ void reg_ovl(const VT<double> (&x)[kNR], const GateDev& g, uint64_t gl,
+                                        double* acc) {
+  constexpr int xr = (A >= 0 ? 1 << A : 0) | (B >= 0 ? 1 << B : 0);
+  constexpr int sm = 1 << SEL;
+  const int zr = g.ozreg;
+  const int zp = __popcll(gl & g.ozglob) & 1;
+  const bool gon = CTRL ? (gl & g.cglob) == g.cglob : true;
+  double ov = 0.0;
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) {
+    if (j & sm) continue;
+    const int jp = j ^ xr;
+    const VT<double> l = x[j | sm];
+    const VT<double> w = x[jp];
+    double ux, uy;
+    if (CNT == 0) {
+      ux = w.x;
+      uy = w.y;
+    } else if (CNT == 1) {
+      ux = -w.y;
+      uy = w.x;
+    } else if (CNT == 2) {
+      ux = -w.x;
+      uy = -w.y;
+    } else {
+      ux = w.y;
+      uy = -w.x;
+    }
+    double c = l.x * uy - l.y * ux;
+    if ((zp ^ __popc(jp & zr)) & 1) c = -c;
+    if (CTRL) c = (gon && (j & g.creg) == g.creg) ? c : 0.0;
+    ov += c;
+  }
+  if constexpr (NT >= 64) {
+    ov = wave_total(ov);
+  } else {
+#pragma unroll
+    for (int off = NT / 2; off > 0; off >>= 1) ov += __shfl_xor(ov, off, NT);
+  }
+  if ((threadIdx.x & 63) == 0) acc[threadIdx.x >> 6] = ov;
+}
+
 // ---------------------------------------------------------------- frames --
 // One workgroup, one tile of 2^k amplitudes: the tile id picks the batch
 // item and the global bits outside the tile (uniform), the thread index the
@@ -259,6 +310,7 @@ __device__ __forceinline__ void apply_gate(VT<T> (&x)[kNR], const GateDev& g, ui
 template <typename T>
 struct PassFrame {
   VT<T>* pt;         // amplitude of register index 0 of this thread
+  uint64_t tile;     // tile index over the whole launch (batch items included)
   uint64_t gbase;    // global bits outside the tile
   uint64_t off[kNR]; // offsets of register index i
   uint32_t st_lane;  // swizzled LDS index of the thread's lane part
@@ -275,6 +327,7 @@ __device__ __forceinline__ void pass_load(VT<T>* lds, VT<T>* __restrict__ psi, i
   // tiles beyond one grid row continue in blockIdx.y (states of 2^32 threads
   // and more per pass)
   const uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  f.tile = tile;
   const uint64_t item = tile >> nn;
   const uint64_t tt = tile & ((1ull << nn) - 1ull);
   f.nt = 1 << (pass->k - R);

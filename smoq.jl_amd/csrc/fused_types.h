@@ -47,6 +47,12 @@ enum GKind : int32_t {
   // sigmas.
   GK_U1_ROT = 8,   // [[a, -b], [b, a]] real (Ry and products of Ry)
   GK_U1_RXR = 9,   // [[a, i q], [i q, a]] (Rx and products of Rx)
+  // Adjoint-gradient overlap (run-time compiled passes only): the state
+  // holds ψ and λ as the two halves of qubit `osel`; the gate leaves the
+  // amplitudes alone and adds Im<λ|G|ψ> of its tile, for the Pauli string
+  // G = i^ocnt X^x Z^z (x on slots a, b; controls as for the other kinds),
+  // to overlap slot `oslot`.
+  GK_OVL = 10,
 };
 constexpr int kMaxTabBits = 3;   // non-register qubits of one phase table
 constexpr int kMaxCzShifts = 4;
@@ -72,6 +78,10 @@ struct GateDev {
   int32_t czs[kMaxCzShifts];
   uint64_t czm[kMaxCzShifts];
   uint64_t czcm[kRegQubits];
+  // GK_OVL: selector slot, overlap slot, |x & z| mod 4, the z bits held in
+  // register slots (slot mask) and outside them (global mask)
+  int32_t osel, oslot, ocnt, ozreg;
+  uint64_t ozglob;
   double m[32];
 };
 

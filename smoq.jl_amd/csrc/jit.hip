@@ -177,11 +177,21 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
       << ", 4) smoq_pass_" << p << "(VT<" << T
       << ">* __restrict__ psi, int n, const PassDev* __restrict__ pass,"
          " const StageDev* __restrict__ stages, const GateDev* __restrict__ gates,"
-         " const double2* __restrict__ tabs) {\n"
+         " const double2* __restrict__ tabs, double* __restrict__ opart) {\n"
       << "  typedef " << T << " T;\n"
       << "  extern __shared__ __align__(16) unsigned char smem_raw[];\n"
       << "  VT<T>* lds = reinterpret_cast<VT<T>*>(smem_raw);\n"
       << "  PassFrame<T> f;\n";
+    // adjoint overlaps of this pass: one LDS slot per (overlap, wave)
+    const int ntp = 1 << (pd.k - kRegQubits);
+    const int nwaves = ntp >= 64 ? ntp / 64 : 1;
+    std::vector<int> oslots;
+    for (int s = pd.s0; s < pd.s1; ++s)
+      for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g)
+        if (prog.gates[g].kind == GK_OVL) oslots.push_back(prog.gates[g].oslot);
+    if (!oslots.empty())
+      o << "  __shared__ double ovl_acc[" << (int)oslots.size() * nwaves << "];\n";
+    int ovl_i = 0;
     // Passes of few tiles (small states) run one workgroup per CU, and each
     // stage would wait a full memory latency for its gate records on their
     // first scalar load.  Every thread first loads one dword per 128 B line
@@ -259,6 +269,12 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
             break;
           }
           case GK_DIAGREG: o << "    reg_diagreg<double>(x, " << gd << ");\n"; break;
+          case GK_OVL:
+            o << "    reg_ovl<" << G.a << ", " << G.b << ", " << G.osel << ", " << G.ocnt << ", "
+              << ((G.creg | G.cglob) ? "true" : "false") << ", " << ntp << ">(x, " << gd
+              << ", sf.gl, ovl_acc + " << ovl_i * nwaves << ");\n";
+            ++ovl_i;
+            break;
           case GK_U1_ROT:
           case GK_U1_RXR:
             o << "    reg_u1s<double, " << (G.kind == GK_U1_ROT ? "GK_U1_ROT" : "GK_U1_RXR") << ", "
@@ -277,6 +293,18 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
       o << "  pass_store_scaled<T>(lds, f, pass->scale);\n";
     else
       o << "  pass_store<T>(lds, f);\n";
+    if (!oslots.empty()) {
+      // the tile's overlaps: the waves' sums in order, one partial per
+      // (overlap slot, tile of the launch)
+      o << "  {\n    const int oslot[" << oslots.size() << "] = {";
+      for (size_t i = 0; i < oslots.size(); ++i) o << (i ? ", " : "") << oslots[i];
+      o << "};\n    const uint64_t ntiles = (uint64_t)gridDim.x * gridDim.y;\n"
+        << "    for (int t = threadIdx.x; t < " << oslots.size() << "; t += blockDim.x) {\n"
+        << "      double v = 0.0;\n"
+        << "      for (int w = 0; w < " << nwaves << "; ++w) v += ovl_acc[t * " << nwaves
+        << " + w];\n"
+        << "      opart[(uint64_t)oslot[t] * ntiles + f.tile] = v;\n    }\n  }\n";
+    }
     if (warm) o << "  asm volatile(\"\" :: \"v\"(warm));\n";
     o << "}\n";
   }

@@ -25,18 +25,28 @@ namespace smoq {
 
 using cd = std::complex<double>;
 
+// An overlap reads its Pauli string's support; gates on other qubits commute
+// with it (both halves change by the same unitary), and so do other
+// overlaps, so the selector qubit is not part of what it touches.
+static uint64_t ovl_x(const IGate& g) {
+  return (g.t0 >= 0 ? 1ull << g.t0 : 0) | (g.t1 >= 0 ? 1ull << g.t1 : 0);
+}
+
 static uint64_t touch_mask(const IGate& g) {
   uint64_t m = g.ctrl;
   if (g.kind == IK_U1) m |= 1ull << g.t0;
   if (g.kind == IK_U2) m |= (1ull << g.t0) | (1ull << g.t1);
   if (g.kind == IK_DIAG)
     for (int i = 0; i < g.nd; ++i) m |= 1ull << g.dq[i];
+  if (g.kind == IK_OVL) m |= ovl_x(g) | g.oz;
   return m;
 }
 
+// qubits that must be local to the pass (and register slots of the stage)
 static uint64_t target_mask(const IGate& g) {
   if (g.kind == IK_U1) return 1ull << g.t0;
   if (g.kind == IK_U2) return (1ull << g.t0) | (1ull << g.t1);
+  if (g.kind == IK_OVL) return ovl_x(g) | (1ull << g.sel);
   return 0;
 }
 
@@ -83,6 +93,32 @@ static void emit_nondiag(const IGate& g, const StageCtx& sc, Program& prog) {
       gd.cglob |= 1ull << b;
   }
   std::memcpy(gd.m, g.m, sizeof(gd.m));
+  prog.gates.push_back(gd);
+}
+
+static void emit_ovl(const IGate& g, const StageCtx& sc, Program& prog) {
+  GateDev gd = blank_gate();
+  gd.kind = GK_OVL;
+  gd.a = g.t0 >= 0 ? sc.slot_of(g.t0) : -1;
+  gd.b = g.t1 >= 0 ? sc.slot_of(g.t1) : -1;
+  gd.osel = sc.slot_of(g.sel);
+  gd.oslot = g.oslot;
+  gd.ocnt = g.ocnt;
+  for (int b = 0; b < sc.n; ++b) {
+    const int sl = sc.slot_of(b);
+    if (g.ctrl >> b & 1) {
+      if (sl >= 0)
+        gd.creg |= 1 << sl;
+      else
+        gd.cglob |= 1ull << b;
+    }
+    if (g.oz >> b & 1) {
+      if (sl >= 0)
+        gd.ozreg |= 1 << sl;
+      else
+        gd.ozglob |= 1ull << b;
+    }
+  }
   prog.gates.push_back(gd);
 }
 
@@ -337,6 +373,11 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
         uint32_t need = 0;
         if (g.kind == IK_U1) need = 1u << g2l[g.t0];
         if (g.kind == IK_U2) need = (1u << g2l[g.t0]) | (1u << g2l[g.t1]);
+        if (g.kind == IK_OVL) {
+          need = 1u << g2l[g.sel];
+          if (g.t0 >= 0) need |= 1u << g2l[g.t0];
+          if (g.t1 >= 0) need |= 1u << g2l[g.t1];
+        }
         if (popc(S | need) > kRegQubits) break;
         S |= need;
       }
@@ -370,7 +411,10 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
           emit_diag_run(run, sc, prog);
           run.clear();
         }
-        emit_nondiag(g, sc, prog);
+        if (g.kind == IK_OVL)
+          emit_ovl(g, sc, prog);
+        else
+          emit_nondiag(g, sc, prog);
       }
       if (!run.empty()) emit_diag_run(run, sc, prog);
       sd.g1 = (int)prog.gates.size();

@@ -39,6 +39,10 @@ enum IKind : int32_t {
   IK_U1 = 0,    // 2x2 on t0, applied where (index & ctrl) == ctrl
   IK_U2 = 1,    // 4x4 on (t0 low role, t1 high role), same ctrl rule
   IK_DIAG = 2,  // phase table over nd <= 3 global qubits dq[]
+  // Im<λ|G|ψ> into overlap slot oslot, ψ and λ the halves of qubit sel, G the
+  // Pauli string i^ocnt X^x Z^oz with x on t0 (and t1 when >= 0), applied
+  // where (index & ctrl) == ctrl; the amplitudes stay unchanged
+  IK_OVL = 3,
 };
 
 struct IGate {
@@ -47,6 +51,9 @@ struct IGate {
   int32_t nd = 0;
   int32_t dq[3] = {-1, -1, -1};
   uint64_t ctrl = 0;
+  // IK_OVL only
+  int32_t sel = -1, oslot = -1, ocnt = 0;
+  uint64_t oz = 0;
   double m[32] = {0};
 };
 
@@ -186,8 +193,10 @@ struct ProgLayout {
 ProgLayout program_layout(const Program& prog, int dtype);
 int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayout& L, void* d);
 // jit: the program's compiled pass kernels, or null for the precompiled one
+// opart: per-tile partials of the program's overlap gates (GK_OVL), at
+// opart[oslot * tiles + tile]; compiled passes only
 int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf,
-                   const JitModule* jit);
+                   const JitModule* jit, double* opart = nullptr);
 // run-time compiled pass kernels (jit.hip)
 std::string jit_source(const Program& prog, int dtype);
 // rewrites rotation gates into the factored kinds of run-time compiled

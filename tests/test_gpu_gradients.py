@@ -37,7 +37,16 @@ def oracle_param_shift(tape, theta, N):
     return g
 
 
-def test_adjoint_matches_oracle_parameter_shift(ctx):
+@pytest.fixture(params=[0, 2], ids=["per_gate_sweep", "fused_sweep"])
+def jit_mode(request, ctx):
+    # 0: the gate-by-gate reverse sweep; 2: fused passes with overlap gates
+    # (compiled passes forced, so a compilation failure is an error)
+    ctx.set_jit(request.param)
+    yield request.param
+    ctx.set_jit(1)
+
+
+def test_adjoint_matches_oracle_parameter_shift(ctx, jit_mode):
     from smoq_amd.circuits import hardware_efficient_ansatz
     from smoq_amd.gradients import VQEEnergy
 
@@ -52,7 +61,7 @@ def test_adjoint_matches_oracle_parameter_shift(ctx):
     assert np.max(np.abs(g[0] - want)) < 1e-10
 
 
-def test_all_rotation_kinds_against_parameter_shift_and_fd(ctx):
+def test_all_rotation_kinds_against_parameter_shift_and_fd(ctx, jit_mode):
     from smoq_amd.circuits import Tape
     from smoq_amd.gradients import VQEEnergy, calculate_gradients_finite_difference
 
@@ -106,7 +115,7 @@ def test_batched_adjoint_equals_single(ctx):
     assert abs(v1.energy(thetas[0], one_gate_kernels=True)[0] - v1.energy(thetas[0])[0]) < 1e-12
 
 
-def test_adjoint_complex64(ctx):
+def test_adjoint_complex64(ctx, jit_mode):
     from smoq_amd.circuits import hardware_efficient_ansatz
     from smoq_amd.gradients import VQEEnergy
 
@@ -118,3 +127,40 @@ def test_adjoint_complex64(ctx):
     e64, g64 = VQEEnergy(tape, xs, zs, cs, dtype="c64").energy_and_grad(theta)
     assert abs(e64[0] - e128[0]) < 1e-5
     assert np.max(np.abs(g64[0] - g128[0])) < 1e-5
+
+
+def test_fused_adjoint_many_passes_against_oracle(ctx, jit_mode):
+    # 14 qubits: the reverse sweep's 15-qubit buffer spans several tiles and
+    # passes; every rotation kind, controls and fixed gates in between
+    from smoq_amd.circuits import Tape
+    from smoq_amd.gradients import VQEEnergy
+
+    N = 14
+    rng = np.random.default_rng(11)
+    t = Tape(N)
+    p = 0
+    for layer in range(3):
+        for q in range(N):
+            t.add(["RY", "RZ", "RX"][(q + layer) % 3], [q], param=p)
+            p += 1
+        for q in range(layer % 2, N - 1, 2):
+            t.add("CZ", [q, q + 1])
+        t.add(["RXX", "RYY", "RZZ"][layer], [layer, N - 1 - layer], param=p)
+        p += 1
+        t.add("H", [5]).add("CNOT", [3, 11]).add("T", [13])
+    theta = rng.uniform(0, 2 * math.pi, t.n_params)
+    xs, zs, cs = tfim_terms(N, 0.9, 1.1)
+    v = VQEEnergy(t, xs, zs, cs)
+    e, g = v.energy_and_grad(theta)
+    assert abs(e[0] - oracle_energy(t, theta, N, 0.9, 1.1)) < 1e-11
+    ps = np.zeros(t.n_params)
+    for k in range(t.n_params):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += math.pi / 2
+        tm[k] -= math.pi / 2
+        ps[k] = (oracle_energy(t, tp, N, 0.9, 1.1) - oracle_energy(t, tm, N, 0.9, 1.1)) / 2.0
+    assert np.max(np.abs(g[0] - ps)) < 1e-10
+    # the state keeps U(θ)|0>, and a second call gives the same gradient
+    e2, g2 = v.energy_and_grad(theta)
+    assert np.array_equal(g2, g) and e2[0] == e[0]
+    v.free()
