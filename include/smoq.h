@@ -163,6 +163,31 @@ int smoq_apply_named(smoq_state* s, int gate, const int* qubits, double theta);
 int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops,
                     const double* theta, int n_theta, int* n_passes);
 
+/* ---- autotuning of the fused passes ---------------------------------------
+ * The reference fuses gates by a fixed greedy rule (fuse_gates,
+ * SmoQ.jl_benchmarks/benchmark_random_circuit_sampling/run_benchmark_smoq_Float64.jl:167-200).
+ * smoq_autotune plans the tape under candidate settings (tile qubits 11 or
+ * 12, 4 or 5 contiguous low qubits, a cap of 0 / 28 / 36 / 44 non-diagonal
+ * gates per pass) besides the context's own, compiles their pass kernels,
+ * times each candidate on s (its contents are overwritten) and records the
+ * fastest for the tape's structure (gate kinds and qubits), qubit count,
+ * dtype and batch: later smoq_apply_tape / smoq_plan_create calls with that
+ * structure use it, unless smoq_set_fusion fixed a tile size.  No
+ * reference counterpart (tooling). */
+typedef struct smoq_tune_result {
+  int tile_qubits, low_qubits, pass_max_gates; /* the chosen settings */
+  int n_passes;       /* HBM passes per application with them */
+  int candidates;     /* settings timed (0: nothing to choose) */
+  double ms;          /* one application with the chosen settings */
+  double default_ms;  /* one application with the context's settings */
+} smoq_tune_result;
+int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta,
+                  int n_theta, smoq_tune_result* out);
+/* The context's tuned settings to / from a text file (one line per
+ * structure); loading merges into the context's table. */
+int smoq_tune_save(smoq_ctx* ctx, const char* path);
+int smoq_tune_load(smoq_ctx* ctx, const char* path, int* n_entries);
+
 /* ---- compiled tapes ------------------------------------------------------
  * A plan is a tape planned once (fused passes, device-resident program) for
  * states of one qubit count and dtype; every run replays the same launches,

@@ -342,7 +342,8 @@ int forward(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta, i
     if ((rc = lower_tape(ops, n_ops, theta, n_params, s->n, gs))) return rc;
     Program& prog = s->ctx->tape_prog;
     plan_program(gs, s->n, pass_tile(s->ctx, s->n, s->batch, gs.size()), s->ctx->low_qubits,
-                 s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog);
+                 s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog,
+                 s->ctx->pass_max_gates);
     return run_program(s, prog);
   }
   for (const Step& st : steps) {
@@ -470,7 +471,8 @@ int fused_adjoint(smoq_state* s, const std::vector<Step>& steps, const double* t
   }
   Program& prog = ctx->tape_prog;
   const int rowbits = s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits;
-  plan_program(rg, nW, pass_tile(ctx, nW, 1, rg.size()), ctx->low_qubits, rowbits, prog);
+  plan_program(rg, nW, pass_tile(ctx, nW, 1, rg.size()), ctx->low_qubits, rowbits, prog,
+               ctx->pass_max_gates);
   std::shared_ptr<JitModule> jit;
   if (!prog.passes.empty()) {
     const int jrc = jit_get(ctx, prog, s->dtype, jit);

@@ -13,8 +13,16 @@
 // plans with new angles reuse the compiled module.
 #include <hip/hiprtc.h>
 
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <iterator>
 #include <sstream>
+#include <thread>
 #include <type_traits>
 
 #include "device_common.h"
@@ -330,22 +338,39 @@ JitModule::~JitModule() {
   if (mod) (void)hipModuleUnload(mod);
 }
 
-static int compile(const std::string& src, const Program& pg, std::shared_ptr<JitModule>& out) {
+// hiprtc options: the flags of the precompiled kernels (no packed FP32, no
+// sinking of the per-slot stores)
+static std::vector<std::string> rtc_options() {
+  return {"--gpu-architecture=gfx950",
+          "-O3",
+          "-std=c++17",
+          "-DSMOQ_REG_QUBITS=" + std::to_string(kRegQubits),
+          "-DSMOQ_ROWBITS_C128=" + std::to_string(CT<double>::kRowBits),
+          "-DSMOQ_ROWBITS_C64=" + std::to_string(CT<float>::kRowBits),
+          "-mllvm",
+          "-simplifycfg-sink-common=false",
+          "-Xclang",
+          "-target-feature",
+          "-Xclang",
+          "-packed-fp32-ops"};
+}
+
+// One module's source to a code object.  Only hiprtc calls (no HIP runtime
+// state), so several compile at once on host threads; err receives the
+// compiler's log on failure.
+static bool rtc_compile(const std::string& src, std::vector<char>& code, std::string& err) {
   const char* headers[3] = {kSrcDeviceCommon, kSrcFusedTypes, kSrcFusedOps};
   const char* names[3] = {"device_common.h", "fused_types.h", "fused_ops.h"};
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "smoq_passes.hip", 3, headers, names) !=
-      HIPRTC_SUCCESS)
-    return set_error(SMOQ_EDEVICE, "hiprtcCreateProgram failed");
-  const std::string reg = "-DSMOQ_REG_QUBITS=" + std::to_string(kRegQubits);
-  const std::string rb128 = "-DSMOQ_ROWBITS_C128=" + std::to_string(CT<double>::kRowBits);
-  const std::string rb64 = "-DSMOQ_ROWBITS_C64=" + std::to_string(CT<float>::kRowBits);
-  // the flags of the precompiled kernels: no packed FP32, no sinking of the
-  // per-slot stores
-  const char* opts[] = {"--gpu-architecture=gfx950", "-O3", "-std=c++17", reg.c_str(),
-                        rb128.c_str(), rb64.c_str(), "-mllvm", "-simplifycfg-sink-common=false",
-                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-  const hiprtcResult cr = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+      HIPRTC_SUCCESS) {
+    err = "hiprtcCreateProgram failed";
+    return false;
+  }
+  const std::vector<std::string> opt = rtc_options();
+  std::vector<const char*> opts;
+  for (const std::string& o : opt) opts.push_back(o.c_str());
+  const hiprtcResult cr = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
   if (cr != HIPRTC_SUCCESS) {
     size_t n = 0;
     std::string log;
@@ -354,26 +379,94 @@ static int compile(const std::string& src, const Program& pg, std::shared_ptr<Ji
       (void)hiprtcGetProgramLog(prog, &log[0]);
     }
     (void)hiprtcDestroyProgram(&prog);
-    return set_error(SMOQ_EDEVICE, "pass kernel compilation failed (%s): %.400s",
-                     hiprtcGetErrorString(cr), log.c_str());
+    err = std::string(hiprtcGetErrorString(cr)) + ": " + log.substr(0, 400);
+    return false;
   }
   size_t code_n = 0;
-  std::vector<char> code;
+  code.clear();
   if (hiprtcGetCodeSize(prog, &code_n) == HIPRTC_SUCCESS) {
     code.resize(code_n);
     (void)hiprtcGetCode(prog, code.data());
   }
   (void)hiprtcDestroyProgram(&prog);
-  if (code.empty()) return set_error(SMOQ_EDEVICE, "pass kernel compilation produced no code");
+  if (code.empty()) {
+    err = "compilation produced no code";
+    return false;
+  }
+  return true;
+}
+
+static int load_module(const std::vector<char>& code, size_t n_passes,
+                       std::shared_ptr<JitModule>& out) {
   auto m = std::make_shared<JitModule>();
   SMOQ_HIP(hipModuleLoadData(&m->mod, code.data()));
-  m->fn.resize(pg.passes.size());
-  for (size_t p = 0; p < pg.passes.size(); ++p) {
+  m->fn.resize(n_passes);
+  for (size_t p = 0; p < n_passes; ++p) {
     const std::string name = "smoq_pass_" + std::to_string(p);
     SMOQ_HIP(hipModuleGetFunction(&m->fn[p], m->mod, name.c_str()));
   }
   out = m;
   return SMOQ_OK;
+}
+
+// ------------------------------------------------------- on-disk cache --
+// With SMOQ_JIT_CACHE_DIR set, code objects are also kept in that
+// directory, named by a 128-bit hash of everything that determines them
+// (the generated source, the embedded device headers, the compiler options
+// and hiprtc's version), so a new process skips the compilation of a
+// circuit structure seen before.
+static std::string cache_dir() {
+  const char* d = std::getenv("SMOQ_JIT_CACHE_DIR");
+  return (d && *d) ? std::string(d) : std::string();
+}
+
+static void fnv_mix(uint64_t& h, const void* p, size_t n, uint64_t prime) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= prime;
+  }
+}
+
+static std::string cache_name(const std::string& src) {
+  uint64_t h1 = 0xcbf29ce484222325ull, h2 = 0x84222325cbf29ce4ull;
+  int maj = 0, mnr = 0;
+  (void)hiprtcVersion(&maj, &mnr);
+  auto both = [&](const void* p, size_t n) {
+    fnv_mix(h1, p, n, 0x100000001b3ull);
+    fnv_mix(h2, p, n, 0x00000100000001b3ull ^ 0x9e3779b97f4a7c15ull);
+  };
+  both(&maj, sizeof maj);
+  both(&mnr, sizeof mnr);
+  for (const std::string& o : rtc_options()) both(o.data(), o.size() + 1);
+  both(kSrcDeviceCommon, sizeof kSrcDeviceCommon);
+  both(kSrcFusedTypes, sizeof kSrcFusedTypes);
+  both(kSrcFusedOps, sizeof kSrcFusedOps);
+  both(src.data(), src.size());
+  char name[64];
+  std::snprintf(name, sizeof name, "smoq_pass_%016llx%016llx.co", (unsigned long long)h1,
+                (unsigned long long)h2);
+  return name;
+}
+
+static bool disk_load(const std::string& dir, const std::string& name, std::vector<char>& code) {
+  std::ifstream f(dir + "/" + name, std::ios::binary);
+  if (!f) return false;
+  code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+  return !code.empty();
+}
+
+static void disk_store(const std::string& dir, const std::string& name,
+                       const std::vector<char>& code) {
+  // write then rename, so a reader never sees a partial file
+  const std::string tmp = dir + "/." + name + "." + std::to_string((long long)getpid());
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    if (!f) return;
+    f.write(code.data(), (std::streamsize)code.size());
+    if (!f) return;
+  }
+  (void)std::rename(tmp.c_str(), (dir + "/" + name).c_str());
 }
 
 int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates) {
@@ -396,6 +489,71 @@ bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits) {
   return n_qubits >= kJitMinQubits && prog.gates.size() >= (size_t)kJitMinGates;
 }
 
+int jit_get_many(smoq_ctx* ctx, const std::vector<const Program*>& progs, int dtype,
+                 std::vector<std::shared_ptr<JitModule>>& out) {
+  const size_t np = progs.size();
+  out.assign(np, nullptr);
+  std::vector<std::string> keys(np), srcs(np), names(np);
+  std::vector<std::vector<char>> codes(np);
+  std::vector<size_t> todo;
+  const std::string dir = cache_dir();
+  for (size_t i = 0; i < np; ++i) {
+    jit_key(*progs[i], dtype, keys[i]);
+    auto it = ctx->jit_cache.find(keys[i]);
+    if (it != ctx->jit_cache.end()) {
+      out[i] = it->second;
+      continue;
+    }
+    // the same structure twice in one request: compile it once
+    bool dup = false;
+    for (size_t j : todo) dup |= keys[j] == keys[i];
+    if (dup) continue;
+    srcs[i] = jit_source(*progs[i], dtype);
+    if (!dir.empty()) {
+      names[i] = cache_name(srcs[i]);
+      if (disk_load(dir, names[i], codes[i])) {
+        std::shared_ptr<JitModule> m;
+        if (load_module(codes[i], progs[i]->passes.size(), m) == SMOQ_OK) {
+          ctx->jit_cache.emplace(keys[i], m);
+          out[i] = m;
+          continue;
+        }
+        (void)hipGetLastError();  // a stale or damaged file: compile again
+      }
+    }
+    todo.push_back(i);
+  }
+  // compile what is missing, several modules at once
+  std::vector<std::string> errs(np);
+  std::vector<char> ok(np, 0);
+  if (!todo.empty()) {
+    const size_t hw = std::max<size_t>(1, std::min<size_t>(16, std::thread::hardware_concurrency()));
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+      for (size_t t; (t = next.fetch_add(1)) < todo.size();) {
+        const size_t i = todo[t];
+        ok[i] = rtc_compile(srcs[i], codes[i], errs[i]) ? 1 : 0;
+      }
+    };
+    std::vector<std::thread> th;
+    for (size_t w = 1; w < std::min(hw, todo.size()); ++w) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+  }
+  for (size_t i : todo) {
+    if (!ok[i])
+      return set_error(SMOQ_EDEVICE, "pass kernel compilation failed (%s)", errs[i].c_str());
+    std::shared_ptr<JitModule> m;
+    int rc = load_module(codes[i], progs[i]->passes.size(), m);
+    if (rc) return rc;
+    if (!dir.empty()) disk_store(dir, names[i], codes[i]);
+    ctx->jit_cache.emplace(keys[i], m);
+  }
+  for (size_t i = 0; i < np; ++i)
+    if (!out[i]) out[i] = ctx->jit_cache.at(keys[i]);
+  return SMOQ_OK;
+}
+
 int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out) {
   out.reset();
   std::string& key = ctx->jit_key_buf;
@@ -405,11 +563,10 @@ int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitMo
     out = it->second;
     return SMOQ_OK;
   }
-  std::shared_ptr<JitModule> m;
-  const int rc = compile(jit_source(prog, dtype), prog, m);
+  std::vector<std::shared_ptr<JitModule>> mods;
+  const int rc = jit_get_many(ctx, {&prog}, dtype, mods);
   if (rc) return rc;
-  ctx->jit_cache.emplace(key, m);
-  out = m;
+  out = mods[0];
   return SMOQ_OK;
 }
 

@@ -105,8 +105,9 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
     *out = p;
     return SMOQ_OK;
   }
-  plan_program(gs, n_qubits, pass_tile(ctx, n_qubits, 1, gs.size()), ctx->low_qubits,
-               dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog);
+  const Fusion fu = fusion_for(ctx, n_qubits, dtype, 1, gs);
+  plan_program(gs, n_qubits, fu.tile, fu.low,
+               dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog, fu.cap);
   p->layout = program_layout(p->prog, dtype);
   if (jit_wanted(ctx, p->prog, n_qubits)) {
     Program factored = p->prog;

@@ -294,7 +294,7 @@ static void emit_diag_run(const std::vector<const IGate*>& run, const StageCtx& 
 }
 
 void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int low_qubits,
-                  int row_bits, Program& prog) {
+                  int row_bits, Program& prog, int max_gates) {
   prog.passes.clear();
   prog.stages.clear();
   prog.gates.clear();
@@ -304,12 +304,8 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
   const int m = (k == n) ? k : std::min(low_qubits, k - 2);
   std::vector<int> remaining(gates.size());
   for (size_t i = 0; i < gates.size(); ++i) remaining[i] = (int)i;
-  // optional cap on the non-diagonal gates one pass takes (0: none), read
-  // once from SMOQ_PASS_MAX_GATES: spreads deep passes over more passes
-  static const int max_gates = [] {
-    const char* e = std::getenv("SMOQ_PASS_MAX_GATES");
-    return e ? std::atoi(e) : 0;
-  }();
+  // max_gates: cap on the non-diagonal gates one pass takes (0: none), which
+  // spreads deep passes over more passes
 
   while (!remaining.empty()) {
     uint64_t L = (m >= 64) ? ~0ull : ((1ull << m) - 1);

@@ -112,16 +112,26 @@ __global__ void __launch_bounds__(kRedBlock)
 }
 
 // Second stage: one thread per (item, term), fixed summation order.
-__global__ void k_finish(const double* __restrict__ partial, int nblocks, int nt, double scale,
-                         double* __restrict__ out, int out_stride, const int* __restrict__ out_col,
-                         int n_items) {
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= n_items * nt) return;
-  const int item = id / nt, t = id % nt;
+// One wave per (item, term): lane i adds blocks i, i+64, ... in order, then
+// the fixed-order wave sum.
+constexpr int kFinishBlock = 256;
+__global__ void __launch_bounds__(kFinishBlock)
+    k_finish(const double* __restrict__ partial, int nblocks, int nt, double scale,
+             double* __restrict__ out, int out_stride, const int* __restrict__ out_col,
+             int n_items) {
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wid >= (int64_t)n_items * nt) return;  // whole waves
+  const int item = (int)(wid / nt), t = (int)(wid % nt);
   const double* p = partial + (uint64_t)item * nblocks * nt + t;
   double v = 0.0;
-  for (int b = 0; b < nblocks; ++b) v += p[(uint64_t)b * nt];
-  out[(uint64_t)item * out_stride + out_col[t]] = scale * v;
+  for (int b = lane; b < nblocks; b += 64) v += p[(uint64_t)b * nt];
+  v = wave_total(v);
+  if (lane == 0) out[(uint64_t)item * out_stride + out_col[t]] = scale * v;
+}
+
+static unsigned finish_blocks(int n_outputs) {
+  return (unsigned)(((int64_t)n_outputs * 64 + kFinishBlock - 1) / kFinishBlock);
 }
 
 // ---------------------------------------------------------------------------
@@ -587,9 +597,9 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
                          (const V*)s->d, s->dim, ps, grp_dev, trm_dev, tpartial, tile_terms);
       SMOQ_CHECK_LAUNCH("k_expect_tiles");
     }
-    const int nthreads = B * tile_terms;
-    hipLaunchKernelGGL(k_finish, dim3((nthreads + 255) / 256), dim3(256), 0, ctx->stream,
-                       tpartial, tile_wg, tile_terms, 1.0, out_dev, n_terms, tcols, B);
+    hipLaunchKernelGGL(k_finish, dim3(finish_blocks(B * tile_terms)), dim3(kFinishBlock), 0,
+                       ctx->stream, tpartial, tile_wg, tile_terms, 1.0, out_dev, n_terms, tcols,
+                       B);
     SMOQ_CHECK_LAUNCH("k_finish");
   }
   // the column lists are uploaded per chunk; keep a host mirror
@@ -640,9 +650,8 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
                          s->dim, c.x, h, nt, ts, partial, ref_mode);
     }
     SMOQ_CHECK_LAUNCH("expectation kernel");
-    const int nthreads = B * nt;
-    hipLaunchKernelGGL(k_finish, dim3((nthreads + 255) / 256), dim3(256), 0, ctx->stream, partial,
-                       nb, nt, (c.x && !ref_mode) ? 2.0 : 1.0, out_dev, n_terms,
+    hipLaunchKernelGGL(k_finish, dim3(finish_blocks(B * nt)), dim3(kFinishBlock), 0, ctx->stream,
+                       partial, nb, nt, (c.x && !ref_mode) ? 2.0 : 1.0, out_dev, n_terms,
                        cols + chunk_cols[ci].first, B);
     SMOQ_CHECK_LAUNCH("k_finish");
   }

@@ -356,6 +356,7 @@ int smoq_init(smoq_ctx** out, int device) {
     if (m >= 0 && m <= 2) ctx->jit_mode = m;
   }
   if (const char* o = std::getenv("SMOQ_OBS_TILES")) ctx->obs_tiles = std::atoi(o) != 0;
+  if (const char* g = std::getenv("SMOQ_PASS_MAX_GATES")) ctx->pass_max_gates = std::max(0, std::atoi(g));
   hipError_t se = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (se != hipSuccess) {
     delete ctx;
@@ -609,8 +610,9 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
   gs.clear();
   if ((rc = lower_tape(ops, n_ops, theta, n_theta, s->n, gs))) return rc;
   Program& prog = s->ctx->tape_prog;
-  plan_program(gs, s->n, pass_tile(s->ctx, s->n, s->batch, gs.size()), s->ctx->low_qubits,
-               s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog);
+  const Fusion fu = fusion_for(s->ctx, s->n, s->dtype, s->batch, gs);
+  plan_program(gs, s->n, fu.tile, fu.low,
+               s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog, fu.cap);
   if (n_passes) *n_passes = (int)prog.passes.size();
   if (gs.empty()) return SMOQ_OK;
   if (s->n < kRegQubits) {

@@ -71,8 +71,18 @@ struct Program {
 
 // Builds the pass/stage program for a gate list on an n-qubit state.
 // row_bits: log2 of the amplitudes per LDS swizzle row of the state's dtype.
+// max_gates: at most that many non-diagonal gates per pass (0: no cap).
 void plan_program(const std::vector<IGate>& gates, int n_qubits, int tile_qubits,
-                  int low_qubits, int row_bits, Program& prog);
+                  int low_qubits, int row_bits, Program& prog, int max_gates = 0);
+
+// Fusion settings of a gate list: tile qubits, contiguous low qubits and
+// the per-pass gate cap.
+struct Fusion {
+  int tile = 0, low = 0, cap = 0;
+};
+// 64-bit hash of a lowered gate list's structure (kinds and qubits, not
+// the matrices)
+uint64_t gate_structure_hash(const std::vector<IGate>& gs);
 // LDS index swizzle (the same function the kernels use)
 inline uint32_t swz_host(uint32_t l, int wb) {
   uint32_t f = (l >> wb) ^ (l >> (2 * wb)) ^ (l >> (3 * wb)) ^ (l >> (4 * wb));
@@ -142,6 +152,10 @@ struct smoq_ctx_s {
   static constexpr int kWorkSlots = 4;
   void* work[kWorkSlots] = {};
   size_t work_bytes[kWorkSlots] = {};
+  // per-pass cap on non-diagonal gates (0: none; SMOQ_PASS_MAX_GATES)
+  int pass_max_gates = 0;
+  // autotuned fusion settings by (qubits, dtype, batch, structure hash)
+  std::unordered_map<std::string, smoq::Fusion> tune;
   // states and plans allocated on this context and not yet freed:
   // smoq_fini refuses to release the context while any remain
   int64_t live_objects = 0;
@@ -211,7 +225,15 @@ void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo);
 bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits);
 // tile qubits of the fused passes for a gate list (automatic unless set)
 int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates);
+// the fusion settings for gs: an autotuned entry of the context when there
+// is one, else the context's settings
+Fusion fusion_for(const smoq_ctx* ctx, int n_qubits, int dtype, int batch,
+                  const std::vector<IGate>& gs);
 int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out);
+// the compiled modules of several programs, the missing ones compiled on
+// parallel host threads
+int jit_get_many(smoq_ctx* ctx, const std::vector<const Program*>& progs, int dtype,
+                 std::vector<std::shared_ptr<JitModule>>& out);
 // lowers a tape (angles resolved from theta) to internal gates
 int lower_tape(const smoq_op* ops, int n_ops, const double* theta, int n_theta, int n_qubits,
                std::vector<IGate>& out);

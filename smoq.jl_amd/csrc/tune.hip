@@ -1,0 +1,202 @@
+// Autotuning of the fused-pass geometry.
+//
+// A circuit structure's fused passes depend on three choices: the tile size
+// (2^k amplitudes per LDS tile), the contiguous low qubits every tile keeps
+// (its HBM runs are 2^m amplitudes long) and an optional cap on the
+// non-diagonal gates one pass takes (deep passes are bound by their LDS
+// round trips, so spreading them can pay for an extra pass).  The reference
+// fuses by a fixed greedy rule (fuse_gates,
+// SmoQ.jl_benchmarks/benchmark_random_circuit_sampling/run_benchmark_smoq_Float64.jl:167-200);
+// here smoq_autotune plans the circuit under a set of candidate settings,
+// compiles their pass modules on parallel host threads, times every
+// candidate on the device and keeps the fastest for the structure.  Later
+// applications of a gate list with the same structure (gate kinds and
+// qubits; angles may differ), state size, dtype and batch use it.
+#include <algorithm>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+
+#include "device_common.h"
+#include "smoq_internal.h"
+
+namespace smoq {
+
+uint64_t gate_structure_hash(const std::vector<IGate>& gs) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  auto mix = [&](uint64_t v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (v >> (8 * i)) & 0xff;
+      h *= 0x100000001b3ull;
+    }
+  };
+  mix(gs.size());
+  for (const IGate& g : gs) {
+    mix((uint64_t)(uint32_t)g.kind | ((uint64_t)(uint32_t)g.nd << 32));
+    mix((uint64_t)(uint32_t)g.t0 | ((uint64_t)(uint32_t)g.t1 << 32));
+    mix((uint64_t)(uint32_t)g.dq[0] | ((uint64_t)(uint32_t)g.dq[1] << 32));
+    mix((uint64_t)(uint32_t)g.dq[2] | ((uint64_t)(uint32_t)g.sel << 32));
+    mix(g.ctrl);
+    mix(g.oz);
+  }
+  return h;
+}
+
+static std::string tune_key(int n, int dtype, int batch, uint64_t hash) {
+  char k[96];
+  std::snprintf(k, sizeof k, "gfx950 %d %d %d %016llx", n, dtype, batch,
+                (unsigned long long)hash);
+  return k;
+}
+
+Fusion fusion_for(const smoq_ctx* ctx, int n, int dtype, int batch, const std::vector<IGate>& gs) {
+  Fusion f;
+  f.tile = pass_tile(ctx, n, batch, gs.size());
+  f.low = ctx->low_qubits;
+  f.cap = ctx->pass_max_gates;
+  // explicit settings of the context win over tuned ones
+  if (!ctx->tune.empty() && ctx->tile_qubits == 0) {
+    auto it = ctx->tune.find(tune_key(n, dtype, batch, gate_structure_hash(gs)));
+    if (it != ctx->tune.end()) f = it->second;
+  }
+  return f;
+}
+
+}  // namespace smoq
+
+using namespace smoq;
+
+extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta,
+                             int n_theta, smoq_tune_result* out) {
+  int rc = check_state(s);
+  if (rc) return rc;
+  if (!out) return set_error(SMOQ_EINVAL, "smoq_autotune: null result");
+  std::memset(out, 0, sizeof(*out));
+  smoq_ctx* ctx = s->ctx;
+  std::vector<IGate> gs;
+  if ((rc = lower_tape(ops, n_ops, theta, n_theta, s->n, gs))) return rc;
+  bool fixed = true;
+  for (int i = 0; i < n_ops; ++i) fixed &= ops[i].param < 0;
+  const Fusion dflt{pass_tile(ctx, s->n, s->batch, gs.size()), ctx->low_qubits,
+                    ctx->pass_max_gates};
+  out->tile_qubits = dflt.tile;
+  out->low_qubits = dflt.low;
+  out->pass_max_gates = dflt.cap;
+  if (gs.empty() || s->n < kRegQubits + 2) return SMOQ_OK;  // nothing to choose
+  // candidates: the context's setting first, then tiles of 2^12 and 2^11
+  // (those leaving >= 256 workgroups per pass) with 4 or 5 low qubits and a
+  // few gate caps
+  std::vector<Fusion> cand{dflt};
+  auto add = [&](int k, int m, int c) {
+    if (k > s->n || m > k - 2) return;
+    for (const Fusion& f : cand)
+      if (f.tile == k && f.low == m && f.cap == c) return;
+    cand.push_back({k, m, c});
+  };
+  for (int k : {12, 11}) {
+    if (k < s->n && ((uint64_t)s->batch << (s->n - k)) < 256) continue;
+    if (k == 12) {
+      for (int c : {0, 28, 36, 44}) add(12, 4, c);
+      for (int c : {0, 36}) add(12, 5, c);
+    } else {
+      for (int c : {0, 36}) add(11, 4, c);
+      add(11, 5, 0);
+    }
+  }
+  const int rowbits = s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits;
+  std::vector<Program> progs(cand.size());
+  std::vector<char> use_jit(cand.size(), 0);
+  std::vector<const Program*> to_compile;
+  std::vector<size_t> compile_idx;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    plan_program(gs, s->n, cand[i].tile, cand[i].low, rowbits, progs[i], cand[i].cap);
+    if (jit_wanted(ctx, progs[i], s->n)) {
+      if (fixed) factor_rotations(progs[i], s->dtype);
+      use_jit[i] = 1;
+      to_compile.push_back(&progs[i]);
+      compile_idx.push_back(i);
+    }
+  }
+  std::vector<std::shared_ptr<JitModule>> mods;
+  if (!to_compile.empty() && (rc = jit_get_many(ctx, to_compile, s->dtype, mods))) return rc;
+  std::vector<std::shared_ptr<JitModule>> jit(cand.size());
+  for (size_t t = 0; t < compile_idx.size(); ++t) jit[compile_idx[t]] = mods[t];
+  // time every candidate: one warm application, then the mean of two
+  size_t max_total = 0;
+  for (const Program& p : progs) max_total = std::max(max_total, program_layout(p, s->dtype).total);
+  if ((rc = ensure_prog(ctx, max_total))) return rc;
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&](int code) {
+    ctx->timing = timing;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return code;
+  };
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return cleanup(set_error(SMOQ_EDEVICE, "smoq_autotune: hipEventCreate failed"));
+  if ((rc = smoq_state_set_basis(s, 0))) return cleanup(rc);
+  std::vector<double> ms(cand.size(), 0.0);
+  constexpr int kReps = 2;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    const ProgLayout L = program_layout(progs[i], s->dtype);
+    if ((rc = upload_program(ctx, progs[i], s->dtype, L, ctx->prog))) return cleanup(rc);
+    const JitModule* jm = use_jit[i] ? jit[i].get() : nullptr;
+    if ((rc = launch_program(s, progs[i], L, ctx->prog, jm))) return cleanup(rc);
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess)
+      return cleanup(set_error(SMOQ_EDEVICE, "smoq_autotune: event record failed"));
+    for (int r = 0; r < kReps; ++r)
+      if ((rc = launch_program(s, progs[i], L, ctx->prog, jm))) return cleanup(rc);
+    float el = 0.f;
+    if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&el, e0, e1) != hipSuccess)
+      return cleanup(set_error(SMOQ_EDEVICE, "smoq_autotune: timing failed"));
+    ms[i] = el / kReps;
+  }
+  size_t best = 0;
+  for (size_t i = 1; i < cand.size(); ++i)
+    if (ms[i] < ms[best]) best = i;
+  ctx->tune[tune_key(s->n, s->dtype, s->batch, gate_structure_hash(gs))] = cand[best];
+  out->tile_qubits = cand[best].tile;
+  out->low_qubits = cand[best].low;
+  out->pass_max_gates = cand[best].cap;
+  out->n_passes = (int)progs[best].passes.size();
+  out->ms = ms[best];
+  out->default_ms = ms[0];
+  out->candidates = (int)cand.size();
+  return cleanup(SMOQ_OK);
+}
+
+extern "C" int smoq_tune_save(smoq_ctx* ctx, const char* path) {
+  if (!ctx || !path) return set_error(SMOQ_EINVAL, "smoq_tune_save: null argument");
+  std::ofstream f(path);
+  if (!f) return set_error(SMOQ_EINVAL, "smoq_tune_save: cannot write %s", path);
+  for (const auto& kv : ctx->tune)
+    f << kv.first << ' ' << kv.second.tile << ' ' << kv.second.low << ' ' << kv.second.cap
+      << '\n';
+  return f ? SMOQ_OK : set_error(SMOQ_EINVAL, "smoq_tune_save: write to %s failed", path);
+}
+
+extern "C" int smoq_tune_load(smoq_ctx* ctx, const char* path, int* n_entries) {
+  if (!ctx || !path) return set_error(SMOQ_EINVAL, "smoq_tune_load: null argument");
+  std::ifstream f(path);
+  if (!f) return set_error(SMOQ_EINVAL, "smoq_tune_load: cannot read %s", path);
+  int count = 0;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string arch, hash;
+    int n, dtype, batch;
+    Fusion fu;
+    if (!(is >> arch >> n >> dtype >> batch >> hash >> fu.tile >> fu.low >> fu.cap)) continue;
+    if (arch != "gfx950" || fu.tile < 6 || fu.tile > kMaxTileQubits || fu.low < 3 || fu.low > 6 ||
+        fu.cap < 0)
+      continue;
+    ctx->tune[arch + " " + std::to_string(n) + " " + std::to_string(dtype) + " " +
+              std::to_string(batch) + " " + hash] = fu;
+    ++count;
+  }
+  if (n_entries) *n_entries = count;
+  return SMOQ_OK;
+}

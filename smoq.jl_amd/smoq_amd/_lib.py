@@ -57,6 +57,18 @@ class smoq_op(ctypes.Structure):
     ]
 
 
+class smoq_tune_result(ctypes.Structure):
+    _fields_ = [
+        ("tile_qubits", ctypes.c_int),
+        ("low_qubits", ctypes.c_int),
+        ("pass_max_gates", ctypes.c_int),
+        ("n_passes", ctypes.c_int),
+        ("candidates", ctypes.c_int),
+        ("ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+    ]
+
+
 class smoq_mcwf_group(ctypes.Structure):
     _fields_ = [
         ("psi", ctypes.c_void_p),
@@ -122,6 +134,9 @@ SIGNATURES = {
     "smoq_global_clifford_snapshots": [_P, _i64, _P, _P, _P, _P],
     "smoq_norm2": [_P, _P],
     "smoq_expect_pauli_noalloc": [_P, _i, _P, _P, _P],
+    "smoq_autotune": [_P, _P, _i, _P, _i, _P],
+    "smoq_tune_save": [_P, ctypes.c_char_p],
+    "smoq_tune_load": [_P, ctypes.c_char_p, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
     "smoq_set_jit": [_P, _i],

@@ -39,6 +39,30 @@ class Context:
         call("smoq_pass_timing", self.handle, int(enable), ctypes.byref(ms), ctypes.byref(n))
         return ms.value, n.value
 
+    def autotune(self, tape, psi, theta=None) -> dict:
+        """smoq_autotune: times candidate fusion settings for the tape on psi
+        (overwritten) and keeps the fastest for its structure; returns the
+        chosen settings and timings."""
+        from ._lib import smoq_tune_result
+
+        res = smoq_tune_result()
+        if theta is None:
+            th_p, nth = None, 0
+        else:
+            th = np.ascontiguousarray(theta, np.float64)
+            th_p, nth = th.ctypes.data_as(ctypes.c_void_p), th.size
+        call("smoq_autotune", psi.handle, tape.to_ctypes(), len(tape.ops), th_p, nth,
+             ctypes.byref(res))
+        return {f: getattr(res, f) for f, _ in res._fields_}
+
+    def tune_save(self, path: str) -> None:
+        call("smoq_tune_save", self.handle, str(path).encode())
+
+    def tune_load(self, path: str) -> int:
+        n = ctypes.c_int(0)
+        call("smoq_tune_load", self.handle, str(path).encode(), ctypes.byref(n))
+        return n.value
+
     def close(self) -> None:
         if self.handle:
             call("smoq_fini", self.handle)
