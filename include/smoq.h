@@ -111,6 +111,12 @@ int smoq_set_fusion(smoq_ctx* ctx, int tile_qubits, int low_qubits);
  * fails), 2 always (compilation failure is an error).  The SMOQ_JIT
  * environment variable sets the initial mode of new contexts. */
 int smoq_set_jit(smoq_ctx* ctx, int mode);
+/* Compiled passes may keep a stage's gates on five qubits (three register
+ * slots and lane bits 4 and 5, traded by v_permlane16/32_swap) instead of
+ * three, with fewer LDS round trips for more VALU work: on (1, the default;
+ * SMOQ_LANE_SWAPS=0 in the environment sets new contexts off) or off (0).
+ * A structure tuned by smoq_autotune keeps its tuned choice. */
+int smoq_set_lane_swaps(smoq_ctx* ctx, int on);
 /* Mean kernel time (ms) and launch count of the fused-pass kernel since the
  * last call, measured with HIP events on the context stream when timing is
  * enabled (enable != 0).  Used to report the per-launch roofline. */
@@ -168,7 +174,8 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops,
  * SmoQ.jl_benchmarks/benchmark_random_circuit_sampling/run_benchmark_smoq_Float64.jl:167-200).
  * smoq_autotune plans the tape under candidate settings (tile qubits 11 or
  * 12, 4 or 5 contiguous low qubits, a cap of 0 / 28 / 36 / 44 non-diagonal
- * gates per pass) besides the context's own, compiles their pass kernels,
+ * gates per pass, stages with or without register/lane exchanges) besides
+ * the context's own, compiles their pass kernels,
  * times each candidate on s (its contents are overwritten) and records the
  * fastest for the tape's structure (gate kinds and qubits), qubit count,
  * dtype and batch: later smoq_apply_tape / smoq_plan_create calls with that
@@ -176,6 +183,7 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops,
  * reference counterpart (tooling). */
 typedef struct smoq_tune_result {
   int tile_qubits, low_qubits, pass_max_gates; /* the chosen settings */
+  int lane_swaps;     /* stages trade register and lane qubits (1) or not */
   int n_passes;       /* HBM passes per application with them */
   int candidates;     /* settings timed (0: nothing to choose) */
   double ms;          /* one application with the chosen settings */

@@ -343,7 +343,8 @@ int forward(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta, i
     Program& prog = s->ctx->tape_prog;
     plan_program(gs, s->n, pass_tile(s->ctx, s->n, s->batch, gs.size()), s->ctx->low_qubits,
                  s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog,
-                 s->ctx->pass_max_gates);
+                 s->ctx->pass_max_gates,
+                 s->ctx->lane_swaps && jit_planned(s->ctx, s->n, gs.size()));
     return run_program(s, prog);
   }
   for (const Step& st : steps) {
@@ -472,7 +473,7 @@ int fused_adjoint(smoq_state* s, const std::vector<Step>& steps, const double* t
   Program& prog = ctx->tape_prog;
   const int rowbits = s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits;
   plan_program(rg, nW, pass_tile(ctx, nW, 1, rg.size()), ctx->low_qubits, rowbits, prog,
-               ctx->pass_max_gates);
+               ctx->pass_max_gates, ctx->lane_swaps != 0);
   std::shared_ptr<JitModule> jit;
   if (!prog.passes.empty()) {
     const int jrc = jit_get(ctx, prog, s->dtype, jit);

@@ -161,8 +161,8 @@ int run_program(smoq_state* s, Program& prog, bool fixed_angles) {
     if (fixed_angles) factor_rotations(prog, s->dtype, &undo);
     rc = jit_get(s->ctx, prog, s->dtype, jit);
     // automatic mode keeps the precompiled kernel (unfactored gates) when
-    // compilation fails
-    if (rc && s->ctx->jit_mode >= 2) return rc;
+    // compilation fails, unless the program exchanges lanes
+    if (rc && (s->ctx->jit_mode >= 2 || program_has_swaps(prog))) return rc;
     if (rc) unfactor_rotations(prog, undo);
   }
   if ((rc = upload_program(s->ctx, prog, s->dtype, L, s->ctx->prog))) return rc;

@@ -433,6 +433,52 @@ __device__ __forceinline__ void stage_load_c(const VT<T>* lds, uint32_t l_lane, 
   }
 }
 
+// The stage's mapping after a lane exchange: the swizzled LDS index of the
+// thread's lane part, its global index and the register patterns' swizzled
+// offsets, all fixed when the kernel is compiled.
+__device__ __forceinline__ void stage_remap(uint32_t sl, uint64_t gl, const uint32_t (&sr)[kNR],
+                                            StageFrame& sf) {
+  sf.sl = sl;
+  sf.gl = gl;
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) sf.sr[i] = sr[i];
+}
+
+// v_permlane16_swap (LB = 4) / v_permlane32_swap (LB = 5) on the two dwords
+// of a double: a's lanes with lane bit LB set trade values with b's lanes
+// with it clear (partner lane: lane ^ 2^LB).
+template <int LB>
+__device__ __forceinline__ void permlane_swap(double& a, double& b) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  u32x2 lo, hi;
+  if constexpr (LB == 4) {
+    lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+    hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  } else {
+    lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+    hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  }
+  a = __longlong_as_double((long long)(((unsigned long long)hi.x << 32) | lo.x));
+  b = __longlong_as_double((long long)(((unsigned long long)hi.y << 32) | lo.y));
+}
+
+// Trades the qubit of register slot R with the qubit of lane bit LB: for
+// every pair (j, j | 2^R) the pattern-j amplitudes of lanes with bit LB set
+// swap with the pattern-(j | 2^R) amplitudes of their partner lanes, which
+// is the exchange of the two index bits.
+template <int R, int LB>
+__device__ __forceinline__ void lane_swap(VT<double> (&x)[kNR]) {
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) {
+    if (j & (1 << R)) continue;
+    const int h = j | (1 << R);
+    permlane_swap<LB>(x[j].x, x[h].x);
+    permlane_swap<LB>(x[j].y, x[h].y);
+  }
+}
+
 // registers -> LDS (rounded to T), then a barrier
 template <typename T>
 __device__ __forceinline__ void stage_store(VT<T>* lds, const StageFrame& sf,

@@ -96,6 +96,12 @@ struct StageDev {
   uint32_t sw_tid[kMaxTileQubits - kRegQubits];  // swizzled LDS index of bit p_i
   uint32_t sw_reg[kRegQubits];       // swizzled LDS index of slot bit q_j
   uint64_t g_tid[kMaxTileQubits - kRegQubits];   // global index bit of p_i
+  // local position p_i of thread-index bit i
+  int8_t tpos[kMaxTileQubits - kRegQubits];
+  // Lane exchanges (run-time compiled passes): cont = 1 continues the
+  // previous stage without an LDS round trip, after register slot
+  // swap_slot has traded its qubit with lane bit swap_bit (4 or 5).
+  int8_t cont, swap_slot, swap_bit;
 };
 
 struct PassDev {

@@ -217,30 +217,44 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
     o << "  pass_load<T>(lds, psi, n, pass, f);\n";
     for (int s = pd.s0; s < pd.s1; ++s) {
       const StageDev& st = prog.stages[s];
-      // thread bit i -> the i-th local position outside the register slots
-      // (ascending) -> its global bit
+      const bool seg_end = s + 1 == pd.s1 || !prog.stages[s + 1].cont;
+      // thread bit i -> local position tpos[i] -> its global bit
       std::vector<int> lpos, gpos;
-      for (int b = 0; b < pd.k; ++b) {
-        bool reg = false;
-        for (int r = 0; r < kRegQubits; ++r) reg |= st.q[r] == b;
-        if (!reg) {
-          lpos.push_back(b);
-          gpos.push_back(pd.map[b]);
-        }
+      for (int i = 0; i < st.ntb; ++i) {
+        lpos.push_back(st.tpos[i]);
+        gpos.push_back(pd.map[st.tpos[i]]);
       }
-      o << "  {\n    VT<double> x[kNR];\n    StageFrame sf;\n"
-        << "    const uint32_t tid = threadIdx.x;\n    const uint32_t sr[kNR] = {";
+      if (!st.cont)
+        o << "  {\n    VT<double> x[kNR];\n    StageFrame sf;\n"
+          << "    const uint32_t tid = threadIdx.x;\n";
+      else
+        o << "    lane_swap<" << (int)st.swap_slot << ", " << (int)st.swap_bit << ">(x);\n";
+      o << "    {\n      const uint32_t sr[kNR] = {";
       for (int i = 0; i < (1 << kRegQubits); ++i) {
         uint32_t r = 0;
         for (int j = 0; j < kRegQubits; ++j)
           if ((i >> j) & 1) r ^= st.sw_reg[j];
         o << (i ? ", " : "") << r << "u";
       }
-      o << "};\n    stage_load_c<T>(lds, ";
-      bit_map_emit(o, lpos, false);
-      o << ", f.gbase | ";
-      bit_map_emit(o, gpos, true);
-      o << ", sr, x, sf);\n";
+      if (!st.cont) {
+        o << "};\n      stage_load_c<T>(lds, ";
+        bit_map_emit(o, lpos, false);
+        o << ", f.gbase | ";
+        bit_map_emit(o, gpos, true);
+        o << ", sr, x, sf);\n    }\n";
+      } else {
+        // the swizzled lane part of the new mapping: swz is linear, so the
+        // thread's part is the XOR of its bits' images
+        o << "};\n      stage_remap(";
+        std::vector<uint32_t> sw;
+        for (int i = 0; i < st.ntb; ++i) sw.push_back(st.sw_tid[i]);
+        o << "0u";
+        for (int i = 0; i < st.ntb; ++i)
+          o << " ^ ((tid >> " << i << ") & 1u ? " << sw[i] << "u : 0u)";
+        o << ", f.gbase | ";
+        bit_map_emit(o, gpos, true);
+        o << ", sr, sf);\n    }\n";
+      }
       for (int g = st.g0; g < st.g1; ++g) {
         const GateDev& G = prog.gates[g];
         const GateRef gd{g};
@@ -291,7 +305,7 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
           default: o << "    reg_diaglane<double, " << G.nu << ">(x, " << gd << ", sf.gl);\n"; break;
         }
       }
-      o << "    stage_store<T>(lds, sf, x);\n  }\n";
+      if (seg_end) o << "    stage_store<T>(lds, sf, x);\n  }\n";
     }
     bool scaled = false;
     for (int s = pd.s0; s < pd.s1; ++s)
@@ -471,8 +485,7 @@ static void disk_store(const std::string& dir, const std::string& name,
 
 int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates) {
   if (ctx->tile_qubits > 0) return ctx->tile_qubits;
-  const bool jit = ctx->jit_mode >= 2 || (ctx->jit_mode == 1 && n_qubits >= kJitMinQubits &&
-                                          n_gates >= (size_t)kJitMinGates);
+  const bool jit = jit_planned(ctx, n_qubits, n_gates);
   // the precompiled kernel is fastest on 2^10 tiles; compiled passes go to
   // the largest tile that still leaves >= 2048 workgroups (fewer passes)
   if (!jit) return 10;
@@ -481,7 +494,13 @@ int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates) {
   return 10;
 }
 
+bool jit_planned(const smoq_ctx* ctx, int n_qubits, size_t n_gates) {
+  return ctx->jit_mode >= 2 || (ctx->jit_mode == 1 && n_qubits >= kJitMinQubits &&
+                                n_gates >= (size_t)kJitMinGates);
+}
+
 bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits) {
+  if (program_has_swaps(prog)) return true;  // only compiled passes exchange lanes
   if (ctx->jit_mode == 0 || prog.passes.empty()) return false;
   if (ctx->jit_mode >= 2) return true;
   // automatic: large states with enough gates to amortise ~0.1 s of
@@ -576,5 +595,11 @@ extern "C" int smoq_set_jit(smoq_ctx* ctx, int mode) {
   if (!ctx) return smoq::set_error(SMOQ_EINVAL, "smoq_set_jit: null context");
   if (mode < 0 || mode > 2) return smoq::set_error(SMOQ_EINVAL, "smoq_set_jit: mode %d", mode);
   ctx->jit_mode = mode;
+  return SMOQ_OK;
+}
+
+extern "C" int smoq_set_lane_swaps(smoq_ctx* ctx, int on) {
+  if (!ctx) return smoq::set_error(SMOQ_EINVAL, "smoq_set_lane_swaps: null context");
+  ctx->lane_swaps = on != 0;
   return SMOQ_OK;
 }

@@ -107,14 +107,16 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
   }
   const Fusion fu = fusion_for(ctx, n_qubits, dtype, 1, gs);
   plan_program(gs, n_qubits, fu.tile, fu.low,
-               dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog, fu.cap);
+               dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog, fu.cap,
+               fu.swaps != 0);
   p->layout = program_layout(p->prog, dtype);
   if (jit_wanted(ctx, p->prog, n_qubits)) {
     Program factored = p->prog;
     factor_rotations(factored, dtype);
     rc = jit_get(ctx, factored, dtype, p->jit);
-    // automatic mode keeps the precompiled kernel when compilation fails
-    if (rc && ctx->jit_mode >= 2) {
+    // automatic mode keeps the precompiled kernel when compilation fails,
+    // unless the plan exchanges lanes (compiled passes only)
+    if (rc && (ctx->jit_mode >= 2 || program_has_swaps(p->prog))) {
       delete p;
       return rc;
     }

@@ -293,8 +293,203 @@ static void emit_diag_run(const std::vector<const IGate*>& run, const StageCtx& 
   if (!mixed.empty()) emit_tables(mixed, sc, prog);
 }
 
+// local positions a gate needs in register slots (bit mask)
+static uint32_t need_local(const IGate& g, const int* g2l) {
+  uint32_t need = 0;
+  if (g.kind == IK_U1) need = 1u << g2l[g.t0];
+  if (g.kind == IK_U2) need = (1u << g2l[g.t0]) | (1u << g2l[g.t1]);
+  if (g.kind == IK_OVL) {
+    need = 1u << g2l[g.sel];
+    if (g.t0 >= 0) need |= 1u << g2l[g.t0];
+    if (g.t1 >= 0) need |= 1u << g2l[g.t1];
+  }
+  return need;
+}
+
+bool program_has_swaps(const Program& prog) {
+  for (const StageDev& st : prog.stages)
+    if (st.cont) return true;
+  return false;
+}
+
+namespace {
+// Where a stage keeps the tile's local positions: reg[slot], tb[thread bit].
+struct Mapping {
+  int reg[kRegQubits];
+  int tb[kMaxTileQubits - kRegQubits];
+  int ntb;
+};
+}  // namespace
+
+static StageDev stage_of(const Mapping& M, const PassDev& pd, int row_bits) {
+  StageDev sd;
+  std::memset(&sd, 0, sizeof(sd));
+  for (int r = 0; r < kRegQubits; ++r) {
+    sd.q[r] = M.reg[r];
+    sd.sw_reg[r] = swz_host(1u << M.reg[r], row_bits);
+  }
+  for (int i = 0; i < M.ntb; ++i) {
+    sd.tpos[i] = (int8_t)M.tb[i];
+    sd.sw_tid[i] = swz_host(1u << M.tb[i], row_bits);
+    sd.g_tid[i] = 1ull << pd.map[M.tb[i]];
+  }
+  sd.ntb = M.ntb;
+  return sd;
+}
+
+// Emits the gates of one stage's run under mapping M (slot assignments
+// from it), diagonal runs merged as usual.
+namespace {
+struct GateSink {
+  const std::vector<IGate>& gates;
+  const int* g2l;
+  int n;
+  Program& prog;
+  std::vector<const IGate*> run;
+  int l2slot[kMaxTileQubits];
+  void set_mapping(const Mapping& M, int k) {
+    for (int b = 0; b < k; ++b) l2slot[b] = -1;
+    for (int r = 0; r < kRegQubits; ++r) l2slot[M.reg[r]] = r;
+  }
+  void flush() {
+    if (run.empty()) return;
+    StageCtx sc{g2l, l2slot, n};
+    emit_diag_run(run, sc, prog);
+    run.clear();
+  }
+  void add(const IGate& g) {
+    if (g.kind == IK_DIAG) {
+      run.push_back(&g);
+      return;
+    }
+    flush();
+    StageCtx sc{g2l, l2slot, n};
+    if (g.kind == IK_OVL)
+      emit_ovl(g, sc, prog);
+    else
+      emit_nondiag(g, sc, prog);
+  }
+};
+}  // namespace
+
+// Stages with lane exchanges: a stage (one LDS round trip) takes consecutive
+// gates whose targets span at most five local positions, three in register
+// slots and two on lane bits 4 and 5.  A gate on a lane-bit qubit first
+// swaps it with the register slot whose qubit is needed furthest ahead
+// (v_permlane16/32_swap, no LDS); each swap starts a continuation stage with
+// the new mapping.  Lane bits 0..row_bits-1 get positions of distinct
+// residues mod row_bits where possible, so that a row of the swizzled LDS
+// tile is read without bank conflicts.
+static void form_stages_swapping(const std::vector<IGate>& gates, const std::vector<int>& taken,
+                                 const int* g2l, int n, int k, int row_bits, const PassDev& pd,
+                                 Program& prog) {
+  const int ntb = k - kRegQubits;
+  GateSink sink{gates, g2l, n, prog, {}, {}};
+  size_t i = 0;
+  while (i < taken.size()) {
+    // the segment's positions in order of first need (at most 5)
+    std::vector<int> F;
+    size_t j = i;
+    for (; j < taken.size(); ++j) {
+      const uint32_t need = need_local(gates[taken[j]], g2l);
+      std::vector<int> add;
+      for (int b = 0; b < k; ++b)
+        if ((need >> b & 1) && std::find(F.begin(), F.end(), b) == F.end()) add.push_back(b);
+      if (F.size() + add.size() > (size_t)kRegQubits + 2) break;
+      F.insert(F.end(), add.begin(), add.end());
+    }
+    Mapping M;
+    M.ntb = ntb;
+    std::vector<char> used(k, 0);
+    int nr = 0;
+    for (size_t f = 0; f < F.size() && nr < kRegQubits; ++f) {
+      M.reg[nr++] = F[f];
+      used[F[f]] = 1;
+    }
+    for (int b = 0; b < k && nr < kRegQubits; ++b)
+      if (!used[b]) {
+        M.reg[nr++] = b;
+        used[b] = 1;
+      }
+    for (int t = 0; t < ntb; ++t) M.tb[t] = -1;
+    for (int l = 0; l < 2; ++l) {
+      const size_t f = kRegQubits + l;
+      if (f < F.size()) {
+        M.tb[4 + l] = F[f];
+        used[F[f]] = 1;
+      }
+    }
+    // lane bits 0..row_bits-1: distinct residues mod row_bits first
+    std::vector<char> res_taken(row_bits, 0);
+    for (int t = 0; t < row_bits && t < 4; ++t) {
+      int pick = -1;
+      for (int b = 0; b < k && pick < 0; ++b)
+        if (!used[b] && !res_taken[b % row_bits]) pick = b;
+      for (int b = 0; b < k && pick < 0; ++b)
+        if (!used[b]) pick = b;
+      M.tb[t] = pick;
+      used[pick] = 1;
+      res_taken[pick % row_bits] = 1;
+    }
+    for (int t = 0; t < ntb; ++t) {
+      if (M.tb[t] >= 0) continue;
+      for (int b = 0; b < k; ++b)
+        if (!used[b]) {
+          M.tb[t] = b;
+          used[b] = 1;
+          break;
+        }
+    }
+    sink.set_mapping(M, k);
+    StageDev sd = stage_of(M, pd, row_bits);
+    sd.g0 = (int)prog.gates.size();
+    for (size_t t = i; t < j; ++t) {
+      const IGate& g = gates[taken[t]];
+      const uint32_t need = need_local(g, g2l);
+      for (int b = 0; b < k; ++b) {
+        if (!(need >> b & 1)) continue;
+        bool in_reg = false;
+        for (int r = 0; r < kRegQubits; ++r) in_reg |= M.reg[r] == b;
+        if (in_reg) continue;
+        const int lb = M.tb[4] == b ? 4 : 5;
+        // victim: the slot not needed now whose next need is furthest
+        int victim = -1;
+        size_t best = 0;
+        for (int r = 0; r < kRegQubits; ++r) {
+          if (need >> M.reg[r] & 1) continue;
+          size_t nxt = taken.size() + 1;
+          for (size_t u = t + 1; u < j; ++u)
+            if (need_local(gates[taken[u]], g2l) >> M.reg[r] & 1) {
+              nxt = u;
+              break;
+            }
+          if (victim < 0 || nxt > best) {
+            victim = r;
+            best = nxt;
+          }
+        }
+        sink.flush();
+        sd.g1 = (int)prog.gates.size();
+        prog.stages.push_back(sd);
+        std::swap(M.reg[victim], M.tb[lb]);
+        sink.set_mapping(M, k);
+        sd = stage_of(M, pd, row_bits);
+        sd.cont = 1;
+        sd.swap_slot = (int8_t)victim;
+        sd.swap_bit = (int8_t)lb;
+        sd.g0 = (int)prog.gates.size();
+      }
+      sink.add(g);
+    }
+    sink.flush();
+    sd.g1 = (int)prog.gates.size();
+    prog.stages.push_back(sd);
+    i = j;
+  }
+}
+
 void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int low_qubits,
-                  int row_bits, Program& prog, int max_gates) {
+                  int row_bits, Program& prog, int max_gates, bool lane_swaps) {
   prog.passes.clear();
   prog.stages.clear();
   prog.gates.clear();
@@ -359,6 +554,15 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
     for (int j = 0; j < kRegQubits; ++j) pd.hb[j] = 1ull << pd.map[k - kRegQubits + j];
     pd.s0 = (int)prog.stages.size();
 
+    // register stages with lane exchanges (tiles of at least one wave)
+    if (lane_swaps && k - kRegQubits >= 6) {
+      form_stages_swapping(gates, taken, g2l, n, k, row_bits, pd, prog);
+      pd.s1 = (int)prog.stages.size();
+      prog.passes.push_back(pd);
+      remaining.swap(deferred);
+      continue;
+    }
+
     // register stages: consecutive gates whose targets fit kRegQubits slots
     size_t i = 0;
     while (i < taken.size()) {
@@ -390,6 +594,7 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
           sd.q[slot++] = b;
         } else {
           sd.sw_tid[ti] = swz_host(1u << b, row_bits);
+          sd.tpos[ti] = (int8_t)b;
           sd.g_tid[ti++] = 1ull << pd.map[b];
         }
       }

@@ -357,6 +357,7 @@ int smoq_init(smoq_ctx** out, int device) {
   }
   if (const char* o = std::getenv("SMOQ_OBS_TILES")) ctx->obs_tiles = std::atoi(o) != 0;
   if (const char* g = std::getenv("SMOQ_PASS_MAX_GATES")) ctx->pass_max_gates = std::max(0, std::atoi(g));
+  if (const char* w = std::getenv("SMOQ_LANE_SWAPS")) ctx->lane_swaps = std::atoi(w) != 0;
   hipError_t se = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (se != hipSuccess) {
     delete ctx;
@@ -612,7 +613,8 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
   Program& prog = s->ctx->tape_prog;
   const Fusion fu = fusion_for(s->ctx, s->n, s->dtype, s->batch, gs);
   plan_program(gs, s->n, fu.tile, fu.low,
-               s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog, fu.cap);
+               s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog, fu.cap,
+               fu.swaps != 0);
   if (n_passes) *n_passes = (int)prog.passes.size();
   if (gs.empty()) return SMOQ_OK;
   if (s->n < kRegQubits) {

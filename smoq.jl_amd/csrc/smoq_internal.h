@@ -72,13 +72,19 @@ struct Program {
 // Builds the pass/stage program for a gate list on an n-qubit state.
 // row_bits: log2 of the amplitudes per LDS swizzle row of the state's dtype.
 // max_gates: at most that many non-diagonal gates per pass (0: no cap).
+// lane_swaps: stages may trade register qubits with lane bits 4 and 5 in
+// registers instead of an LDS round trip (run-time compiled passes only).
 void plan_program(const std::vector<IGate>& gates, int n_qubits, int tile_qubits,
-                  int low_qubits, int row_bits, Program& prog, int max_gates = 0);
+                  int low_qubits, int row_bits, Program& prog, int max_gates = 0,
+                  bool lane_swaps = false);
+// whether a program has stages that continue through lane exchanges
+bool program_has_swaps(const Program& prog);
 
 // Fusion settings of a gate list: tile qubits, contiguous low qubits and
 // the per-pass gate cap.
 struct Fusion {
   int tile = 0, low = 0, cap = 0;
+  int swaps = 0;  // lane exchanges in compiled passes
 };
 // 64-bit hash of a lowered gate list's structure (kinds and qubits, not
 // the matrices)
@@ -154,6 +160,9 @@ struct smoq_ctx_s {
   size_t work_bytes[kWorkSlots] = {};
   // per-pass cap on non-diagonal gates (0: none; SMOQ_PASS_MAX_GATES)
   int pass_max_gates = 0;
+  // compiled passes may exchange register and lane qubits instead of LDS
+  // round trips (SMOQ_LANE_SWAPS=0: off)
+  int lane_swaps = 1;
   // autotuned fusion settings by (qubits, dtype, batch, structure hash)
   std::unordered_map<std::string, smoq::Fusion> tune;
   // states and plans allocated on this context and not yet freed:
@@ -223,6 +232,9 @@ struct FactorUndo {
 void factor_rotations(Program& prog, int dtype, std::vector<FactorUndo>* undo = nullptr);
 void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo);
 bool jit_wanted(const smoq_ctx* ctx, const Program& prog, int n_qubits);
+// whether a gate list of this size will run as compiled passes (decided
+// before planning, so that the planner may use lane exchanges)
+bool jit_planned(const smoq_ctx* ctx, int n_qubits, size_t n_gates);
 // tile qubits of the fused passes for a gate list (automatic unless set)
 int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates);
 // the fusion settings for gs: an autotuned entry of the context when there

@@ -1,10 +1,12 @@
 // Autotuning of the fused-pass geometry.
 //
-// A circuit structure's fused passes depend on three choices: the tile size
+// A circuit structure's fused passes depend on four choices: the tile size
 // (2^k amplitudes per LDS tile), the contiguous low qubits every tile keeps
-// (its HBM runs are 2^m amplitudes long) and an optional cap on the
-// non-diagonal gates one pass takes (deep passes are bound by their LDS
-// round trips, so spreading them can pay for an extra pass).  The reference
+// (its HBM runs are 2^m amplitudes long), an optional cap on the
+// non-diagonal gates one pass takes (deep passes are bound by their VALU
+// work and LDS round trips, so spreading them can pay for an extra pass)
+// and whether stages trade register and lane qubits in registers (fewer LDS
+// round trips for more VALU work).  The reference
 // fuses by a fixed greedy rule (fuse_gates,
 // SmoQ.jl_benchmarks/benchmark_random_circuit_sampling/run_benchmark_smoq_Float64.jl:167-200);
 // here smoq_autotune plans the circuit under a set of candidate settings,
@@ -54,6 +56,7 @@ Fusion fusion_for(const smoq_ctx* ctx, int n, int dtype, int batch, const std::v
   f.tile = pass_tile(ctx, n, batch, gs.size());
   f.low = ctx->low_qubits;
   f.cap = ctx->pass_max_gates;
+  f.swaps = ctx->lane_swaps && jit_planned(ctx, n, gs.size());
   // explicit settings of the context win over tuned ones
   if (!ctx->tune.empty() && ctx->tile_qubits == 0) {
     auto it = ctx->tune.find(tune_key(n, dtype, batch, gate_structure_hash(gs)));
@@ -77,30 +80,34 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
   if ((rc = lower_tape(ops, n_ops, theta, n_theta, s->n, gs))) return rc;
   bool fixed = true;
   for (int i = 0; i < n_ops; ++i) fixed &= ops[i].param < 0;
+  const bool compiled = jit_planned(ctx, s->n, gs.size());
   const Fusion dflt{pass_tile(ctx, s->n, s->batch, gs.size()), ctx->low_qubits,
-                    ctx->pass_max_gates};
+                    ctx->pass_max_gates, ctx->lane_swaps && compiled};
   out->tile_qubits = dflt.tile;
   out->low_qubits = dflt.low;
   out->pass_max_gates = dflt.cap;
+  out->lane_swaps = dflt.swaps;
   if (gs.empty() || s->n < kRegQubits + 2) return SMOQ_OK;  // nothing to choose
   // candidates: the context's setting first, then tiles of 2^12 and 2^11
   // (those leaving >= 256 workgroups per pass) with 4 or 5 low qubits and a
   // few gate caps
   std::vector<Fusion> cand{dflt};
-  auto add = [&](int k, int m, int c) {
+  auto add = [&](int k, int m, int c, int w) {
     if (k > s->n || m > k - 2) return;
+    w = w && compiled;  // lane exchanges exist in compiled passes only
     for (const Fusion& f : cand)
-      if (f.tile == k && f.low == m && f.cap == c) return;
-    cand.push_back({k, m, c});
+      if (f.tile == k && f.low == m && f.cap == c && f.swaps == w) return;
+    cand.push_back({k, m, c, w});
   };
   for (int k : {12, 11}) {
     if (k < s->n && ((uint64_t)s->batch << (s->n - k)) < 256) continue;
     if (k == 12) {
-      for (int c : {0, 28, 36, 44}) add(12, 4, c);
-      for (int c : {0, 36}) add(12, 5, c);
+      for (int c : {0, 28, 36, 44}) add(12, 4, c, 1);
+      for (int c : {0, 36}) add(12, 4, c, 0);
+      for (int c : {0, 36}) add(12, 5, c, 1);
     } else {
-      for (int c : {0, 36}) add(11, 4, c);
-      add(11, 5, 0);
+      for (int c : {0, 36}) add(11, 4, c, 1);
+      add(11, 5, 0, 1);
     }
   }
   const int rowbits = s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits;
@@ -109,7 +116,8 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
   std::vector<const Program*> to_compile;
   std::vector<size_t> compile_idx;
   for (size_t i = 0; i < cand.size(); ++i) {
-    plan_program(gs, s->n, cand[i].tile, cand[i].low, rowbits, progs[i], cand[i].cap);
+    plan_program(gs, s->n, cand[i].tile, cand[i].low, rowbits, progs[i], cand[i].cap,
+                 cand[i].swaps != 0);
     if (jit_wanted(ctx, progs[i], s->n)) {
       if (fixed) factor_rotations(progs[i], s->dtype);
       use_jit[i] = 1;
@@ -161,6 +169,7 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
   out->tile_qubits = cand[best].tile;
   out->low_qubits = cand[best].low;
   out->pass_max_gates = cand[best].cap;
+  out->lane_swaps = cand[best].swaps;
   out->n_passes = (int)progs[best].passes.size();
   out->ms = ms[best];
   out->default_ms = ms[0];
@@ -174,7 +183,7 @@ extern "C" int smoq_tune_save(smoq_ctx* ctx, const char* path) {
   if (!f) return set_error(SMOQ_EINVAL, "smoq_tune_save: cannot write %s", path);
   for (const auto& kv : ctx->tune)
     f << kv.first << ' ' << kv.second.tile << ' ' << kv.second.low << ' ' << kv.second.cap
-      << '\n';
+      << ' ' << kv.second.swaps << '\n';
   return f ? SMOQ_OK : set_error(SMOQ_EINVAL, "smoq_tune_save: write to %s failed", path);
 }
 
@@ -189,7 +198,8 @@ extern "C" int smoq_tune_load(smoq_ctx* ctx, const char* path, int* n_entries) {
     std::string arch, hash;
     int n, dtype, batch;
     Fusion fu;
-    if (!(is >> arch >> n >> dtype >> batch >> hash >> fu.tile >> fu.low >> fu.cap)) continue;
+    if (!(is >> arch >> n >> dtype >> batch >> hash >> fu.tile >> fu.low >> fu.cap >> fu.swaps))
+      continue;
     if (arch != "gfx950" || fu.tile < 6 || fu.tile > kMaxTileQubits || fu.low < 3 || fu.low > 6 ||
         fu.cap < 0)
       continue;

@@ -62,6 +62,7 @@ class smoq_tune_result(ctypes.Structure):
         ("tile_qubits", ctypes.c_int),
         ("low_qubits", ctypes.c_int),
         ("pass_max_gates", ctypes.c_int),
+        ("lane_swaps", ctypes.c_int),
         ("n_passes", ctypes.c_int),
         ("candidates", ctypes.c_int),
         ("ms", ctypes.c_double),
@@ -140,6 +141,7 @@ SIGNATURES = {
     "smoq_sample": [_P, _P, _i64, _i, _P],
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
     "smoq_set_jit": [_P, _i],
+    "smoq_set_lane_swaps": [_P, _i],
     "smoq_mcwf_jump": [_P, _i, _d, _P, _P, _P],
     "smoq_mcwf_trotter_evolve": [_P, _P, _i, _d, _P, _i, _P, _P, _i, _i, _P, _P, _P],
 }

@@ -32,6 +32,10 @@ class Context:
         """Run-time compiled fused passes: 0 never, 1 automatic, 2 always."""
         call("smoq_set_jit", self.handle, int(mode))
 
+    def set_lane_swaps(self, on: bool) -> None:
+        """Stages of compiled passes trading register and lane qubits."""
+        call("smoq_set_lane_swaps", self.handle, int(bool(on)))
+
     def pass_timing(self, enable: bool):
         """Returns (mean fused-pass kernel ms, launches) since the last call."""
         ms = ctypes.c_double()

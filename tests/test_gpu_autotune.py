@@ -38,6 +38,7 @@ def test_autotune_choice_keeps_results(ctx, tmp_path):
     assert res["ms"] > 0 and res["ms"] <= res["default_ms"]
     assert res["tile_qubits"] in (11, 12) and res["low_qubits"] in (4, 5)
     assert res["pass_max_gates"] in (0, 28, 36, 44)
+    assert res["lane_swaps"] in (0, 1)
     s.set_basis(0)
     n_pass = tape.apply(s)
     assert n_pass == res["n_passes"]

@@ -28,18 +28,21 @@ def test_shared_circuit_amplitudes_and_samples(ctx, N):
     tape = load_circuit_file(path(N))
     r = uniforms(4200 + N, 100)
     want_psi, want_idx = orc.run_circuit_sampling(N, orc.read_circuit_file(path(N)), r)
-    for jit in (0, 2):
+    # precompiled passes; compiled passes without and with lane exchanges
+    for jit, swaps in ((0, 0), (2, 0), (2, 1)):
         ctx.set_jit(jit)
+        ctx.set_lane_swaps(swaps)
         s = GPUState.basis(N, 0)
         tape.apply(s)
         got = s.to_numpy()
         s.free()
-        assert np.max(np.abs(got - want_psi)) < 1e-12, (N, jit)
+        assert np.max(np.abs(got - want_psi)) < 1e-12, (N, jit, swaps)
         idx, bits = run_circuit_sampling(tape, r)
         np.testing.assert_array_equal(idx, want_idx)
         assert bits.shape == (100, N)
         np.testing.assert_array_equal(bits @ (1 << np.arange(N)), want_idx)
     ctx.set_jit(1)
+    ctx.set_lane_swaps(1)
 
 
 def test_shared_circuit_complex64(ctx):
