@@ -350,6 +350,76 @@ int smoq_dm_apply_dense(smoq_state* rho, const smoq_state* u);
 int smoq_dm_qubit_channel(smoq_state* rho, int n_qubits, const int* qubits, int kind,
                           const double* uniforms, int32_t* out);
 
+/* ---- circuits of gates, noise, measurements and resets ---------------------
+ * smoq_apply_circuit replaces apply_circuit!(ψ, circuit, θ) and
+ * apply_circuit!(ρ, circuit, θ) (utils/cpu/cpuVariationalQuantumCircuitExecutor.jl:95-117,
+ * 252-262) over the reference's CircuitOperation list
+ * (utils/cpu/cpuVariationalQuantumCircuitBuilder.jl:118-132), one record per
+ * operation; the op's qubits are qubits[q_first .. q_first + q_count - 1],
+ * 0-based.
+ * State vector (density_matrix = 0, any batch): every maximal run of gates
+ * is one smoq_apply_tape; noise (apply_noise_mcwf!, :187-204; phase damping
+ * acts as dephasing, thermal relaxation does nothing), measurement
+ * (projective_measurement! with the op's basis, outcomes recorded) and reset
+ * (reset_state!) are smoq_qubit_channel calls.  Each of those ops takes
+ * q_count * batch uniforms from `uniforms`, in op order, laid out like
+ * smoq_qubit_channel's ([k * batch + b]); running out returns SMOQ_EINVAL
+ * with the ops before it applied.  Outcome i (0-based, one per measured
+ * qubit) of item b goes to outcomes[i * batch + b]; if_outcome = k > 0 runs
+ * the op only when outcome k (1-based) is 1, as the reference does, and
+ * needs batch 1.  trace_out and lindblad_step are skipped (the reference
+ * warns and skips).
+ * Density matrix (density_matrix = 1, s = vec(ρ) of 2N qubits): the whole
+ * circuit is one tape of maps on vec(ρ), with the reference's ρ executor:
+ * gates as U ρ U† except Y, S and T, which its apply_gate_rho! leaves out;
+ * per qubit depolarizing (1-p)ρ + (p/3)(XρX + YρY + ZρZ), bit flip
+ * (1-p)ρ + pXρX, dephasing (1-p/2)ρ + (p/2)ZρZ, amplitude damping by the
+ * first-order σ⁻ dissipator (apply_sigma_minus_dissipator_rho!); phase
+ * damping and thermal relaxation nothing; measure and reset zero the
+ * qubit's coherences; no uniforms, no outcomes, if_outcome not consulted;
+ * trace_out is an unknown operation (SMOQ_EINVAL, nothing applied). */
+enum smoq_circuit_kind {
+  SMOQ_CK_RX = 0,
+  SMOQ_CK_RY = 1,
+  SMOQ_CK_RZ = 2,
+  SMOQ_CK_H = 3,
+  SMOQ_CK_X = 4,
+  SMOQ_CK_Y = 5,
+  SMOQ_CK_Z = 6,
+  SMOQ_CK_S = 7,
+  SMOQ_CK_T = 8,
+  SMOQ_CK_CZ = 9,
+  SMOQ_CK_CNOT = 10, /* also :cx; qubits (control, target) */
+  SMOQ_CK_RXX = 11,
+  SMOQ_CK_RYY = 12,
+  SMOQ_CK_RZZ = 13,
+  SMOQ_CK_DEPOLARIZING = 16,
+  SMOQ_CK_AMPLITUDE_DAMPING = 17,
+  SMOQ_CK_DEPHASING = 18,
+  SMOQ_CK_BIT_FLIP = 19,
+  SMOQ_CK_PHASE_DAMPING = 20,
+  SMOQ_CK_THERMAL_RELAXATION = 21,
+  SMOQ_CK_MEASURE = 32,   /* arg: basis 0 z, 1 x, 2 y */
+  SMOQ_CK_RESET = 33,     /* arg: target 0 zero, 1 one, 2 plus, 3 minus */
+  SMOQ_CK_TRACE_OUT = 34,
+  SMOQ_CK_LINDBLAD_STEP = 35
+};
+typedef struct smoq_circuit_op {
+  int32_t kind;
+  int32_t param;       /* 0-based index into theta (the reference's param_idx - 1), -1: angle */
+  int32_t q_first;
+  int32_t q_count;
+  int32_t if_outcome;  /* 0: always */
+  int32_t arg;         /* measurement basis or reset target */
+  double angle;        /* fixed_angle */
+  double p;            /* noise probability */
+} smoq_circuit_op;
+int smoq_apply_circuit(smoq_state* s, int density_matrix, const smoq_circuit_op* ops, int n_ops,
+                       const int32_t* qubits, int n_qubit_entries, const double* theta,
+                       int n_theta, const double* uniforms, int64_t n_uniforms,
+                       int32_t* outcomes, int max_outcomes, int* n_outcomes,
+                       int64_t* n_uniforms_used);
+
 /* ---- Monte Carlo wave-function trajectories (σ⁻ jumps) -------------------
  * One group = the trajectories of one qubit count: `psi` holds them as batch
  * items.  Each step of lindblad_mcwf_batched_step!

@@ -893,3 +893,148 @@ def pauli_matrix(xmask: int, zmask: int, N: int):
             m = Z
         M = np.kron(M, m)
     return M
+
+
+# ------------------------------------------------ circuit executor (VQC) --
+# The op records read here have the fields of the reference's
+# CircuitOperation (type, qubits (1-based), param_idx, fixed_angle, p,
+# target_state, basis, if_outcome), as attributes.
+_EXEC_GATES = {"rx", "ry", "rz", "h", "x", "y", "z", "s", "t", "cz", "cnot", "cx", "rxx", "ryy",
+               "rzz"}
+_EXEC_NOISE = {"depolarizing", "amplitude_damping", "dephasing", "bit_flip", "phase_damping",
+               "thermal_relaxation"}
+_EXEC_MEASURE = {"measure_z", "measure_x", "measure_y", "measure"}
+
+
+def _exec_angle(op, theta):
+    return float(theta[op.param_idx - 1]) if op.param_idx > 0 else float(op.fixed_angle)
+
+
+def measure_basis(psi, k, basis, N, u):
+    """One qubit of projective_measurement!(ψ, qubits, basis::Symbol)
+    (utils/cpu/cpuQuantumStateMeasurements.jl:319-345): :z directly, :x as
+    H, Z-measure, H, :y as S†, H, Z-measure, H, S."""
+    if basis == "z":
+        return measure_z(psi, k, N, u)
+    if basis == "x":
+        apply_named(psi, "h", [k], 0.0, N)
+        o = measure_z(psi, k, N, u)
+        apply_named(psi, "h", [k], 0.0, N)
+        return o
+    if basis == "y":
+        apply_named(psi, "sdg", [k], 0.0, N)
+        apply_named(psi, "h", [k], 0.0, N)
+        o = measure_z(psi, k, N, u)
+        apply_named(psi, "h", [k], 0.0, N)
+        apply_named(psi, "s", [k], 0.0, N)
+        return o
+    raise ValueError(basis)
+
+
+def circuit_psi(psi, ops, theta, N, uniforms):
+    """apply_circuit!(ψ, circuit, θ) (utils/cpu/cpuVariationalQuantumCircuitExecutor.jl:95-117)
+    with apply_operation! (:124-140): an op with if_outcome = k > 0 runs only
+    when outcomes[k] == 1; gates by apply_gate_psi! (:146-181); noise by
+    apply_noise_mcwf! (:187-204: phase_damping as dephasing, thermal
+    relaxation nothing) with one uniform per listed qubit; measure by
+    projective_measurement! with op.basis, outcomes appended; reset by
+    reset_state! (one uniform per qubit); trace_out and lindblad_step do
+    nothing.  `uniforms` is read in op order.  Returns (outcomes, used)."""
+    outcomes, cur = [], 0
+    for op in ops:
+        if op.if_outcome > 0 and (op.if_outcome > len(outcomes)
+                                  or outcomes[op.if_outcome - 1] != 1):
+            continue
+        t, qs = op.type, list(op.qubits)
+        if t in _EXEC_GATES:
+            apply_named(psi, "cnot" if t == "cx" else t, qs, _exec_angle(op, theta), N)
+        elif t in _EXEC_NOISE:
+            kind = {"phase_damping": "dephasing"}.get(t, t)
+            if kind == "thermal_relaxation":
+                continue
+            for q in qs:
+                psi_channel(psi, kind, op.p, q, N, float(uniforms[cur]))
+                cur += 1
+        elif t in _EXEC_MEASURE:
+            for q in qs:
+                outcomes.append(measure_basis(psi, q, op.basis, N, float(uniforms[cur])))
+                cur += 1
+        elif t == "reset":
+            for q in qs:
+                measure_z(psi, q, N, float(uniforms[cur]))
+                force_qubit(psi, q, op.target_state, N)
+                cur += 1
+        elif t in ("trace_out", "lindblad_step"):
+            continue
+        else:
+            raise ValueError(t)
+    return outcomes, cur
+
+
+def gate_unitary(gate, qubits, theta, N):
+    """Dense 2^N matrix of a named gate: apply_named on every basis vector."""
+    dim = 1 << N
+    U = np.zeros((dim, dim), np.complex128)
+    for j in range(dim):
+        e = np.zeros(dim, np.complex128)
+        e[j] = 1.0
+        U[:, j] = apply_named(e, gate, qubits, theta, N)
+    return U
+
+
+def _pauli_on(P, k, N):
+    return dense_1q(np.asarray(P, np.complex128), k - 1, N)
+
+
+def circuit_rho(rho, ops, theta, N):
+    """apply_circuit!(ρ, circuit, θ) (…Executor.jl:252-262), no if_outcome
+    check: gates U ρ U† for the branches of apply_gate_rho! (:290-318; y, s
+    and t have none); noise per qubit (:327-343): depolarizing (1-p)ρ +
+    (p/3)(XρX + (XZ)ρ(XZ)† + ZρZ) (:346-368), bit flip (1-p)ρ + pXρX
+    (:371-375), dephasing (1-p/2)ρ + (p/2)ZρZ (:378-382), amplitude damping
+    by apply_sigma_minus_dissipator_rho! (cpuQuantumChannelGates.jl:833-854);
+    measure and reset zero the qubit's coherences (:387-427); lindblad_step
+    nothing; trace_out is an unknown operation."""
+    X = np.array([[0, 1], [1, 0]], np.complex128)
+    Z = np.array([[1, 0], [0, -1]], np.complex128)
+    idx = np.arange(1 << N)
+    for op in ops:
+        t, qs = op.type, list(op.qubits)
+        if t in _EXEC_GATES:
+            if t in ("y", "s", "t"):
+                continue
+            U = gate_unitary("cnot" if t == "cx" else t, qs, _exec_angle(op, theta), N)
+            rho[:, :] = U @ rho @ U.conj().T
+        elif t in _EXEC_NOISE:
+            for k in qs:
+                p = op.p
+                if t == "depolarizing":
+                    Xk, Zk = _pauli_on(X, k, N), _pauli_on(Z, k, N)
+                    XZ = Xk @ Zk
+                    rho[:, :] = ((1 - p) * rho + (p / 3) * (Xk @ rho @ Xk)
+                                 + (p / 3) * (XZ @ rho @ XZ.conj().T) + (p / 3) * (Zk @ rho @ Zk))
+                elif t == "bit_flip":
+                    Xk = _pauli_on(X, k, N)
+                    rho[:, :] = (1 - p) * rho + p * (Xk @ rho @ Xk)
+                elif t == "dephasing":
+                    Zk = _pauli_on(Z, k, N)
+                    rho[:, :] = (1 - p / 2) * rho + (p / 2) * (Zk @ rho @ Zk)
+                elif t == "amplitude_damping":
+                    m = 1 << (k - 1)
+                    for j in range(1 << N):
+                        for i in range(1 << N):
+                            bi, bj = (i >> (k - 1)) & 1, (j >> (k - 1)) & 1
+                            if bi == 1 and bj == 1:
+                                rho[i ^ m, j ^ m] += p * rho[i, j]
+                                rho[i, j] *= 1 - p
+                            elif bi == 1 or bj == 1:
+                                rho[i, j] *= 1 - 0.5 * p
+        elif t in _EXEC_MEASURE or t == "reset":
+            for k in qs:
+                b = (idx >> (k - 1)) & 1
+                rho[b[:, None] != b[None, :]] = 0
+        elif t == "lindblad_step":
+            continue
+        else:
+            raise ValueError(f"Unknown operation type: {t}")
+    return rho

@@ -57,6 +57,26 @@ class smoq_op(ctypes.Structure):
     ]
 
 
+# smoq_circuit_kind: the reference's CircuitOperation types
+CK = dict(rx=0, ry=1, rz=2, h=3, x=4, y=5, z=6, s=7, t=8, cz=9, cnot=10, cx=10, rxx=11,
+          ryy=12, rzz=13, depolarizing=16, amplitude_damping=17, dephasing=18, bit_flip=19,
+          phase_damping=20, thermal_relaxation=21, measure=32, reset=33, trace_out=34,
+          lindblad_step=35)
+
+
+class smoq_circuit_op(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("param", ctypes.c_int32),
+        ("q_first", ctypes.c_int32),
+        ("q_count", ctypes.c_int32),
+        ("if_outcome", ctypes.c_int32),
+        ("arg", ctypes.c_int32),
+        ("angle", ctypes.c_double),
+        ("p", ctypes.c_double),
+    ]
+
+
 class smoq_tune_result(ctypes.Structure):
     _fields_ = [
         ("tile_qubits", ctypes.c_int),
@@ -136,6 +156,7 @@ SIGNATURES = {
     "smoq_norm2": [_P, _P],
     "smoq_expect_pauli_noalloc": [_P, _i, _P, _P, _P],
     "smoq_autotune": [_P, _P, _i, _P, _i, _P],
+    "smoq_apply_circuit": [_P, _i, _P, _i, _P, _i, _P, _i, _P, ctypes.c_int64, _P, _i, _P, _P],
     "smoq_tune_save": [_P, ctypes.c_char_p],
     "smoq_tune_load": [_P, ctypes.c_char_p, _P],
     "smoq_sample": [_P, _P, _i64, _i, _P],
