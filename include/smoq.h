@@ -499,6 +499,33 @@ int smoq_tape_energy(smoq_state* s, const smoq_op* ops, int n_ops, const double*
                      int n_params, int n_terms, const uint64_t* xmask, const uint64_t* zmask,
                      const double* coeff, int flags, double* energy);
 
+/* Adjoint gradients of many parameter sets in one persistent launch; each
+ * item has its own qubit count, tape, angles and Hamiltonian, so VQE runs of
+ * different sizes (the parameter loop of
+ * scripts_variational_quantum_circuits/demo_variational_quantum_eigensolver.jl:175-289
+ * with calculate_gradients, utils/cpu/cpuVariationalQuantumCircuitGradients.jl:88-163)
+ * share the device.  Per item, from |0…0> of n_qubits: the forward sweep,
+ * W = (ψ, Hψ) interleaved, the reverse sweep with its overlaps, the
+ * per-parameter sums, all by one workgroup that owns the item (the items
+ * are dealt to the workgroups largest first); one launch runs every item.
+ * Outputs as smoq_adjoint_gradient for one item: *energy and grad[p].
+ * n_qubits 3 .. 30; the states live in the context's work buffers. */
+typedef struct smoq_vqe_item {
+  int32_t n_qubits;
+  int32_t n_ops;
+  const smoq_op* ops;
+  int32_t n_params;
+  int32_t n_terms;
+  const double* theta;
+  const uint64_t* xmask;
+  const uint64_t* zmask;
+  const double* coeff;
+  double* energy;
+  double* grad;
+} smoq_vqe_item;
+int smoq_adjoint_gradient_batch(smoq_ctx* ctx, int dtype, const smoq_vqe_item* items,
+                                int n_items);
+
 /* ---- observables --------------------------------------------------------
  * <P_t> for Pauli strings P_t = i^{|x_t & z_t|} X^{x_t} Z^{z_t} (x/z bit
  * masks; Y on qubit q sets bit q in both).  out has batch*n_terms entries

@@ -9,18 +9,11 @@
 #include <algorithm>
 #include <cmath>
 
-#include "device_common.h"
-#include "smoq_internal.h"
+#include "adjoint_common.h"
 
 namespace smoq {
 
 constexpr int kAdjBlock = 256;
-
-struct PauliRot {
-  uint64_t x, z, ctrl;
-  int cnt;  // |x & z| mod 4
-  int h;    // highest bit of x (x != 0)
-};
 
 __device__ inline double2 ipow_c(int cnt) {
   double2 r;
@@ -146,12 +139,6 @@ __global__ void k_rowsum(const double* __restrict__ in, int bpi, int rows, doubl
   out[r] = v;
 }
 
-// A Pauli term of H with its phase folded in: c * i^{|x&z|}.
-struct AdjTerm {
-  uint64_t x, z;
-  double fre, fim;
-};
-
 // Start of the fused reverse sweep: W (n+1 qubits per item) interleaves ψ
 // and λ = Hψ — W[2k] = ψ_k, W[2k+1] = λ_k, so W's qubit 0 selects the
 // vector and its qubit q+1 is ψ's qubit q — and the block partials of
@@ -219,18 +206,7 @@ __global__ void __launch_bounds__(kOvlRedBlock)
   if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
-namespace {
-
-// one step of a sweep: a fixed gate (and its inverse) or a Pauli rotation
-struct Step {
-  bool param = false;
-  int pidx = -1;
-  PauliRot rot{};
-  smoq_op op{};  // the tape op of a parametrised rotation
-  IGate fwd, inv;
-};
-
-IGate inverse_of(const IGate& g) {
+static IGate inverse_of(const IGate& g) {
   IGate r = g;
   if (g.kind == IK_U1 || g.kind == IK_U2) {
     const int d = g.kind == IK_U1 ? 2 : 4;
@@ -293,6 +269,93 @@ int build_steps(const smoq_op* ops, int n_ops, int n_params, int n, std::vector<
   }
   return SMOQ_OK;
 }
+
+IGate shifted_up(const IGate& g) {
+  IGate r = g;
+  if (r.t0 >= 0) r.t0 += 1;
+  if (r.t1 >= 0) r.t1 += 1;
+  for (int i = 0; i < r.nd; ++i) r.dq[i] += 1;
+  r.ctrl <<= 1;
+  r.oz <<= 1;
+  return r;
+}
+
+int adj_terms(int n_terms, const uint64_t* xm, const uint64_t* zm, const double* coeff, int n,
+              std::vector<AdjTerm>& out) {
+  out.clear();
+  if (n_terms < 0 || (n_terms > 0 && (!xm || !zm || !coeff)))
+    return set_error(SMOQ_EINVAL, "adjoint: bad term arrays");
+  const uint64_t dim = 1ull << n;
+  for (int t = 0; t < n_terms; ++t) {
+    if ((xm[t] | zm[t]) & ~(dim - 1))
+      return set_error(SMOQ_EINVAL, "Pauli sum: term %d acts outside the %d qubits", t, n);
+    const int cnt = __builtin_popcountll(xm[t] & zm[t]) & 3;
+    const double re[4] = {1, 0, -1, 0}, im[4] = {0, 1, 0, -1};
+    out.push_back({xm[t], zm[t], coeff[t] * re[cnt], coeff[t] * im[cnt]});
+  }
+  return SMOQ_OK;
+}
+
+int forward_gates(const std::vector<Step>& steps, const double* theta, int n,
+                  std::vector<IGate>& out) {
+  out.clear();
+  for (const Step& st : steps) {
+    if (!st.param) {
+      out.push_back(st.fwd);
+      continue;
+    }
+    const int rc = lower_op(st.op, theta[st.pidx], n, out);
+    if (rc) return rc;
+  }
+  return SMOQ_OK;
+}
+
+int reverse_gates(const std::vector<Step>& steps, const double* theta, int n,
+                  std::vector<IGate>& rg, int* n_slots_out) {
+  rg.clear();
+  int n_slots = 0;
+  for (const Step& st : steps) n_slots += st.param;
+  int slot = 0;
+  std::vector<IGate> tmp;
+  for (int i = (int)steps.size() - 1; i >= 0; --i) {
+    const Step& st = steps[i];
+    if (!st.param) {
+      rg.push_back(shifted_up(st.inv));
+      continue;
+    }
+    IGate o;
+    o.kind = IK_OVL;
+    const uint64_t x = st.rot.x;
+    o.t0 = x ? __builtin_ctzll(x) : -1;
+    o.t1 = __builtin_popcountll(x) == 2 ? 63 - __builtin_clzll(x) : -1;
+    o.ctrl = st.rot.ctrl;
+    o.oz = st.rot.z;
+    o.ocnt = st.rot.cnt;
+    o.oslot = slot++;
+    IGate os = shifted_up(o);
+    os.sel = 0;
+    rg.push_back(os);
+    tmp.clear();
+    const int rc = lower_op(st.op, -theta[st.pidx], n, tmp);
+    if (rc) return rc;
+    for (const IGate& g : tmp) rg.push_back(shifted_up(g));
+  }
+  *n_slots_out = n_slots;
+  return SMOQ_OK;
+}
+
+void grad_from_slots(const std::vector<Step>& steps, const double* hs, int n_slots, int n_params,
+                     double* grad) {
+  for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
+  int fwd = 0;
+  for (const Step& st : steps) {
+    if (!st.param) continue;
+    grad[st.pidx] += hs[n_slots - 1 - fwd];
+    ++fwd;
+  }
+}
+
+namespace {
 
 int blocks_per_item(const smoq_state* s) {
   const uint64_t work = s->dim >> 1;
@@ -406,17 +469,6 @@ int energy_of(smoq_state* s, int n_terms, const uint64_t* x, const uint64_t* z, 
   return SMOQ_OK;
 }
 
-// g with every qubit moved up by one: W's qubit 0 is the ψ/λ selector
-IGate shifted_up(const IGate& g) {
-  IGate r = g;
-  if (r.t0 >= 0) r.t0 += 1;
-  if (r.t1 >= 0) r.t1 += 1;
-  for (int i = 0; i < r.nd; ++i) r.dq[i] += 1;
-  r.ctrl <<= 1;
-  r.oz <<= 1;
-  return r;
-}
-
 // The reverse sweep as fused passes over W (ψ and λ interleaved, one state
 // of n+1 qubits): walking the gates backwards, a parametrised rotation
 // becomes an overlap gate (Im<λ|G|ψ> into its slot) followed by R(-θ), any
@@ -430,46 +482,13 @@ int fused_adjoint(smoq_state* s, const std::vector<Step>& steps, const double* t
   smoq_ctx* ctx = s->ctx;
   const int nW = s->n + 1;
   if (s->batch != 1 || ctx->jit_mode == 0 || nW < 6 || nW > kMaxQubits) return SMOQ_OK;
-  if (n_terms < 0 || (n_terms > 0 && (!xm || !zm || !coeff)))
-    return set_error(SMOQ_EINVAL, "smoq_adjoint_gradient: bad term arrays");
   std::vector<AdjTerm> tv;
-  for (int t = 0; t < n_terms; ++t) {
-    if ((xm[t] | zm[t]) & ~(s->dim - 1))
-      return set_error(SMOQ_EINVAL, "Pauli sum: term %d acts outside the %d qubits", t, s->n);
-    const int cnt = __builtin_popcountll(xm[t] & zm[t]) & 3;
-    const double re[4] = {1, 0, -1, 0}, im[4] = {0, 1, 0, -1};
-    tv.push_back({xm[t], zm[t], coeff[t] * re[cnt], coeff[t] * im[cnt]});
-  }
+  int rc = adj_terms(n_terms, xm, zm, coeff, s->n, tv);
+  if (rc) return rc;
   // the reverse program
   std::vector<IGate>& rg = ctx->tape_gates;
-  rg.clear();
   int n_slots = 0;
-  for (const Step& st : steps) n_slots += st.param;
-  int slot = 0;
-  std::vector<IGate> tmp;
-  for (int i = (int)steps.size() - 1; i >= 0; --i) {
-    const Step& st = steps[i];
-    if (!st.param) {
-      rg.push_back(shifted_up(st.inv));
-      continue;
-    }
-    IGate o;
-    o.kind = IK_OVL;
-    const uint64_t x = st.rot.x;
-    o.t0 = x ? __builtin_ctzll(x) : -1;
-    o.t1 = __builtin_popcountll(x) == 2 ? 63 - __builtin_clzll(x) : -1;
-    o.ctrl = st.rot.ctrl;
-    o.oz = st.rot.z;
-    o.ocnt = st.rot.cnt;
-    o.oslot = slot++;
-    IGate os = shifted_up(o);
-    os.sel = 0;
-    rg.push_back(os);
-    tmp.clear();
-    int rc = lower_op(st.op, -theta[st.pidx], s->n, tmp);
-    if (rc) return rc;
-    for (const IGate& g : tmp) rg.push_back(shifted_up(g));
-  }
+  if ((rc = reverse_gates(steps, theta, s->n, rg, &n_slots))) return rc;
   Program& prog = ctx->tape_prog;
   const int rowbits = s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits;
   plan_program(rg, nW, pass_tile(ctx, nW, 1, rg.size()), ctx->low_qubits, rowbits, prog,
@@ -484,7 +503,7 @@ int fused_adjoint(smoq_state* s, const std::vector<Step>& steps, const double* t
   }
   // buffers: W (2S) | overlap partials, slot sums, energy partials | terms
   void* wbuf = nullptr;
-  int rc = ensure_work(ctx, 1, 2 * s->bytes, &wbuf);
+  rc = ensure_work(ctx, 1, 2 * s->bytes, &wbuf);
   if (rc) return rc;
   const uint64_t ntiles = prog.passes.empty() ? 1 : (1ull << prog.passes[0].n_nonlocal);
   const int bpi = blocks_per_item(s);
@@ -533,13 +552,7 @@ int fused_adjoint(smoq_state* s, const std::vector<Step>& steps, const double* t
   SMOQ_HIP(hipMemcpyAsync(energy, esum, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   SMOQ_HIP(hipStreamSynchronize(ctx->stream));
   // each parameter's slots in gate order (slot numbers run backwards)
-  for (int p = 0; p < n_params; ++p) grad[p] = 0.0;
-  int fwd = 0;
-  for (const Step& st : steps) {
-    if (!st.param) continue;
-    grad[st.pidx] += hs[n_slots - 1 - fwd];
-    ++fwd;
-  }
+  grad_from_slots(steps, hs.data(), n_slots, n_params, grad);
   *done = true;
   return SMOQ_OK;
 }

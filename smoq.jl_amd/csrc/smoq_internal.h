@@ -155,7 +155,7 @@ struct smoq_ctx_s {
   std::vector<std::pair<size_t, void*>> state_pool;
   // grow-only device buffers of the gradient calls (angle tables, λ, the
   // reverse sweep's copy of ψ, partial sums), reused across calls
-  static constexpr int kWorkSlots = 4;
+  static constexpr int kWorkSlots = 6;
   void* work[kWorkSlots] = {};
   size_t work_bytes[kWorkSlots] = {};
   // per-pass cap on non-diagonal gates (0: none; SMOQ_PASS_MAX_GATES)

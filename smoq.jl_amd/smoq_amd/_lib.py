@@ -77,6 +77,22 @@ class smoq_circuit_op(ctypes.Structure):
     ]
 
 
+class smoq_vqe_item(ctypes.Structure):
+    _fields_ = [
+        ("n_qubits", ctypes.c_int32),
+        ("n_ops", ctypes.c_int32),
+        ("ops", ctypes.c_void_p),
+        ("n_params", ctypes.c_int32),
+        ("n_terms", ctypes.c_int32),
+        ("theta", ctypes.c_void_p),
+        ("xmask", ctypes.c_void_p),
+        ("zmask", ctypes.c_void_p),
+        ("coeff", ctypes.c_void_p),
+        ("energy", ctypes.c_void_p),
+        ("grad", ctypes.c_void_p),
+    ]
+
+
 class smoq_tune_result(ctypes.Structure):
     _fields_ = [
         ("tile_qubits", ctypes.c_int),
@@ -156,6 +172,7 @@ SIGNATURES = {
     "smoq_norm2": [_P, _P],
     "smoq_expect_pauli_noalloc": [_P, _i, _P, _P, _P],
     "smoq_autotune": [_P, _P, _i, _P, _i, _P],
+    "smoq_adjoint_gradient_batch": [_P, _i, _P, _i],
     "smoq_apply_circuit": [_P, _i, _P, _i, _P, _i, _P, _i, _P, ctypes.c_int64, _P, _i, _P, _P],
     "smoq_tune_save": [_P, ctypes.c_char_p],
     "smoq_tune_load": [_P, ctypes.c_char_p, _P],

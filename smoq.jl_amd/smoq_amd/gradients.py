@@ -67,6 +67,49 @@ class VQEEnergy:
         self.psi.free()
 
 
+def adjoint_gradient_batch(items, dtype="c128", ctx=None):
+    """Adjoint gradients of several parameter sets, each with its own qubit
+    count, in one persistent launch (smoq_adjoint_gradient_batch).
+
+    items: sequence of (tape, theta, xmask, zmask, coeff) — the ansatz tape
+    (its own n), one parameter vector and H = Σ coeff_t P_t.  Returns
+    (energies[n_items], [grad_i]) with grad_i of length tape_i.n_params."""
+    from ._lib import smoq_vqe_item
+    from .state import _DTYPES, default_context
+
+    ctx = ctx or default_context()
+    items = list(items)
+    arr = (smoq_vqe_item * max(1, len(items)))()
+    keep = []
+    energies = np.zeros(len(items), np.float64)
+    grads = []
+    for i, (tape, theta, xm, zm, c) in enumerate(items):
+        th = np.ascontiguousarray(theta, np.float64).reshape(-1)
+        if th.size != tape.n_params:
+            raise ValueError(f"item {i}: {th.size} angles for {tape.n_params} parameters")
+        xm = np.ascontiguousarray(xm, np.uint64)
+        zm = np.ascontiguousarray(zm, np.uint64)
+        c = np.ascontiguousarray(c, np.float64)
+        g = np.zeros(max(1, tape.n_params), np.float64)
+        ops = tape.to_ctypes()
+        keep += [th, xm, zm, c, g, ops]
+        grads.append(g[:tape.n_params])
+        r = arr[i]
+        r.n_qubits = tape.n
+        r.n_ops = len(tape.ops)
+        r.ops = ctypes.addressof(ops)
+        r.n_params = tape.n_params
+        r.n_terms = c.size
+        r.theta = th.ctypes.data
+        r.xmask = xm.ctypes.data
+        r.zmask = zm.ctypes.data
+        r.coeff = c.ctypes.data
+        r.energy = energies.ctypes.data + 8 * i
+        r.grad = g.ctypes.data
+    call("smoq_adjoint_gradient_batch", ctx.handle, _DTYPES[dtype], arr, len(items))
+    return energies, grads
+
+
 def calculate_gradients_parameter_shift(cost_fn, theta, shift: float = math.pi / 2):
     """calculate_gradients_parameter_shift (…Gradients.jl:138-163)."""
     theta = np.asarray(theta, np.float64)
