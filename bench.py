@@ -194,6 +194,62 @@ def adjoint_vs_parameter_shift(N=24, layers=20, shift_params=4, batch_n=20, batc
     eb, gb = vb.energy_and_grad(thb)
     t_batch = time.perf_counter() - t0
     vb.free()
+    # the same parameter sets through the one-launch batch (one workgroup per
+    # set), and one after another through the single-set fused sweep
+    from smoq_amd.gradients import adjoint_gradient_batch
+
+    def timed_batch(items, route):
+        if route:
+            os.environ["SMOQ_VQE_ROUTE"] = route
+        try:
+            adjoint_gradient_batch(items)
+            ctx.sync()
+            t0 = time.perf_counter()
+            out = adjoint_gradient_batch(items)
+            return time.perf_counter() - t0, out
+        finally:
+            os.environ.pop("SMOQ_VQE_ROUTE", None)
+
+    items = [(tb, thb[i], xb, zb, cb) for i in range(batch)]
+    t_one, (e1, g1) = timed_batch(items, "one_launch")
+    t_auto, (e2, g2) = timed_batch(items, None)
+    d_one = max(float(np.max(np.abs(g1[i] - gb[i]))) for i in range(batch))
+    d_auto = max(float(np.max(np.abs(g2[i] - gb[i]))) for i in range(batch))
+    vs = VQEEnergy(tb, xb, zb, cb)
+    vs.energy_and_grad(thb[0])
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(batch):
+        vs.energy_and_grad(thb[i])
+    t_seq = time.perf_counter() - t0
+    vs.free()
+    # uneven qubit counts: the same ansatz depth at N = 12, 14, 16, 18, 20
+    sizes = [12, 14, 16, 18, 20] * (batch // 5)
+    uneven = []
+    for i, n in enumerate(sizes):
+        tu = hardware_efficient_ansatz(n, layers)
+        xu, zu, cu = xxz_terms(n, 0.0, 0.0, 1.0, 1.0)
+        uneven.append((tu, 2 * np.pi * streams.uniforms(seed + 100 + i, tu.n_params), xu, zu, cu))
+    t_uneven, _ = timed_batch(uneven, "one_launch")
+    t_uneven_auto, _ = timed_batch(uneven, None)
+    # many small parameter sets (the VQE demo's sizes): N = 6 .. 12, 8 layers
+    small = []
+    for i in range(256):
+        n = 6 + i % 7
+        tu = hardware_efficient_ansatz(n, 8)
+        xu, zu, cu = xxz_terms(n, 0.0, 0.0, 1.0, 1.0)
+        small.append((tu, 2 * np.pi * streams.uniforms(seed + 1000 + i, tu.n_params), xu, zu, cu))
+    t_small, _ = timed_batch(small, None)
+    vq = [VQEEnergy(tu, xu, zu, cu) for (tu, _, xu, zu, cu) in small[:7]]
+    for v_, (tu, th_, *_r) in zip(vq, small[:7]):
+        v_.energy_and_grad(th_)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i, (tu, th_, *_r) in enumerate(small):
+        vq[i % 7].energy_and_grad(th_)
+    t_small_seq = time.perf_counter() - t0
+    for v_ in vq:
+        v_.free()
     return {
         "workload": f"TFIM energy gradient, N={N}, {layers} layers, {P} parameters, c128",
         "adjoint_s_per_gradient": t_adj,
@@ -206,7 +262,24 @@ def adjoint_vs_parameter_shift(N=24, layers=20, shift_params=4, batch_n=20, batc
         "energy": float(e[0]),
         "batched": {"n_qubits": batch_n, "batch": batch, "params_per_set": tb.n_params,
                     "s_per_batched_gradient": t_batch,
-                    "gradients_per_s": batch / t_batch},
+                    "gradients_per_s": batch / t_batch,
+                    "kernels": "per-gate sweep, one launch per gate for the whole batch"},
+        "batched_one_launch": {"n_qubits": batch_n, "batch": batch, "s": t_one,
+                               "gradients_per_s": batch / t_one,
+                               "max_abs_diff_vs_per_gate_batch": d_one,
+                               "auto_s": t_auto, "auto_gradients_per_s": batch / t_auto,
+                               "auto_max_abs_diff_vs_per_gate_batch": d_auto,
+                               "sequential_fused_s": t_seq,
+                               "sequential_fused_gradients_per_s": batch / t_seq},
+        "uneven_one_launch": {"n_qubits": sorted(set(sizes)), "items": len(sizes),
+                              "layers": layers, "s": t_uneven,
+                              "gradients_per_s": len(sizes) / t_uneven,
+                              "auto_s": t_uneven_auto,
+                              "auto_gradients_per_s": len(sizes) / t_uneven_auto},
+        "small_sets": {"n_qubits": list(range(6, 13)), "items": len(small), "layers": 8,
+                       "s": t_small, "gradients_per_s": len(small) / t_small,
+                       "sequential_fused_s": t_small_seq,
+                       "sequential_fused_gradients_per_s": len(small) / t_small_seq},
     }
 
 
@@ -733,6 +806,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--probe", choices=["traffic"], default=None)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--no-mcwf", action="store_true")
     ap.add_argument("--mcwf-only", action="store_true")
     ap.add_argument("--no-reference", action="store_true")
@@ -771,6 +845,9 @@ def main():
         return
     if args.trotter_only:
         print(json.dumps({"trotter_xxz": trotter_xxz()}), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_vs_parameter_shift()}), flush=True)
         return
     if args.analytics_only:
         print(json.dumps({"partial_trace": reduced_states(),
@@ -924,7 +1001,13 @@ def main():
         result.setdefault("extras", {})["adjoint"] = adj
         log(f"adjoint N=24: {adj['adjoint_s_per_gradient'] * 1e3:.1f} ms/gradient, "
             f"{adj['speedup_vs_param_shift']:.0f}x parameter shift; batch 64 x N=20: "
-            f"{adj['batched']['gradients_per_s']:.0f} gradients/s")
+            f"{adj['batched']['gradients_per_s']:.0f} gradients/s per-gate, "
+            f"{adj['batched_one_launch']['gradients_per_s']:.0f}/s one launch, "
+            f"{adj['batched_one_launch']['auto_gradients_per_s']:.0f}/s routed "
+            f"(sequential fused {adj['batched_one_launch']['sequential_fused_gradients_per_s']:.0f}/s), "
+            f"uneven N=12..20: {adj['uneven_one_launch']['auto_gradients_per_s']:.0f}/s routed; "
+            f"256 sets N=6..12: {adj['small_sets']['gradients_per_s']:.0f}/s "
+            f"(sequential {adj['small_sets']['sequential_fused_gradients_per_s']:.0f}/s)")
         if not args.no_checks:
             adj["full_check"] = adjoint_full_check()
     if not args.no_trotter and not args.no_extras:
@@ -971,6 +1054,8 @@ def summary(result):
         out["adjoint_ms"] = a["adjoint_s_per_gradient"] * 1e3
         out["adjoint_speedup_vs_ps"] = a["speedup_vs_param_shift"]
         out["adjoint_batch_grads_per_s"] = a["batched"]["gradients_per_s"]
+        out["adjoint_batch_one_launch_grads_per_s"] = a["batched_one_launch"]["gradients_per_s"]
+        out["adjoint_uneven_one_launch_grads_per_s"] = a["uneven_one_launch"]["gradients_per_s"]
         if "full_check" in a:
             out["adjoint_max_diff_ps_all"] = a["full_check"]["max_abs_diff_vs_param_shift"]
             out["adjoint_max_diff_fd"] = a["full_check"]["max_abs_diff_vs_central_fd"]

@@ -383,6 +383,11 @@ int smoq_fini(smoq_ctx* ctx) {
   for (int i = 0; i < smoq_ctx::kWorkSlots; ++i)
     if (ctx->work[i]) (void)hipFree(ctx->work[i]);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  if (ctx->aux_stream) {
+    (void)hipStreamSynchronize(ctx->aux_stream);
+    (void)hipStreamDestroy(ctx->aux_stream);
+  }
+  if (ctx->res_pinned) (void)hipHostFree(ctx->res_pinned);
   blas_release(ctx);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);

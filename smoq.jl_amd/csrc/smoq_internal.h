@@ -133,6 +133,11 @@ struct smoq_ctx_s {
   size_t prog_bytes = 0;
   void* host_pinned = nullptr;
   size_t host_pinned_bytes = 0;
+  // second stream (default priority) and pinned result buffer of the batched
+  // gradients, whose one-launch part runs beside whole-device launches
+  hipStream_t aux_stream = nullptr;
+  void* res_pinned = nullptr;
+  size_t res_pinned_bytes = 0;
   // fused-pass timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;   // begin/end pairs recorded on the stream

@@ -17,6 +17,8 @@
 // idle threads only join the barriers).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <thread>
 
 #include "adjoint_common.h"
@@ -316,13 +318,18 @@ struct ItemPlan {
 
 size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Enqueues the one-launch part on stream st: plans, uploads (waiting for the
+// upload, so the context's pinned staging buffer is free again on return),
+// launches, and queues the copies of every item's sums into the context's
+// pinned result buffer; finish_batch waits for them and fills the outputs.
 template <typename T>
-int run_batch(smoq_ctx* ctx, int dtype, const smoq_vqe_item* items, int n_items) {
+int run_batch(smoq_ctx* ctx, int dtype, const smoq_vqe_item* items, int n_items, hipStream_t st,
+              std::vector<ItemPlan>& plans) {
   using V = VT<T>;
   const int rowbits = CT<T>::kRowBits;
   const int tile_req = ctx->tile_qubits > 0 ? std::min(ctx->tile_qubits, kMaxTileQubits) : kVqeTile;
   // plan every item (host threads over the items)
-  std::vector<ItemPlan> plans(n_items);
+  plans.assign(n_items, ItemPlan());
   auto plan_one = [&](int i) {
     const smoq_vqe_item& it = items[i];
     ItemPlan& P = plans[i];
@@ -451,7 +458,9 @@ int run_batch(smoq_ctx* ctx, int dtype, const smoq_vqe_item* items, int n_items)
   if (!rc) rc = ensure_work(ctx, 5, tb, &tbuf);
   if (!rc) rc = ensure_pinned(ctx, tb);
   if (rc) return rc;
-  SMOQ_HIP(hipStreamSynchronize(ctx->stream));  // the pinned buffer may feed an earlier copy
+  // the pinned buffer may feed an earlier copy on either stream
+  SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+  SMOQ_HIP(hipStreamSynchronize(st));
   unsigned char* sd = (unsigned char*)sbuf;
   unsigned char* td = (unsigned char*)tbuf;
   unsigned char* h = (unsigned char*)ctx->host_pinned;
@@ -491,34 +500,100 @@ int run_batch(smoq_ctx* ctx, int dtype, const smoq_vqe_item* items, int n_items)
   std::memcpy(h + off_items, dev.data(), sizeof(VqeItemDev) * n_items);
   std::memcpy(h + off_first, first.data(), sizeof(int) * first.size());
   std::memcpy(h + off_list, list.data(), sizeof(int) * list.size());
-  SMOQ_HIP(hipMemcpyAsync(td, h, tb, hipMemcpyHostToDevice, ctx->stream));
+  SMOQ_HIP(hipMemcpyAsync(td, h, tb, hipMemcpyHostToDevice, st));
+  SMOQ_HIP(hipStreamSynchronize(st));
   // the table area is passed twice: read-only (programs, terms, item
   // records, lists: never written by the kernel, so scalar loads may read
   // them) and writable (the partials and sums, in other byte ranges)
-  hipLaunchKernelGGL(k_vqe_batch<T>, dim3((unsigned)grid), dim3(block), shm, ctx->stream,
+  hipLaunchKernelGGL(k_vqe_batch<T>, dim3((unsigned)grid), dim3(block), shm, st,
                      (const unsigned char*)td, td, sd, (const VqeItemDev*)(td + off_items),
                      (const int*)(td + off_first), (const int*)(td + off_list), tile_elems);
   SMOQ_CHECK_LAUNCH("k_vqe_batch");
-  // every item's sums
+  // every item's sums, into the pinned result buffer
   size_t res_n = 0;
   for (const ItemPlan& P : plans) res_n += P.n_slots + 1;
-  std::vector<double> res(res_n);
+  if (res_n * sizeof(double) > ctx->res_pinned_bytes) {
+    if (ctx->res_pinned) SMOQ_HIP(hipHostFree(ctx->res_pinned));
+    ctx->res_pinned = nullptr;
+    ctx->res_pinned_bytes = 0;
+    SMOQ_HIP(hipHostMalloc(&ctx->res_pinned, res_n * sizeof(double), hipHostMallocDefault));
+    ctx->res_pinned_bytes = res_n * sizeof(double);
+  }
+  double* res = (double*)ctx->res_pinned;
   size_t r = 0;
   for (int i = 0; i < n_items; ++i) {
     const size_t cnt = plans[i].n_slots + 1;
-    SMOQ_HIP(hipMemcpyAsync(res.data() + r, td + off_s[i], cnt * sizeof(double),
-                            hipMemcpyDeviceToHost, ctx->stream));
+    SMOQ_HIP(hipMemcpyAsync(res + r, td + off_s[i], cnt * sizeof(double), hipMemcpyDeviceToHost,
+                            st));
     r += cnt;
   }
-  SMOQ_HIP(hipStreamSynchronize(ctx->stream));
-  r = 0;
+  return SMOQ_OK;
+}
+
+int finish_batch(hipStream_t st, const double* res, const smoq_vqe_item* items, int n_items,
+                 const std::vector<ItemPlan>& plans) {
+  SMOQ_HIP(hipStreamSynchronize(st));
+  size_t r = 0;
   for (int i = 0; i < n_items; ++i) {
     const ItemPlan& P = plans[i];
-    grad_from_slots(P.steps, res.data() + r, P.n_slots, items[i].n_params, items[i].grad);
+    grad_from_slots(P.steps, res + r, P.n_slots, items[i].n_params, items[i].grad);
     *items[i].energy = res[r + P.n_slots];
     r += P.n_slots + 1;
   }
   return SMOQ_OK;
+}
+
+// Which items take the one launch and which the whole-device sweep one after
+// another: a workgroup moves an item's 2^n amplitudes through its passes
+// alone (about 8.6e-10 s per amplitude and tape op, measured at n = 20),
+// while the whole-device sweep costs about 1.8e-12 s per amplitude and op
+// with a floor of about 6e-7 s per op (launches).  The two run side by side
+// on two streams, so the most expensive items move to the sweep, as many as
+// make the longer of the two shortest.  SMOQ_VQE_ROUTE=one_launch keeps
+// every item in the launch, =sweep sends every item to the sweep.
+std::vector<char> route_items(const smoq_ctx* ctx, const smoq_vqe_item* items, int n_items) {
+  std::vector<char> big(n_items, 0);
+  if (const char* e = std::getenv("SMOQ_VQE_ROUTE")) {
+    if (!std::strcmp(e, "one_launch")) return big;
+    if (!std::strcmp(e, "sweep")) return std::vector<char>(n_items, 1);
+  }
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+      hipSuccess || n_cu <= 0)
+    n_cu = 256;
+  std::vector<double> tp(n_items), ts(n_items);
+  std::vector<int> order(n_items);
+  for (int i = 0; i < n_items; ++i) {
+    const double amps = std::ldexp(1.0, items[i].n_qubits);
+    const double ops = std::max(1, items[i].n_ops);
+    tp[i] = 8.6e-10 * amps * ops;
+    ts[i] = std::max(6e-7 * ops, 1.8e-12 * amps * ops);
+    order[i] = i;
+  }
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return tp[a] > tp[b]; });
+  double sum_p = 0.0, sum_s = 0.0;
+  for (int i = 0; i < n_items; ++i) sum_p += tp[i];
+  auto launch_time = [&](size_t from) {
+    // the remaining items (order[from..]) on up to n_cu workgroups
+    if (from >= order.size()) return 0.0;
+    const double per = sum_p / std::min<double>((double)n_cu, (double)(order.size() - from));
+    return std::max(tp[order[from]], per);
+  };
+  // move the k most expensive items, for the k that minimises the longer of
+  // the two streams' times
+  size_t best_k = 0;
+  double best = launch_time(0);
+  for (size_t k = 1; k <= order.size(); ++k) {
+    sum_p -= tp[order[k - 1]];
+    sum_s += ts[order[k - 1]];
+    const double t = std::max(launch_time(k), sum_s);
+    if (t < best) {
+      best = t;
+      best_k = k;
+    }
+  }
+  for (size_t k = 0; k < best_k; ++k) big[order[k]] = 1;
+  return big;
 }
 
 }  // namespace
@@ -542,6 +617,42 @@ extern "C" int smoq_adjoint_gradient_batch(smoq_ctx* ctx, int dtype, const smoq_
     if (it.n_params < 0 || (it.n_params > 0 && (!it.theta || !it.grad)) || !it.energy)
       return set_error(SMOQ_EINVAL, "smoq_adjoint_gradient_batch: item %d: bad arrays", i);
   }
-  return dtype == SMOQ_C128 ? run_batch<double>(ctx, dtype, items, n_items)
-                            : run_batch<float>(ctx, dtype, items, n_items);
+  const std::vector<char> big = route_items(ctx, items, n_items);
+  std::vector<smoq_vqe_item> small;
+  for (int i = 0; i < n_items; ++i)
+    if (!big[i]) small.push_back(items[i]);
+  std::vector<ItemPlan> plans;
+  int rc = SMOQ_OK;
+  if (!small.empty()) {
+    if (!ctx->aux_stream)
+      SMOQ_HIP(hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+    rc = dtype == SMOQ_C128
+             ? run_batch<double>(ctx, dtype, small.data(), (int)small.size(), ctx->aux_stream, plans)
+             : run_batch<float>(ctx, dtype, small.data(), (int)small.size(), ctx->aux_stream, plans);
+    if (rc) {
+      (void)hipStreamSynchronize(ctx->aux_stream);
+      return rc;
+    }
+  }
+  // the large items, each through the single-set sweep on the context's
+  // stream, while the launch runs
+  for (int i = 0; i < n_items && !rc; ++i) {
+    if (!big[i]) continue;
+    const smoq_vqe_item& it = items[i];
+    smoq_state* s = nullptr;
+    rc = smoq_state_alloc(ctx, it.n_qubits, dtype, 1, &s);
+    if (!rc)
+      rc = smoq_adjoint_gradient(s, it.ops, it.n_ops, it.theta, it.n_params, it.n_terms, it.xmask,
+                                 it.zmask, it.coeff, it.energy, it.grad);
+    if (s) {
+      const int frc = smoq_state_free(s);
+      if (!rc) rc = frc;
+    }
+  }
+  if (!small.empty()) {
+    const int frc = finish_batch(ctx->aux_stream, (const double*)ctx->res_pinned, small.data(),
+                                 (int)small.size(), plans);
+    if (!rc) rc = frc;
+  }
+  return rc;
 }

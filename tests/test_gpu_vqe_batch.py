@@ -4,6 +4,7 @@ oracle's parameter-shift gradient and energy (≤1e-10 / 1e-12), against the
 single-state adjoint (smoq_adjoint_gradient) and complex64 against
 complex128 (≤1e-5)."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -78,22 +79,29 @@ def test_uneven_sizes_match_oracle_parameter_shift(ctx):
         assert np.max(np.abs(g - oracle_param_shift(tape, th, N))) < 1e-10
 
 
-@pytest.mark.parametrize("tile", [0, 12])
-def test_batch_matches_single_state_adjoint(ctx, tile):
+@pytest.mark.parametrize("tile,route", [(0, "one_launch"), (12, "one_launch"), (0, "auto"),
+                                        (0, "sweep")])
+def test_batch_matches_single_state_adjoint(ctx, tile, route):
+    """One launch for every item (at the default tile and at 2^12), the
+    automatic split between the launch and the whole-device sweep, and the
+    sweep alone, against the single-state adjoint."""
     from smoq_amd.gradients import VQEEnergy, adjoint_gradient_batch
 
-    sizes = [10, 13, 11, 16, 12, 10, 14]
+    sizes = [10, 13, 11, 16, 12, 10, 14, 18]
     items = []
     for i, N in enumerate(sizes):
         tape = mixed_tape(N, i)
         th = np.random.default_rng(100 + i).uniform(0, 2 * math.pi, tape.n_params)
         items.append((tape, th) + tuple(tfim_terms(N)))
-    print(f"[gpu] vqe batch vs single sizes={sizes} tile={tile}", flush=True)
+    print(f"[gpu] vqe batch vs single sizes={sizes} tile={tile} route={route}", flush=True)
     ctx.set_fusion(tile if tile else -1, 0)
+    if route != "auto":
+        os.environ["SMOQ_VQE_ROUTE"] = route
     try:
         E, G = adjoint_gradient_batch(items)
     finally:
         ctx.set_fusion(-1, 0)
+        os.environ.pop("SMOQ_VQE_ROUTE", None)
     for (tape, th, xs, zs, cs), e, g in zip(items, E, G):
         v = VQEEnergy(tape, xs, zs, cs)
         e1, g1 = v.energy_and_grad(th)
