@@ -205,6 +205,14 @@ __device__ __forceinline__ void reg_diagreg(VT<T> (&x)[kNR], const GateDev& g) {
   for (int j = 0; j < kNR; ++j) x[j] = cmul(ld_c<T>(g.m, j), x[j]);
 }
 
+// register diagonal with pattern 0 normalised to 1: patterns in MASK only
+template <typename T, uint32_t MASK>
+__device__ __forceinline__ void reg_diagreg_m(VT<T> (&x)[kNR], const GateDev& g) {
+#pragma unroll
+  for (int j = 0; j < kNR; ++j)
+    if ((MASK >> j) & 1u) x[j] = cmul(ld_c<T>(g.m, j), x[j]);
+}
+
 // diagonal on non-register qubits only: one phase per lane (NU as above)
 template <typename T, int NU = -1>
 __device__ __forceinline__ void reg_diaglane(VT<T> (&x)[kNR], const GateDev& g, uint64_t gl) {
@@ -317,6 +325,26 @@ struct PassFrame {
   int nt;            // threads per tile
 };
 
+// Tile loads and stores of a pass touch every amplitude once: nontemporal
+// (streaming) accesses keep them from displacing cached lines.
+template <typename T>
+__device__ __forceinline__ VT<T> ld_stream(const VT<T>* p) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  VT<T> r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(VT<T>* p, VT<T> a) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 v;
+  v.x = a.x;
+  v.y = a.y;
+  __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
+}
+
 // HBM -> LDS for the workgroup's tile; ends with a barrier
 template <typename T>
 __device__ __forceinline__ void pass_load(VT<T>* lds, VT<T>* __restrict__ psi, int n,
@@ -345,7 +373,7 @@ __device__ __forceinline__ void pass_load(VT<T>* lds, VT<T>* __restrict__ psi, i
   f.st_lane = swz<WB>((uint32_t)threadIdx.x);
   VT<T> x[kNR];
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) x[i] = f.pt[f.off[i]];
+  for (int i = 0; i < kNR; ++i) x[i] = ld_stream<T>(f.pt + f.off[i]);
 #pragma unroll
   for (int i = 0; i < kNR; ++i) lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))] = x[i];
   __syncthreads();
@@ -359,7 +387,7 @@ __device__ __forceinline__ void pass_store(const VT<T>* lds, const PassFrame<T>&
 #pragma unroll
   for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))];
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) f.pt[f.off[i]] = x[i];
+  for (int i = 0; i < kNR; ++i) st_stream<T>(f.pt + f.off[i], x[i]);
 }
 
 // LDS -> HBM with the pass's factored-out scalar applied
@@ -375,7 +403,25 @@ __device__ __forceinline__ void pass_store_scaled(const VT<T>* lds, const PassFr
   for (int i = 0; i < kNR; ++i) {
     x[i].x *= sc;
     x[i].y *= sc;
-    f.pt[f.off[i]] = x[i];
+    st_stream<T>(f.pt + f.off[i], x[i]);
+  }
+}
+
+// LDS -> HBM with the pass's complex factor (rotation scalars and the
+// phases taken out of its register diagonals) applied
+template <typename T>
+__device__ __forceinline__ void pass_store_cscaled(const VT<T>* lds, const PassFrame<T>& f,
+                                                   double sre, double sim) {
+  constexpr int WB = CT<T>::kRowBits;
+  VT<T> x[kNR];
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))];
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) {
+    VT<double> v;
+    v.x = (double)x[i].x * sre - (double)x[i].y * sim;
+    v.y = (double)x[i].x * sim + (double)x[i].y * sre;
+    st_stream<T>(f.pt + f.off[i], make_c<T>(v.x, v.y));
   }
 }
 

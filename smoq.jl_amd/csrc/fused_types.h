@@ -53,6 +53,11 @@ enum GKind : int32_t {
   // G = i^ocnt X^x Z^z (x on slots a, b; controls as for the other kinds),
   // to overlap slot `oslot`.
   GK_OVL = 10,
+  // GK_DIAGREG with pattern 0's phase taken out into the pass's complex
+  // scale (run-time compiled passes of circuits with fixed angles): only the
+  // register patterns in the mask b are multiplied (a one-qubit phase on
+  // slot t touches the four patterns with bit t set)
+  GK_DIAGREG1 = 11,
 };
 constexpr int kMaxTabBits = 3;   // non-register qubits of one phase table
 constexpr int kMaxCzShifts = 4;
@@ -112,7 +117,8 @@ struct PassDev {
   uint64_t tid_mask;      // global bits of local positions 0..k-R-1 (thread index)
   uint64_t hb[kRegQubits];  // global bits of local positions k-R..k-1
   int32_t map[kMaxTileQubits];       // local bit -> global bit
-  double scale;                      // product of the factored-out rotation scalars
+  // product of the factored-out rotation scalars and diagonal phases
+  double scale, scale_im;
 };
 
 }  // namespace smoq

@@ -16,6 +16,8 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <complex>
+#include <cstring>
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
@@ -121,14 +123,40 @@ static void bit_map_emit(O& o, const std::vector<int>& dst, bool wide) {
 // keep them in float.  Factoring therefore stops once the product of a
 // pass's sigmas would fall below 2^-40 (complex64) or 2^-500 (complex128);
 // later rotations of that pass stay unfactored.
+//
+// Register diagonals (GK_DIAGREG) of unit-modulus pattern-0 phase become
+// GK_DIAGREG1: that phase joins the pass's complex scale and every pattern
+// whose phase equals it exactly is skipped, so that a one-qubit phase costs
+// four complex products per thread instead of eight.
 void factor_rotations(Program& prog, int dtype, std::vector<FactorUndo>* undo) {
   const int max_log2 = dtype == SMOQ_C64 ? 40 : 500;
   for (PassDev& pd : prog.passes) {
     double scale = 1.0;
     int scale_exp = 0;  // scale = frexp mantissa * 2^scale_exp, tracked exactly
+    std::complex<double> phase(1.0, 0.0);
     for (int s = pd.s0; s < pd.s1; ++s) {
       for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g) {
         GateDev& G = prog.gates[g];
+        if (G.kind == GK_DIAGREG) {
+          const std::complex<double> p0(G.m[0], G.m[1]);
+          if (!(std::fabs(std::abs(p0) - 1.0) <= 1e-6)) continue;
+          FactorUndo u{g, G.kind, G.b, G.m[0], {}};
+          std::memcpy(u.md, G.m, sizeof u.md);
+          uint32_t mask = 0;
+          for (int j = 0; j < (1 << kRegQubits); ++j) {
+            const std::complex<double> pj(G.m[2 * j], G.m[2 * j + 1]);
+            const std::complex<double> q =
+                (pj.real() == p0.real() && pj.imag() == p0.imag()) ? 1.0 : pj / p0;
+            if (q != 1.0) mask |= 1u << j;
+            G.m[2 * j] = q.real();
+            G.m[2 * j + 1] = q.imag();
+          }
+          if (undo) undo->push_back(u);
+          G.kind = GK_DIAGREG1;
+          G.b = (int32_t)mask;
+          phase *= p0;
+          continue;
+        }
         if (G.creg != 0 || G.cglob != 0) continue;
         const double* m = G.m;
         double a, b;
@@ -150,7 +178,7 @@ void factor_rotations(Program& prog, int dtype, std::vector<FactorUndo>* undo) {
         int e = 0;
         const double mant = std::frexp(scale * sigma, &e);
         if (-(scale_exp + e) >= max_log2) continue;
-        if (undo) undo->push_back({g, G.kind, G.b, G.m[0]});
+        if (undo) undo->push_back({g, G.kind, G.b, G.m[0], {}});
         G.kind = kind;
         G.b = diag ? 0 : 1;
         G.m[0] = t;
@@ -158,18 +186,24 @@ void factor_rotations(Program& prog, int dtype, std::vector<FactorUndo>* undo) {
         scale_exp += e;
       }
     }
-    pd.scale = std::ldexp(scale, scale_exp);
+    const double mag = std::ldexp(scale, scale_exp);
+    pd.scale = mag * phase.real();
+    pd.scale_im = mag * phase.imag();
   }
 }
 
 void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo) {
   for (const FactorUndo& u : undo) {
     GateDev& G = prog.gates[u.g];
+    if (G.kind == GK_DIAGREG1) std::memcpy(G.m, u.md, sizeof u.md);
     G.kind = u.kind;
     G.b = u.b;
     G.m[0] = u.m0;
   }
-  for (PassDev& pd : prog.passes) pd.scale = 1.0;
+  for (PassDev& pd : prog.passes) {
+    pd.scale = 1.0;
+    pd.scale_im = 0.0;
+  }
 }
 
 
@@ -291,6 +325,9 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
             break;
           }
           case GK_DIAGREG: o << "    reg_diagreg<double>(x, " << gd << ");\n"; break;
+          case GK_DIAGREG1:
+            if (G.b) o << "    reg_diagreg_m<double, " << G.b << "u>(x, " << gd << ");\n";
+            break;
           case GK_OVL:
             o << "    reg_ovl<" << G.a << ", " << G.b << ", " << G.osel << ", " << G.ocnt << ", "
               << ((G.creg | G.cglob) ? "true" : "false") << ", " << ntp << ">(x, " << gd
@@ -307,11 +344,15 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
       }
       if (seg_end) o << "    stage_store<T>(lds, sf, x);\n  }\n";
     }
-    bool scaled = false;
+    bool scaled = false, phased = false;
     for (int s = pd.s0; s < pd.s1; ++s)
-      for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g)
+      for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g) {
         scaled |= prog.gates[g].kind == GK_U1_ROT || prog.gates[g].kind == GK_U1_RXR;
-    if (scaled)
+        phased |= prog.gates[g].kind == GK_DIAGREG1;
+      }
+    if (phased)
+      o << "  pass_store_cscaled<T>(lds, f, pass->scale, pass->scale_im);\n";
+    else if (scaled)
       o << "  pass_store_scaled<T>(lds, f, pass->scale);\n";
     else
       o << "  pass_store<T>(lds, f);\n";

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <complex>
 #include <cstdlib>
+#include <iterator>
 #include <map>
 
 #include "smoq_internal.h"
@@ -468,7 +469,9 @@ static void form_stages_swapping(const std::vector<IGate>& gates, const std::vec
             best = nxt;
           }
         }
-        sink.flush();
+        // a pending diagonal run may be applied under either mapping: it
+        // stays pending, so that it is emitted with the register slots of
+        // the gate that ends it
         sd.g1 = (int)prog.gates.size();
         prog.stages.push_back(sd);
         std::swap(M.reg[victim], M.tb[lb]);
@@ -481,11 +484,53 @@ static void form_stages_swapping(const std::vector<IGate>& gates, const std::vec
       }
       sink.add(g);
     }
-    sink.flush();
+    // a diagonal run still pending at the segment's end moves on to the
+    // next segment (the pass's last one emits it)
+    if (j == taken.size()) sink.flush();
     sd.g1 = (int)prog.gates.size();
     prog.stages.push_back(sd);
     i = j;
   }
+}
+
+// A diagonal gate other than a CZ commutes with every gate that has none of
+// its qubits as a target, so inside a pass it moves down to just before the
+// first later gate that does (or to the pass's end).  There that gate's
+// target is a register slot, so a one-qubit phase becomes a register
+// diagonal merged with the others that wait for the same gate, instead of a
+// per-lane phase of its own.  CZs stay where they are: the CZs of a layer
+// stay together and merge into one sign mask.  The diagonal gates that no
+// later gate of the pass stops come back in `tail` (in gate order), at the
+// end of `taken` unless tail_out is set.
+static void sink_diagonals(const std::vector<IGate>& gates, std::vector<int>& taken,
+                           std::vector<int>* tail_out) {
+  std::vector<int> order, pending, keep;
+  order.reserve(taken.size());
+  for (int gi : taken) {
+    const IGate& g = gates[gi];
+    if (g.kind == IK_DIAG && !is_cz(g)) {
+      pending.push_back(gi);
+      continue;
+    }
+    // qubits on which g does not commute with a diagonal gate
+    const uint64_t nc = g.kind == IK_DIAG ? 0 : g.kind == IK_OVL ? touch_mask(g) : target_mask(g);
+    if (nc && !pending.empty()) {
+      keep.clear();
+      for (int d : pending) {
+        if (touch_mask(gates[d]) & nc)
+          order.push_back(d);
+        else
+          keep.push_back(d);
+      }
+      pending.swap(keep);
+    }
+    order.push_back(gi);
+  }
+  if (tail_out)
+    tail_out->swap(pending);
+  else
+    order.insert(order.end(), pending.begin(), pending.end());
+  taken.swap(order);
 }
 
 void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int low_qubits,
@@ -532,6 +577,26 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
     }
     // pad the local set to exactly k qubits with the lowest free bits
     for (int b = 0; b < n && popc(L) < k; ++b) L |= 1ull << b;
+    // Diagonal gates that sink to the pass's end move on to the next pass
+    // when there is one (some non-diagonal gate was deferred): every gate
+    // before them that they do not commute with is in this pass or an
+    // earlier one, so the next pass takes them again, and there they sink
+    // to the gate that targets their qubit.
+    bool more = false;
+    for (int gi : deferred) more |= gates[gi].kind != IK_DIAG;
+    if (more) {
+      std::vector<int> tail;
+      sink_diagonals(gates, taken, &tail);
+      if (!tail.empty()) {
+        std::vector<int> merged;
+        merged.reserve(deferred.size() + tail.size());
+        std::merge(deferred.begin(), deferred.end(), tail.begin(), tail.end(),
+                   std::back_inserter(merged));
+        deferred.swap(merged);
+      }
+    } else {
+      sink_diagonals(gates, taken, nullptr);
+    }
 
     PassDev pd;
     std::memset(&pd, 0, sizeof(pd));
@@ -580,6 +645,14 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
         }
         if (popc(S | need) > kRegQubits) break;
         S |= need;
+      }
+      // diagonal gates after the stage's last non-diagonal gate wait for
+      // the next stage, where the gate that ends them has its target in a
+      // register slot (the pass's last stage keeps them)
+      if (j < taken.size()) {
+        size_t e = j;
+        while (e > i && gates[taken[e - 1]].kind == IK_DIAG) --e;
+        if (e > i) j = e;
       }
       for (int b = 0; b < k && popc(S) < kRegQubits; ++b) S |= 1u << b;
       StageDev sd;

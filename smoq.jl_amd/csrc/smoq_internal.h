@@ -233,6 +233,7 @@ struct FactorUndo {
   int g;
   int32_t kind, b;
   double m0;
+  double md[16];  // a normalised register diagonal's phases (GK_DIAGREG1)
 };
 void factor_rotations(Program& prog, int dtype, std::vector<FactorUndo>* undo = nullptr);
 void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo);
