@@ -188,6 +188,11 @@ typedef struct smoq_tune_result {
   int candidates;     /* settings timed (0: nothing to choose) */
   double ms;          /* one application with the chosen settings */
   double default_ms;  /* one application with the context's settings */
+  /* per-pass HBM orders (states of 2^12 or more tiles per pass): passes given
+   * a non-default tile order, and the sum of the passes' times with them
+   * (0: not searched) */
+  int io_passes;
+  double io_ms;
 } smoq_tune_result;
 int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const double* theta,
                   int n_theta, smoq_tune_result* out);

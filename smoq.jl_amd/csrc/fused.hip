@@ -89,7 +89,8 @@ int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayo
 
 template <typename T>
 static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout& L,
-                            const void* dbuf, const JitModule* jit, double* opart) {
+                            const void* dbuf, const JitModule* jit, double* opart,
+                            int only_pass) {
   smoq_ctx* ctx = s->ctx;
   int rc = SMOQ_OK;
   const unsigned char* d = (const unsigned char*)dbuf;
@@ -107,6 +108,7 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
   VT<T>* psi = (VT<T>*)s->d;
   int n = s->n;
   for (size_t p = 0; p < prog.passes.size(); ++p) {
+    if (only_pass >= 0 && (int)p != only_pass) continue;
     const PassDev& pd = prog.passes[p];
     const uint64_t tiles = (uint64_t)s->batch << pd.n_nonlocal;
     const int nt = 1 << (pd.k - kRegQubits);
@@ -146,9 +148,9 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
 }
 
 int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf,
-                   const JitModule* jit, double* opart) {
-  return s->dtype == SMOQ_C128 ? launch_program_t<double>(s, prog, L, dbuf, jit, opart)
-                               : launch_program_t<float>(s, prog, L, dbuf, jit, opart);
+                   const JitModule* jit, double* opart, int only_pass) {
+  return s->dtype == SMOQ_C128 ? launch_program_t<double>(s, prog, L, dbuf, jit, opart, only_pass)
+                               : launch_program_t<float>(s, prog, L, dbuf, jit, opart, only_pass);
 }
 
 int run_program(smoq_state* s, Program& prog, bool fixed_angles) {

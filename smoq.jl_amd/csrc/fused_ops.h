@@ -322,8 +322,18 @@ struct PassFrame {
   uint64_t gbase;    // global bits outside the tile
   uint64_t off[kNR]; // offsets of register index i
   uint32_t st_lane;  // swizzled LDS index of the thread's lane part
-  int nt;            // threads per tile
+  uint32_t sreg[kNR];  // swizzled LDS index of register index i's part
 };
+
+// the tile of launch index tt (nn bits): rotated left by tord & 31 after a
+// bit reversal when tord & 32 (a bijection on the nn-bit range)
+__device__ __forceinline__ uint64_t tile_order(uint64_t tt, int nn, int tord) {
+  if (nn <= 0 || tord == 0) return tt;
+  if (tord & 32) tt = __brevll(tt) >> (64 - nn);
+  const int r = (tord & 31) % nn;
+  if (r) tt = ((tt << r) | (tt >> (nn - r))) & ((1ull << nn) - 1ull);
+  return tt;
+}
 
 // Tile loads and stores of a pass touch every amplitude once: nontemporal
 // (streaming) accesses keep them from displacing cached lines.
@@ -355,10 +365,9 @@ __device__ __forceinline__ void pass_load(VT<T>* lds, VT<T>* __restrict__ psi, i
   // tiles beyond one grid row continue in blockIdx.y (states of 2^32 threads
   // and more per pass)
   const uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
-  f.tile = tile;
   const uint64_t item = tile >> nn;
-  const uint64_t tt = tile & ((1ull << nn) - 1ull);
-  f.nt = 1 << (pass->k - R);
+  const uint64_t tt = tile_order(tile & ((1ull << nn) - 1ull), nn, pass->tord);
+  f.tile = (item << nn) | tt;
   f.gbase = pdep64(tt, pass->nl_mask);
   f.pt = psi + (item << n) + (f.gbase | pdep64((uint64_t)threadIdx.x, pass->tid_mask));
 #pragma unroll
@@ -370,22 +379,25 @@ __device__ __forceinline__ void pass_load(VT<T>* lds, VT<T>* __restrict__ psi, i
   }
   // The swizzle is linear over GF(2): swz(a | b) = swz(a) ^ swz(b) for
   // disjoint a, b, so every LDS address is a per-lane part XOR a uniform part.
-  f.st_lane = swz<WB>((uint32_t)threadIdx.x);
+  const uint32_t ltm = pass->ltm;
+  const uint32_t lrm = ((1u << pass->k) - 1u) & ~ltm;
+  f.st_lane = swz<WB>(pdep32((uint32_t)threadIdx.x, ltm));
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) f.sreg[i] = swz<WB>(pdep32((uint32_t)i, lrm));
   VT<T> x[kNR];
 #pragma unroll
   for (int i = 0; i < kNR; ++i) x[i] = ld_stream<T>(f.pt + f.off[i]);
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))] = x[i];
+  for (int i = 0; i < kNR; ++i) lds[f.st_lane ^ f.sreg[i]] = x[i];
   __syncthreads();
 }
 
 // LDS -> HBM for the tile (after the last stage's barrier)
 template <typename T>
 __device__ __forceinline__ void pass_store(const VT<T>* lds, const PassFrame<T>& f) {
-  constexpr int WB = CT<T>::kRowBits;
   VT<T> x[kNR];
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))];
+  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ f.sreg[i]];
 #pragma unroll
   for (int i = 0; i < kNR; ++i) st_stream<T>(f.pt + f.off[i], x[i]);
 }
@@ -394,11 +406,10 @@ __device__ __forceinline__ void pass_store(const VT<T>* lds, const PassFrame<T>&
 template <typename T>
 __device__ __forceinline__ void pass_store_scaled(const VT<T>* lds, const PassFrame<T>& f,
                                                   double scale) {
-  constexpr int WB = CT<T>::kRowBits;
   const T sc = (T)scale;
   VT<T> x[kNR];
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))];
+  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ f.sreg[i]];
 #pragma unroll
   for (int i = 0; i < kNR; ++i) {
     x[i].x *= sc;
@@ -412,10 +423,9 @@ __device__ __forceinline__ void pass_store_scaled(const VT<T>* lds, const PassFr
 template <typename T>
 __device__ __forceinline__ void pass_store_cscaled(const VT<T>* lds, const PassFrame<T>& f,
                                                    double sre, double sim) {
-  constexpr int WB = CT<T>::kRowBits;
   VT<T> x[kNR];
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ swz<WB>((uint32_t)(i * f.nt))];
+  for (int i = 0; i < kNR; ++i) x[i] = lds[f.st_lane ^ f.sreg[i]];
 #pragma unroll
   for (int i = 0; i < kNR; ++i) {
     VT<double> v;

@@ -119,6 +119,14 @@ struct PassDev {
   int32_t map[kMaxTileQubits];       // local bit -> global bit
   // product of the factored-out rotation scalars and diagonal phases
   double scale, scale_im;
+  // HBM order of the tiles and of each tile's amplitudes (they move
+  // nothing but the time a pass takes): the launch's tile index is rotated
+  // left by tord & 31 bits, after a bit reversal when tord & 32, inside its
+  // n_nonlocal bits; ltm marks the local positions that the thread index
+  // covers when the tile moves between HBM and LDS (tid_mask holds their
+  // global bits, hb the other three positions' bits)
+  int32_t tord;
+  uint32_t ltm;
 };
 
 }  // namespace smoq

@@ -109,6 +109,7 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
   plan_program(gs, n_qubits, fu.tile, fu.low,
                dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog, fu.cap,
                fu.swaps != 0);
+  apply_pass_io(p->prog, fu.io, fu.low);
   p->layout = program_layout(p->prog, dtype);
   if (jit_wanted(ctx, p->prog, n_qubits)) {
     Program factored = p->prog;

@@ -533,6 +533,36 @@ static void sink_diagonals(const std::vector<IGate>& gates, std::vector<int>& ta
   taken.swap(order);
 }
 
+void set_pass_io(PassDev& pd, int io, int low) {
+  const int k = pd.k;
+  // the three register positions of the HBM <-> LDS moves: the tile's top
+  // three (io bit 6 clear) or the three above its contiguous low qubits
+  int r0 = k - kRegQubits;
+  if ((io & 64) && low + kRegQubits <= k - kRegQubits) r0 = low;
+  pd.ltm = 0;
+  pd.tid_mask = 0;
+  int hj = 0;
+  for (int b = 0; b < k; ++b) {
+    if (b >= r0 && b < r0 + kRegQubits) {
+      pd.hb[hj++] = 1ull << pd.map[b];
+    } else {
+      pd.ltm |= 1u << b;
+      pd.tid_mask |= 1ull << pd.map[b];
+    }
+  }
+  pd.tord = io & 63;
+}
+
+void apply_pass_io(Program& prog, const std::vector<uint8_t>& io, int low) {
+  if (io.size() != prog.passes.size()) return;
+  for (size_t p = 0; p < io.size(); ++p) {
+    PassDev& pd = prog.passes[p];
+    // the contiguous low qubits as plan_program keeps them
+    const int m = pd.n_nonlocal == 0 ? pd.k : std::min(low, pd.k - 2);
+    set_pass_io(pd, io[p], m);
+  }
+}
+
 void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int low_qubits,
                   int row_bits, Program& prog, int max_gates, bool lane_swaps) {
   prog.passes.clear();
@@ -615,8 +645,7 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
       }
     }
     pd.n_nonlocal = ni;
-    for (int j = 0; j < k - kRegQubits; ++j) pd.tid_mask |= 1ull << pd.map[j];
-    for (int j = 0; j < kRegQubits; ++j) pd.hb[j] = 1ull << pd.map[k - kRegQubits + j];
+    set_pass_io(pd, 0, m);
     pd.s0 = (int)prog.stages.size();
 
     // register stages with lane exchanges (tiles of at least one wave)

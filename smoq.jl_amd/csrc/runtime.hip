@@ -620,6 +620,7 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
   plan_program(gs, s->n, fu.tile, fu.low,
                s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog, fu.cap,
                fu.swaps != 0);
+  apply_pass_io(prog, fu.io, fu.low);
   if (n_passes) *n_passes = (int)prog.passes.size();
   if (gs.empty()) return SMOQ_OK;
   if (s->n < kRegQubits) {

@@ -79,12 +79,19 @@ void plan_program(const std::vector<IGate>& gates, int n_qubits, int tile_qubits
                   bool lane_swaps = false);
 // whether a program has stages that continue through lane exchanges
 bool program_has_swaps(const Program& prog);
+// a pass's HBM order (io & 63: the tile order of PassDev::tord; io & 64:
+// registers on the three positions above the `low` contiguous qubits
+// instead of the tile's top three when moving it between HBM and LDS)
+void set_pass_io(PassDev& pd, int io, int low);
+// set_pass_io of every pass when io has one entry per pass
+void apply_pass_io(Program& prog, const std::vector<uint8_t>& io, int low);
 
 // Fusion settings of a gate list: tile qubits, contiguous low qubits and
 // the per-pass gate cap.
 struct Fusion {
   int tile = 0, low = 0, cap = 0;
   int swaps = 0;  // lane exchanges in compiled passes
+  std::vector<uint8_t> io;  // per-pass HBM order (set_pass_io), empty: default
 };
 // 64-bit hash of a lowered gate list's structure (kinds and qubits, not
 // the matrices)
@@ -223,8 +230,9 @@ int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayo
 // jit: the program's compiled pass kernels, or null for the precompiled one
 // opart: per-tile partials of the program's overlap gates (GK_OVL), at
 // opart[oslot * tiles + tile]; compiled passes only
+// only_pass >= 0: that pass alone
 int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf,
-                   const JitModule* jit, double* opart = nullptr);
+                   const JitModule* jit, double* opart = nullptr, int only_pass = -1);
 // run-time compiled pass kernels (jit.hip)
 std::string jit_source(const Program& prog, int dtype);
 // rewrites rotation gates into the factored kinds of run-time compiled

@@ -165,6 +165,68 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
   size_t best = 0;
   for (size_t i = 1; i < cand.size(); ++i)
     if (ms[i] < ms[best]) best = i;
+  // Per pass, the HBM order of its tiles and of the registers that move
+  // them (set_pass_io): they change no result, but the time of a pass
+  // depends on which address bits its concurrently running tiles span, by
+  // up to a quarter on some qubit maps.  Every pass is timed alone under
+  // each order and keeps the fastest (the default unless 1% faster).
+  Program& bp = progs[best];
+  const int nn = bp.passes.empty() ? 0 : bp.passes[0].n_nonlocal;
+  if (nn >= 12 && s->batch == 1) {
+    const ProgLayout L = program_layout(bp, s->dtype);
+    const JitModule* jm = use_jit[best] ? jit[best].get() : nullptr;
+    std::vector<int> codes;
+    for (int rev : {0, 32})
+      for (int r : {0, 3, 6, 9, 12, 15}) codes.push_back(rev | r);
+    for (int rev : {0, 32})
+      for (int r : {0, 6, 12}) codes.push_back(64 | rev | r);
+    std::vector<uint8_t> io(bp.passes.size(), 0);
+    const int low = cand[best].low;
+    auto time_pass = [&](size_t p, float* out_ms) -> int {
+      int r2;
+      if ((r2 = upload_program(ctx, bp, s->dtype, L, ctx->prog))) return r2;
+      if (hipEventRecord(e0, ctx->stream) != hipSuccess)
+        return set_error(SMOQ_EDEVICE, "smoq_autotune: event record failed");
+      for (int r = 0; r < kReps; ++r)
+        if ((r2 = launch_program(s, bp, L, ctx->prog, jm, nullptr, (int)p))) return r2;
+      if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(out_ms, e0, e1) != hipSuccess)
+        return set_error(SMOQ_EDEVICE, "smoq_autotune: timing failed");
+      return SMOQ_OK;
+    };
+    for (size_t p = 0; p < bp.passes.size(); ++p) {
+      // one untimed launch of the pass first
+      if ((rc = upload_program(ctx, bp, s->dtype, L, ctx->prog))) return cleanup(rc);
+      if ((rc = launch_program(s, bp, L, ctx->prog, jm, nullptr, (int)p))) return cleanup(rc);
+      float base = 0.f;
+      if ((rc = time_pass(p, &base))) return cleanup(rc);
+      float bt = base;
+      int bc = 0;
+      for (int c : codes) {
+        if (c == 0) continue;
+        if ((c & 31) >= bp.passes[p].n_nonlocal) continue;
+        set_pass_io(bp.passes[p], c, low);
+        float t = 0.f;
+        if ((rc = time_pass(p, &t))) return cleanup(rc);
+        if (t < bt) {
+          bt = t;
+          bc = c;
+        }
+      }
+      if (bt > 0.99f * base) bc = 0;
+      io[p] = (uint8_t)bc;
+      set_pass_io(bp.passes[p], bc, low);
+    }
+    cand[best].io = io;
+    for (uint8_t c : io) out->io_passes += c != 0;
+    float total = 0.f;
+    for (size_t p = 0; p < bp.passes.size(); ++p) {
+      float t = 0.f;
+      if ((rc = time_pass(p, &t))) return cleanup(rc);
+      total += t / kReps;
+    }
+    out->io_ms = total;
+  }
   ctx->tune[tune_key(s->n, s->dtype, s->batch, gate_structure_hash(gs))] = cand[best];
   out->tile_qubits = cand[best].tile;
   out->low_qubits = cand[best].low;
@@ -182,8 +244,16 @@ extern "C" int smoq_tune_save(smoq_ctx* ctx, const char* path) {
   std::ofstream f(path);
   if (!f) return set_error(SMOQ_EINVAL, "smoq_tune_save: cannot write %s", path);
   for (const auto& kv : ctx->tune)
+  {
     f << kv.first << ' ' << kv.second.tile << ' ' << kv.second.low << ' ' << kv.second.cap
-      << ' ' << kv.second.swaps << '\n';
+      << ' ' << kv.second.swaps;
+    // per-pass HBM orders, when tuned
+    if (!kv.second.io.empty()) {
+      f << ' ' << kv.second.io.size();
+      for (uint8_t c : kv.second.io) f << ' ' << (int)c;
+    }
+    f << '\n';
+  }
   return f ? SMOQ_OK : set_error(SMOQ_EINVAL, "smoq_tune_save: write to %s failed", path);
 }
 
@@ -203,6 +273,17 @@ extern "C" int smoq_tune_load(smoq_ctx* ctx, const char* path, int* n_entries) {
     if (arch != "gfx950" || fu.tile < 6 || fu.tile > kMaxTileQubits || fu.low < 3 || fu.low > 6 ||
         fu.cap < 0)
       continue;
+    size_t nio = 0;
+    if (is >> nio) {
+      if (nio > 4096) continue;
+      bool ok = true;
+      for (size_t p = 0; p < nio && ok; ++p) {
+        int c = -1;
+        ok = (bool)(is >> c) && c >= 0 && c < 128;
+        fu.io.push_back((uint8_t)c);
+      }
+      if (!ok) continue;
+    }
     ctx->tune[arch + " " + std::to_string(n) + " " + std::to_string(dtype) + " " +
               std::to_string(batch) + " " + hash] = fu;
     ++count;

@@ -103,6 +103,8 @@ class smoq_tune_result(ctypes.Structure):
         ("candidates", ctypes.c_int),
         ("ms", ctypes.c_double),
         ("default_ms", ctypes.c_double),
+        ("io_passes", ctypes.c_int),
+        ("io_ms", ctypes.c_double),
     ]
 
 

@@ -79,3 +79,48 @@ def test_jit_disk_cache_round_trip(monkeypatch, tmp_path):
         assert len(files) >= 1
     assert np.array_equal(got[0], got[1])
     assert np.linalg.norm(got[0] - ref) < 1e-12
+
+
+def test_pass_hbm_orders_keep_results(tmp_path):
+    """Per-pass tile orders and load mappings (the tuned HBM order of each
+    pass, set here through a saved table) move no amplitude: the result is
+    bit-identical to the default order and matches the oracle."""
+    from smoq_amd import GPUState
+    from smoq_amd.circuits import random_rotation_brick
+    from smoq_amd.state import Context
+
+    N = 20
+    tape = random_rotation_brick(N, 12, seed=47)
+    ref = oracle_state(tape, N)
+    c = Context(0)
+    s = GPUState(N, ctx=c)
+    res = c.autotune(tape, s)
+    s.set_basis(0)
+    tape.apply(s)
+    plain = s.to_numpy()
+    assert np.linalg.norm(plain - ref) < 1e-12
+    path = tmp_path / "tune.txt"
+    c.tune_save(path)
+    s.free()
+    c.close()
+    line = path.read_text().split("\n")[0].split()
+    base = line[:9]  # arch n dtype batch hash tile low cap swaps
+    n_pass = res["n_passes"]
+    # rotations, reversals (32) and registers above the low qubits (64)
+    codes = [3, 32 | 5, 64, 64 | 32 | 7, 6, 32, 64 | 2, 1]
+    io = [codes[p % len(codes)] for p in range(n_pass)]
+    path.write_text(" ".join(base + [str(n_pass)] + [str(x) for x in io]) + "\n")
+    c2 = Context(0)
+    assert c2.tune_load(path) == 1
+    s2 = GPUState.basis(N, 0, ctx=c2)
+    assert tape.apply(s2) == n_pass
+    got = s2.to_numpy()
+    assert np.array_equal(got, plain)
+    # the same through a compiled plan
+    plan = tape.compile(ctx=c2)
+    s2.set_basis(0)
+    plan.run(s2)
+    assert np.array_equal(s2.to_numpy(), plain)
+    plan.free()
+    s2.free()
+    c2.close()
