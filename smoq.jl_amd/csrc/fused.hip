@@ -136,7 +136,7 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
       const PassDev* pp = dp + p;
       void* args[] = {&psi, &n, &pp, (void*)&ds, (void*)&dg, (void*)&dt, &opart};
       SMOQ_HIP(hipModuleLaunchKernel(jit->fn[p], (unsigned)gx, (unsigned)gy, 1, nt, 1, 1,
-                                     (unsigned)shm, ctx->stream, args, nullptr));
+                                     0u, ctx->stream, args, nullptr));
     } else {
       hipLaunchKernelGGL(k_fused_pass<T>, dim3((unsigned)gx, (unsigned)gy), dim3(nt), shm,
                          ctx->stream, psi, n, dp + p, ds, dg, dt);

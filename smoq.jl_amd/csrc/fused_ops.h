@@ -437,11 +437,22 @@ __device__ __forceinline__ void pass_store_cscaled(const VT<T>* lds, const PassF
 
 // One register stage: 2^R amplitudes of the thread (its lane part plus the
 // stage's register qubits) move from LDS into complex128 registers.
+// LDS addresses are kept in bytes: an access is then one XOR of the lane
+// part with a uniform (mostly compile-time) part.
 struct StageFrame {
-  uint32_t sl;       // lane part of the swizzled LDS index
+  uint32_t sl;       // lane part of the swizzled LDS address (bytes)
   uint64_t gl;       // global index of the thread's register index 0
-  uint32_t sr[kNR];  // swizzled LDS offsets of the register patterns
+  uint32_t sr[kNR];  // swizzled LDS offsets of the register patterns (bytes)
 };
+
+template <typename T>
+__device__ __forceinline__ const VT<T>& lds_at(const VT<T>* lds, uint32_t byte_off) {
+  return *reinterpret_cast<const VT<T>*>(reinterpret_cast<const unsigned char*>(lds) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ VT<T>& lds_at(VT<T>* lds, uint32_t byte_off) {
+  return *reinterpret_cast<VT<T>*>(reinterpret_cast<unsigned char*>(lds) + byte_off);
+}
 
 // NTB >= 0: the stage's thread-index bits, known when the kernel is compiled
 template <typename T, int NTB = -1>
@@ -462,15 +473,16 @@ __device__ __forceinline__ void stage_load(const VT<T>* lds, const StageDev& st,
       sf.gl |= st.g_tid[i] & (0ull - (uint64_t)b);
     }
   }
+  sf.sl *= (uint32_t)sizeof(VT<T>);
 #pragma unroll
   for (int i = 0; i < kNR; ++i) {
     uint32_t r = 0;
 #pragma unroll
     for (int j = 0; j < R; ++j) r ^= ((i >> j) & 1) ? st.sw_reg[j] : 0u;
-    sf.sr[i] = r;
+    sf.sr[i] = r * (uint32_t)sizeof(VT<T>);
   }
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) x[i] = to_d2(lds[sf.sl ^ sf.sr[i]]);
+  for (int i = 0; i < kNR; ++i) x[i] = to_d2(lds_at<T>(lds, sf.sl ^ sf.sr[i]));
 }
 
 // The same with the stage's addressing known when the kernel is compiled:
@@ -480,24 +492,25 @@ template <typename T>
 __device__ __forceinline__ void stage_load_c(const VT<T>* lds, uint32_t l_lane, uint64_t gl,
                                              const uint32_t (&sr)[kNR], VT<double> (&x)[kNR],
                                              StageFrame& sf) {
-  sf.sl = swz<CT<T>::kRowBits>(l_lane);
+  sf.sl = swz<CT<T>::kRowBits>(l_lane) * (uint32_t)sizeof(VT<T>);
   sf.gl = gl;
 #pragma unroll
   for (int i = 0; i < kNR; ++i) {
-    sf.sr[i] = sr[i];
-    x[i] = to_d2(lds[sf.sl ^ sr[i]]);
+    sf.sr[i] = sr[i] * (uint32_t)sizeof(VT<T>);
+    x[i] = to_d2(lds_at<T>(lds, sf.sl ^ sf.sr[i]));
   }
 }
 
 // The stage's mapping after a lane exchange: the swizzled LDS index of the
 // thread's lane part, its global index and the register patterns' swizzled
 // offsets, all fixed when the kernel is compiled.
+template <typename T>
 __device__ __forceinline__ void stage_remap(uint32_t sl, uint64_t gl, const uint32_t (&sr)[kNR],
                                             StageFrame& sf) {
-  sf.sl = sl;
+  sf.sl = sl * (uint32_t)sizeof(VT<T>);
   sf.gl = gl;
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) sf.sr[i] = sr[i];
+  for (int i = 0; i < kNR; ++i) sf.sr[i] = sr[i] * (uint32_t)sizeof(VT<T>);
 }
 
 // v_permlane16_swap (LB = 4) / v_permlane32_swap (LB = 5) on the two dwords
@@ -540,7 +553,7 @@ template <typename T>
 __device__ __forceinline__ void stage_store(VT<T>* lds, const StageFrame& sf,
                                             const VT<double> (&x)[kNR]) {
 #pragma unroll
-  for (int i = 0; i < kNR; ++i) lds[sf.sl ^ sf.sr[i]] = make_c<T>(x[i].x, x[i].y);
+  for (int i = 0; i < kNR; ++i) lds_at<T>(lds, sf.sl ^ sf.sr[i]) = make_c<T>(x[i].x, x[i].y);
   __syncthreads();
 }
 

@@ -221,8 +221,10 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
          " const StageDev* __restrict__ stages, const GateDev* __restrict__ gates,"
          " const double2* __restrict__ tabs, double* __restrict__ opart) {\n"
       << "  typedef " << T << " T;\n"
-      << "  extern __shared__ __align__(16) unsigned char smem_raw[];\n"
-      << "  VT<T>* lds = reinterpret_cast<VT<T>*>(smem_raw);\n"
+      // the tile as a static LDS array: its address is known when the
+      // kernel is compiled, so an LDS access is one XOR of a lane part with
+      // a uniform part (the pass is launched without dynamic LDS)
+      << "  __shared__ __align__(16) VT<T> lds[" << (1 << pd.k) << "];\n"
       << "  PassFrame<T> f;\n";
     // adjoint overlaps of this pass: one LDS slot per (overlap, wave)
     const int ntp = 1 << (pd.k - kRegQubits);
@@ -279,7 +281,7 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
       } else {
         // the swizzled lane part of the new mapping: swz is linear, so the
         // thread's part is the XOR of its bits' images
-        o << "};\n      stage_remap(";
+        o << "};\n      stage_remap<T>(";
         std::vector<uint32_t> sw;
         for (int i = 0; i < st.ntb; ++i) sw.push_back(st.sw_tid[i]);
         o << "0u";

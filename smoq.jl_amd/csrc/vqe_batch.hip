@@ -174,7 +174,7 @@ __device__ __forceinline__ void vqe_pass_tile(VT<T>* lds, double* ovl, VT<T>* __
     }
     if (act) {
 #pragma unroll
-      for (int i = 0; i < kNR; ++i) lds[sf.sl ^ sf.sr[i]] = make_c<T>(x[i].x, x[i].y);
+      for (int i = 0; i < kNR; ++i) lds_at<T>(lds, sf.sl ^ sf.sr[i]) = make_c<T>(x[i].x, x[i].y);
     }
     __syncthreads();
   }
