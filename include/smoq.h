@@ -184,6 +184,7 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops,
 typedef struct smoq_tune_result {
   int tile_qubits, low_qubits, pass_max_gates; /* the chosen settings */
   int lane_swaps;     /* stages trade register and lane qubits (1) or not */
+  int searched;       /* passes' local qubit sets by search (1) or greedy (0) */
   int n_passes;       /* HBM passes per application with them */
   int candidates;     /* settings timed (0: nothing to choose) */
   double ms;          /* one application with the chosen settings */

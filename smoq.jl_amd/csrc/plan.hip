@@ -108,7 +108,8 @@ int smoq_plan_create(smoq_ctx* ctx, int n_qubits, int dtype, const smoq_op* ops,
   const Fusion fu = fusion_for(ctx, n_qubits, dtype, 1, gs);
   plan_program(gs, n_qubits, fu.tile, fu.low,
                dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, p->prog, fu.cap,
-               fu.swaps != 0);
+               fu.swaps != 0,
+               fu.locals.empty() ? nullptr : &fu.locals);
   apply_pass_io(p->prog, fu.io, fu.low);
   p->layout = program_layout(p->prog, dtype);
   if (jit_wanted(ctx, p->prog, n_qubits)) {

@@ -563,8 +563,119 @@ void apply_pass_io(Program& prog, const std::vector<uint8_t>& io, int low) {
   }
 }
 
+// One pass's gates in order: a gate joins when none of its qubits is
+// touched by a gate already deferred (so it commutes with all it overtakes),
+// the pass's cap on non-diagonal gates allows it and its targets fit the
+// local set (grown up to k qubits when `grow`, else fixed).  Returns the
+// local set.
+static uint64_t take_gates(const std::vector<IGate>& gates, const std::vector<int>& remaining,
+                           uint64_t L, bool grow, int k, int max_gates, std::vector<int>& taken,
+                           std::vector<int>& deferred) {
+  uint64_t blocked = 0;
+  int nondiag = 0;
+  for (int gi : remaining) {
+    const IGate& g = gates[gi];
+    const uint64_t touch = touch_mask(g);
+    if (touch & blocked) {
+      deferred.push_back(gi);
+      blocked |= touch;
+      continue;
+    }
+    if (max_gates > 0 && g.kind != IK_DIAG && nondiag >= max_gates && !taken.empty()) {
+      deferred.push_back(gi);
+      blocked |= touch;
+      continue;
+    }
+    const uint64_t nl = L | target_mask(g);
+    if (nl == L || (grow && popc(nl) <= k)) {
+      L = nl;
+      taken.push_back(gi);
+      nondiag += g.kind != IK_DIAG;
+    } else {
+      deferred.push_back(gi);
+      blocked |= touch;
+    }
+  }
+  return L;
+}
+
+// non-diagonal gates a pass with the fixed local set L would take
+static int count_taken(const std::vector<IGate>& gates, const std::vector<int>& remaining,
+                       uint64_t L, int max_gates) {
+  uint64_t blocked = 0;
+  int nondiag = 0, taken = 0;
+  for (int gi : remaining) {
+    const IGate& g = gates[gi];
+    const uint64_t touch = touch_mask(g);
+    if (touch & blocked) {
+      blocked |= touch;
+      continue;
+    }
+    if (max_gates > 0 && g.kind != IK_DIAG && nondiag >= max_gates && taken > 0) {
+      blocked |= touch;
+      continue;
+    }
+    if ((L | target_mask(g)) == L) {
+      ++taken;
+      nondiag += g.kind != IK_DIAG;
+    } else {
+      blocked |= touch;
+    }
+  }
+  return nondiag;
+}
+
+void search_local_sets(const std::vector<IGate>& gates, int n, int tile_qubits, int low_qubits,
+                       int max_gates, int iters, uint64_t seed, std::vector<uint64_t>& locals) {
+  locals.clear();
+  const int k = std::min(tile_qubits, n);
+  if (k >= n) return;  // one tile holds the state: nothing to choose
+  const int m = std::min(low_qubits, k - 2);
+  const uint64_t low = (1ull << m) - 1;
+  std::vector<int> remaining(gates.size()), taken, deferred;
+  for (size_t i = 0; i < gates.size(); ++i) remaining[i] = (int)i;
+  uint64_t rng = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto next = [&]() {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return rng;
+  };
+  auto pick = [&](uint64_t mask) {  // a uniformly chosen set bit of mask
+    int c = (int)(next() % (uint64_t)popc(mask));
+    for (int b = 0; b < 64; ++b)
+      if ((mask >> b & 1) && c-- == 0) return b;
+    return -1;
+  };
+  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
+  while (!remaining.empty()) {
+    taken.clear();
+    deferred.clear();
+    uint64_t cur = take_gates(gates, remaining, low, true, k, max_gates, taken, deferred);
+    for (int b = 0; b < n && popc(cur) < k; ++b) cur |= 1ull << b;
+    int best = count_taken(gates, remaining, cur, max_gates);
+    // random exchanges of one non-low local qubit with one other qubit,
+    // kept when the pass takes at least as many non-diagonal gates
+    for (int it = 0; it < iters && (all & ~cur); ++it) {
+      const int out = pick(cur & ~low), in = pick(all & ~cur);
+      const uint64_t cand = (cur & ~(1ull << out)) | (1ull << in);
+      const int sc = count_taken(gates, remaining, cand, max_gates);
+      if (sc >= best) {
+        best = sc;
+        cur = cand;
+      }
+    }
+    taken.clear();
+    deferred.clear();
+    take_gates(gates, remaining, cur, false, k, max_gates, taken, deferred);
+    locals.push_back(cur);
+    remaining.swap(deferred);
+  }
+}
+
 void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int low_qubits,
-                  int row_bits, Program& prog, int max_gates, bool lane_swaps) {
+                  int row_bits, Program& prog, int max_gates, bool lane_swaps,
+                  const std::vector<uint64_t>* locals) {
   prog.passes.clear();
   prog.stages.clear();
   prog.gates.clear();
@@ -577,34 +688,24 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
   // max_gates: cap on the non-diagonal gates one pass takes (0: none), which
   // spreads deep passes over more passes
 
+  size_t pass_no = 0;
   while (!remaining.empty()) {
-    uint64_t L = (m >= 64) ? ~0ull : ((1ull << m) - 1);
-    uint64_t blocked = 0;
-    int nondiag = 0;
     std::vector<int> taken, deferred;
-    for (int gi : remaining) {
-      const IGate& g = gates[gi];
-      uint64_t touch = touch_mask(g);
-      if (touch & blocked) {
-        deferred.push_back(gi);
-        blocked |= touch;
-        continue;
-      }
-      if (max_gates > 0 && g.kind != IK_DIAG && nondiag >= max_gates && !taken.empty()) {
-        deferred.push_back(gi);
-        blocked |= touch;
-        continue;
-      }
-      uint64_t nl = L | target_mask(g);
-      if (popc(nl) <= k) {
-        L = nl;
-        taken.push_back(gi);
-        nondiag += g.kind != IK_DIAG;
-      } else {
-        deferred.push_back(gi);
-        blocked |= touch;
+    uint64_t L;
+    if (locals && pass_no < locals->size()) {
+      // a local set chosen beforehand (search_local_sets): it stays fixed
+      L = take_gates(gates, remaining, (*locals)[pass_no], false, k, max_gates, taken, deferred);
+      if (taken.empty()) {
+        // not this gate list's sets: greedy from here on
+        taken.clear();
+        deferred.clear();
+        locals = nullptr;
       }
     }
+    if (!locals || pass_no >= locals->size())
+      L = take_gates(gates, remaining, (m >= 64) ? ~0ull : ((1ull << m) - 1), true, k, max_gates,
+                     taken, deferred);
+    ++pass_no;
     // pad the local set to exactly k qubits with the lowest free bits
     for (int b = 0; b < n && popc(L) < k; ++b) L |= 1ull << b;
     // Diagonal gates that sink to the pass's end move on to the next pass

@@ -619,7 +619,8 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops, const double* 
   const Fusion fu = fusion_for(s->ctx, s->n, s->dtype, s->batch, gs);
   plan_program(gs, s->n, fu.tile, fu.low,
                s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits, prog, fu.cap,
-               fu.swaps != 0);
+               fu.swaps != 0,
+               fu.locals.empty() ? nullptr : &fu.locals);
   apply_pass_io(prog, fu.io, fu.low);
   if (n_passes) *n_passes = (int)prog.passes.size();
   if (gs.empty()) return SMOQ_OK;

@@ -74,9 +74,20 @@ struct Program {
 // max_gates: at most that many non-diagonal gates per pass (0: no cap).
 // lane_swaps: stages may trade register qubits with lane bits 4 and 5 in
 // registers instead of an LDS round trip (run-time compiled passes only).
+// locals: the local qubit set of each pass, chosen beforehand
+// (search_local_sets); passes beyond it, or after a set that takes no gate,
+// grow their sets greedily.
 void plan_program(const std::vector<IGate>& gates, int n_qubits, int tile_qubits,
                   int low_qubits, int row_bits, Program& prog, int max_gates = 0,
-                  bool lane_swaps = false);
+                  bool lane_swaps = false, const std::vector<uint64_t>* locals = nullptr);
+// Local qubit sets of the passes by search: from each pass's greedy set,
+// `iters` random exchanges of one non-low local qubit with another qubit,
+// kept when the pass then takes at least as many non-diagonal gates (the
+// greedy rule fixes the first qubits it meets, which can cost whole passes
+// later).  Deterministic for a seed.
+void search_local_sets(const std::vector<IGate>& gates, int n_qubits, int tile_qubits,
+                       int low_qubits, int max_gates, int iters, uint64_t seed,
+                       std::vector<uint64_t>& locals);
 // whether a program has stages that continue through lane exchanges
 bool program_has_swaps(const Program& prog);
 // a pass's HBM order (io & 63: the tile order of PassDev::tord; io & 64:
@@ -92,6 +103,7 @@ struct Fusion {
   int tile = 0, low = 0, cap = 0;
   int swaps = 0;  // lane exchanges in compiled passes
   std::vector<uint8_t> io;  // per-pass HBM order (set_pass_io), empty: default
+  std::vector<uint64_t> locals;  // per-pass local qubit sets, empty: greedy
 };
 // 64-bit hash of a lowered gate list's structure (kinds and qubits, not
 // the matrices)

@@ -16,6 +16,7 @@
 // qubits; angles may differ), state size, dtype and batch use it.
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <sstream>
 
@@ -23,6 +24,9 @@
 #include "smoq_internal.h"
 
 namespace smoq {
+
+// exchanges tried per pass when searching its local qubit set
+constexpr int kSearchIters = 400;
 
 uint64_t gate_structure_hash(const std::vector<IGate>& gs) {
   uint64_t h = 0xcbf29ce484222325ull;
@@ -96,8 +100,16 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
     if (k > s->n || m > k - 2) return;
     w = w && compiled;  // lane exchanges exist in compiled passes only
     for (const Fusion& f : cand)
-      if (f.tile == k && f.low == m && f.cap == c && f.swaps == w) return;
-    cand.push_back({k, m, c, w});
+      if (f.tile == k && f.low == m && f.cap == c && f.swaps == w && f.locals.empty()) return;
+    cand.push_back({k, m, c, w, {}, {}});
+  };
+  // the same settings with searched local qubit sets (fewer passes where
+  // the greedy sets cost some)
+  auto add_searched = [&](int k, int m, int c, int w) {
+    if (k >= s->n || m > k - 2) return;
+    Fusion f{k, m, c, w && compiled, {}, {}};
+    search_local_sets(gs, s->n, k, m, c, kSearchIters, 1, f.locals);
+    cand.push_back(f);
   };
   for (int k : {12, 11}) {
     if (k < s->n && ((uint64_t)s->batch << (s->n - k)) < 256) continue;
@@ -105,6 +117,7 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
       for (int c : {0, 28, 36, 44}) add(12, 4, c, 1);
       for (int c : {0, 36}) add(12, 4, c, 0);
       for (int c : {0, 36}) add(12, 5, c, 1);
+      for (int c : {0, 30, 36, 44}) add_searched(12, 4, c, 1);
     } else {
       for (int c : {0, 36}) add(11, 4, c, 1);
       add(11, 5, 0, 1);
@@ -117,7 +130,7 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
   std::vector<size_t> compile_idx;
   for (size_t i = 0; i < cand.size(); ++i) {
     plan_program(gs, s->n, cand[i].tile, cand[i].low, rowbits, progs[i], cand[i].cap,
-                 cand[i].swaps != 0);
+                 cand[i].swaps != 0, cand[i].locals.empty() ? nullptr : &cand[i].locals);
     if (jit_wanted(ctx, progs[i], s->n)) {
       if (fixed) factor_rotations(progs[i], s->dtype);
       use_jit[i] = 1;
@@ -165,6 +178,15 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
   size_t best = 0;
   for (size_t i = 1; i < cand.size(); ++i)
     if (ms[i] < ms[best]) best = i;
+  // SMOQ_TUNE_LOG=1: every candidate's settings and time on stderr
+  const char* log = std::getenv("SMOQ_TUNE_LOG");
+  if (log && std::atoi(log) != 0)
+    for (size_t i = 0; i < cand.size(); ++i)
+      std::fprintf(stderr,
+                   "smoq_autotune: tile 2^%d low %d cap %d swaps %d %s: %zu passes %.3f ms%s\n",
+                   cand[i].tile, cand[i].low, cand[i].cap, cand[i].swaps,
+                   cand[i].locals.empty() ? "greedy" : "searched", progs[i].passes.size(), ms[i],
+                   i == best ? " (best)" : "");
   // Per pass, the HBM order of its tiles and of the registers that move
   // them (set_pass_io): they change no result, but the time of a pass
   // depends on which address bits its concurrently running tiles span, by
@@ -232,6 +254,7 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
   out->low_qubits = cand[best].low;
   out->pass_max_gates = cand[best].cap;
   out->lane_swaps = cand[best].swaps;
+  out->searched = !cand[best].locals.empty();
   out->n_passes = (int)progs[best].passes.size();
   out->ms = ms[best];
   out->default_ms = ms[0];
@@ -247,10 +270,15 @@ extern "C" int smoq_tune_save(smoq_ctx* ctx, const char* path) {
   {
     f << kv.first << ' ' << kv.second.tile << ' ' << kv.second.low << ' ' << kv.second.cap
       << ' ' << kv.second.swaps;
-    // per-pass HBM orders, when tuned
+    // per-pass HBM orders and searched local sets, when there are any
     if (!kv.second.io.empty()) {
-      f << ' ' << kv.second.io.size();
+      f << " io " << kv.second.io.size();
       for (uint8_t c : kv.second.io) f << ' ' << (int)c;
+    }
+    if (!kv.second.locals.empty()) {
+      f << " loc " << kv.second.locals.size() << std::hex;
+      for (uint64_t l : kv.second.locals) f << ' ' << l;
+      f << std::dec;
     }
     f << '\n';
   }
@@ -273,17 +301,24 @@ extern "C" int smoq_tune_load(smoq_ctx* ctx, const char* path, int* n_entries) {
     if (arch != "gfx950" || fu.tile < 6 || fu.tile > kMaxTileQubits || fu.low < 3 || fu.low > 6 ||
         fu.cap < 0)
       continue;
-    size_t nio = 0;
-    if (is >> nio) {
-      if (nio > 4096) continue;
-      bool ok = true;
-      for (size_t p = 0; p < nio && ok; ++p) {
-        int c = -1;
-        ok = (bool)(is >> c) && c >= 0 && c < 128;
-        fu.io.push_back((uint8_t)c);
+    bool ok = true;
+    std::string tag;
+    while (ok && (is >> tag)) {
+      size_t cnt = 0;
+      ok = (bool)(is >> cnt) && cnt <= 4096 && (tag == "io" || tag == "loc");
+      for (size_t p = 0; p < cnt && ok; ++p) {
+        if (tag == "io") {
+          int c = -1;
+          ok = (bool)(is >> c) && c >= 0 && c < 128;
+          fu.io.push_back((uint8_t)c);
+        } else {
+          uint64_t l = 0;
+          ok = (bool)(is >> std::hex >> l >> std::dec) && l != 0 && (n >= 64 || (l >> n) == 0);
+          fu.locals.push_back(l);
+        }
       }
-      if (!ok) continue;
     }
+    if (!ok) continue;
     ctx->tune[arch + " " + std::to_string(n) + " " + std::to_string(dtype) + " " +
               std::to_string(batch) + " " + hash] = fu;
     ++count;

@@ -99,6 +99,7 @@ class smoq_tune_result(ctypes.Structure):
         ("low_qubits", ctypes.c_int),
         ("pass_max_gates", ctypes.c_int),
         ("lane_swaps", ctypes.c_int),
+        ("searched", ctypes.c_int),
         ("n_passes", ctypes.c_int),
         ("candidates", ctypes.c_int),
         ("ms", ctypes.c_double),

@@ -38,7 +38,7 @@ def test_autotune_choice_keeps_results(ctx, tmp_path):
     assert res["ms"] > 0 and res["ms"] <= res["default_ms"]
     assert res["tile_qubits"] in (11, 12) and res["low_qubits"] in (4, 5)
     assert res["pass_max_gates"] in (0, 28, 36, 44)
-    assert res["lane_swaps"] in (0, 1)
+    assert res["lane_swaps"] in (0, 1) and res["searched"] in (0, 1)
     s.set_basis(0)
     n_pass = tape.apply(s)
     assert n_pass == res["n_passes"]
@@ -104,12 +104,17 @@ def test_pass_hbm_orders_keep_results(tmp_path):
     s.free()
     c.close()
     line = path.read_text().split("\n")[0].split()
-    base = line[:9]  # arch n dtype batch hash tile low cap swaps
+    # arch n dtype batch hash tile low cap swaps, then the tuned local sets
+    # when the search won (kept), without the tuned orders
+    base = line[:9]
+    if "loc" in line:
+        i = line.index("loc")
+        base += line[i:i + 2 + int(line[i + 1])]
     n_pass = res["n_passes"]
     # rotations, reversals (32) and registers above the low qubits (64)
     codes = [3, 32 | 5, 64, 64 | 32 | 7, 6, 32, 64 | 2, 1]
     io = [codes[p % len(codes)] for p in range(n_pass)]
-    path.write_text(" ".join(base + [str(n_pass)] + [str(x) for x in io]) + "\n")
+    path.write_text(" ".join(base + ["io", str(n_pass)] + [str(x) for x in io]) + "\n")
     c2 = Context(0)
     assert c2.tune_load(path) == 1
     s2 = GPUState.basis(N, 0, ctx=c2)
