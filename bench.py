@@ -667,7 +667,7 @@ def shadow_snapshots():
 
 
 def trotter_xxz(N=28, steps=1000, dt=0.01, obs_every=10, J=1.0, delta=0.5, seed=28,
-                t_cheb=10.0, timed_steps=20):
+                t_cheb=10.0, timed_steps=20, tune=True):
     """Second-order Trotter evolution of the open XXZ chain (J = 1, Δ = 0.5,
     h_i uniform in [-1, 1] from the seeded stream) from the Néel state:
     `steps` steps of dt in complex128 and in complex64, the staggered
@@ -696,9 +696,17 @@ def trotter_xxz(N=28, steps=1000, dt=0.01, obs_every=10, J=1.0, delta=0.5, seed=
            "gates_per_step": len(seq)}
     recs, finals = {}, {}
     for dtype in ("c128", "c64"):
-        plan = tape.compile(dtype)
         psi = GPUState.basis(N, neel, dtype)
         S = psi.nbytes
+        tuned = None
+        if tune:
+            # the autotuner's choice for the step's structure (untimed); the
+            # plan below is built with it
+            t0 = time.perf_counter()
+            tuned = ctx.autotune(tape, psi)
+            tuned["seconds"] = time.perf_counter() - t0
+            psi.set_basis(neel)
+        plan = tape.compile(dtype)
         # per-launch roofline of the step's passes (events, no graph)
         ctx.pass_timing(True)
         plan.run(psi, timed_steps)
@@ -716,6 +724,7 @@ def trotter_xxz(N=28, steps=1000, dt=0.01, obs_every=10, J=1.0, delta=0.5, seed=
         passes = plan.passes
         out[dtype] = {
             "seconds": t, "ms_per_step": t * 1e3 / steps, "passes_per_step": passes,
+            "fusion": tuned if tuned else "default",
             "gate_ops_per_s": len(seq) * steps / t,
             "hbm_GBps_whole_run": passes * 2.0 * S * steps / t / 1e9,
             "pass_roofline": {"mean_launch_ms": pass_ms, "launches": launches,
@@ -844,7 +853,7 @@ def main():
         print(json.dumps({"reference_workloads": reference_benchmarks()}), flush=True)
         return
     if args.trotter_only:
-        print(json.dumps({"trotter_xxz": trotter_xxz()}), flush=True)
+        print(json.dumps({"trotter_xxz": trotter_xxz(tune=not args.no_tune)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_vs_parameter_shift()}), flush=True)
@@ -970,6 +979,13 @@ def main():
         # complex64 sweep of the same circuit
         psi.free()
         p64 = GPUState(N, "c64")
+        tuned64 = None
+        if not args.no_tune and not (args.tile or args.low):
+            t0 = time.perf_counter()
+            tuned64 = ctx.autotune(tape, p64)
+            tuned64["seconds"] = time.perf_counter() - t0
+            log(f"c64 autotune: {tuned64['n_passes']} passes, {tuned64['ms']:.1f} ms vs "
+                f"{tuned64['default_ms']:.1f} ms default")
         p64.set_basis(0)
         tape.apply(p64)
         ctx.sync()
@@ -990,6 +1006,7 @@ def main():
                          "first_indices": [int(v) for v in idx[:4]],
                          "oracle_check": samp_check},
             "c64_sweep": {"ms_per_circuit": t64 * 1e3, "gate_ops_per_s": n_gates / t64,
+                          "fusion": tuned64 if tuned64 else "default",
                           "max_abs_err_Z_ZZ_vs_c128": float(np.max(np.abs(obs64 - obs128)))},
         }
     else:
@@ -1012,7 +1029,7 @@ def main():
             adj["full_check"] = adjoint_full_check()
     if not args.no_trotter and not args.no_extras:
         log("second-order Trotter XXZ N=28 ...")
-        result.setdefault("extras", {})["trotter_xxz"] = trotter_xxz()
+        result.setdefault("extras", {})["trotter_xxz"] = trotter_xxz(tune=not args.no_tune)
     if not args.no_mcwf and not args.no_extras:
         log("batched MCWF ...")
         mc = mcwf_batch()

@@ -137,6 +137,47 @@ __device__ __forceinline__ VT<T> u2_row(const double* m, int r, VT<T> a0, VT<T> 
   return cadd(acc, cmul(ld_c<T>(m, 4 * r + 3), a3));
 }
 
+// row r of a 4x4 with the exactly-zero entries of Z left out
+template <typename T, uint32_t Z>
+__device__ __forceinline__ VT<T> u2_row_z(const double* m, int r, VT<T> a0, VT<T> a1, VT<T> a2,
+                                          VT<T> a3) {
+  const VT<T> in[4] = {a0, a1, a2, a3};
+  VT<T> acc = make_c<T>(0.0, 0.0);
+  bool first = true;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if ((Z >> (4 * r + c)) & 1u) continue;
+    const VT<T> p = cmul(ld_c<T>(m, 4 * r + c), in[c]);
+    acc = first ? p : cadd(acc, p);
+    first = false;
+  }
+  return acc;
+}
+
+// 4x4 on slots (A low role, B high role) whose zero entries Z are known
+// when the kernel is compiled (a two-qubit XXZ bond conserves the number of
+// ones: six entries of sixteen)
+template <typename T, int A, int B, uint32_t Z>
+__device__ __forceinline__ void reg_u2z(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
+  if constexpr (A >= kRegQubits || B >= kRegQubits) return;
+  const int creg = g.creg;
+#pragma unroll
+  for (int j = 0; j < kNR; ++j) {
+    if (j & ((1 << A) | (1 << B))) continue;
+    const int id0 = j, id1 = j | (1 << A), id2 = j | (1 << B), id3 = j | (1 << A) | (1 << B);
+    const VT<T> in0 = x[id0], in1 = x[id1], in2 = x[id2], in3 = x[id3];
+    const bool on = gon && ((j & creg) == creg);
+    const VT<T> o0 = u2_row_z<T, Z>(g.m, 0, in0, in1, in2, in3);
+    const VT<T> o1 = u2_row_z<T, Z>(g.m, 1, in0, in1, in2, in3);
+    const VT<T> o2 = u2_row_z<T, Z>(g.m, 2, in0, in1, in2, in3);
+    const VT<T> o3 = u2_row_z<T, Z>(g.m, 3, in0, in1, in2, in3);
+    x[id0] = csel(on, o0, in0);
+    x[id1] = csel(on, o1, in1);
+    x[id2] = csel(on, o2, in2);
+    x[id3] = csel(on, o3, in3);
+  }
+}
+
 // 4x4 on slots (A low role, B high role)
 template <typename T, int A, int B>
 __device__ __forceinline__ void reg_u2(VT<T> (&x)[kNR], const GateDev& g, bool gon) {

@@ -87,6 +87,10 @@ struct GateDev {
   // register slots (slot mask) and outside them (global mask)
   int32_t osel, oslot, ocnt, ozreg;
   uint64_t ozglob;
+  // GK_U2 in run-time compiled passes of fixed-angle circuits: bit 4r + c
+  // set when entry (r, c) is exactly zero (its product is left out)
+  uint32_t zero;
+  int32_t pad2;
   double m[32];
 };
 

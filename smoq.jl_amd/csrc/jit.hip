@@ -157,6 +157,17 @@ void factor_rotations(Program& prog, int dtype, std::vector<FactorUndo>* undo) {
           phase *= p0;
           continue;
         }
+        if (G.kind == GK_U2) {
+          // the exactly-zero entries become structure
+          uint32_t z = 0;
+          for (int e = 0; e < 16; ++e)
+            if (G.m[2 * e] == 0.0 && G.m[2 * e + 1] == 0.0) z |= 1u << e;
+          if (z) {
+            if (undo) undo->push_back({g, G.kind, G.b, G.m[0], {}});
+            G.zero = z;
+          }
+          continue;
+        }
         if (G.creg != 0 || G.cglob != 0) continue;
         const double* m = G.m;
         double a, b;
@@ -196,6 +207,7 @@ void unfactor_rotations(Program& prog, const std::vector<FactorUndo>& undo) {
   for (const FactorUndo& u : undo) {
     GateDev& G = prog.gates[u.g];
     if (G.kind == GK_DIAGREG1) std::memcpy(G.m, u.md, sizeof u.md);
+    G.zero = 0;
     G.kind = u.kind;
     G.b = u.b;
     G.m[0] = u.m0;
@@ -306,8 +318,12 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
                 << gd << ", (sf.gl & " << gd << ".cglob) == " << gd << ".cglob);\n";
             break;
           case GK_U2:
-            o << "    reg_u2<double, " << G.a << ", " << G.b << ">(x, " << gd << ", (sf.gl & " << gd
-              << ".cglob) == " << gd << ".cglob);\n";
+            if (G.zero)
+              o << "    reg_u2z<double, " << G.a << ", " << G.b << ", " << G.zero << "u>(x, " << gd
+                << ", (sf.gl & " << gd << ".cglob) == " << gd << ".cglob);\n";
+            else
+              o << "    reg_u2<double, " << G.a << ", " << G.b << ">(x, " << gd << ", (sf.gl & "
+                << gd << ".cglob) == " << gd << ".cglob);\n";
             break;
           case GK_DIAGTAB:
             o << "    reg_diagtab<double, " << G.nu << ">(x, " << gd << ", sf.gl, tabs);\n";

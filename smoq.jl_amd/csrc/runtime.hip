@@ -168,17 +168,38 @@ int lower_op(const smoq_op& op, double theta, int n, std::vector<IGate>& out) {
   switch (op.gate) {
     case SMOQ_G_U1:
       if ((rc = need(1))) return rc;
+      if (op.m[2] == 0.0 && op.m[3] == 0.0 && op.m[4] == 0.0 && op.m[5] == 0.0) {
+        // a diagonal matrix is a phase gate: it needs no register slot
+        const double ph[4] = {op.m[0], op.m[1], op.m[6], op.m[7]};
+        set_diag(g, 1, op.q, ph);
+        break;
+      }
       g.kind = IK_U1;
       g.t0 = op.q[0];
       std::memcpy(g.m, op.m, sizeof(double) * 8);
       break;
-    case SMOQ_G_U2:
+    case SMOQ_G_U2: {
       if ((rc = need(2))) return rc;
+      bool diag = true;
+      for (int r = 0; r < 4; ++r)
+        for (int c2 = 0; c2 < 4; ++c2)
+          if (r != c2 && (op.m[2 * (4 * r + c2)] != 0.0 || op.m[2 * (4 * r + c2) + 1] != 0.0))
+            diag = false;
+      if (diag) {
+        double ph[8];
+        for (int e = 0; e < 4; ++e) {
+          ph[2 * e] = op.m[2 * (5 * e)];
+          ph[2 * e + 1] = op.m[2 * (5 * e) + 1];
+        }
+        set_diag(g, 2, op.q, ph);
+        break;
+      }
       g.kind = IK_U2;
       g.t0 = op.q[0];
       g.t1 = op.q[1];
       std::memcpy(g.m, op.m, sizeof(double) * 32);
       break;
+    }
     case SMOQ_G_H:
       if ((rc = need(1))) return rc;
       set_u1(g, op.q[0], r2, 0, r2, 0, r2, 0, -r2, 0);

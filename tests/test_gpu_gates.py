@@ -306,3 +306,72 @@ def test_product_states_and_initial_psi(ctx):
     s.free()
     with pytest.raises(ValueError):
         make_product_ket(["zero", "up"])
+
+
+def structured_tape(N, n_ops, seed):
+    """Rotations, CZs and matrices with exact zeros: diagonal 2x2 and 4x4
+    (phase gates, lowered without a register slot), number-conserving 4x4
+    (an XXZ bond: phases on |00>, |11> and a block on |01>, |10>) and dense
+    4x4."""
+    from scipy.linalg import expm
+
+    from smoq_amd.circuits import Tape
+
+    rng = np.random.default_rng(seed)
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sy = np.array([[0, -1j], [1j, 0]], complex)
+    sz = np.diag([1.0, -1.0]).astype(complex)
+    t = Tape(N)
+    for _ in range(n_ops):
+        k = int(rng.integers(6))
+        if k == 0:
+            t.add("U1", [int(rng.integers(N))],
+                  m=np.diag(np.exp(1j * rng.uniform(0, 6.3, 2))))
+        elif k == 1:
+            q = [int(x) for x in rng.choice(N, 2, replace=False)]
+            t.add("U2", q, m=np.diag(np.exp(1j * rng.uniform(0, 6.3, 4))))
+        elif k == 2:
+            q = [int(x) for x in rng.choice(N, 2, replace=False)]
+            H = np.kron(sx, sx) + np.kron(sy, sy) + rng.uniform(-1, 1) * np.kron(sz, sz)
+            t.add("U2", q, m=expm(-1j * rng.uniform(0.01, 0.5) * H))
+        elif k == 3:
+            q = [int(x) for x in rng.choice(N, 2, replace=False)]
+            t.add("U2", q, m=np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))[0])
+        elif k == 4:
+            t.add(str(rng.choice(["RX", "RY", "RZ"])), [int(rng.integers(N))],
+                  float(rng.uniform(0, 2 * math.pi)))
+        else:
+            q = [int(x) for x in rng.choice(N, 2, replace=False)]
+            t.add("CZ", q)
+    return t
+
+
+@pytest.mark.parametrize("jit", [0, 2])
+@pytest.mark.parametrize("N,tile", [(14, 10), (16, 12)])
+def test_structured_matrices_in_fused_passes(ctx, N, tile, jit):
+    """Gates with exactly-zero entries (phase gates given as matrices, XXZ
+    bonds): lowered and compiled with those zeros as structure, the result
+    is the oracle's, in complex128 and complex64."""
+    from smoq_amd import GPUState
+
+    ctx.set_fusion(tile, 4)
+    ctx.set_jit(jit)
+    try:
+        tape = structured_tape(N, 160, seed=5 * N + tile + jit)
+        psi = rand_state(N, N + 3)
+        ref = oracle_run_tape(tape, psi.copy())
+        s = GPUState.from_numpy(psi)
+        tape.apply(s)
+        assert rel(s.to_numpy(), ref) < TOL128
+        s64 = GPUState.from_numpy(psi.astype(np.complex64))
+        tape.apply(s64)
+        assert rel(s64.to_numpy(), ref) < 1e-5
+        # a compiled plan (fixed angles: zero entries as structure)
+        plan = tape.compile()
+        s.upload(psi)
+        plan.run(s)
+        assert rel(s.to_numpy(), ref) < TOL128
+        plan.free()
+    finally:
+        ctx.set_fusion(-1, 4)
+        ctx.set_jit(1)
