@@ -225,6 +225,13 @@ __device__ __forceinline__ void czset_apply(VT<T> (&x)[kNR], uint32_t neg) {
   }
 }
 
+// XOR of the sign bits of both components of a register amplitude with s
+// (0 or 0x80000000)
+__device__ __forceinline__ void czsign(VT<double>& a, uint32_t s) {
+  a.x = __hiloint2double(__double2hiint(a.x) ^ (int)s, __double2loint(a.x));
+  a.y = __hiloint2double(__double2hiint(a.y) ^ (int)s, __double2loint(a.y));
+}
+
 // merged CZs: the lane builds a 16-bit mask of the slot patterns to negate
 // (lane-lane pairs, register-register pairs, register-lane pairs)
 template <typename T>
@@ -550,6 +557,20 @@ __device__ __forceinline__ void stage_remap(uint32_t sl, uint64_t gl, const uint
                                             StageFrame& sf) {
   sf.sl = sl * (uint32_t)sizeof(VT<T>);
   sf.gl = gl;
+#pragma unroll
+  for (int i = 0; i < kNR; ++i) sf.sr[i] = sr[i] * (uint32_t)sizeof(VT<T>);
+}
+
+// The mapping after a lane exchange that moved thread bit LB to another
+// local position: the lane part and the global index change by that bit's
+// images dsw (swizzled LDS index) and dg (global bits); the register
+// patterns' offsets are the new mapping's constants.
+template <typename T>
+__device__ __forceinline__ void stage_swap_remap(uint32_t tid, int lb, uint32_t dsw, uint64_t dg,
+                                                 const uint32_t (&sr)[kNR], StageFrame& sf) {
+  const bool b = (tid >> lb) & 1u;
+  sf.sl ^= b ? dsw * (uint32_t)sizeof(VT<T>) : 0u;
+  sf.gl ^= b ? dg : 0ull;
 #pragma unroll
   for (int i = 0; i < kNR; ++i) sf.sr[i] = sr[i] * (uint32_t)sizeof(VT<T>);
 }

@@ -291,17 +291,30 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
         bit_map_emit(o, gpos, true);
         o << ", sr, x, sf);\n    }\n";
       } else {
-        // the swizzled lane part of the new mapping: swz is linear, so the
-        // thread's part is the XOR of its bits' images
-        o << "};\n      stage_remap<T>(";
-        std::vector<uint32_t> sw;
-        for (int i = 0; i < st.ntb; ++i) sw.push_back(st.sw_tid[i]);
-        o << "0u";
-        for (int i = 0; i < st.ntb; ++i)
-          o << " ^ ((tid >> " << i << ") & 1u ? " << sw[i] << "u : 0u)";
-        o << ", f.gbase | ";
-        bit_map_emit(o, gpos, true);
-        o << ", sr, sf);\n    }\n";
+        // after a lane exchange only thread bit swap_bit maps to another
+        // local position: the lane part and the global index change by that
+        // bit's images (swz is linear over GF(2)), the register offsets are
+        // constants
+        const StageDev& pv = prog.stages[s - 1];
+        const int lb = st.swap_bit;
+        bool one_bit = pv.ntb == st.ntb;
+        for (int i = 0; i < st.ntb && one_bit; ++i)
+          if (i != lb) one_bit = pv.sw_tid[i] == st.sw_tid[i] && pv.g_tid[i] == st.g_tid[i];
+        if (one_bit) {
+          o << "};\n      stage_swap_remap<T>(tid, " << lb << ", "
+            << (pv.sw_tid[lb] ^ st.sw_tid[lb]) << "u, 0x" << std::hex
+            << (pv.g_tid[lb] ^ st.g_tid[lb]) << std::dec << "ull, sr, sf);\n    }\n";
+        } else {
+          // the swizzled lane part of the new mapping: the XOR of its
+          // thread bits' images
+          o << "};\n      stage_remap<T>(";
+          o << "0u";
+          for (int i = 0; i < st.ntb; ++i)
+            o << " ^ ((tid >> " << i << ") & 1u ? " << st.sw_tid[i] << "u : 0u)";
+          o << ", f.gbase | ";
+          bit_map_emit(o, gpos, true);
+          o << ", sr, sf);\n    }\n";
+        }
       }
       for (int g = st.g0; g < st.g1; ++g) {
         const GateDev& G = prog.gates[g];
@@ -329,17 +342,39 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
             o << "    reg_diagtab<double, " << G.nu << ">(x, " << gd << ", sf.gl, tabs);\n";
             break;
           case GK_CZSET: {
-            // the CZ pattern is structure: shifts and masks become immediates
-            o << "    {\n      int par = 0;\n";
-            for (int i = 0; i < G.ncz; ++i)
-              o << "      par ^= __popcll(sf.gl & (sf.gl >> " << G.czs[i] << ") & 0x" << std::hex
-                << G.czm[i] << std::dec << "ull);\n";
-            o << "      uint32_t neg = ((par & 1) ? kAllPatterns : 0u) ^ " << G.czreg << "u;\n";
+            // The CZ pattern is structure.  Register pattern j is negated by
+            // the parity of the lane-lane pairs (sign word sp), by each slot
+            // t set in j whose lane partners have odd parity (sc<t>) and by
+            // the register-register pairs (a constant): only the patterns a
+            // per-lane term reaches get an XOR of the sign bits, constant
+            // negations are left to the compiler.
+            o << "    {\n";
+            if (G.ncz > 0) {
+              o << "      int par = 0;\n";
+              for (int i = 0; i < G.ncz; ++i)
+                o << "      par ^= __popcll(sf.gl & (sf.gl >> " << G.czs[i] << ") & 0x"
+                  << std::hex << G.czm[i] << std::dec << "ull);\n";
+              o << "      const uint32_t sp = (uint32_t)(par & 1) << 31;\n";
+            }
             for (int t = 0; t < kRegQubits; ++t)
               if (G.czcm[t])
-                o << "      neg ^= (__popcll(sf.gl & 0x" << std::hex << G.czcm[t] << std::dec
-                  << "ull) & 1) ? slot_patterns(" << t << ") : 0u;\n";
-            o << "      czset_apply<double>(x, neg);\n    }\n";
+                o << "      const uint32_t sc" << t << " = (uint32_t)(__popcll(sf.gl & 0x"
+                  << std::hex << G.czcm[t] << std::dec << "ull) & 1) << 31;\n";
+            for (int j = 0; j < (1 << kRegQubits); ++j) {
+              std::string terms;
+              if (G.ncz > 0) terms = "sp";
+              for (int t = 0; t < kRegQubits; ++t)
+                if (G.czcm[t] && (j >> t & 1)) terms += (terms.empty() ? "sc" : " ^ sc") + std::to_string(t);
+              const bool cneg = (G.czreg >> j) & 1;
+              if (terms.empty()) {
+                if (cneg) o << "      x[" << j << "].x = -x[" << j << "].x;\n      x[" << j
+                            << "].y = -x[" << j << "].y;\n";
+              } else {
+                o << "      czsign(x[" << j << "], " << terms << (cneg ? " ^ 0x80000000u" : "")
+                  << ");\n";
+              }
+            }
+            o << "    }\n";
             break;
           }
           case GK_DIAGREG: o << "    reg_diagreg<double>(x, " << gd << ");\n"; break;
