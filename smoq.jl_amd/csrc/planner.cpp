@@ -632,8 +632,7 @@ void search_local_sets(const std::vector<IGate>& gates, int n, int tile_qubits, 
   if (k >= n) return;  // one tile holds the state: nothing to choose
   const int m = std::min(low_qubits, k - 2);
   const uint64_t low = (1ull << m) - 1;
-  std::vector<int> remaining(gates.size()), taken, deferred;
-  for (size_t i = 0; i < gates.size(); ++i) remaining[i] = (int)i;
+  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
   uint64_t rng = seed * 0x9E3779B97F4A7C15ull + 1;
   auto next = [&]() {
     rng ^= rng << 13;
@@ -647,29 +646,73 @@ void search_local_sets(const std::vector<IGate>& gates, int n, int tile_qubits, 
       if ((mask >> b & 1) && c-- == 0) return b;
     return -1;
   };
-  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1);
-  while (!remaining.empty()) {
+  std::vector<int> taken, deferred;
+  // one pass's set: the greedy set, then random exchanges of one non-low
+  // local qubit with another qubit, kept when the pass takes at least as
+  // many non-diagonal gates
+  auto one_set = [&](const std::vector<int>& rem) {
     taken.clear();
     deferred.clear();
-    uint64_t cur = take_gates(gates, remaining, low, true, k, max_gates, taken, deferred);
+    uint64_t cur = take_gates(gates, rem, low, true, k, max_gates, taken, deferred);
     for (int b = 0; b < n && popc(cur) < k; ++b) cur |= 1ull << b;
-    int best = count_taken(gates, remaining, cur, max_gates);
-    // random exchanges of one non-low local qubit with one other qubit,
-    // kept when the pass takes at least as many non-diagonal gates
+    int best = count_taken(gates, rem, cur, max_gates);
     for (int it = 0; it < iters && (all & ~cur); ++it) {
       const int out = pick(cur & ~low), in = pick(all & ~cur);
       const uint64_t cand = (cur & ~(1ull << out)) | (1ull << in);
-      const int sc = count_taken(gates, remaining, cand, max_gates);
+      const int sc = count_taken(gates, rem, cand, max_gates);
       if (sc >= best) {
         best = sc;
         cur = cand;
       }
     }
-    taken.clear();
-    deferred.clear();
-    take_gates(gates, remaining, cur, false, k, max_gates, taken, deferred);
-    locals.push_back(cur);
-    remaining.swap(deferred);
+    return cur;
+  };
+  // Beam search over the passes: each kept plan prefix tries kBeamTries
+  // searched sets for its next pass; the kBeamWidth prefixes that leave the
+  // fewest gates (distinct remainders) go on.
+  constexpr int kBeamWidth = 4, kBeamTries = 6;
+  struct Prefix {
+    std::vector<int> rem;
+    std::vector<uint64_t> sets;
+  };
+  std::vector<Prefix> beam(1);
+  beam[0].rem.resize(gates.size());
+  for (size_t i = 0; i < gates.size(); ++i) beam[0].rem[i] = (int)i;
+  while (true) {
+    for (const Prefix& p : beam)
+      if (p.rem.empty()) {
+        locals = p.sets;
+        return;
+      }
+    std::vector<Prefix> next_beam;
+    for (const Prefix& p : beam) {
+      std::vector<uint64_t> tried;
+      for (int t = 0; t < kBeamTries; ++t) {
+        const uint64_t L = one_set(p.rem);
+        if (std::find(tried.begin(), tried.end(), L) != tried.end()) continue;
+        tried.push_back(L);
+        taken.clear();
+        deferred.clear();
+        take_gates(gates, p.rem, L, false, k, max_gates, taken, deferred);
+        if (taken.empty()) continue;
+        Prefix q;
+        q.rem = deferred;
+        q.sets = p.sets;
+        q.sets.push_back(L);
+        next_beam.push_back(std::move(q));
+      }
+    }
+    std::stable_sort(next_beam.begin(), next_beam.end(),
+                     [](const Prefix& a, const Prefix& b) { return a.rem.size() < b.rem.size(); });
+    beam.clear();
+    for (Prefix& q : next_beam) {
+      bool dup = false;
+      for (const Prefix& b : beam) dup |= b.rem == q.rem;
+      if (dup) continue;
+      beam.push_back(std::move(q));
+      if ((int)beam.size() == kBeamWidth) break;
+    }
+    if (beam.empty()) return;  // no progress (cannot happen: the greedy set takes a gate)
   }
 }
 

@@ -80,11 +80,12 @@ struct Program {
 void plan_program(const std::vector<IGate>& gates, int n_qubits, int tile_qubits,
                   int low_qubits, int row_bits, Program& prog, int max_gates = 0,
                   bool lane_swaps = false, const std::vector<uint64_t>* locals = nullptr);
-// Local qubit sets of the passes by search: from each pass's greedy set,
-// `iters` random exchanges of one non-low local qubit with another qubit,
-// kept when the pass then takes at least as many non-diagonal gates (the
-// greedy rule fixes the first qubits it meets, which can cost whole passes
-// later).  Deterministic for a seed.
+// Local qubit sets of the passes by search: a beam of plan prefixes, each
+// extended by sets found from the greedy set with `iters` random exchanges
+// of one non-low local qubit with another qubit (kept when the pass then
+// takes at least as many non-diagonal gates); the prefixes leaving the
+// fewest gates go on.  The greedy rule fixes the first qubits it meets,
+// which can cost whole passes later.  Deterministic for a seed.
 void search_local_sets(const std::vector<IGate>& gates, int n_qubits, int tile_qubits,
                        int low_qubits, int max_gates, int iters, uint64_t seed,
                        std::vector<uint64_t>& locals);

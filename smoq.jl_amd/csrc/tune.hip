@@ -26,7 +26,9 @@
 namespace smoq {
 
 // exchanges tried per pass when searching its local qubit set
-constexpr int kSearchIters = 400;
+constexpr int kSearchIters = 300;
+// searches (seeds) per candidate setting
+constexpr uint64_t kSearchSeeds = 3;
 
 uint64_t gate_structure_hash(const std::vector<IGate>& gs) {
   uint64_t h = 0xcbf29ce484222325ull;
@@ -108,7 +110,12 @@ extern "C" int smoq_autotune(smoq_state* s, const smoq_op* ops, int n_ops, const
   auto add_searched = [&](int k, int m, int c, int w) {
     if (k >= s->n || m > k - 2) return;
     Fusion f{k, m, c, w && compiled, {}, {}};
-    search_local_sets(gs, s->n, k, m, c, kSearchIters, 1, f.locals);
+    // a few seeds of the search; the fewest passes wins
+    std::vector<uint64_t> loc;
+    for (uint64_t seed = 1; seed <= kSearchSeeds; ++seed) {
+      search_local_sets(gs, s->n, k, m, c, kSearchIters, seed, loc);
+      if (f.locals.empty() || loc.size() < f.locals.size()) f.locals = loc;
+    }
     cand.push_back(f);
   };
   for (int k : {12, 11}) {
