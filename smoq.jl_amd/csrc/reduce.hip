@@ -148,28 +148,34 @@ static unsigned finish_blocks(int n_outputs) {
 // (N - 12) / 7 + 1 passes instead of one launch per x mask.
 constexpr int kObsTileBits = 12;
 constexpr int kObsLowBits = 4;
-constexpr int kObsMaxTerms = 128;  // terms of one pass
+constexpr int kObsMaxTerms = 32;  // terms of one pass: one register accumulator each
 constexpr int kObsBlockBits = 9;
 constexpr int kObsBlock = 1 << kObsBlockBits;  // 2 workgroups (4 waves per SIMD) per CU
-constexpr int kObsPairIters = (1 << (kObsTileBits - 1)) / kObsBlock;
+constexpr int kObsPairIters = (1 << (kObsTileBits - 1)) / kObsBlock;  // 4: bits 9, 10 of q
+constexpr int kObsJ = (1 << kObsTileBits) / kObsBlock;                // 8: bits 9..11 of l
 
 struct ObsTerm {
   uint64_t z;     // global z mask (base parity)
   uint32_t lz;    // z mask in local tile bits
   int32_t kind;   // -1 diagonal; else |x & z| mod 4 as in k_expect_pair
-};
-// Terms sharing an x mask.  Pair groups (lx != 0) list first the terms whose
-// sign is fixed within a tile (lz has no bit besides hl, so it is the base
-// parity alone: they need only the group's sums of Re w and Im w), then the
-// rest.  Diagonal groups (lx == 0) hold the terms with one value m of the z
-// bits above the low kObsBlockBits tile bits.
-struct ObsGroup {
-  uint32_t lx;    // x mask in local tile bits (0: diagonal group)
+  uint32_t lx;    // x mask in local tile bits (0: diagonal)
   int32_t hl;     // highest local bit of lx
-  int32_t t0;     // first term (index into the pass-ordered term array)
+  // the sign (-1)^{|l & lz|} of a thread's amplitude (pair: of its pair's
+  // first index) splits into the thread-index part (-1)^{|tid & la|},
+  // applied once after the last tile, and the part of its iteration bits
+  // (pair: q bits 9, 10 as lb; diagonal: l bits 9..11 as lb)
+  uint32_t la;
+  uint32_t lb;
+};
+// Terms sharing an x mask (lx != 0), or diagonal terms (lx == 0) sharing
+// the z bits m above the low kObsBlockBits tile bits (planning only).
+struct ObsGroup {
+  uint32_t lx;
+  int32_t hl;
+  int32_t t0;
   int32_t nt;
-  int32_t nconst; // pair groups: leading terms with a tile-constant sign
-  uint32_t m;     // diagonal groups: lz >> kObsBlockBits shared by the terms
+  int32_t nconst;
+  uint32_t m;
 };
 struct ObsPass {
   int k, lowb, n_cbits;
@@ -181,63 +187,25 @@ struct ObsPass {
 
 __device__ inline int parity64(uint64_t v) { return __popcll(v) & 1; }
 
-// Terms of a pair group whose sign varies within the tile, CH at a time.
-template <typename T, int CH>
-__device__ inline void obs_pair_varying(const typename CT<T>::V* tile, int k, uint64_t base,
-                                        const ObsGroup& G, const ObsTerm* terms,
-                                        int c, int nc, int local0,
-                                        double (*acc_s)[kObsMaxTerms]) {
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  uint32_t lz[CH];
-  int sgn0[CH], kd[CH];
-#pragma unroll
-  for (int j = 0; j < CH; ++j) {
-    const ObsTerm tm = terms[G.t0 + c + (j < nc ? j : 0)];
-    lz[j] = tm.lz;
-    kd[j] = tm.kind;
-    sgn0[j] = parity64(base & tm.z) ^ ((tm.kind == 1 || tm.kind == 2) ? 1 : 0);
-  }
-  double acc[CH];
-#pragma unroll
-  for (int j = 0; j < CH; ++j) acc[j] = 0.0;
-  auto add_pair = [&](uint32_t l, const typename CT<T>::V& a, const typename CT<T>::V& b) {
-    const double wr = (double)b.x * (double)a.x + (double)b.y * (double)a.y;
-    const double wi = (double)b.x * (double)a.y - (double)b.y * (double)a.x;
-#pragma unroll
-    for (int j = 0; j < CH; ++j) {
-      const double v = (kd[j] & 1) ? wi : wr;
-      acc[j] += ((__popc(l & lz[j]) ^ sgn0[j]) & 1) ? -v : v;
-    }
-  };
-  const uint32_t hl = (uint32_t)G.hl;
-#pragma unroll
-  for (int it = 0; it < kObsPairIters; ++it) {
-    const int p = tid + it * kObsBlock;
-    if (p >= (1 << (k - 1))) break;
-    const uint32_t l = ((uint32_t)p >> hl << (hl + 1)) | ((uint32_t)p & ((1u << hl) - 1u));
-    add_pair(l, tile[l], tile[l ^ G.lx]);
-  }
-#pragma unroll
-  for (int j = 0; j < CH; ++j) {
-    if (j >= nc) continue;
-    const double s = wave_total(acc[j]);
-    if (lane == 0) acc_s[wid][G.t0 + c + j - local0] += s;
-  }
-}
-
+// One pass of the tiled reduction.  Every workgroup walks its tiles of one
+// batch item; each thread keeps one FP64 accumulator per term of the pass
+// in registers (term s in acc[s]; the tile-invariant signs are factored
+// out), so a term costs a few operations per tile and one wave sum per
+// workgroup at the end.  Terms of one x mask are consecutive: their pair
+// products are computed once per tile.
 template <typename T>
-__global__ void __launch_bounds__(kObsBlock)
+__global__ void __launch_bounds__(kObsBlock, 4)
     k_expect_tiles(const typename CT<T>::V* __restrict__ psi, uint64_t dim, ObsPass P,
                    const ObsGroup* __restrict__ groups, const ObsTerm* __restrict__ terms,
                    double* __restrict__ partial, int part_terms) {
   using V = typename CT<T>::V;
-  constexpr int kJ = (1 << kObsTileBits) / kObsBlock;  // amplitudes per thread (diagonal)
+  (void)groups;
   __shared__ V tile[1 << kObsTileBits];
   __shared__ uint64_t dep_hi[1 << (kObsTileBits - kObsLowBits)];
   __shared__ double acc_s[kObsBlock / 64][kObsMaxTerms];
-  // this pass's descriptors, term indices relative to the pass
-  __shared__ ObsGroup s_groups[kObsMaxTerms];
-  __shared__ ObsTerm s_terms[kObsMaxTerms];
+  // the pass's term descriptors are read from global memory with uniform
+  // addresses (scalar loads)
+  const ObsTerm* __restrict__ s_terms = terms + P.term0;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const uint64_t item = blockIdx.y;
   const V* p = psi + item * dim;
@@ -249,108 +217,116 @@ __global__ void __launch_bounds__(kObsBlock)
       if ((h >> b) & 1) off |= 1ull << P.tpos[lowb + b];
     dep_hi[h] = off;
   }
-  for (int t = tid; t < (kObsBlock / 64) * kObsMaxTerms; t += kObsBlock)
-    (&acc_s[0][0])[t] = 0.0;
-  for (int g = tid; g < P.n_groups; g += kObsBlock) {
-    ObsGroup G = groups[P.group0 + g];
-    G.t0 -= P.term0;
-    s_groups[g] = G;
-  }
-  for (int t = tid; t < P.n_terms; t += kObsBlock) s_terms[t] = terms[P.term0 + t];
+  double acc[kObsMaxTerms];
+#pragma unroll
+  for (int t = 0; t < kObsMaxTerms; ++t) acc[t] = 0.0;
   __syncthreads();
+  const int nterms = P.n_terms;
   const uint64_t n_tiles = dim >> k;
   const int lowmask = (1 << lowb) - 1;
-  const int n_groups = P.n_groups;
   for (uint64_t tt = blockIdx.x; tt < n_tiles; tt += gridDim.x) {
     uint64_t base = 0;
     for (int b = 0; b < P.n_cbits; ++b)
       if ((tt >> b) & 1) base |= 1ull << P.cpos[b];
-    for (int l = tid; l < ntile; l += kObsBlock)
-      tile[l] = p[base | (uint64_t)(l & lowmask) | dep_hi[l >> lowb]];
-    __syncthreads();
-    // probabilities of this thread's amplitudes l = tid + kObsBlock j
-    double pr[kJ];
+    // the thread's loads (in two halves: the registers of all eight would
+    // not fit beside the accumulators), the LDS stores and the plain sum of
+    // its |psi_l|^2, l = tid + kObsBlock j
+    double pr0 = 0.0;
 #pragma unroll
-    for (int j = 0; j < kJ; ++j) {
-      const int l = tid + kObsBlock * j;
-      pr[j] = 0.0;
-      if (l < ntile) {
-        const V a = tile[l];
-        pr[j] = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+    for (int h = 0; h < kObsJ; h += kObsJ / 2) {
+      V v[kObsJ / 2];
+#pragma unroll
+      for (int j = 0; j < kObsJ / 2; ++j) {
+        const int l = tid + kObsBlock * (h + j);
+        if (l < ntile) v[j] = p[base | (uint64_t)(l & lowmask) | dep_hi[l >> lowb]];
+      }
+#pragma unroll
+      for (int j = 0; j < kObsJ / 2; ++j) {
+        const int l = tid + kObsBlock * (h + j);
+        if (l < ntile) {
+          tile[l] = v[j];
+          pr0 += (double)v[j].x * (double)v[j].x + (double)v[j].y * (double)v[j].y;
+        }
       }
     }
-    for (int g = 0; g < n_groups; ++g) {
-      const ObsGroup G = s_groups[g];
-      const uint32_t lx = __builtin_amdgcn_readfirstlane(G.lx);
-      const int hl = __builtin_amdgcn_readfirstlane(G.hl);
-      const int t0 = __builtin_amdgcn_readfirstlane(G.t0);
-      const int nt = __builtin_amdgcn_readfirstlane(G.nt);
-      const int nconst = __builtin_amdgcn_readfirstlane(G.nconst);
+    __syncthreads();
+    // per x mask the plain sums of conj(psi_{l ^ lx}) psi_l over the thread's
+    // pairs (l = q with a zero inserted at hl, q = tid + kObsBlock it); terms
+    // whose sign also depends on the iteration bits (lb != 0, rare) sum
+    // their own
+    double sr = 0.0, si = 0.0;
+    uint32_t cur = 0;
+#pragma unroll
+    for (int s = 0; s < kObsMaxTerms; ++s) {
+      if (s >= nterms) continue;
+      const ObsTerm& tm = s_terms[s];
+      const uint32_t lx = tm.lx;
+      const uint32_t lb = tm.lb;
+      const int kind = tm.kind;
+      const uint64_t z = tm.z;
+      double u = 0.0;
       if (lx == 0) {
-        // sum over j with the sign of the high tile bits, then over the
-        // wave with the sign of the lane and wave bits, term by term
-        const uint32_t m = __builtin_amdgcn_readfirstlane(G.m);
-        double w = 0.0;
+        if (lb == 0) {
+          u = pr0;
+        } else {
 #pragma unroll
-        for (int j = 0; j < kJ; ++j) w += (__popc((uint32_t)j & m) & 1) ? -pr[j] : pr[j];
-        for (int t = 0; t < nt; ++t) {
-          const ObsTerm tm = s_terms[t0 + t];
-          const uint32_t lo = tm.lz & ((1u << kObsBlockBits) - 1u);
-          const int s = (__popc((uint32_t)tid & lo) ^ parity64(base & tm.z)) & 1;
-          const double v = wave_total(s ? -w : w);
-          if (lane == 0) acc_s[wid][t0 + t] += v;
+          for (int j = 0; j < kObsJ; ++j) {
+            const int l = tid + kObsBlock * j;
+            if (l < ntile) {
+              const V a = tile[l];
+              const double pj = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+              u += (__popc((uint32_t)j & lb) & 1) ? -pj : pj;
+            }
+          }
         }
-        continue;
-      }
-      if (nconst > 0) {
-        double sr = 0.0, si = 0.0;
+      } else {
+        const int hl = tm.hl;
+        if (lx != cur || lb != 0) {
+          double ar = 0.0, ai = 0.0;
 #pragma unroll
-        for (int it = 0; it < kObsPairIters; ++it) {
-          const int q = tid + it * kObsBlock;
-          if (q >= (ntile >> 1)) break;
-          const uint32_t l = ((uint32_t)q >> hl << (hl + 1)) | ((uint32_t)q & ((1u << hl) - 1u));
-          const V a = tile[l], b = tile[l ^ lx];
-          sr += (double)b.x * (double)a.x + (double)b.y * (double)a.y;
-          si += (double)b.x * (double)a.y - (double)b.y * (double)a.x;
-        }
-        sr = wave_total(sr);
-        si = wave_total(si);
-        // one lane per constant-sign term
-        if (lane < nconst) {
-          const ObsTerm tm = s_terms[t0 + lane];
-          const double v = (tm.kind & 1) ? si : sr;
-          const int s = parity64(base & tm.z) ^ ((tm.kind == 1 || tm.kind == 2) ? 1 : 0);
-          acc_s[wid][t0 + lane] += s ? -v : v;
-        }
-        for (int t = t0 + 64; t < t0 + nconst; ++t) {  // more than 64 (rare)
-          if (lane != 0) break;
-          const ObsTerm tm = s_terms[t];
-          const double v = (tm.kind & 1) ? si : sr;
-          const int s = parity64(base & tm.z) ^ ((tm.kind == 1 || tm.kind == 2) ? 1 : 0);
-          acc_s[wid][t] += s ? -v : v;
+          for (int it = 0; it < kObsPairIters; ++it) {
+            const int q = tid + it * kObsBlock;
+            if (q < (ntile >> 1)) {
+              const uint32_t l =
+                  ((uint32_t)q >> hl << (hl + 1)) | ((uint32_t)q & ((1u << hl) - 1u));
+              const V a = tile[l], b = tile[l ^ lx];
+              double wr = (double)b.x * (double)a.x + (double)b.y * (double)a.y;
+              double wi = (double)b.x * (double)a.y - (double)b.y * (double)a.x;
+              if (__popc((uint32_t)it & lb) & 1) {
+                wr = -wr;
+                wi = -wi;
+              }
+              ar += wr;
+              ai += wi;
+            }
+          }
+          if (lb == 0) {
+            cur = lx;
+            sr = ar;
+            si = ai;
+          }
+          u = (kind & 1) ? ai : ar;
+        } else {
+          u = (kind & 1) ? si : sr;
         }
       }
-      ObsGroup Gu;
-      Gu.lx = lx;
-      Gu.hl = hl;
-      Gu.t0 = t0;
-      Gu.nt = nt;
-      Gu.nconst = nconst;
-      Gu.m = 0;
-      for (int c = nconst; c < nt; c += 16) {
-        const int nc = min(16, nt - c);
-        if (nc <= 2)
-          obs_pair_varying<T, 2>(tile, k, base, Gu, s_terms, c, nc, 0, acc_s);
-        else if (nc <= 4)
-          obs_pair_varying<T, 4>(tile, k, base, Gu, s_terms, c, nc, 0, acc_s);
-        else
-          obs_pair_varying<T, 16>(tile, k, base, Gu, s_terms, c, nc, 0, acc_s);
-      }
+      const int sb = parity64(base & z) ^ ((kind == 1 || kind == 2) ? 1 : 0);
+      acc[s] += sb ? -u : u;
     }
     __syncthreads();
   }
+  // the thread-index signs, then one wave sum per term, the waves in order
+#pragma unroll
+  for (int t = 0; t < kObsMaxTerms; ++t) {
+    if (t < nterms) {
+      const uint32_t la = s_terms[t].la;
+      double v = (__popc((uint32_t)tid & la) & 1) ? -acc[t] : acc[t];
+      v = wave_total(v);
+      if (lane == 0) acc_s[wid][t] = v;
+    }
+  }
   __syncthreads();
-  for (int t = tid; t < P.n_terms; t += kObsBlock) {
+  for (int t = tid; t < nterms; t += kObsBlock) {
     double v = 0.0;
 #pragma unroll
     for (int w = 0; w < kObsBlock / 64; ++w) v += acc_s[w][t];
@@ -375,7 +351,9 @@ static uint32_t pext64(uint64_t v, const uint8_t* pos, int k) {
 }
 
 // Greedy cover of the distinct x masks (ordered by highest, then lowest
-// bit) by tile bit sets of k bits that always hold bits 0..lowb-1.
+// bit) by tile bit sets of k bits that always hold bits 0..lowb-1, at most
+// kObsMaxTerms terms a pass; the diagonal terms fill the passes' spare
+// term slots, then passes of their own.
 static void plan_obs_tiles(int n, int n_terms, const uint64_t* xm, const uint64_t* zm,
                            ObsPlan& P) {
   const int k = std::min(n, kObsTileBits), lowb = std::min(k, kObsLowBits);
@@ -397,16 +375,10 @@ static void plan_obs_tiles(int n, int n_terms, const uint64_t* xm, const uint64_
   struct Draft {
     uint64_t hi = 0;  // tile bits above the low run
     std::vector<uint64_t> xs;
+    std::vector<int> diag;
     int n_terms = 0;
   };
-  std::vector<Draft> drafts(1);
-  std::vector<int> diag;
-  if (by_x.count(0)) diag = by_x[0];
-  if ((int)diag.size() > kObsMaxTerms) {
-    P.rest.insert(P.rest.end(), diag.begin() + kObsMaxTerms, diag.end());
-    diag.resize(kObsMaxTerms);
-  }
-  drafts[0].n_terms = (int)diag.size();
+  std::vector<Draft> drafts;
   for (uint64_t x : pair_xs) {
     const uint64_t bits = x & ~lowset;
     const int nt = (int)by_x[x].size();
@@ -414,8 +386,9 @@ static void plan_obs_tiles(int n, int n_terms, const uint64_t* xm, const uint64_
       P.rest.insert(P.rest.end(), by_x[x].begin(), by_x[x].end());
       continue;
     }
-    Draft& d = drafts.back();
-    if (__builtin_popcountll(d.hi | bits) <= k - lowb && d.n_terms + nt <= kObsMaxTerms) {
+    if (!drafts.empty() && __builtin_popcountll(drafts.back().hi | bits) <= k - lowb &&
+        drafts.back().n_terms + nt <= kObsMaxTerms) {
+      Draft& d = drafts.back();
       d.hi |= bits;
       d.xs.push_back(x);
       d.n_terms += nt;
@@ -427,10 +400,24 @@ static void plan_obs_tiles(int n, int n_terms, const uint64_t* xm, const uint64_
       drafts.push_back(nd);
     }
   }
-  for (size_t di = 0; di < drafts.size(); ++di) {
-    Draft& d = drafts[di];
-    if (di > 0 && d.xs.empty()) continue;
-    if (di == 0 && d.xs.empty() && diag.empty()) continue;
+  if (by_x.count(0)) {
+    const std::vector<int>& diag = by_x[0];
+    size_t i = 0;
+    for (Draft& d : drafts)
+      while (i < diag.size() && d.n_terms < kObsMaxTerms) {
+        d.diag.push_back(diag[i++]);
+        d.n_terms++;
+      }
+    while (i < diag.size()) {
+      Draft nd;
+      while (i < diag.size() && nd.n_terms < kObsMaxTerms) {
+        nd.diag.push_back(diag[i++]);
+        nd.n_terms++;
+      }
+      drafts.push_back(nd);
+    }
+  }
+  for (Draft& d : drafts) {
     // fill the tile up to k bits with the lowest free positions
     uint64_t hi = d.hi;
     for (int b = lowb; b < n && __builtin_popcountll(hi) < k - lowb; ++b) hi |= 1ull << b;
@@ -448,52 +435,59 @@ static void plan_obs_tiles(int n, int n_terms, const uint64_t* xm, const uint64_
     ps.n_cbits = c;
     ps.group0 = (int)P.groups.size();
     ps.term0 = (int)P.terms.size();
-    auto push_term = [&](uint64_t x, int t) {
+    auto push_term = [&](uint64_t x, int t, int hl) {
       ObsTerm tm;
+      std::memset(&tm, 0, sizeof(tm));
       tm.z = zm[t];
       tm.lz = pext64(zm[t], ps.tpos, k);
       tm.kind = x ? (__builtin_popcountll(x & zm[t]) & 3) : -1;
+      tm.lx = x ? pext64(x, ps.tpos, k) : 0u;
+      tm.hl = hl;
+      if (!x) {
+        // l = tid + kObsBlock j: thread bits are l's low bits, j its bits 9..11
+        tm.la = tm.lz & ((1u << kObsBlockBits) - 1u);
+        tm.lb = tm.lz >> kObsBlockBits;
+      } else {
+        // l = q with a zero at hl, q = tid + kObsBlock it
+        for (int i = 0; i < kObsBlockBits + 2; ++i) {
+          const int pos = i < hl ? i : i + 1;
+          if (!((tm.lz >> pos) & 1)) continue;
+          if (i < kObsBlockBits)
+            tm.la |= 1u << i;
+          else
+            tm.lb |= 1u << (i - kObsBlockBits);
+        }
+      }
       P.terms.push_back(tm);
       P.cols.push_back(t);
     };
-    auto add_group = [&](uint64_t x, const std::vector<int>& ts) {
-      if (x == 0) {
-        // one group per value of the z bits above the low kObsBlockBits tile bits
-        std::vector<uint32_t> ms;
-        std::map<uint32_t, std::vector<int>> by_m;
-        for (int t : ts) {
-          const uint32_t m = pext64(zm[t], ps.tpos, k) >> kObsBlockBits;
-          if (!by_m.count(m)) ms.push_back(m);
-          by_m[m].push_back(t);
-        }
-        for (uint32_t m : ms) {
-          ObsGroup g{};
-          g.m = m;
-          g.t0 = (int)P.terms.size();
-          g.nt = (int)by_m[m].size();
-          for (int t : by_m[m]) push_term(0, t);
-          P.groups.push_back(g);
-        }
-        return;
+    if (!d.diag.empty()) {
+      // one group per value of the z bits above the low kObsBlockBits tile bits
+      std::vector<uint32_t> ms;
+      std::map<uint32_t, std::vector<int>> by_m;
+      for (int t : d.diag) {
+        const uint32_t m = pext64(zm[t], ps.tpos, k) >> kObsBlockBits;
+        if (!by_m.count(m)) ms.push_back(m);
+        by_m[m].push_back(t);
       }
+      for (uint32_t m : ms) {
+        ObsGroup g{};
+        g.m = m;
+        g.t0 = (int)P.terms.size();
+        g.nt = (int)by_m[m].size();
+        for (int t : by_m[m]) push_term(0, t, 0);
+        P.groups.push_back(g);
+      }
+    }
+    for (uint64_t x : d.xs) {
       ObsGroup g{};
       g.lx = pext64(x, ps.tpos, k);
       g.hl = 31 - __builtin_clz(g.lx);
       g.t0 = (int)P.terms.size();
-      g.nt = (int)ts.size();
-      const uint32_t keep = ~(1u << g.hl);
-      for (int pass_const = 1; pass_const >= 0; --pass_const)
-        for (int t : ts) {
-          const bool is_const = (pext64(zm[t], ps.tpos, k) & keep) == 0;
-          if (is_const == (bool)pass_const) {
-            push_term(x, t);
-            g.nconst += is_const;
-          }
-        }
+      g.nt = (int)by_x[x].size();
+      for (int t : by_x[x]) push_term(x, t, g.hl);
       P.groups.push_back(g);
-    };
-    if (di == 0 && !diag.empty()) add_group(0, diag);
-    for (uint64_t x : d.xs) add_group(x, by_x[x]);
+    }
     ps.n_groups = (int)P.groups.size() - ps.group0;
     ps.n_terms = (int)P.terms.size() - ps.term0;
     P.passes.push_back(ps);
