@@ -194,6 +194,29 @@ __device__ void mc_gates(VT_alias<T>* lds, int tq, int pos0, int site0, int cnt,
   }
 }
 
+// A full 2^kMcTileQ tile into LDS with eight of the thread's loads in
+// flight at a time (a loop of single loads waits out one memory latency per
+// iteration).
+template <typename T>
+__device__ __forceinline__ void mc_load_full_tile(VT_alias<T>* lds, const VT_alias<T>* psi,
+                                                  const TileMap& tm, uint64_t t) {
+  constexpr int WB = CT<T>::kRowBits;
+  constexpr int kPer = (1 << kMcTileQ) / kMcBlock;
+  // plain vector registers (the HIP vector type's copies can go through
+  // private memory)
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2* src = reinterpret_cast<const v2*>(psi);
+  v2* dst = reinterpret_cast<v2*>(lds);
+#pragma unroll
+  for (int h = 0; h < kPer; h += 8) {
+    v2 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = src[tm.g(t, threadIdx.x + (uint32_t)(h + i) * kMcBlock)];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dst[swz<WB>(threadIdx.x + (uint32_t)(h + i) * kMcBlock)] = v[i];
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
   using V = VT_alias<T>;
@@ -221,9 +244,13 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
   const int cntA = n < kMcTileQ ? n : kMcTileQ;
 
   auto load_tile = [&](const TileMap& tm, uint64_t t) {
-    for (uint32_t l = threadIdx.x; l < tile; l += kMcBlock) {
-      const uint64_t gi = tm.g(t, l);
-      lds[swz<WB>(l)] = gi < D ? psi[gi] : make_c<T>(0.0, 0.0);
+    if (!resident) {
+      mc_load_full_tile<T>(lds, psi, tm, t);
+    } else {
+      for (uint32_t l = threadIdx.x; l < tile; l += kMcBlock) {
+        const uint64_t gi = tm.g(t, l);
+        lds[swz<WB>(l)] = gi < D ? psi[gi] : make_c<T>(0.0, 0.0);
+      }
     }
     __syncthreads();
   };
@@ -237,15 +264,30 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
     double tot = 0.0, hi[kMcMaxQ - 8];
 #pragma unroll
     for (int i = 0; i < kMcMaxQ - 8; ++i) hi[i] = 0.0;
-    for (uint64_t j = 0; (j << 8) < D; ++j) {
-      const uint64_t k = (j << 8) | threadIdx.x;
-      if (k >= D) break;
-      const V v = resident ? lds[swz<WB>((uint32_t)k)] : psi[k];
+    auto add = [&](uint64_t j, const V& v) {
       const double p = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
       tot += p;
 #pragma unroll
       for (int i = 0; i < kMcMaxQ - 8; ++i)
         if ((j >> i) & 1) hi[i] += p;
+    };
+    if (resident) {
+      for (uint64_t j = 0; (j << 8) < D; ++j) {
+        const uint64_t k = (j << 8) | threadIdx.x;
+        if (k >= D) break;
+        add(j, lds[swz<WB>((uint32_t)k)]);
+      }
+    } else {
+      // 2^n >= 2^13 amplitudes: eight loads of the thread in flight at once
+      typedef T v2 __attribute__((ext_vector_type(2)));
+      const v2* src = reinterpret_cast<const v2*>(psi);
+      for (uint64_t j0 = 0; (j0 << 8) < D; j0 += 8) {
+        v2 v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = src[((j0 + i) << 8) | threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) add(j0 + i, make_c<T>(v[i].x, v[i].y));
+      }
     }
     double v[kMcMaxQ + 1];
 #pragma unroll
@@ -307,14 +349,27 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
       const int k = s_site;
       new_norm_sq = sums[k];
       const uint64_t m = 1ull << k;
-      for (uint64_t p = threadIdx.x; p < (D >> 1); p += kMcBlock) {
-        const uint64_t i0 = insert_zero(p, k), i1 = i0 | m;
-        if (resident) {
+      if (resident) {
+        for (uint64_t p = threadIdx.x; p < (D >> 1); p += kMcBlock) {
+          const uint64_t i0 = insert_zero(p, k), i1 = i0 | m;
           lds[swz<WB>((uint32_t)i0)] = lds[swz<WB>((uint32_t)i1)];
           lds[swz<WB>((uint32_t)i1)] = make_c<T>(0.0, 0.0);
-        } else {
-          psi[i0] = psi[i1];
-          psi[i1] = make_c<T>(0.0, 0.0);
+        }
+      } else {
+        // 2^(n-1) >= 2^12 pairs: eight of the thread's loads in flight at once
+        typedef T v2 __attribute__((ext_vector_type(2)));
+        v2* pv = reinterpret_cast<v2*>(psi);
+        const v2 zero = {(T)0, (T)0};
+        for (uint64_t p0 = threadIdx.x; p0 < (D >> 1); p0 += 8 * kMcBlock) {
+          v2 v[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) v[i] = pv[insert_zero(p0 + (uint64_t)i * kMcBlock, k) | m];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const uint64_t i0 = insert_zero(p0 + (uint64_t)i * kMcBlock, k);
+            pv[i0] = v[i];
+            pv[i0 | m] = zero;
+          }
         }
       }
       __syncthreads();
