@@ -4,6 +4,7 @@
 // at column k ^ x: i^{|x&z|} (-1)^{|(k ^ x) & z|}.  Each thread owns output
 // amplitudes and writes each of them once, as apply_hamiltonian_bitwise!
 // (utils/cpu/cpuQuantumChannelUnitaryEvolutionChebyshev.jl:55-152) does.
+#include <algorithm>
 #include <cmath>
 
 #include "device_common.h"
@@ -39,11 +40,19 @@ __global__ void __launch_bounds__(256)
   const uint64_t k = i & (dim - 1);
   const uint64_t base = i - k;
   double hr = 0.0, hi = 0.0;
+  // terms come sorted by x: the amplitude at k ^ x is loaded once for all
+  // the terms that share x (the diagonal terms share x = 0)
+  uint64_t cur = ~0ull;
+  double vr = 0.0, vi = 0.0;
   for (int t = 0; t < nt; ++t) {
     const TermDev tm = terms[t];
     const uint64_t j = k ^ tm.x;
-    const VT_alias<T> v = in[base + j];
-    const double vr = (double)v.x, vi = (double)v.y;
+    if (tm.x != cur) {
+      cur = tm.x;
+      const VT_alias<T> v = in[base + j];
+      vr = (double)v.x;
+      vi = (double)v.y;
+    }
     const bool neg = __popcll(j & tm.z) & 1;
     const double sr = neg ? -vr : vr, si = neg ? -vi : vi;
     hr += tm.fre * sr - tm.fim * si;
@@ -99,6 +108,10 @@ static int check_terms(const smoq_state* s, int n_terms, const uint64_t* x, cons
     const double re[4] = {1, 0, -1, 0}, im[4] = {0, 1, 0, -1};
     tv.push_back({x[t], z[t], c[t] * re[cnt], c[t] * im[cnt]});
   }
+  // terms sharing an x mask next to each other (k_hpsi loads their common
+  // amplitude once); the order inside a group is the caller's
+  std::stable_sort(tv.begin(), tv.end(),
+                   [](const TermDev& a, const TermDev& b) { return a.x < b.x; });
   return SMOQ_OK;
 }
 
