@@ -960,22 +960,46 @@ def main():
         ts = float(np.median(tss))
         log(f"sampling: {args.shots} draws in {ts * 1e3:.2f} ms (median of 3: "
             f"{', '.join(f'{t * 1e3:.2f}' for t in tss)})")
+        # the sequential rule (sample_from_probabilities / projective
+        # measurement): its running sum in parallel chunks, bit for bit
+        me.sample_indices(psi, r, "sequential")
+        ctx.sync()
+        tsq = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            idx_seq = me.sample_indices(psi, r, "sequential")
+            tsq.append(time.perf_counter() - t0)
+        ts_seq = float(np.median(tsq))
+        log(f"sequential-rule sampling: {args.shots} draws in {ts_seq * 1e3:.2f} ms")
         zs = [1 << i for i in range(N)] + [3 << i for i in range(N - 1)]
         obs128 = ob.expect_pauli(psi, [0] * len(zs), zs)
         samp_check = None
+        seq_check = None
         if not args.no_checks:
-            # the oracle's cumsum + searchsortedfirst on the downloaded state
-            # and the same uniforms (outside the timed draws)
+            # the oracle's cumsum + searchsortedfirst (and its sequential
+            # running sum) on the downloaded state and the same uniforms
+            # (outside the timed draws)
             from oracle import oracle as orc
 
             t0 = time.perf_counter()
             host = psi.to_numpy()
             ref = orc.sample_cumsum_search(host, N, r)
-            del host
             mism = int(np.count_nonzero(ref != idx.astype(np.int64)))
             samp_check = {"oracle_mismatches": mism, "draws_compared": int(r.size),
                           "seconds": time.perf_counter() - t0}
             log(f"sampling check vs oracle at N={N}: {mism} mismatches in {r.size} draws")
+            # the oracle's running sum, then the first index with r <= c_i
+            # (the last index when none): the rule of its per-draw loop
+            # (orc.sample_sequential, O(2^N) per draw) in one pass
+            t0 = time.perf_counter()
+            cseq = orc.cumsum_sequential(orc.probs(host))
+            del host
+            ref = np.minimum(np.searchsorted(cseq, r, side="left"), cseq.size - 1)
+            del cseq
+            mism_seq = int(np.count_nonzero(ref != idx_seq.astype(np.int64)))
+            seq_check = {"oracle_mismatches": mism_seq, "draws_compared": int(r.size),
+                         "seconds": time.perf_counter() - t0}
+            log(f"sequential-rule check vs oracle at N={N}: {mism_seq} mismatches")
         # complex64 sweep of the same circuit
         psi.free()
         p64 = GPUState(N, "c64")
@@ -1004,7 +1028,10 @@ def main():
                          "timing": "wall clock of smoq_sample from host uniforms to host "
                                    "indices (8 MB each way over PCIe included)",
                          "first_indices": [int(v) for v in idx[:4]],
-                         "oracle_check": samp_check},
+                         "oracle_check": samp_check,
+                         "sequential_rule": {"ms": ts_seq * 1e3, "oracle_check": seq_check,
+                                             "note": "running sum in parallel 4096-term "
+                                                     "chunks, starts shifted from a guess"}},
             "c64_sweep": {"ms_per_circuit": t64 * 1e3, "gate_ops_per_s": n_gates / t64,
                           "fusion": tuned64 if tuned64 else "default",
                           "max_abs_err_Z_ZZ_vs_c128": float(np.max(np.abs(obs64 - obs128)))},
@@ -1063,6 +1090,11 @@ def summary(result):
         chk = x["sampling"].get("oracle_check")
         if chk:
             out["sampling_oracle_mismatches"] = chk["oracle_mismatches"]
+        sq = x["sampling"].get("sequential_rule")
+        if sq:
+            out["sequential_sampling_ms"] = sq["ms"]
+            if sq.get("oracle_check"):
+                out["sequential_oracle_mismatches"] = sq["oracle_check"]["oracle_mismatches"]
     if "c64_sweep" in x:
         out["c64_ms"] = x["c64_sweep"]["ms_per_circuit"]
         out["c64_err"] = x["c64_sweep"]["max_abs_err_Z_ZZ_vs_c128"]

@@ -261,6 +261,229 @@ __global__ void k_seq_prefix(const typename CT<T>::V* __restrict__ psi, uint64_t
   }
 }
 
+// ------------------------------------------ sequential prefix, in parallel --
+// The reference's running sum s_i = fl(s_{i-1} + p_i) is one dependent
+// chain.  It is reproduced bit for bit in chunks of kSeqChunk terms:
+//
+//  1. every chunk is summed from zero, and an exclusive scan of those sums
+//     gives each chunk a guess G of its true start S;
+//  2. every chunk runs the chain from its guess to its end E and notes
+//     whether any step rounded a tie;
+//  3. one workgroup walks the chunks in order.  While the chain stays inside
+//     one binade [2^e, 2^(e+1)) its grid spacing u is fixed, and adding p
+//     to a value and to the same value shifted by a multiple k u of the
+//     grid rounds to results shifted by the same k u, except for a tie when
+//     k is odd (round-half-to-even flips with the grid index's parity).  So
+//     when G, S, E and E + (S - G) share a binade and no tie meets an odd
+//     k, the chunk's true end is E + (S - G), exact; any other chunk is run
+//     again from S (the few that start near or cross a power of two);
+//  4. every chunk runs the chain from its exact start, writing s_i.
+constexpr uint32_t kSeqChunk = 4096;
+constexpr uint64_t kSeqParallelMin = 1ull << 18;  // shorter: one thread
+constexpr int kSeqFixThreads = 256;
+constexpr uint32_t kSeqFixBlock = 2048;  // chunk records staged per round
+
+template <typename T>
+__device__ __forceinline__ double seq_term(const typename CT<T>::V* __restrict__ psi, uint64_t i,
+                                           uint64_t diag_step) {
+  return diag_step ? (double)psi[i * diag_step].x : prob_of<T>(psi[i]);
+}
+
+// pass 1: the chain of each chunk from zero (kSeqChunk terms, loads of a
+// group ahead of its sums)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_seq_chunk_sums(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
+                     uint64_t nch, double* __restrict__ sums) {
+#pragma clang fp contract(off)
+  const uint64_t ch = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ch >= nch) return;
+  const uint64_t i0 = ch * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
+  double s = 0.0;
+  uint64_t i = i0;
+  for (; i + kBatch <= i1; i += kBatch) {
+    double p[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) p[u] = seq_term<T>(psi, i + u, diag_step);
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) s = s + p[u];
+  }
+  for (; i < i1; ++i) s = s + seq_term<T>(psi, i, diag_step);
+  sums[ch] = s;
+}
+
+// exclusive prefix of the chunk sums (a guess only: any order will do), one
+// workgroup: each thread sums a range, the workgroup scans the range sums
+__global__ void __launch_bounds__(1024)
+    k_seq_guess(const double* __restrict__ sums, uint64_t nch, double* __restrict__ guess) {
+  __shared__ double part[1024];
+  const uint64_t per = (nch + 1023) / 1024;
+  const uint64_t b0 = threadIdx.x * per, b1 = b0 + per < nch ? b0 + per : nch;
+  double t = 0.0;
+  for (uint64_t c = b0; c < b1; ++c) t += sums[c];
+  part[threadIdx.x] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double run = 0.0;
+    for (int k = 0; k < 1024; ++k) {
+      const double v = part[k];
+      part[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  double run = part[threadIdx.x];
+  for (uint64_t c = b0; c < b1; ++c) {
+    guess[c] = run;
+    run += sums[c];
+  }
+}
+
+__device__ __forceinline__ uint64_t dbits(double x) { return (uint64_t)__double_as_longlong(x); }
+__device__ __forceinline__ int dexp(double x) { return (int)((dbits(x) >> 52) & 0x7ff); }
+
+// pass 2: the chain of each chunk from its guess; tie[ch] = 1 when a step's
+// exact sum lay halfway between grid points (TwoSum error, compared with the
+// spacing above and below the result)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_seq_chunk_ends(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
+                     uint64_t nch, const double* __restrict__ guess, double* __restrict__ ends,
+                     uint8_t* __restrict__ tie) {
+#pragma clang fp contract(off)
+  const uint64_t ch = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ch >= nch) return;
+  const uint64_t i0 = ch * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
+  double a = guess[ch];
+  bool t = false;
+  auto step = [&](double p) {
+    const double s = a + p;
+    const double bp = s - a;
+    const double err = (a - (s - bp)) + (p - bp);
+    if (err != 0.0) {
+      const double e2 = 2.0 * fabs(err);
+      const double up = __longlong_as_double((long long)(dbits(s) + 1)) - s;
+      t |= e2 == up || 2.0 * e2 == up;
+    }
+    a = s;
+  };
+  uint64_t i = i0;
+  for (; i + kBatch <= i1; i += kBatch) {
+    double p[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) p[u] = seq_term<T>(psi, i + u, diag_step);
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) step(p[u]);
+  }
+  for (; i < i1; ++i) step(seq_term<T>(psi, i, diag_step));
+  ends[ch] = a;
+  tie[ch] = t ? 1 : 0;
+}
+
+// pass 3: the exact start of every chunk, chunks in order (one workgroup;
+// thread 0 walks the staged records, all threads stage a chunk's terms when
+// it has to run again)
+template <typename T>
+__global__ void __launch_bounds__(kSeqFixThreads)
+    k_seq_fix(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
+              uint64_t nch, const double* __restrict__ guess, const double* __restrict__ ends,
+              const uint8_t* __restrict__ tie, double* __restrict__ start,
+              unsigned long long* __restrict__ reruns) {
+#pragma clang fp contract(off)
+  __shared__ double sg[kSeqFixBlock], se[kSeqFixBlock];
+  __shared__ uint8_t st[kSeqFixBlock];
+  __shared__ double terms[kSeqChunk];
+  __shared__ double s_cur;
+  __shared__ long long s_next;  // next chunk to walk (block-relative), or -1 - chunk to rerun
+  if (threadIdx.x == 0) s_cur = 0.0;
+  unsigned long long nrerun = 0;
+  for (uint64_t b0 = 0; b0 < nch; b0 += kSeqFixBlock) {
+    const uint32_t nb = (uint32_t)(nch - b0 < kSeqFixBlock ? nch - b0 : kSeqFixBlock);
+    for (uint32_t k = threadIdx.x; k < nb; k += kSeqFixThreads) {
+      sg[k] = guess[b0 + k];
+      se[k] = ends[b0 + k];
+      st[k] = tie[b0 + k];
+    }
+    if (threadIdx.x == 0) s_next = 0;
+    __syncthreads();
+    while (true) {
+      if (threadIdx.x == 0) {
+        double S = s_cur;
+        long long k = s_next;
+        for (; k < (long long)nb; ++k) {
+          const double g = sg[k], e = se[k];
+          start[b0 + k] = S;
+          if (S == g) {
+            S = e;
+            continue;
+          }
+          // shift by d = S - g: k_ulp grid steps of g's binade (bit patterns
+          // of positive doubles in one binade differ by that count)
+          const int eg = dexp(g);
+          if (!(S > 0.0) || !(g > 0.0) || !(e > 0.0) || eg == 0 || eg == 0x7ff ||
+              dexp(S) != eg || dexp(e) != eg) {
+            break;
+          }
+          const long long kd = (long long)(dbits(S) - dbits(g));
+          if (st[k] && (kd & 1)) break;
+          const uint64_t rb = dbits(e) + (uint64_t)kd;
+          if ((int)((rb >> 52) & 0x7ff) != eg) break;
+          S = __longlong_as_double((long long)rb);
+        }
+        s_cur = S;
+        s_next = k < (long long)nb ? -1 - k : (long long)nb;
+      }
+      __syncthreads();
+      const long long nx = s_next;
+      if (nx >= 0) break;  // block done
+      // run chunk b0 + r again from the exact start s_cur
+      const long long r = -1 - nx;
+      const uint64_t ch = b0 + (uint64_t)r;
+      const uint64_t i0 = ch * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
+      for (uint64_t i = i0 + threadIdx.x; i < i1; i += kSeqFixThreads)
+        terms[i - i0] = seq_term<T>(psi, i, diag_step);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double S = s_cur;
+        for (uint64_t i = 0; i < i1 - i0; ++i) S = S + terms[i];
+        s_cur = S;
+        s_next = r + 1;
+        ++nrerun;
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *reruns = nrerun;
+}
+
+// pass 4: every chunk's chain from its exact start, writing the sums
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_seq_write(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
+                uint64_t nch, const double* __restrict__ start, double* __restrict__ c) {
+#pragma clang fp contract(off)
+  const uint64_t ch = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ch >= nch) return;
+  const uint64_t i0 = ch * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
+  double s = start[ch];
+  uint64_t i = i0;
+  for (; i + kBatch <= i1; i += kBatch) {
+    double p[kBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) p[u] = seq_term<T>(psi, i + u, diag_step);
+#pragma unroll
+    for (uint32_t u = 0; u < kBatch; ++u) {
+      s = s + p[u];
+      c[i + u] = s;
+    }
+  }
+  for (; i < i1; ++i) {
+    s = s + seq_term<T>(psi, i, diag_step);
+    c[i] = s;
+  }
+}
+
 // first i with r <= c[i] (c is non-decreasing), else n-1
 __global__ void k_search_le(const double* __restrict__ c, uint64_t n, const double* __restrict__ r,
                             int64_t nd, uint64_t* __restrict__ out) {
@@ -603,7 +826,10 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   const size_t off_o = off_r + up(nd * sizeof(double));
   const size_t off_c = off_o + up(nd * sizeof(uint64_t));
   const size_t off_levels = off_c + up((leaf_ends ? 1 : n) * sizeof(double));
-  int rc = ensure_scratch(ctx, off_levels + (leaf_ends ? 0 : tree_bytes));
+  // the parallel sequential prefix keeps per-chunk records after c
+  const uint64_t nch_seq = (!tree && n >= kSeqParallelMin) ? (n + kSeqChunk - 1) / kSeqChunk : 0;
+  const size_t seq_bytes = nch_seq ? up(nch_seq * (4 * sizeof(double) + 1) + 16) : 0;
+  int rc = ensure_scratch(ctx, off_levels + (leaf_ends ? 0 : tree_bytes) + seq_bytes);
   if (rc) return rc;
   unsigned char* base = (unsigned char*)ctx->scratch;
   double* rd = (double*)(base + off_r);
@@ -612,8 +838,29 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   if ((rc = stage_draws(ctx, r_host, nd, rd))) return rc;
 
   if (!tree) {
-    hipLaunchKernelGGL(k_seq_prefix<T>, dim3(1), dim3(1), 0, ctx->stream, psi, n, c, diag_step);
-    SMOQ_CHECK_LAUNCH("k_seq_prefix");
+    if (n < kSeqParallelMin) {
+      hipLaunchKernelGGL(k_seq_prefix<T>, dim3(1), dim3(1), 0, ctx->stream, psi, n, c, diag_step);
+      SMOQ_CHECK_LAUNCH("k_seq_prefix");
+    } else {
+      const uint64_t nch = (n + kSeqChunk - 1) / kSeqChunk;
+      double* sums = (double*)(base + off_levels);
+      double* guess = sums + nch;
+      double* ends = guess + nch;
+      double* start = ends + nch;
+      unsigned long long* reruns = (unsigned long long*)(start + nch);
+      uint8_t* tie = (uint8_t*)(reruns + 1);
+      const unsigned gb = (unsigned)((nch + 255) / 256);
+      hipLaunchKernelGGL(k_seq_chunk_sums<T>, dim3(gb), dim3(256), 0, ctx->stream, psi, n,
+                         diag_step, nch, sums);
+      hipLaunchKernelGGL(k_seq_guess, dim3(1), dim3(1024), 0, ctx->stream, sums, nch, guess);
+      hipLaunchKernelGGL(k_seq_chunk_ends<T>, dim3(gb), dim3(256), 0, ctx->stream, psi, n,
+                         diag_step, nch, guess, ends, tie);
+      hipLaunchKernelGGL(k_seq_fix<T>, dim3(1), dim3(kSeqFixThreads), 0, ctx->stream, psi, n,
+                         diag_step, nch, guess, ends, tie, start, reruns);
+      hipLaunchKernelGGL(k_seq_write<T>, dim3(gb), dim3(256), 0, ctx->stream, psi, n, diag_step,
+                         nch, start, c);
+      SMOQ_CHECK_LAUNCH("sequential prefix kernels");
+    }
     if (sem != SMOQ_SAMPLE_CUMSUM_SEARCH) {
       hipLaunchKernelGGL(k_search_le, dim3(nblk(nd)), dim3(256), 0, ctx->stream, c, n, rd, nd, od);
     } else {

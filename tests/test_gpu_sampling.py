@@ -99,3 +99,36 @@ def test_long_vector_sampling_repeats_and_batch_item_zero(ctx):
     want = orc.sample_cumsum_search(a, N, r)
     assert np.array_equal(first, want)
     assert np.array_equal(again, want)
+
+
+def tie_state(N):
+    """Every |psi_i|^2 = 2^-20 + 2^-54 (amplitude 2^-10 + 2^-45): the running
+    sum is exact below 1/2 and every step above 1/2 rounds a tie."""
+    return np.full(1 << N, 2.0 ** -10 + 2.0 ** -45, np.complex128)
+
+
+@pytest.mark.parametrize("kind", ["gauss", "lognormal", "blocks", "tail", "ties"])
+@pytest.mark.parametrize("N", [19, 20])
+def test_sequential_prefix_in_chunks_bit_exact(ctx, kind, N):
+    """The reference's sequential running sum (sample_from_probabilities),
+    computed in parallel chunks whose starts are shifted from a guess, is
+    the oracle's chain bit for bit: draws placed exactly on the oracle's
+    prefix values and one grid step either side of them land where the
+    oracle's do.  States cross many binades (lognormal, blocks), end in
+    zeros (tail) or round a tie at every step above 1/2 (ties)."""
+    from smoq_amd import GPUState
+    from smoq_amd import measurements as me
+
+    psi = tie_state(N) if kind == "ties" else structured_state(kind, N, 90 + N)
+    c = orc.cumsum_sequential(orc.probs(psi))
+    rng = np.random.default_rng(N)
+    pos = np.concatenate([rng.integers(0, psi.size, 600), np.arange(0, psi.size, 4096),
+                          np.arange(4095, psi.size, 4096)])
+    pos = pos[pos < psi.size]
+    r = np.concatenate([c[pos], np.nextafter(c[pos], 0.0), np.nextafter(c[pos], 2.0),
+                        [0.0, 1.0 - 2.0 ** -53]])
+    r = r[(r >= 0.0) & (r < 1.0)]
+    s = GPUState.from_numpy(psi)
+    got = me.sample_indices(s, r, "sequential").astype(np.int64)
+    want = orc.sample_sequential(psi, N, r)
+    assert np.array_equal(got, want)

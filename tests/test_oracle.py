@@ -320,3 +320,18 @@ def test_product_ket_matches_kron_chain():
     # |0>|+>|1> (qubit 1 = |0>): amplitudes at indices with bit0 = 0, bit2 = 1
     v = orc.product_ket([(1, 0), (r, r), (0, 1)])
     assert np.allclose(v, [0, 0, 0, 0, r, 0, r, 0])
+
+
+def test_sequential_rule_as_running_sum_and_search():
+    """The oracle's per-draw sequential loop (sample_from_probabilities) is
+    the first index of its running sum at or above the draw, the last index
+    when none is: the form the full-size check uses."""
+    rng = np.random.default_rng(5)
+    N = 10
+    psi = rng.normal(size=1 << N) + 1j * rng.normal(size=1 << N)
+    psi /= np.linalg.norm(psi)
+    c = orc.cumsum_sequential(orc.probs(psi))
+    r = np.concatenate([rng.random(300), c[::37], np.nextafter(c[::41], 2.0), [0.0, 2.0]])
+    want = orc.sample_sequential(psi, N, r)
+    got = np.minimum(np.searchsorted(c, r, side="left"), c.size - 1)
+    assert np.array_equal(got, want)
