@@ -138,6 +138,32 @@ def traffic_probe(N=28):
           f"state {s.nbytes} bytes", flush=True)
 
 
+def observables_probe(N=20, items=250, calls=10):
+    """Program for the kernel trace of the tiled Pauli reduction: the read-out
+    of the sigma_z-dephasing MCWF demo (X, Y, Z on every site, XX, YY, ZZ on
+    every bond) over a batch of random states, one untimed call, then
+    `calls` timed ones."""
+    import smoq_amd
+    from smoq_amd import GPUState
+    from smoq_amd import observables as ob
+
+    ctx = smoq_amd.default_context()
+    xs, zs = ob.qrc_observable_masks(N, 1)
+    rng = np.random.default_rng(1)
+    h = rng.normal(size=(items, 1 << N)) + 1j * rng.normal(size=(items, 1 << N))
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    s = GPUState.from_numpy(h.astype(np.complex128))
+    ob.expect_pauli(s, xs, zs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        ob.expect_pauli(s, xs, zs)
+    ctx.sync()
+    t = (time.perf_counter() - t0) / calls
+    print(f"observables probe: N={N}, {items} items ({s.nbytes} bytes), {len(xs)} terms: "
+          f"{t * 1e3:.3f} ms per call", flush=True)
+
+
 def measured_traffic_ratio():
     """HBM traffic / algorithmic bytes of the fused pass, from the committed
     counter summary (profiles/fused_passes_traffic.json), or None."""
@@ -818,7 +844,7 @@ def adjoint_full_check(N=24, layers=20, seed=24, fd_params=16, fd_eps=1e-6):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--probe", choices=["traffic"], default=None)
+    ap.add_argument("--probe", choices=["traffic", "observables"], default=None)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--no-mcwf", action="store_true")
@@ -847,6 +873,9 @@ def main():
 
     if args.probe == "traffic":
         traffic_probe()
+        return
+    if args.probe == "observables":
+        observables_probe()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))

@@ -174,9 +174,11 @@ int smoq_apply_tape(smoq_state* s, const smoq_op* ops, int n_ops,
  * SmoQ.jl_benchmarks/benchmark_random_circuit_sampling/run_benchmark_smoq_Float64.jl:167-200).
  * smoq_autotune plans the tape under candidate settings (tile qubits 11 or
  * 12, 4 or 5 contiguous low qubits, a cap of 0 / 28 / 36 / 44 non-diagonal
- * gates per pass, stages with or without register/lane exchanges) besides
- * the context's own, compiles their pass kernels,
- * times each candidate on s (its contents are overwritten) and records the
+ * gates per pass, stages with or without register/lane exchanges, local
+ * qubit sets from a beam search over plan prefixes) besides the context's
+ * own, compiles their pass kernels, times each candidate on s (its
+ * contents are overwritten), then times HBM orders (tile-index rotations and
+ * reversals, load maps) pass by pass for the winner, and records the
  * fastest for the tape's structure (gate kinds and qubits), qubit count,
  * dtype and batch: later smoq_apply_tape / smoq_plan_create calls with that
  * structure use it, unless smoq_set_fusion fixed a tile size.  No
