@@ -117,6 +117,12 @@ int smoq_set_jit(smoq_ctx* ctx, int mode);
  * SMOQ_LANE_SWAPS=0 in the environment sets new contexts off) or off (0).
  * A structure tuned by smoq_autotune keeps its tuned choice. */
 int smoq_set_lane_swaps(smoq_ctx* ctx, int on);
+/* Pauli term lists (smoq_expect_pauli, measure_all_observables) through
+ * reduction passes compiled for the term list: 0 off (precompiled tiled
+ * passes), 1 automatic (states of at least 12 qubits and 2^22 amplitudes
+ * over the batch; the default, or SMOQ_OBS_JIT), 2 for every state of at
+ * least 12 qubits.  No reference counterpart (tuning). */
+int smoq_set_obs_jit(smoq_ctx* ctx, int mode);
 /* Mean kernel time (ms) and launch count of the fused-pass kernel since the
  * last call, measured with HIP events on the context stream when timing is
  * enabled (enable != 0).  Used to report the per-launch roofline. */

@@ -505,12 +505,12 @@ static bool rtc_compile(const std::string& src, std::vector<char>& code, std::st
 }
 
 static int load_module(const std::vector<char>& code, size_t n_passes,
-                       std::shared_ptr<JitModule>& out) {
+                       std::shared_ptr<JitModule>& out, const char* prefix = "smoq_pass_") {
   auto m = std::make_shared<JitModule>();
   SMOQ_HIP(hipModuleLoadData(&m->mod, code.data()));
   m->fn.resize(n_passes);
   for (size_t p = 0; p < n_passes; ++p) {
-    const std::string name = "smoq_pass_" + std::to_string(p);
+    const std::string name = prefix + std::to_string(p);
     SMOQ_HIP(hipModuleGetFunction(&m->fn[p], m->mod, name.c_str()));
   }
   out = m;
@@ -667,6 +667,39 @@ int jit_get_many(smoq_ctx* ctx, const std::vector<const Program*>& progs, int dt
   return SMOQ_OK;
 }
 
+int jit_get_source(smoq_ctx* ctx, const std::string& src, size_t n_fn, const char* prefix,
+                   std::shared_ptr<JitModule>& out) {
+  out.reset();
+  const std::string key = std::string(prefix) + "\n" + src;
+  auto it = ctx->jit_cache.find(key);
+  if (it != ctx->jit_cache.end()) {
+    out = it->second;
+    return SMOQ_OK;
+  }
+  const std::string dir = cache_dir();
+  const std::string name = dir.empty() ? std::string() : cache_name(src);
+  std::vector<char> code;
+  if (!dir.empty() && disk_load(dir, name, code)) {
+    std::shared_ptr<JitModule> m;
+    if (load_module(code, n_fn, m, prefix) == SMOQ_OK) {
+      ctx->jit_cache.emplace(key, m);
+      out = m;
+      return SMOQ_OK;
+    }
+    (void)hipGetLastError();  // a stale or damaged file: compile again
+  }
+  std::string err;
+  if (!rtc_compile(src, code, err))
+    return set_error(SMOQ_EDEVICE, "kernel compilation failed (%s)", err.c_str());
+  std::shared_ptr<JitModule> m;
+  const int rc = load_module(code, n_fn, m, prefix);
+  if (rc) return rc;
+  if (!dir.empty()) disk_store(dir, name, code);
+  ctx->jit_cache.emplace(key, m);
+  out = m;
+  return SMOQ_OK;
+}
+
 int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out) {
   out.reset();
   std::string& key = ctx->jit_key_buf;
@@ -689,6 +722,14 @@ extern "C" int smoq_set_jit(smoq_ctx* ctx, int mode) {
   if (!ctx) return smoq::set_error(SMOQ_EINVAL, "smoq_set_jit: null context");
   if (mode < 0 || mode > 2) return smoq::set_error(SMOQ_EINVAL, "smoq_set_jit: mode %d", mode);
   ctx->jit_mode = mode;
+  return SMOQ_OK;
+}
+
+extern "C" int smoq_set_obs_jit(smoq_ctx* ctx, int mode) {
+  if (!ctx) return smoq::set_error(SMOQ_EINVAL, "smoq_set_obs_jit: null context");
+  if (mode < 0 || mode > 2)
+    return smoq::set_error(SMOQ_EINVAL, "smoq_set_obs_jit: mode %d", mode);
+  ctx->obs_jit = mode;
   return SMOQ_OK;
 }
 

@@ -516,8 +516,10 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
   // x mask would (true values only; the reference-rule path stays per mask)
   ObsPlan plan;
   bool tiled = false;
+  // compiled passes for this term list (not in the entry point that
+  // allocates nothing: loading a code object allocates device memory)
+  std::shared_ptr<ObsJitPlan> jplan;
   if (!ref_mode && ctx->obs_tiles) {
-    plan_obs_tiles(s->n, n_terms, xm, zm, plan);
     auto launches = [&](const std::vector<int>& ts) {
       std::map<uint64_t, int> per_x;
       for (int t : ts) per_x[xm[t]]++;
@@ -527,10 +529,45 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
     };
     std::vector<int> all(n_terms);
     for (int t = 0; t < n_terms; ++t) all[t] = t;
-    tiled = !plan.passes.empty() && plan.passes.size() + launches(plan.rest) < launches(all);
+    const bool want_jit =
+        !fixed.ptr && s->n >= kOjTileBits &&
+        (ctx->obs_jit >= 2 ||
+         (ctx->obs_jit == 1 && ((uint64_t)s->dim * (uint64_t)B) >= (1ull << kOjMinLogAmps)));
+    if (want_jit) {
+      std::string key((const char*)&s->n, sizeof(int));
+      key.append((const char*)&s->dtype, sizeof(int));
+      key.append((const char*)xm, sizeof(uint64_t) * n_terms);
+      key.append((const char*)zm, sizeof(uint64_t) * n_terms);
+      auto it = ctx->obs_jit_plans.find(key);
+      if (it != ctx->obs_jit_plans.end()) {
+        jplan = it->second;
+      } else {
+        auto jp = std::make_shared<ObsJitPlan>();
+        plan_obs_jit(s->n, s->dtype, n_terms, xm, zm, *jp);
+        if (!jp->passes.empty()) {
+          const int rc = jit_get_source(ctx, jp->src, jp->passes.size(), "smoq_obs_", jp->mod);
+          if (rc) return rc;
+        }
+        jp->src.clear();
+        jp->src.shrink_to_fit();
+        ctx->obs_jit_plans.emplace(key, jp);
+        jplan = jp;
+      }
+      if (jplan->passes.empty() ||
+          jplan->passes.size() + launches(jplan->rest) >= launches(all))
+        jplan.reset();
+    }
+    if (jplan) {
+      tiled = true;
+    } else {
+      plan_obs_tiles(s->n, n_terms, xm, zm, plan);
+      tiled = !plan.passes.empty() && plan.passes.size() + launches(plan.rest) < launches(all);
+    }
   }
   std::vector<int> plain_terms;
-  if (tiled) {
+  if (jplan) {
+    plain_terms = jplan->rest;
+  } else if (tiled) {
     plain_terms = plan.rest;
   } else {
     plain_terms.resize(n_terms);
@@ -544,8 +581,9 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
     groups[xm[t]].push_back(t);
   }
   const int maxblocks = 2048;
-  const int tile_terms = tiled ? (int)plan.terms.size() : 0;
-  const uint64_t n_tiles = tiled ? (s->dim >> plan.passes[0].k) : 0;
+  const int tile_terms = jplan ? (int)jplan->cols.size() : tiled ? (int)plan.terms.size() : 0;
+  const uint64_t n_tiles =
+      jplan ? (s->dim >> kOjTileBits) : tiled ? (s->dim >> plan.passes[0].k) : 0;
   const int tile_wg = tiled ? (int)std::max<uint64_t>(
                                   1, std::min<uint64_t>(n_tiles, (4096 + B - 1) / B))
                             : 0;
@@ -579,7 +617,23 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
   ObsGroup* grp_dev = (ObsGroup*)(sc += up(col_bytes));
   ObsTerm* trm_dev = (ObsTerm*)(sc += up(grp_bytes));
   using V = typename CT<T>::V;
-  if (tiled) {
+  if (jplan) {
+    SMOQ_HIP(hipMemcpyAsync(tcols, jplan->cols.data(), jplan->cols.size() * sizeof(int),
+                            hipMemcpyHostToDevice, ctx->stream));
+    const V* psi = (const V*)s->d;
+    unsigned long long dim = s->dim, ntl = n_tiles;
+    int pt = tile_terms;
+    for (size_t p = 0; p < jplan->passes.size(); ++p) {
+      int term0 = jplan->passes[p].term0;
+      void* args[] = {&psi, &dim, &ntl, &tpartial, &pt, &term0};
+      SMOQ_HIP(hipModuleLaunchKernel(jplan->mod->fn[p], (unsigned)tile_wg, (unsigned)B, 1,
+                                     kOjThreads, 1, 1, 0, ctx->stream, args, nullptr));
+    }
+    hipLaunchKernelGGL(k_finish, dim3(finish_blocks(B * tile_terms)), dim3(kFinishBlock), 0,
+                       ctx->stream, tpartial, tile_wg, tile_terms, 1.0, out_dev, n_terms, tcols,
+                       B);
+    SMOQ_CHECK_LAUNCH("k_finish");
+  } else if (tiled) {
     SMOQ_HIP(hipMemcpyAsync(grp_dev, plan.groups.data(), grp_bytes, hipMemcpyHostToDevice,
                             ctx->stream));
     SMOQ_HIP(hipMemcpyAsync(trm_dev, plan.terms.data(), trm_bytes, hipMemcpyHostToDevice,

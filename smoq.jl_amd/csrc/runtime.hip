@@ -377,6 +377,10 @@ int smoq_init(smoq_ctx** out, int device) {
     if (m >= 0 && m <= 2) ctx->jit_mode = m;
   }
   if (const char* o = std::getenv("SMOQ_OBS_TILES")) ctx->obs_tiles = std::atoi(o) != 0;
+  if (const char* o = std::getenv("SMOQ_OBS_JIT")) {
+    const int m = std::atoi(o);
+    if (m >= 0 && m <= 2) ctx->obs_jit = m;
+  }
   if (const char* g = std::getenv("SMOQ_PASS_MAX_GATES")) ctx->pass_max_gates = std::max(0, std::atoi(g));
   if (const char* w = std::getenv("SMOQ_LANE_SWAPS")) ctx->lane_swaps = std::atoi(w) != 0;
   hipError_t se = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);

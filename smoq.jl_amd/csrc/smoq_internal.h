@@ -130,6 +130,53 @@ constexpr size_t kStatePoolMaxBytes = 64ull << 20;
 constexpr size_t kStatePoolMax = 8;
 constexpr int kJitMinQubits = 16;
 constexpr int kJitMinGates = 16;
+
+// Compiled tiled Pauli reductions (obs_jit.cpp): tiles of 2^kOjTileBits
+// amplitudes, kOjThreads threads with 2^kOjRegBits amplitudes each, at
+// most kOjMaxSlots FP64 accumulators per thread and pass.
+constexpr int kOjTileBits = 12;
+constexpr int kOjRegBits = 4;
+constexpr int kOjThreads = 1 << (kOjTileBits - kOjRegBits);
+constexpr int kOjMaxSlots = 48;
+constexpr int kOjMaxSlotsC64 = 40;  // complex64 converts in registers
+// automatic mode: at least 2^kOjMinLogAmps amplitudes over the batch
+constexpr int kOjMinLogAmps = 22;
+struct OjStage {
+  uint32_t rmask;        // register tile bits
+  int rpos[kOjRegBits];  // the same, ascending
+  int tbit[kOjTileBits - kOjRegBits];  // tile bit of each thread-index bit
+};
+struct OjSlot {
+  int stage;
+  int kind;  // 0 diagonal, 1 register pairs, 2 partner from LDS
+  uint32_t m;  // kind 1: x mask in register bits; kind 2: in tile bits
+  int comp;    // 0 real part, 1 imaginary part
+  uint32_t zr;  // z bits among the register bits (sign per register index)
+  uint64_t zb;  // z bits outside the tile (sign per tile)
+};
+struct OjTerm {
+  int col;    // the caller's term index
+  int slot;
+  int stage;
+  uint32_t lzt;  // z bits among the stage's thread bits (sign per thread)
+  int scale;     // +-1, +-2
+};
+struct OjPass {
+  int tpos[kOjTileBits];  // global bit of each tile bit
+  std::vector<OjStage> st;
+  std::vector<OjSlot> sl;
+  std::vector<OjTerm> tm;
+  int term0 = 0;
+};
+struct ObsJitPlan {
+  std::vector<OjPass> passes;
+  std::vector<int> cols;  // pass-ordered term -> caller's term index
+  std::vector<int> rest;  // terms left to the per-mask kernels
+  std::string src;
+  std::shared_ptr<JitModule> mod;
+};
+void plan_obs_jit(int n, int dtype, int n_terms, const uint64_t* xm, const uint64_t* zm,
+                  ObsJitPlan& out);
 }  // namespace smoq
 
 // ------------------------------------------------------------ handles --
@@ -144,6 +191,10 @@ struct smoq_ctx_s {
   std::string jit_key_buf;  // the last key (capacity reused across calls)
   // multi-mask Pauli term lists through tiled passes (SMOQ_OBS_TILES=0: off)
   int obs_tiles = 1;
+  // ... compiled for the term list: 0 off, 1 automatic (large states), 2
+  // always (states of at least 12 qubits); plans cached by term list
+  int obs_jit = 1;
+  std::unordered_map<std::string, std::shared_ptr<smoq::ObsJitPlan>> obs_jit_plans;
   // rocBLAS handle for dense density-matrix propagation, created on first use
   void* blas = nullptr;
   // grow-only device scratch, reused across calls on this context
@@ -269,6 +320,10 @@ int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates);
 Fusion fusion_for(const smoq_ctx* ctx, int n_qubits, int dtype, int batch,
                   const std::vector<IGate>& gs);
 int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out);
+// a module from generated source with kernels <prefix>0 .. <prefix>(n_fn-1),
+// cached per context (and on disk) by its source
+int jit_get_source(smoq_ctx* ctx, const std::string& src, size_t n_fn, const char* prefix,
+                   std::shared_ptr<JitModule>& out);
 // the compiled modules of several programs, the missing ones compiled on
 // parallel host threads
 int jit_get_many(smoq_ctx* ctx, const std::vector<const Program*>& progs, int dtype,

@@ -183,6 +183,7 @@ SIGNATURES = {
     "smoq_mcwf_evolve": [_P, _P, _i, _d, _i, _i],
     "smoq_set_jit": [_P, _i],
     "smoq_set_lane_swaps": [_P, _i],
+    "smoq_set_obs_jit": [_P, _i],
     "smoq_mcwf_jump": [_P, _i, _d, _P, _P, _P],
     "smoq_mcwf_trotter_evolve": [_P, _P, _i, _d, _P, _i, _P, _P, _i, _i, _P, _P, _P],
 }

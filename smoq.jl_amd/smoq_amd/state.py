@@ -32,6 +32,11 @@ class Context:
         """Run-time compiled fused passes: 0 never, 1 automatic, 2 always."""
         call("smoq_set_jit", self.handle, int(mode))
 
+    def set_obs_jit(self, mode: int) -> None:
+        """Pauli reductions compiled for the term list: 0 off, 1 automatic
+        (large states), 2 for every state of at least 12 qubits."""
+        call("smoq_set_obs_jit", self.handle, int(mode))
+
     def set_lane_swaps(self, on: bool) -> None:
         """Stages of compiled passes trading register and lane qubits."""
         call("smoq_set_lane_swaps", self.handle, int(bool(on)))
