@@ -142,7 +142,8 @@ def observables_probe(N=20, items=250, calls=10):
     """Program for the kernel trace of the tiled Pauli reduction: the read-out
     of the sigma_z-dephasing MCWF demo (X, Y, Z on every site, XX, YY, ZZ on
     every bond) over a batch of random states, one untimed call, then
-    `calls` timed ones."""
+    `calls` timed ones; then `calls` steps of that demo's MCWF evolution of
+    the same size with the read-out after every step."""
     import smoq_amd
     from smoq_amd import GPUState
     from smoq_amd import observables as ob
@@ -162,6 +163,24 @@ def observables_probe(N=20, items=250, calls=10):
     t = (time.perf_counter() - t0) / calls
     print(f"observables probe: N={N}, {items} items ({s.nbytes} bytes), {len(xs)} terms: "
           f"{t * 1e3:.3f} ms per call", flush=True)
+    s.free()
+    # the sigma_z-dephasing MCWF step of the same demo with the read-out
+    # after every step: 2 steps (compilation), then `calls` steps
+    from smoq_amd.mcwf import evolve_mcwf_trotter, xxz_field_trotter_gates
+
+    gates = xxz_field_trotter_gates(N, 0.05)
+    st = GPUState.basis(N, 0, batch=items)
+    evolve_mcwf_trotter(st, gates, N, items, 2, 0.1, 0.05, jump="sigma_z", seed=1,
+                        obs_every=1, xmask=xs, zmask=zs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    evolve_mcwf_trotter(st, gates, N, items, calls, 0.1, 0.05, jump="sigma_z", seed=2,
+                        obs_every=1, xmask=xs, zmask=zs)
+    ctx.sync()
+    t = (time.perf_counter() - t0) / calls
+    st.free()
+    print(f"observables probe: MCWF sigma_z dephasing, {items} trajectories, "
+          f"{t * 1e3:.3f} ms per step with the read-out", flush=True)
 
 
 def measured_traffic_ratio():
