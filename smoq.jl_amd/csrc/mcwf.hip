@@ -91,6 +91,15 @@ struct TileMap {
   }
 };
 
+// the step's small tables, copied into LDS by the kernel: the stages read
+// them at every tile, and from global memory each read would wait out a
+// cache latency
+struct McTabs {
+  const double2* site_u;     // 4 per site
+  const double2* chain_tab;  // uniform chain: n + 1 products
+  const double* decay_pow;   // n + 1
+};
+
 struct StageOpts {
   double pre_scale;  // applied on load
   bool bonds;        // bond phases before the site gates
@@ -101,8 +110,8 @@ struct StageOpts {
 // tile positions, site gates on the positions with site[s] >= 0.
 template <typename T>
 __device__ void mc_stage(VT_alias<T>* lds, int tq, const int (&pos)[3], const int (&site)[3],
-                         const McwfGroupDev& G, const TileMap& tm, uint64_t t, const StageOpts& o,
-                         uint64_t dmask, double& norm_acc) {
+                         const McwfGroupDev& G, const McTabs& tb, const TileMap& tm, uint64_t t,
+                         const StageOpts& o, uint64_t dmask, double& norm_acc) {
   constexpr int WB = CT<T>::kRowBits;
   const uint32_t tmask = (1u << tq) - 1u;
   const uint32_t pm = (1u << pos[0]) | (1u << pos[1]) | (1u << pos[2]);
@@ -125,7 +134,7 @@ __device__ void mc_stage(VT_alias<T>* lds, int tq, const int (&pos)[3], const in
         const uint64_t gi = tm.g(t, li[i]) & dmask;
         if (G.uniform_chain) {
           const int nd = __popcll((gi ^ (gi >> 1)) & (dmask >> 1));
-          x[i] = cmul(G.chain_tab[nd], x[i]);
+          x[i] = cmul(tb.chain_tab[nd], x[i]);
         } else {
           for (int b = 0; b < G.n_bonds; ++b) {
             const int e = (int)((gi >> G.bond_q[2 * b]) & 1) |
@@ -138,7 +147,7 @@ __device__ void mc_stage(VT_alias<T>* lds, int tq, const int (&pos)[3], const in
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
       if (site[s] < 0) continue;
-      const double2* u = G.site_u + 4 * site[s];
+      const double2* u = tb.site_u + 4 * site[s];
       const double2 u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
@@ -154,7 +163,7 @@ __device__ void mc_stage(VT_alias<T>* lds, int tq, const int (&pos)[3], const in
       if (o.decay) {
         // the norm is that of the amplitudes as stored
         const int ne = __popcll(tm.g(t, li[i]) & dmask);
-        const double d = ne > 0 ? G.decay_pow[ne] : 1.0;
+        const double d = ne > 0 ? tb.decay_pow[ne] : 1.0;
         const VT_alias<T> r = make_c<T>(x[i].x * d, x[i].y * d);
         lds[swz<WB>(li[i])] = r;
         norm_acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
@@ -171,8 +180,8 @@ __device__ void mc_stage(VT_alias<T>* lds, int tq, const int (&pos)[3], const in
 // the last
 template <typename T>
 __device__ void mc_gates(VT_alias<T>* lds, int tq, int pos0, int site0, int cnt,
-                         const McwfGroupDev& G, const TileMap& tm, uint64_t t, StageOpts o,
-                         uint64_t dmask, double& norm_acc) {
+                         const McwfGroupDev& G, const McTabs& tb, const TileMap& tm, uint64_t t,
+                         StageOpts o, uint64_t dmask, double& norm_acc) {
   const int stages = cnt > 3 ? (cnt + 2) / 3 : 1;
   for (int st = 0; st < stages; ++st) {
     int pos[3], site[3];
@@ -190,7 +199,7 @@ __device__ void mc_gates(VT_alias<T>* lds, int tq, int pos0, int site0, int cnt,
       used |= 1u << b;
     }
     StageOpts so{st == 0 ? o.pre_scale : 1.0, o.bonds && st == 0, o.decay && st == stages - 1};
-    mc_stage<T>(lds, tq, pos, site, G, tm, t, so, dmask, norm_acc);
+    mc_stage<T>(lds, tq, pos, site, G, tb, tm, t, so, dmask, norm_acc);
   }
 }
 
@@ -231,8 +240,20 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
   const McwfTrajDev tr = A.trajs[blockIdx.x];
   const McwfGroupDev G = A.groups[tr.group];
   const int n = G.n;
+  __shared__ double2 tab_u[4 * kMcMaxQ], tab_chain[kMcMaxQ + 1];
+  __shared__ double tab_decay[kMcMaxQ + 1];
+  for (int i = threadIdx.x; i < 4 * n; i += kMcBlock) tab_u[i] = G.site_u[i];
+  for (int i = threadIdx.x; i <= n; i += kMcBlock) {
+    tab_decay[i] = G.decay_pow[i];
+    tab_chain[i] = G.uniform_chain ? G.chain_tab[i] : make_double2(1.0, 0.0);
+  }
+  __syncthreads();
+  const McTabs tb{tab_u, tab_chain, tab_decay};
   const uint64_t D = 1ull << n, dmask = D - 1;
-  V* psi = reinterpret_cast<V*>(G.psi) + (uint64_t)tr.idx * D;
+  // the state through a global-memory pointer (a pointer read from memory is
+  // generic, and generic loads also count against the LDS wait counter)
+  typedef __attribute__((address_space(1))) V GV;
+  V* psi = (V*)((GV*)G.psi + (uint64_t)tr.idx * D);
   const bool resident = n <= kMcTileQ;
   const int tq = resident ? (n < 3 ? 3 : n) : kMcTileQ;  // tiny states pad the tile with zeros
   const int h = resident ? 0 : n - kMcTileQ;
@@ -307,19 +328,19 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
     const double sc = s_scale;
     double norm_acc = 0.0;
     if (resident) {
-      mc_gates<T>(lds, tq, 0, 0, cntA, G, tmA, 0, StageOpts{sc, true, true}, dmask,
+      mc_gates<T>(lds, tq, 0, 0, cntA, G, tb, tmA, 0, StageOpts{sc, true, true}, dmask,
                   norm_acc);
     } else {
       double unused = 0.0;
       for (uint64_t t = 0; t < ntiles; ++t) {
         load_tile(tmA, t);
-        mc_gates<T>(lds, tq, 0, 0, cntA, G, tmA, t, StageOpts{sc, true, false}, dmask,
+        mc_gates<T>(lds, tq, 0, 0, cntA, G, tb, tmA, t, StageOpts{sc, true, false}, dmask,
                     unused);
         store_tile(tmA, t);
       }
       for (uint64_t t = 0; t < ntiles; ++t) {
         load_tile(tmB, t);
-        mc_gates<T>(lds, tq, kMcTileQ - h, kMcTileQ, h, G, tmB, t, StageOpts{1.0, false, true}, dmask,
+        mc_gates<T>(lds, tq, kMcTileQ - h, kMcTileQ, h, G, tb, tmB, t, StageOpts{1.0, false, true}, dmask,
                     norm_acc);
         store_tile(tmB, t);
       }
