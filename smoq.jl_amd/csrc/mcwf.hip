@@ -27,7 +27,8 @@ constexpr int kMcWaves = kMcBlock / 64;
 
 struct McwfGroupDev {
   void* psi;
-  int n, n_bonds, uniform_chain, pad;
+  int n, n_bonds, uniform_chain;
+  int site_rx;  // every site matrix real on the diagonal, imaginary off it
   const int32_t* bond_q;
   const double2* bond_diag;
   const double2* site_u;
@@ -91,6 +92,16 @@ struct TileMap {
   }
 };
 
+// a * p + i b * q for real a, b: (a p.x - b q.y, a p.y + b q.x), each
+// product rounded on its own as in the general complex form
+__device__ inline double2 rx_pair(double a, double b, double2 p, double2 q) {
+#pragma clang fp contract(off)
+  double2 r;
+  r.x = a * p.x - b * q.y;
+  r.y = a * p.y + b * q.x;
+  return r;
+}
+
 // the step's small tables, copied into LDS by the kernel: the stages read
 // them at every tile, and from global memory each read would wait out a
 // cache latency
@@ -147,6 +158,21 @@ __device__ void mc_stage(VT_alias<T>* lds, int tq, const int (&pos)[3], const in
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
       if (site[s] < 0) continue;
+      if (G.site_rx) {
+        // the products with the matrix's zero parts left out: the same
+        // roundings as the general form below
+        const double2* u = tb.site_u + 4 * site[s];
+        const double a0 = u[0].x, b0 = u[1].y, b1 = u[2].y, a1 = u[3].x;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          if (i & (1 << s)) continue;
+          const int j = i | (1 << s);
+          const double2 a = x[i], b = x[j];
+          x[i] = rx_pair(a0, b0, a, b);
+          x[j] = rx_pair(a1, b1, b, a);
+        }
+        continue;
+      }
       const double2* u = tb.site_u + 4 * site[s];
       const double2 u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
 #pragma unroll
@@ -763,7 +789,11 @@ extern "C" int smoq_mcwf_evolve(smoq_ctx* ctx, const smoq_mcwf_group* groups, in
     d.n = G.psi->n;
     d.n_bonds = G.n_bonds;
     d.uniform_chain = o.chain_ok ? 1 : 0;
-    d.pad = 0;
+    d.site_rx = 1;
+    for (int q = 0; q < d.n && d.site_rx; ++q) {
+      const double* u = G.site_u + 8 * q;  // (re, im) of u00, u01, u10, u11
+      d.site_rx = u[1] == 0.0 && u[2] == 0.0 && u[4] == 0.0 && u[7] == 0.0;
+    }
     d.bond_q = (const int32_t*)(dev + o.bond_q);
     d.bond_diag = (const double2*)(dev + o.bond_diag);
     d.site_u = (const double2*)(dev + o.site_u);
