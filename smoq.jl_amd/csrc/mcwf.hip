@@ -252,8 +252,85 @@ __device__ __forceinline__ void mc_load_full_tile(VT_alias<T>* lds, const VT_ali
   }
 }
 
+// Register stages of 16 amplitudes per thread (kMcBlock threads over one
+// 2^kMcTileQ tile), register index i over four tile positions: 0..3, 4..7
+// or 8..11, the other eight from the thread index.  The last map is the
+// coalesced one (16 contiguous amplitudes per group of 16 lanes).
+__device__ __forceinline__ uint32_t mc_l0(uint32_t tx, uint32_t i) { return (tx << 4) | i; }
+__device__ __forceinline__ uint32_t mc_l4(uint32_t tx, uint32_t i) {
+  return (tx & 15u) | (i << 4) | ((tx >> 4) << 8);
+}
+__device__ __forceinline__ uint32_t mc_l8(uint32_t tx, uint32_t i) { return tx | (i << 8); }
+// the thread index as a value the compiler cannot hoist out of a loop: the
+// 16-amplitude maps are then recomputed per tile (a few integer operations)
+// instead of held in registers across it
+__device__ __forceinline__ uint32_t mc_tx() {
+  uint32_t tx = threadIdx.x;
+  asm volatile("" : "+v"(tx));
+  return tx;
+}
+
+// the site gate of `site` on register bit B of the thread's 16 amplitudes
+template <int B>
+__device__ __forceinline__ void mc_site16(double2 (&x)[16], int site, const McwfGroupDev& G,
+                                          const McTabs& tb) {
+  const double2* u = tb.site_u + 4 * site;
+  if (G.site_rx) {
+    const double a0 = u[0].x, b0 = u[1].y, b1 = u[2].y, a1 = u[3].x;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i & (1 << B)) continue;
+      const int j = i | (1 << B);
+      const double2 a = x[i], b = x[j];
+      x[i] = rx_pair(a0, b0, a, b);
+      x[j] = rx_pair(a1, b1, b, a);
+    }
+  } else {
+    const double2 u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i & (1 << B)) continue;
+      const int j = i | (1 << B);
+      const double2 a = x[i], b = x[j];
+      x[i] = cfma2(u0, a, u1, b);
+      x[j] = cfma2(u2, a, u3, b);
+    }
+  }
+}
+
+// site gates on register bits lo..hi-1 in ascending order, site site0 + b
+// on bit b
+__device__ __forceinline__ void mc_sites16(double2 (&x)[16], int lo, int hi, int site0,
+                                           const McwfGroupDev& G, const McTabs& tb) {
+  // scheduling barriers keep the compiler from interleaving the four gates
+  // over all 16 amplitudes (more live values than the register budget)
+  if (lo <= 0 && 0 < hi) mc_site16<0>(x, site0, G, tb);
+  __builtin_amdgcn_sched_barrier(0);
+  if (lo <= 1 && 1 < hi) mc_site16<1>(x, site0 + 1, G, tb);
+  __builtin_amdgcn_sched_barrier(0);
+  if (lo <= 2 && 2 < hi) mc_site16<2>(x, site0 + 2, G, tb);
+  __builtin_amdgcn_sched_barrier(0);
+  if (lo <= 3 && 3 < hi) mc_site16<3>(x, site0 + 3, G, tb);
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// the bond phases of global index gi
+__device__ __forceinline__ double2 mc_bonds(double2 v, uint64_t gi, uint64_t dmask,
+                                            const McwfGroupDev& G, const McTabs& tb) {
+  gi &= dmask;
+  if (G.uniform_chain) {
+    const int nd = __popcll((gi ^ (gi >> 1)) & (dmask >> 1));
+    return cmul(tb.chain_tab[nd], v);
+  }
+  for (int b = 0; b < G.n_bonds; ++b) {
+    const int e = (int)((gi >> G.bond_q[2 * b]) & 1) | ((int)((gi >> G.bond_q[2 * b + 1]) & 1) << 1);
+    v = cmul(G.bond_diag[4 * b + e], v);
+  }
+  return v;
+}
+
 template <typename T>
-__global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
+__global__ void __launch_bounds__(kMcBlock, 2) k_mcwf(McwfArgs A) {
   using V = VT_alias<T>;
   constexpr int WB = CT<T>::kRowBits;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -357,18 +434,92 @@ __global__ void __launch_bounds__(kMcBlock) k_mcwf(McwfArgs A) {
       mc_gates<T>(lds, tq, 0, 0, cntA, G, tb, tmA, 0, StageOpts{sc, true, true}, dmask,
                   norm_acc);
     } else {
-      double unused = 0.0;
+      // Two passes per step, 16 amplitudes per thread in registers; every
+      // amplitude sees the same operations in the same order as in the
+      // gate list (scale, bonds, sites 0..n-1, decay).
+      typedef T v2 __attribute__((ext_vector_type(2)));
+      const v2* gsrc = reinterpret_cast<const v2*>(psi);
+      v2* ldv = reinterpret_cast<v2*>(lds);
+      double2 x[16];
+      // pass A: tiles of the contiguous qubits 0..11 (sites 0..11)
       for (uint64_t t = 0; t < ntiles; ++t) {
-        load_tile(tmA, t);
-        mc_gates<T>(lds, tq, 0, 0, cntA, G, tb, tmA, t, StageOpts{sc, true, false}, dmask,
-                    unused);
-        store_tile(tmA, t);
+        const uint64_t gb = t << kMcTileQ;
+        const uint32_t tx = mc_tx();
+        {
+          v2 v[16];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) v[i] = gsrc[gb | mc_l8(tx, i)];
+          __syncthreads();  // the previous tile's readers are done with LDS
+#pragma unroll
+          for (int i = 0; i < 16; ++i) ldv[swz<WB>(mc_l8(tx, i))] = v[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const v2 w = ldv[swz<WB>(mc_l0(tx, i))];
+          x[i] = make_double2((double)w.x * sc, (double)w.y * sc);
+          x[i] = mc_bonds(x[i], gb | mc_l0(tx, i), dmask, G, tb);
+        }
+        mc_sites16(x, 0, 4, 0, G, tb);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const VT_alias<T> r = make_c<T>(x[i].x, x[i].y);
+          lds[swz<WB>(mc_l0(tx, i))] = r;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[swz<WB>(mc_l4(tx, i))]);
+        mc_sites16(x, 0, 4, 4, G, tb);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) lds[swz<WB>(mc_l4(tx, i))] = make_c<T>(x[i].x, x[i].y);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[swz<WB>(mc_l8(tx, i))]);
+        mc_sites16(x, 0, 4, 8, G, tb);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) psi[gb | mc_l8(tx, i)] = make_c<T>(x[i].x, x[i].y);
       }
+      __syncthreads();  // pass A's stores are seen by every thread
+      // pass B: tiles of the low lb = 12 - h qubits and the high h qubits
+      // (sites 12..n-1 at tile positions lb..11), decay and norm
+      const int lb = kMcTileQ - h;
+      const uint32_t lowm = (1u << lb) - 1u;
+      auto gB = [&](uint64_t t, uint32_t l) {
+        return (uint64_t)(l & lowm) | (t << lb) | ((uint64_t)(l >> lb) << kMcTileQ);
+      };
       for (uint64_t t = 0; t < ntiles; ++t) {
-        load_tile(tmB, t);
-        mc_gates<T>(lds, tq, kMcTileQ - h, kMcTileQ, h, G, tb, tmB, t, StageOpts{1.0, false, true}, dmask,
-                    norm_acc);
-        store_tile(tmB, t);
+        const uint32_t tx = mc_tx();
+        if (lb >= 8) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const v2 w = gsrc[gB(t, mc_l8(tx, i))];
+            x[i] = make_double2((double)w.x, (double)w.y);
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const v2 w = gsrc[gB(t, mc_l4(tx, i))];
+            x[i] = make_double2((double)w.x, (double)w.y);
+          }
+          mc_sites16(x, lb - 4, 4, kMcTileQ - (lb - 4), G, tb);
+          __syncthreads();  // the previous tile's readers are done with LDS
+#pragma unroll
+          for (int i = 0; i < 16; ++i) lds[swz<WB>(mc_l4(tx, i))] = make_c<T>(x[i].x, x[i].y);
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[swz<WB>(mc_l8(tx, i))]);
+        }
+        const int lo8 = lb > 8 ? lb - 8 : 0;
+        mc_sites16(x, lo8, 4, kMcTileQ + 8 - lb, G, tb);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const uint64_t gi = gB(t, mc_l8(tx, i));
+          const int ne = __popcll(gi & dmask);
+          const double d = ne > 0 ? tb.decay_pow[ne] : 1.0;
+          const VT_alias<T> r = make_c<T>(x[i].x * d, x[i].y * d);
+          psi[gi] = r;
+          norm_acc += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+        }
       }
     }
     double nv[1] = {norm_acc};
