@@ -261,6 +261,12 @@ __device__ __forceinline__ uint32_t mc_l4(uint32_t tx, uint32_t i) {
   return (tx & 15u) | (i << 4) | ((tx >> 4) << 8);
 }
 __device__ __forceinline__ uint32_t mc_l8(uint32_t tx, uint32_t i) { return tx | (i << 8); }
+// LDS row of tile index l in the 16-amplitude stages: the low four bits
+// XORed with the two higher nibbles, so that for each of the three maps the
+// 16 lanes of a row group read 16 distinct 16-byte bank groups
+__device__ __forceinline__ uint32_t mc_swz(uint32_t l) {
+  return l ^ (((l >> 4) ^ (l >> 8)) & 15u);
+}
 // the thread index as a value the compiler cannot hoist out of a loop: the
 // 16-amplitude maps are then recomputed per tile (a few integer operations)
 // instead of held in registers across it
@@ -451,12 +457,12 @@ __global__ void __launch_bounds__(kMcBlock, 2) k_mcwf(McwfArgs A) {
           for (int i = 0; i < 16; ++i) v[i] = gsrc[gb | mc_l8(tx, i)];
           __syncthreads();  // the previous tile's readers are done with LDS
 #pragma unroll
-          for (int i = 0; i < 16; ++i) ldv[swz<WB>(mc_l8(tx, i))] = v[i];
+          for (int i = 0; i < 16; ++i) ldv[mc_swz(mc_l8(tx, i))] = v[i];
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-          const v2 w = ldv[swz<WB>(mc_l0(tx, i))];
+          const v2 w = ldv[mc_swz(mc_l0(tx, i))];
           x[i] = make_double2((double)w.x * sc, (double)w.y * sc);
           x[i] = mc_bonds(x[i], gb | mc_l0(tx, i), dmask, G, tb);
         }
@@ -464,17 +470,17 @@ __global__ void __launch_bounds__(kMcBlock, 2) k_mcwf(McwfArgs A) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const VT_alias<T> r = make_c<T>(x[i].x, x[i].y);
-          lds[swz<WB>(mc_l0(tx, i))] = r;
+          lds[mc_swz(mc_l0(tx, i))] = r;
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[swz<WB>(mc_l4(tx, i))]);
+        for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[mc_swz(mc_l4(tx, i))]);
         mc_sites16(x, 0, 4, 4, G, tb);
 #pragma unroll
-        for (int i = 0; i < 16; ++i) lds[swz<WB>(mc_l4(tx, i))] = make_c<T>(x[i].x, x[i].y);
+        for (int i = 0; i < 16; ++i) lds[mc_swz(mc_l4(tx, i))] = make_c<T>(x[i].x, x[i].y);
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[swz<WB>(mc_l8(tx, i))]);
+        for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[mc_swz(mc_l8(tx, i))]);
         mc_sites16(x, 0, 4, 8, G, tb);
 #pragma unroll
         for (int i = 0; i < 16; ++i) psi[gb | mc_l8(tx, i)] = make_c<T>(x[i].x, x[i].y);
@@ -504,10 +510,10 @@ __global__ void __launch_bounds__(kMcBlock, 2) k_mcwf(McwfArgs A) {
           mc_sites16(x, lb - 4, 4, kMcTileQ - (lb - 4), G, tb);
           __syncthreads();  // the previous tile's readers are done with LDS
 #pragma unroll
-          for (int i = 0; i < 16; ++i) lds[swz<WB>(mc_l4(tx, i))] = make_c<T>(x[i].x, x[i].y);
+          for (int i = 0; i < 16; ++i) lds[mc_swz(mc_l4(tx, i))] = make_c<T>(x[i].x, x[i].y);
           __syncthreads();
 #pragma unroll
-          for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[swz<WB>(mc_l8(tx, i))]);
+          for (int i = 0; i < 16; ++i) x[i] = to_d2(lds[mc_swz(mc_l8(tx, i))]);
         }
         const int lo8 = lb > 8 ? lb - 8 : 0;
         mc_sites16(x, lo8, 4, kMcTileQ + 8 - lb, G, tb);
