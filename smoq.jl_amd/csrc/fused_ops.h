@@ -311,11 +311,12 @@ __device__ __forceinline__ void apply_gate(VT<T> (&x)[kNR], const GateDev& g, ui
 // SEL clear hold ψ, those with it set λ at the same index.  For every ψ
 // pattern j (index k) the thread adds Im[conj(λ_k) (Gψ)_k] with
 // (Gψ)_k = i^CNT (-1)^{|(k ^ x) & z|} ψ_{k ^ x}, x on slots A, B (-1: none),
-// for k inside the controls.  The wave's sum goes to acc[wave] (written once
-// per tile; the pass adds the waves in order at its end).
-// NT: threads per tile; below a wave, the sum is a butterfly over the
-// tile's lanes.
-template <int A, int B, int SEL, int CNT, bool CTRL, int NT>
+// for k inside the controls.  With ROWS each 16-lane row's sum goes to
+// acc[row of the workgroup] (four DPP steps, no cross-row reads), else each
+// wave's sum to acc[wave]; written once per tile, the pass adds the slots in
+// order at its end.  NT: threads per tile; below a wave, the sum is a
+// butterfly over the tile's lanes into acc[0].
+template <int A, int B, int SEL, int CNT, bool CTRL, int NT, bool ROWS>
 __device__ __forceinline__ void reg_ovl(const VT<double> (&x)[kNR], const GateDev& g, uint64_t gl,
                                         double* acc) {
   constexpr int xr = (A >= 0 ? 1 << A : 0) | (B >= 0 ? 1 << B : 0);
@@ -349,13 +350,20 @@ __device__ __forceinline__ void reg_ovl(const VT<double> (&x)[kNR], const GateDe
     if (CTRL) c = (gon && (j & g.creg) == g.creg) ? c : 0.0;
     ov += c;
   }
-  if constexpr (NT >= 64) {
+  if constexpr (NT >= 64 && ROWS) {
+    ov += dpp_move<0xB1>(ov);   // quad_perm [1,0,3,2]
+    ov += dpp_move<0x4E>(ov);   // quad_perm [2,3,0,1]
+    ov += dpp_move<0x141>(ov);  // row_half_mirror
+    ov += dpp_move<0x140>(ov);  // row_mirror
+    if ((threadIdx.x & 15) == 0) acc[threadIdx.x >> 4] = ov;
+  } else if constexpr (NT >= 64) {
     ov = wave_total(ov);
+    if ((threadIdx.x & 63) == 0) acc[threadIdx.x >> 6] = ov;
   } else {
 #pragma unroll
     for (int off = NT / 2; off > 0; off >>= 1) ov += __shfl_xor(ov, off, NT);
+    if (threadIdx.x == 0) acc[0] = ov;
   }
-  if ((threadIdx.x & 63) == 0) acc[threadIdx.x >> 6] = ov;
 }
 
 // ---------------------------------------------------------------- frames --

@@ -238,13 +238,16 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
       // a uniform part (the pass is launched without dynamic LDS)
       << "  __shared__ __align__(16) VT<T> lds[" << (1 << pd.k) << "];\n"
       << "  PassFrame<T> f;\n";
-    // adjoint overlaps of this pass: one LDS slot per (overlap, wave)
+    // adjoint overlaps of this pass: one LDS slot per (overlap, 16-lane row),
+    // or per (overlap, wave) when that many row slots would cost the second
+    // workgroup per CU its LDS
     const int ntp = 1 << (pd.k - kRegQubits);
-    const int nwaves = ntp >= 64 ? ntp / 64 : 1;
     std::vector<int> oslots;
     for (int s = pd.s0; s < pd.s1; ++s)
       for (int g = prog.stages[s].g0; g < prog.stages[s].g1; ++g)
         if (prog.gates[g].kind == GK_OVL) oslots.push_back(prog.gates[g].oslot);
+    const bool ovl_rows = ntp >= 64 && oslots.size() * (size_t)(ntp / 16) * 8 <= 12288;
+    const int nwaves = ntp >= 64 ? (ovl_rows ? ntp / 16 : ntp / 64) : 1;
     if (!oslots.empty())
       o << "  __shared__ double ovl_acc[" << (int)oslots.size() * nwaves << "];\n";
     int ovl_i = 0;
@@ -383,7 +386,8 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
             break;
           case GK_OVL:
             o << "    reg_ovl<" << G.a << ", " << G.b << ", " << G.osel << ", " << G.ocnt << ", "
-              << ((G.creg | G.cglob) ? "true" : "false") << ", " << ntp << ">(x, " << gd
+              << ((G.creg | G.cglob) ? "true" : "false") << ", " << ntp << ", "
+              << (ovl_rows ? "true" : "false") << ">(x, " << gd
               << ", sf.gl, ovl_acc + " << ovl_i * nwaves << ");\n";
             ++ovl_i;
             break;
@@ -670,19 +674,13 @@ int jit_get_many(smoq_ctx* ctx, const std::vector<const Program*>& progs, int dt
 int jit_get_source(smoq_ctx* ctx, const std::string& src, size_t n_fn, const char* prefix,
                    std::shared_ptr<JitModule>& out) {
   out.reset();
-  const std::string key = std::string(prefix) + "\n" + src;
-  auto it = ctx->jit_cache.find(key);
-  if (it != ctx->jit_cache.end()) {
-    out = it->second;
-    return SMOQ_OK;
-  }
+  // the caller keeps the module (not the context's pass-module table)
   const std::string dir = cache_dir();
   const std::string name = dir.empty() ? std::string() : cache_name(src);
   std::vector<char> code;
   if (!dir.empty() && disk_load(dir, name, code)) {
     std::shared_ptr<JitModule> m;
     if (load_module(code, n_fn, m, prefix) == SMOQ_OK) {
-      ctx->jit_cache.emplace(key, m);
       out = m;
       return SMOQ_OK;
     }
@@ -695,7 +693,6 @@ int jit_get_source(smoq_ctx* ctx, const std::string& src, size_t n_fn, const cha
   const int rc = load_module(code, n_fn, m, prefix);
   if (rc) return rc;
   if (!dir.empty()) disk_store(dir, name, code);
-  ctx->jit_cache.emplace(key, m);
   out = m;
   return SMOQ_OK;
 }

@@ -550,6 +550,13 @@ static int expect_t(smoq_state* s, int n_terms, const uint64_t* xm, const uint64
         }
         jp->src.clear();
         jp->src.shrink_to_fit();
+        // a bounded table: many distinct term lists would otherwise keep a
+        // code object each (in-flight launches hold no reference: the
+        // stream is drained first)
+        if (ctx->obs_jit_plans.size() >= kObsJitPlansMax) {
+          SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+          ctx->obs_jit_plans.clear();
+        }
         ctx->obs_jit_plans.emplace(key, jp);
         jplan = jp;
       }

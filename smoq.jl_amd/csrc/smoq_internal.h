@@ -141,6 +141,7 @@ constexpr int kOjMaxSlots = 48;
 constexpr int kOjMaxSlotsC64 = 40;  // complex64 converts in registers
 // automatic mode: at least 2^kOjMinLogAmps amplitudes over the batch
 constexpr int kOjMinLogAmps = 22;
+constexpr size_t kObsJitPlansMax = 64;
 struct OjStage {
   uint32_t rmask;        // register tile bits
   int rpos[kOjRegBits];  // the same, ascending
@@ -192,7 +193,8 @@ struct smoq_ctx_s {
   // multi-mask Pauli term lists through tiled passes (SMOQ_OBS_TILES=0: off)
   int obs_tiles = 1;
   // ... compiled for the term list: 0 off, 1 automatic (large states), 2
-  // always (states of at least 12 qubits); plans cached by term list
+  // always (states of at least 12 qubits); plans and their modules cached
+  // by term list, at most kObsJitPlansMax (then the table starts over)
   int obs_jit = 1;
   std::unordered_map<std::string, std::shared_ptr<smoq::ObsJitPlan>> obs_jit_plans;
   // rocBLAS handle for dense density-matrix propagation, created on first use
@@ -320,8 +322,8 @@ int pass_tile(const smoq_ctx* ctx, int n_qubits, int batch, size_t n_gates);
 Fusion fusion_for(const smoq_ctx* ctx, int n_qubits, int dtype, int batch,
                   const std::vector<IGate>& gs);
 int jit_get(smoq_ctx* ctx, const Program& prog, int dtype, std::shared_ptr<JitModule>& out);
-// a module from generated source with kernels <prefix>0 .. <prefix>(n_fn-1),
-// cached per context (and on disk) by its source
+// a module from generated source with kernels <prefix>0 .. <prefix>(n_fn-1)
+// (compiled, or loaded from the on-disk cache); the caller keeps it
 int jit_get_source(smoq_ctx* ctx, const std::string& src, size_t n_fn, const char* prefix,
                    std::shared_ptr<JitModule>& out);
 // the compiled modules of several programs, the missing ones compiled on
