@@ -275,8 +275,9 @@ __global__ void k_seq_prefix(const typename CT<T>::V* __restrict__ psi, uint64_t
 //     grid rounds to results shifted by the same k u, except for a tie when
 //     k is odd (round-half-to-even flips with the grid index's parity).  So
 //     when G, S, E and E + (S - G) share a binade and no tie meets an odd
-//     k, the chunk's true end is E + (S - G), exact; any other chunk is run
-//     again from S (the few that start near or cross a power of two);
+//     k, the chunk's true end is E + (S - G), exact; a chunk whose terms are
+//     all zero ends where it starts; any other chunk is run again from S
+//     (the few that start near or cross a power of two);
 //  4. every chunk runs the chain from its exact start, writing s_i.
 constexpr uint32_t kSeqChunk = 4096;
 constexpr uint64_t kSeqParallelMin = 1ull << 18;  // shorter: one thread
@@ -386,9 +387,9 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 __global__ void __launch_bounds__(kSeqFixThreads)
     k_seq_fix(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
-              uint64_t nch, const double* __restrict__ guess, const double* __restrict__ ends,
-              const uint8_t* __restrict__ tie, double* __restrict__ start,
-              unsigned long long* __restrict__ reruns) {
+              uint64_t nch, const double* __restrict__ sums, const double* __restrict__ guess,
+              const double* __restrict__ ends, const uint8_t* __restrict__ tie,
+              double* __restrict__ start, unsigned long long* __restrict__ reruns) {
 #pragma clang fp contract(off)
   __shared__ double sg[kSeqFixBlock], se[kSeqFixBlock];
   __shared__ uint8_t st[kSeqFixBlock];
@@ -402,7 +403,9 @@ __global__ void __launch_bounds__(kSeqFixThreads)
     for (uint32_t k = threadIdx.x; k < nb; k += kSeqFixThreads) {
       sg[k] = guess[b0 + k];
       se[k] = ends[b0 + k];
-      st[k] = tie[b0 + k];
+      // bit 1: every term of the chunk is zero (its sum from zero is zero,
+      // the terms being >= 0), so the chain leaves any start unchanged
+      st[k] = tie[b0 + k] | (sums[b0 + k] == 0.0 ? 2 : 0);
     }
     if (threadIdx.x == 0) s_next = 0;
     __syncthreads();
@@ -417,6 +420,7 @@ __global__ void __launch_bounds__(kSeqFixThreads)
             S = e;
             continue;
           }
+          if (st[k] & 2) continue;  // zero terms: the end is the start
           // shift by d = S - g: k_ulp grid steps of g's binade (bit patterns
           // of positive doubles in one binade differ by that count)
           const int eg = dexp(g);
@@ -425,7 +429,7 @@ __global__ void __launch_bounds__(kSeqFixThreads)
             break;
           }
           const long long kd = (long long)(dbits(S) - dbits(g));
-          if (st[k] && (kd & 1)) break;
+          if ((st[k] & 1) && (kd & 1)) break;
           const uint64_t rb = dbits(e) + (uint64_t)kd;
           if ((int)((rb >> 52) & 0x7ff) != eg) break;
           S = __longlong_as_double((long long)rb);
@@ -856,7 +860,7 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
       hipLaunchKernelGGL(k_seq_chunk_ends<T>, dim3(gb), dim3(256), 0, ctx->stream, psi, n,
                          diag_step, nch, guess, ends, tie);
       hipLaunchKernelGGL(k_seq_fix<T>, dim3(1), dim3(kSeqFixThreads), 0, ctx->stream, psi, n,
-                         diag_step, nch, guess, ends, tie, start, reruns);
+                         diag_step, nch, sums, guess, ends, tie, start, reruns);
       hipLaunchKernelGGL(k_seq_write<T>, dim3(gb), dim3(256), 0, ctx->stream, psi, n, diag_step,
                          nch, start, c);
       SMOQ_CHECK_LAUNCH("sequential prefix kernels");

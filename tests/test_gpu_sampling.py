@@ -26,6 +26,10 @@ def structured_state(kind, N, seed):
     elif kind == "blocks":
         v = rng.normal(size=d) + 1j * rng.normal(size=d)
         v[rng.random(d) < 0.7] *= 1e-12
+    elif kind == "zero_chunks":  # every third 4096-term chunk all zero, and a zero tail
+        v = rng.normal(size=d) + 1j * rng.normal(size=d)
+        v[(np.arange(d) // 4096) % 3 == 1] = 0
+        v[d - d // 5:] = 0
     else:  # zero tail
         v = rng.normal(size=d) + 1j * rng.normal(size=d)
         v[d // 3:] = 0
@@ -107,7 +111,7 @@ def tie_state(N):
     return np.full(1 << N, 2.0 ** -10 + 2.0 ** -45, np.complex128)
 
 
-@pytest.mark.parametrize("kind", ["gauss", "lognormal", "blocks", "tail", "ties"])
+@pytest.mark.parametrize("kind", ["gauss", "lognormal", "blocks", "tail", "zero_chunks", "ties"])
 @pytest.mark.parametrize("N", [19, 20])
 def test_sequential_prefix_in_chunks_bit_exact(ctx, kind, N):
     """The reference's sequential running sum (sample_from_probabilities),
@@ -115,7 +119,8 @@ def test_sequential_prefix_in_chunks_bit_exact(ctx, kind, N):
     the oracle's chain bit for bit: draws placed exactly on the oracle's
     prefix values and one grid step either side of them land where the
     oracle's do.  States cross many binades (lognormal, blocks), end in
-    zeros (tail) or round a tie at every step above 1/2 (ties)."""
+    zeros (tail), hold whole chunks of zeros (zero_chunks) or round a tie at
+    every step above 1/2 (ties)."""
     from smoq_amd import GPUState
     from smoq_amd import measurements as me
 
