@@ -281,8 +281,8 @@ __global__ void k_seq_prefix(const typename CT<T>::V* __restrict__ psi, uint64_t
 //  4. every chunk runs the chain from its exact start, writing s_i.
 constexpr uint32_t kSeqChunk = 4096;
 constexpr uint64_t kSeqParallelMin = 1ull << 18;  // shorter: one thread
-constexpr int kSeqFixThreads = 256;
-constexpr uint32_t kSeqFixBlock = 2048;  // chunk records staged per round
+constexpr int kSeqFixThreads = 1024;
+constexpr uint64_t kSeqFixWin = 16 * kSeqFixThreads;  // chunks per scan window
 
 template <typename T>
 __device__ __forceinline__ double seq_term(const typename CT<T>::V* __restrict__ psi, uint64_t i,
@@ -290,27 +290,89 @@ __device__ __forceinline__ double seq_term(const typename CT<T>::V* __restrict__
   return diag_step ? (double)psi[i * diag_step].x : prob_of<T>(psi[i]);
 }
 
-// pass 1: the chain of each chunk from zero (kSeqChunk terms, loads of a
-// group ahead of its sums)
+// The chunk passes: one thread per chunk, kSeqStageBlock chunks per
+// workgroup.  Each round stages the next kSeqStageTerms terms of all the
+// workgroup's chunks through LDS with coalesced loads (16 lanes read one
+// chunk's 16 contiguous terms), then every thread adds its chunk's terms in
+// order.  The writing pass puts its prefixes back into the same buffer and
+// stores them coalesced.
+__device__ __forceinline__ uint64_t dbits(double x) { return (uint64_t)__double_as_longlong(x); }
+__device__ __forceinline__ int dexp(double x) { return (int)((dbits(x) >> 52) & 0x7ff); }
+
+constexpr int kSeqStageBlock = 256;
+constexpr int kSeqStageTerms = 16;
+static_assert(kSeqChunk % kSeqStageTerms == 0, "chunk length");
+
 template <typename T>
-__global__ void __launch_bounds__(256)
-    k_seq_chunk_sums(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
-                     uint64_t nch, double* __restrict__ sums) {
-#pragma clang fp contract(off)
-  const uint64_t ch = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ch >= nch) return;
-  const uint64_t i0 = ch * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
-  double s = 0.0;
-  uint64_t i = i0;
-  for (; i + kBatch <= i1; i += kBatch) {
-    double p[kBatch];
+__device__ __forceinline__ void seq_stage(const typename CT<T>::V* __restrict__ psi, uint64_t n,
+                                          uint64_t diag_step, uint64_t ch0, uint32_t off,
+                                          double (*buf)[kSeqStageTerms + 1]) {
+  double v[kSeqStageTerms];
 #pragma unroll
-    for (uint32_t u = 0; u < kBatch; ++u) p[u] = seq_term<T>(psi, i + u, diag_step);
-#pragma unroll
-    for (uint32_t u = 0; u < kBatch; ++u) s = s + p[u];
+  for (int j = 0; j < kSeqStageTerms; ++j) {
+    const uint32_t L = (uint32_t)j * kSeqStageBlock + threadIdx.x;
+    const uint64_t i = (ch0 + L / kSeqStageTerms) * kSeqChunk + off + L % kSeqStageTerms;
+    v[j] = i < n ? seq_term<T>(psi, i, diag_step) : 0.0;
   }
-  for (; i < i1; ++i) s = s + seq_term<T>(psi, i, diag_step);
-  sums[ch] = s;
+#pragma unroll
+  for (int j = 0; j < kSeqStageTerms; ++j) {
+    const uint32_t L = (uint32_t)j * kSeqStageBlock + threadIdx.x;
+    buf[L / kSeqStageTerms][L % kSeqStageTerms] = v[j];
+  }
+}
+
+// MODE 0: the chain of each chunk from zero -> sums[ch]
+// MODE 1: from guess[ch] -> ends[ch], tie[ch] = 1 when a step's exact sum lay
+//         halfway between grid points (TwoSum error, compared with the
+//         spacing above and below the result)
+// MODE 2: from start[ch], writing every prefix to c
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kSeqStageBlock)
+    k_seq_chunks(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
+                 uint64_t nch, const double* __restrict__ from, double* __restrict__ out,
+                 uint8_t* __restrict__ tie) {
+#pragma clang fp contract(off)
+  __shared__ double buf[kSeqStageBlock][kSeqStageTerms + 1];
+  const uint64_t ch0 = (uint64_t)blockIdx.x * kSeqStageBlock;
+  const uint64_t ch = ch0 + threadIdx.x;
+  const bool mine = ch < nch;
+  double a = MODE == 0 ? 0.0 : (mine ? from[ch] : 0.0);
+  bool t = false;
+  for (uint32_t off = 0; off < kSeqChunk; off += kSeqStageTerms) {
+    seq_stage<T>(psi, n, diag_step, ch0, off, buf);
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int u = 0; u < kSeqStageTerms; ++u) {
+        const double p = buf[threadIdx.x][u];
+        const double s = a + p;
+        if (MODE == 1) {
+          const double bp = s - a;
+          const double err = (a - (s - bp)) + (p - bp);
+          if (err != 0.0) {
+            const double e2 = 2.0 * fabs(err);
+            const double up = __longlong_as_double((long long)(dbits(s) + 1)) - s;
+            t |= e2 == up || 2.0 * e2 == up;
+          }
+        }
+        a = s;
+        if (MODE == 2) buf[threadIdx.x][u] = s;
+      }
+    }
+    __syncthreads();
+    if (MODE == 2) {
+#pragma unroll
+      for (int j = 0; j < kSeqStageTerms; ++j) {
+        const uint32_t L = (uint32_t)j * kSeqStageBlock + threadIdx.x;
+        const uint64_t c = ch0 + L / kSeqStageTerms;
+        const uint64_t i = c * kSeqChunk + off + L % kSeqStageTerms;
+        if (c < nch && i < n) out[i] = buf[L / kSeqStageTerms][L % kSeqStageTerms];
+      }
+      __syncthreads();
+    }
+  }
+  if (mine && MODE != 2) out[ch] = a;
+  if (mine && MODE == 1) tie[ch] = t ? 1 : 0;
 }
 
 // exclusive prefix of the chunk sums (a guess only: any order will do), one
@@ -340,50 +402,26 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-__device__ __forceinline__ uint64_t dbits(double x) { return (uint64_t)__double_as_longlong(x); }
-__device__ __forceinline__ int dexp(double x) { return (int)((dbits(x) >> 52) & 0x7ff); }
 
-// pass 2: the chain of each chunk from its guess; tie[ch] = 1 when a step's
-// exact sum lay halfway between grid points (TwoSum error, compared with the
-// spacing above and below the result)
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_seq_chunk_ends(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
-                     uint64_t nch, const double* __restrict__ guess, double* __restrict__ ends,
-                     uint8_t* __restrict__ tie) {
-#pragma clang fp contract(off)
-  const uint64_t ch = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ch >= nch) return;
-  const uint64_t i0 = ch * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
-  double a = guess[ch];
-  bool t = false;
-  auto step = [&](double p) {
-    const double s = a + p;
-    const double bp = s - a;
-    const double err = (a - (s - bp)) + (p - bp);
-    if (err != 0.0) {
-      const double e2 = 2.0 * fabs(err);
-      const double up = __longlong_as_double((long long)(dbits(s) + 1)) - s;
-      t |= e2 == up || 2.0 * e2 == up;
-    }
-    a = s;
-  };
-  uint64_t i = i0;
-  for (; i + kBatch <= i1; i += kBatch) {
-    double p[kBatch];
-#pragma unroll
-    for (uint32_t u = 0; u < kBatch; ++u) p[u] = seq_term<T>(psi, i + u, diag_step);
-#pragma unroll
-    for (uint32_t u = 0; u < kBatch; ++u) step(p[u]);
-  }
-  for (; i < i1; ++i) step(seq_term<T>(psi, i, diag_step));
-  ends[ch] = a;
-  tie[ch] = t ? 1 : 0;
+// pass 3: the exact start of every chunk (one workgroup).  Inside a binade
+// the walk is integer arithmetic on bit patterns: B(S_{k+1}) = B(S_k) +
+// (B(E_k) - B(G_k)) for every chunk that meets the conditions above (0 for a
+// chunk of zeros), so the starts over a window of chunks are one prefix sum.
+// The first chunk in the window that fails them is run again from its exact
+// start (its terms staged in LDS, one thread adding them in order), and the
+// next window begins after it.
+__device__ __forceinline__ bool seq_shift_ok(double S, double g, double e, bool tie, bool zero) {
+  if (zero || S == g) return true;
+  const int eg = dexp(g);
+  if (!(S > 0.0) || !(g > 0.0) || !(e > 0.0) || eg == 0 || eg == 0x7ff || dexp(S) != eg ||
+      dexp(e) != eg)
+    return false;
+  const long long kd = (long long)(dbits(S) - dbits(g));
+  if (tie && (kd & 1)) return false;
+  const uint64_t rb = dbits(e) + (uint64_t)kd;
+  return (int)((rb >> 52) & 0x7ff) == eg;
 }
 
-// pass 3: the exact start of every chunk, chunks in order (one workgroup;
-// thread 0 walks the staged records, all threads stage a chunk's terms when
-// it has to run again)
 template <typename T>
 __global__ void __launch_bounds__(kSeqFixThreads)
     k_seq_fix(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
@@ -391,101 +429,84 @@ __global__ void __launch_bounds__(kSeqFixThreads)
               const double* __restrict__ ends, const uint8_t* __restrict__ tie,
               double* __restrict__ start, unsigned long long* __restrict__ reruns) {
 #pragma clang fp contract(off)
-  __shared__ double sg[kSeqFixBlock], se[kSeqFixBlock];
-  __shared__ uint8_t st[kSeqFixBlock];
   __shared__ double terms[kSeqChunk];
-  __shared__ double s_cur;
-  __shared__ long long s_next;  // next chunk to walk (block-relative), or -1 - chunk to rerun
-  if (threadIdx.x == 0) s_cur = 0.0;
+  __shared__ long long wsum[kSeqFixThreads / 64];
+  __shared__ unsigned long long s_bad;
+  __shared__ double s_S;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // the pattern step of chunk k: 0 for a chunk of zeros
+  auto dstep = [&](uint64_t k) -> long long {
+    return sums[k] == 0.0 ? 0ll : (long long)(dbits(ends[k]) - dbits(guess[k]));
+  };
+  if (tid == 0) s_S = 0.0;
+  __syncthreads();
   unsigned long long nrerun = 0;
-  for (uint64_t b0 = 0; b0 < nch; b0 += kSeqFixBlock) {
-    const uint32_t nb = (uint32_t)(nch - b0 < kSeqFixBlock ? nch - b0 : kSeqFixBlock);
-    for (uint32_t k = threadIdx.x; k < nb; k += kSeqFixThreads) {
-      sg[k] = guess[b0 + k];
-      se[k] = ends[b0 + k];
-      // bit 1: every term of the chunk is zero (its sum from zero is zero,
-      // the terms being >= 0), so the chain leaves any start unchanged
-      st[k] = tie[b0 + k] | (sums[b0 + k] == 0.0 ? 2 : 0);
+  uint64_t k0 = 0;
+  while (k0 < nch) {
+    const double S0 = s_S;
+    const uint64_t k1 = nch - k0 < kSeqFixWin ? nch : k0 + kSeqFixWin;
+    const uint64_t per = (k1 - k0 + kSeqFixThreads - 1) / kSeqFixThreads;
+    const uint64_t a0 = k0 + (uint64_t)tid * per;
+    const uint64_t a = a0 < k1 ? a0 : k1, b = a + per < k1 ? a + per : k1;
+    long long loc = 0;
+    for (uint64_t k = a; k < b; ++k) loc += dstep(k);
+    long long incl = loc;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const long long y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
     }
-    if (threadIdx.x == 0) s_next = 0;
+    if (lane == 63) wsum[wid] = incl;
+    if (tid == 0) s_bad = ~0ull;
     __syncthreads();
-    while (true) {
-      if (threadIdx.x == 0) {
-        double S = s_cur;
-        long long k = s_next;
-        for (; k < (long long)nb; ++k) {
-          const double g = sg[k], e = se[k];
-          start[b0 + k] = S;
-          if (S == g) {
-            S = e;
-            continue;
-          }
-          if (st[k] & 2) continue;  // zero terms: the end is the start
-          // shift by d = S - g: k_ulp grid steps of g's binade (bit patterns
-          // of positive doubles in one binade differ by that count)
-          const int eg = dexp(g);
-          if (!(S > 0.0) || !(g > 0.0) || !(e > 0.0) || eg == 0 || eg == 0x7ff ||
-              dexp(S) != eg || dexp(e) != eg) {
-            break;
-          }
-          const long long kd = (long long)(dbits(S) - dbits(g));
-          if ((st[k] & 1) && (kd & 1)) break;
-          const uint64_t rb = dbits(e) + (uint64_t)kd;
-          if ((int)((rb >> 52) & 0x7ff) != eg) break;
-          S = __longlong_as_double((long long)rb);
-        }
-        s_cur = S;
-        s_next = k < (long long)nb ? -1 - k : (long long)nb;
+    long long base = incl - loc;
+    for (int w = 0; w < wid; ++w) base += wsum[w];
+    // the first chunk whose shift is not exact
+    uint64_t bS = dbits(S0) + (uint64_t)base;
+    for (uint64_t k = a; k < b; ++k) {
+      if (!seq_shift_ok(__longlong_as_double((long long)bS), guess[k], ends[k], tie[k] != 0,
+                        sums[k] == 0.0)) {
+        atomicMin(&s_bad, (unsigned long long)k);
+        break;
       }
-      __syncthreads();
-      const long long nx = s_next;
-      if (nx >= 0) break;  // block done
-      // run chunk b0 + r again from the exact start s_cur
-      const long long r = -1 - nx;
-      const uint64_t ch = b0 + (uint64_t)r;
-      const uint64_t i0 = ch * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
-      for (uint64_t i = i0 + threadIdx.x; i < i1; i += kSeqFixThreads)
-        terms[i - i0] = seq_term<T>(psi, i, diag_step);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double S = s_cur;
-        for (uint64_t i = 0; i < i1 - i0; ++i) S = S + terms[i];
-        s_cur = S;
-        s_next = r + 1;
-        ++nrerun;
-      }
-      __syncthreads();
+      bS += (uint64_t)dstep(k);
     }
     __syncthreads();
-  }
-  if (threadIdx.x == 0) *reruns = nrerun;
-}
-
-// pass 4: every chunk's chain from its exact start, writing the sums
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_seq_write(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
-                uint64_t nch, const double* __restrict__ start, double* __restrict__ c) {
-#pragma clang fp contract(off)
-  const uint64_t ch = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ch >= nch) return;
-  const uint64_t i0 = ch * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
-  double s = start[ch];
-  uint64_t i = i0;
-  for (; i + kBatch <= i1; i += kBatch) {
-    double p[kBatch];
-#pragma unroll
-    for (uint32_t u = 0; u < kBatch; ++u) p[u] = seq_term<T>(psi, i + u, diag_step);
-#pragma unroll
-    for (uint32_t u = 0; u < kBatch; ++u) {
-      s = s + p[u];
-      c[i + u] = s;
+    const uint64_t bad = s_bad;
+    bS = dbits(S0) + (uint64_t)base;
+    for (uint64_t k = a; k < b && k <= bad; ++k) {
+      start[k] = __longlong_as_double((long long)bS);
+      if (k == bad) {
+        s_S = __longlong_as_double((long long)bS);
+        break;
+      }
+      bS += (uint64_t)dstep(k);
     }
+    if (bad == ~0ull && tid == kSeqFixThreads - 1) {
+      long long tot = 0;
+      for (int w = 0; w < kSeqFixThreads / 64; ++w) tot += wsum[w];
+      s_S = __longlong_as_double((long long)(dbits(S0) + (uint64_t)tot));
+    }
+    __syncthreads();
+    if (bad == ~0ull) {
+      k0 = k1;
+      continue;
+    }
+    // run chunk `bad` again from its exact start
+    const uint64_t i0 = bad * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
+    for (uint64_t i = i0 + tid; i < i1; i += kSeqFixThreads)
+      terms[i - i0] = seq_term<T>(psi, i, diag_step);
+    __syncthreads();
+    if (tid == 0) {
+      double S = s_S;
+      for (uint64_t i = 0; i < i1 - i0; ++i) S = S + terms[i];
+      s_S = S;
+    }
+    ++nrerun;
+    __syncthreads();
+    k0 = bad + 1;
   }
-  for (; i < i1; ++i) {
-    s = s + seq_term<T>(psi, i, diag_step);
-    c[i] = s;
-  }
+  if (tid == 0) *reruns = nrerun;
 }
 
 // first i with r <= c[i] (c is non-decreasing), else n-1
@@ -853,16 +874,16 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
       double* start = ends + nch;
       unsigned long long* reruns = (unsigned long long*)(start + nch);
       uint8_t* tie = (uint8_t*)(reruns + 1);
-      const unsigned gb = (unsigned)((nch + 255) / 256);
-      hipLaunchKernelGGL(k_seq_chunk_sums<T>, dim3(gb), dim3(256), 0, ctx->stream, psi, n,
-                         diag_step, nch, sums);
+      const unsigned gb = (unsigned)((nch + kSeqStageBlock - 1) / kSeqStageBlock);
+      hipLaunchKernelGGL((k_seq_chunks<T, 0>), dim3(gb), dim3(kSeqStageBlock), 0, ctx->stream,
+                         psi, n, diag_step, nch, nullptr, sums, nullptr);
       hipLaunchKernelGGL(k_seq_guess, dim3(1), dim3(1024), 0, ctx->stream, sums, nch, guess);
-      hipLaunchKernelGGL(k_seq_chunk_ends<T>, dim3(gb), dim3(256), 0, ctx->stream, psi, n,
-                         diag_step, nch, guess, ends, tie);
+      hipLaunchKernelGGL((k_seq_chunks<T, 1>), dim3(gb), dim3(kSeqStageBlock), 0, ctx->stream,
+                         psi, n, diag_step, nch, guess, ends, tie);
       hipLaunchKernelGGL(k_seq_fix<T>, dim3(1), dim3(kSeqFixThreads), 0, ctx->stream, psi, n,
                          diag_step, nch, sums, guess, ends, tie, start, reruns);
-      hipLaunchKernelGGL(k_seq_write<T>, dim3(gb), dim3(256), 0, ctx->stream, psi, n, diag_step,
-                         nch, start, c);
+      hipLaunchKernelGGL((k_seq_chunks<T, 2>), dim3(gb), dim3(kSeqStageBlock), 0, ctx->stream,
+                         psi, n, diag_step, nch, start, c, nullptr);
       SMOQ_CHECK_LAUNCH("sequential prefix kernels");
     }
     if (sem != SMOQ_SAMPLE_CUMSUM_SEARCH) {
