@@ -345,11 +345,16 @@ __device__ __forceinline__ void reg_ovl(const VT<double> (&x)[kNR], const GateDe
       ux = w.y;
       uy = -w.x;
     }
-    double c = l.x * uy - l.y * ux;
-    if ((zp ^ __popc(jp & zr)) & 1) c = -c;
-    if (CTRL) c = (gon && (j & g.creg) == g.creg) ? c : 0.0;
-    ov += c;
+    const double c = l.x * uy - l.y * ux;
+    // the pattern's sign, and with controls whether it counts, are the same
+    // on every lane: a uniform factor of +-1 or 0
+    double sj = (__popc(jp & zr) & 1) ? -1.0 : 1.0;
+    if (CTRL) sj = ((j & g.creg) == g.creg) ? sj : 0.0;
+    ov = fma(c, sj, ov);
   }
+  // the lane's own sign and controls (negation commutes with rounding)
+  if (zp) ov = -ov;
+  if (CTRL && !gon) ov = 0.0;
   if constexpr (NT >= 64 && ROWS) {
     ov += dpp_move<0xB1>(ov);   // quad_perm [1,0,3,2]
     ov += dpp_move<0x4E>(ov);   // quad_perm [2,3,0,1]
