@@ -25,8 +25,9 @@
 // r inside such a step's interval (first prefix of the leaf, last prefix
 // before it] could make the reference's search path-dependent, so those
 // draws replay Julia's halving search itself with every probed prefix
-// recomputed from its leaf.  The intervals are collected on the device and
-// merged on the host, so the check per draw is one binary search.
+// recomputed from its leaf.  The intervals are collected and sorted by lower
+// end on the device, with the running maximum of their upper ends, so the
+// check per draw is one binary search.
 //
 // SMOQ_SAMPLE_SEQUENTIAL follows sample_from_probabilities
 // (utils/cpu/cpuQuantumStateMeasurements.jl:203-213): a strictly sequential
@@ -36,6 +37,8 @@
 #include <vector>
 
 #include "device_common.h"
+#include <hipcub/hipcub.hpp>
+
 #include "smoq_internal.h"
 
 namespace smoq {
@@ -628,8 +631,8 @@ __global__ void k_leaf_ends(LevelSet ls, int D, uint64_t nv, const double* __res
   if (f < e_prev) {
     const unsigned long long k = atomicAdd(n_dips, 1ull);
     if (k < cap) {
-      dips[2 * k] = f;
-      dips[2 * k + 1] = e_prev;
+      dips[k] = f;
+      dips[cap + k] = e_prev;
     }
   }
 }
@@ -680,8 +683,9 @@ __device__ inline double prefix_at(const typename CT<T>::V* __restrict__ psi, co
 }
 
 // One draw per thread: the leaf-end search and the in-leaf recomputation,
-// or, for a draw inside a merged step interval (ulo[k], uhi[k]], Julia's
-// searchsortedfirst loop over the recomputed prefixes.
+// or, for a draw inside a step interval (ulo: lower ends, ascending; uhi:
+// running maximum of the upper ends), Julia's searchsortedfirst loop over
+// the recomputed prefixes.
 template <typename T>
 __global__ void k_search_leaves(const typename CT<T>::V* __restrict__ psi, LevelSet ls, int D,
                                 uint64_t n, const double* __restrict__ E, uint64_t nv,
@@ -694,7 +698,8 @@ __global__ void k_search_leaves(const typename CT<T>::V* __restrict__ psi, Level
   const double x = r[d];
   const double c0 = ls.lv[0].carry[0];
   if (n_union > 0) {
-    // the last interval whose lower end lies below x
+    // the last interval whose lower end lies below x: x is inside one iff
+    // the running maximum of the upper ends there reaches it
     int a = 0, b = n_union;
     while (a < b) {
       const int m = (a + b) >> 1;
@@ -773,25 +778,7 @@ __global__ void k_search_leaves(const typename CT<T>::V* __restrict__ psi, Level
 
 static unsigned nblk(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-// merges the step intervals (lo, hi] of the dip list into sorted disjoint ones
-static void merge_intervals(std::vector<double>& d, std::vector<double>& lo,
-                            std::vector<double>& hi) {
-  const size_t k = d.size() / 2;
-  std::vector<std::pair<double, double>> iv(k);
-  for (size_t i = 0; i < k; ++i) iv[i] = {d[2 * i], d[2 * i + 1]};
-  std::sort(iv.begin(), iv.end());
-  lo.clear();
-  hi.clear();
-  for (const auto& a : iv) {
-    if (!lo.empty() && a.first <= hi.back()) {
-      hi.back() = std::max(hi.back(), a.second);
-    } else {
-      lo.push_back(a.first);
-      hi.push_back(a.second);
-    }
-  }
-}
-
+// step intervals (lo, hi] listed at most
 constexpr unsigned long long kDipCap = 1ull << 20;
 
 // The draws' uniforms go to the device through the context's pinned
@@ -922,7 +909,19 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   // long vectors: the tree and the leaf-end buffers live in the context's
   // sampling buffer, whose node geometry is kept for the next call of the
   // same length
-  const size_t leaf_bytes = up(nv * sizeof(double)) * 2 + up(2 * kDipCap * sizeof(double)) + 256;
+  // the step intervals: lower ends, upper ends, both sorted by lower end,
+  // then the device sort's temporary storage
+  size_t sort_bytes = 0, scan_bytes = 0;
+  SMOQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const double*)nullptr,
+                                              (double*)nullptr, (const double*)nullptr,
+                                              (double*)nullptr, (int)kDipCap, 0, 64,
+                                              ctx->stream));
+  SMOQ_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, scan_bytes, (const double*)nullptr,
+                                             (double*)nullptr, hipcub::Max(), (int)kDipCap,
+                                             ctx->stream));
+  const size_t dip_temp = up(std::max(sort_bytes, scan_bytes));
+  const size_t leaf_bytes = up(nv * sizeof(double)) * 2 + up(4 * kDipCap * sizeof(double)) +
+                            dip_temp + 256;
   if (ctx->tree_dev_bytes < tree_bytes + leaf_bytes) {
     if (ctx->tree_dev) {
       SMOQ_HIP(hipStreamSynchronize(ctx->stream));
@@ -945,8 +944,9 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   double* first = (double*)tl;
   double* E = (double*)(tl + up(nv * sizeof(double)));
   double* dips = (double*)(tl + 2 * up(nv * sizeof(double)));
-  unsigned long long* ndips =
-      (unsigned long long*)(tl + 2 * up(nv * sizeof(double)) + up(2 * kDipCap * sizeof(double)));
+  double* dips_sorted = dips + 2 * kDipCap;
+  void* dip_tmp = (void*)(tl + 2 * up(nv * sizeof(double)) + up(4 * kDipCap * sizeof(double)));
+  unsigned long long* ndips = (unsigned long long*)((unsigned char*)dip_tmp + dip_temp);
   if (ctx->tree_n != n) {
     // root: elements 2..n (1-based) = [1, n) 0-based
     const uint64_t rs = 1;
@@ -990,29 +990,35 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   unsigned long long n_dips = 0;
   SMOQ_HIP(hipMemcpyAsync(&n_dips, ndips, sizeof(n_dips), hipMemcpyDeviceToHost, ctx->stream));
   SMOQ_HIP(hipStreamSynchronize(ctx->stream));
-  // the step intervals, merged; more than the list holds: every draw takes
-  // the reference's search itself (one interval covering every value)
-  std::vector<double> ulo, uhi;
+  // the step intervals sorted by lower end with the running maximum of the
+  // upper ends: a draw x lies in their union iff, for the last interval
+  // whose lower end is below x, that maximum reaches x.  More intervals
+  // than the list holds: every draw takes the reference's search itself
+  // (one interval covering every value).
+  int n_union = 0;
+  const double* ulo = dips_sorted;
+  const double* uhi = dips_sorted + kDipCap;
   if (n_dips > kDipCap) {
-    ulo.push_back(-HUGE_VAL);
-    uhi.push_back(HUGE_VAL);
-  } else if (n_dips > 0) {
-    std::vector<double> hd(2 * n_dips);
-    SMOQ_HIP(hipMemcpyAsync(hd.data(), dips, hd.size() * sizeof(double), hipMemcpyDeviceToHost,
+    const double all[2] = {-HUGE_VAL, HUGE_VAL};
+    SMOQ_HIP(hipMemcpyAsync(dips_sorted, &all[0], sizeof(double), hipMemcpyHostToDevice,
                             ctx->stream));
-    SMOQ_HIP(hipStreamSynchronize(ctx->stream));
-    merge_intervals(hd, ulo, uhi);
-  }
-  const int n_union = (int)ulo.size();
-  if (n_union > 0) {
-    // the dip list is read: its space holds the merged intervals now
-    SMOQ_HIP(hipMemcpyAsync(dips, ulo.data(), n_union * sizeof(double), hipMemcpyHostToDevice,
-                            ctx->stream));
-    SMOQ_HIP(hipMemcpyAsync(dips + kDipCap, uhi.data(), n_union * sizeof(double),
+    SMOQ_HIP(hipMemcpyAsync(dips_sorted + kDipCap, &all[1], sizeof(double),
                             hipMemcpyHostToDevice, ctx->stream));
+    SMOQ_HIP(hipStreamSynchronize(ctx->stream));  // the host values live on the stack
+    n_union = 1;
+  } else if (n_dips > 0) {
+    const int nk = (int)n_dips;
+    size_t tb_sort = dip_temp, tb_scan = dip_temp;
+    SMOQ_HIP(hipcub::DeviceRadixSort::SortPairs(dip_tmp, tb_sort, (const double*)dips,
+                                                dips_sorted, (const double*)(dips + kDipCap),
+                                                dips, nk, 0, 64, ctx->stream));
+    SMOQ_HIP(hipcub::DeviceScan::InclusiveScan(dip_tmp, tb_scan, (const double*)dips,
+                                               dips_sorted + kDipCap, hipcub::Max(), nk,
+                                               ctx->stream));
+    n_union = nk;
   }
   hipLaunchKernelGGL(k_search_leaves<T>, dim3(nblk(nd)), dim3(256), 0, ctx->stream, psi, ls, D, n,
-                     E, nv, dips, dips + kDipCap, n_union, rd, nd, od);
+                     E, nv, ulo, uhi, n_union, rd, nd, od);
   SMOQ_CHECK_LAUNCH("k_search_leaves");
   return finish_draws(ctx, od, nd, out_host);
 }
