@@ -1129,8 +1129,11 @@ def main():
         result["cpu_baseline"] = cpu_baseline(tape, N)
         log(f"CPU baseline: {result['cpu_baseline']['value']:.3f} gate-ops/s, "
             f"{result['cpu_baseline']['sample']}")
+    # the headline figures of every extra again, last on the line, where a
+    # log tail shows them
+    result["summary"] = summary(result)
     print(json.dumps(result), flush=True)
-    log("summary " + json.dumps(summary(result)))
+    log("summary " + json.dumps(result["summary"]))
 
 
 def summary(result):
@@ -1170,6 +1173,15 @@ def summary(result):
     if "mcwf" in x:
         out["mcwf_s"] = x["mcwf"]["seconds"]
         out["mcwf_GBps"] = x["mcwf"]["hbm_GBps_two_pass_sizes"]
+    if "reference_workloads" in x:
+        rw = x["reference_workloads"]
+        for key, name in (("random_circuit_sampling", "ref_circuit_sampling"),
+                          ("expectation_values", "ref_expectation_values"),
+                          ("mcwf_dephasing", "ref_mcwf_dephasing")):
+            by = rw.get(key, {}).get("by_N", {})
+            if by:
+                n_max = max(by, key=int)
+                out[name + "_speedup_N" + n_max] = by[n_max].get("speedup")
     if "cpu_baseline" in result:
         out["cpu_gate_ops_per_s"] = result["cpu_baseline"]["value"]
     return out
