@@ -157,46 +157,61 @@ __device__ __forceinline__ VT<T> u2_row_z(const double* m, int r, VT<T> a0, VT<T
 // 4x4 on slots (A low role, B high role) whose zero entries Z are known
 // when the kernel is compiled (a two-qubit XXZ bond conserves the number of
 // ones: six entries of sixteen)
-template <typename T, int A, int B, uint32_t Z>
+template <typename T, int A, int B, uint32_t Z, bool CTRL = true>
 __device__ __forceinline__ void reg_u2z(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
   if constexpr (A >= kRegQubits || B >= kRegQubits) return;
-  const int creg = g.creg;
+  const int creg = CTRL ? g.creg : 0;
 #pragma unroll
   for (int j = 0; j < kNR; ++j) {
     if (j & ((1 << A) | (1 << B))) continue;
     const int id0 = j, id1 = j | (1 << A), id2 = j | (1 << B), id3 = j | (1 << A) | (1 << B);
     const VT<T> in0 = x[id0], in1 = x[id1], in2 = x[id2], in3 = x[id3];
-    const bool on = gon && ((j & creg) == creg);
     const VT<T> o0 = u2_row_z<T, Z>(g.m, 0, in0, in1, in2, in3);
     const VT<T> o1 = u2_row_z<T, Z>(g.m, 1, in0, in1, in2, in3);
     const VT<T> o2 = u2_row_z<T, Z>(g.m, 2, in0, in1, in2, in3);
     const VT<T> o3 = u2_row_z<T, Z>(g.m, 3, in0, in1, in2, in3);
-    x[id0] = csel(on, o0, in0);
-    x[id1] = csel(on, o1, in1);
-    x[id2] = csel(on, o2, in2);
-    x[id3] = csel(on, o3, in3);
+    if constexpr (CTRL) {
+      const bool on = gon && ((j & creg) == creg);
+      x[id0] = csel(on, o0, in0);
+      x[id1] = csel(on, o1, in1);
+      x[id2] = csel(on, o2, in2);
+      x[id3] = csel(on, o3, in3);
+    } else {
+      x[id0] = o0;
+      x[id1] = o1;
+      x[id2] = o2;
+      x[id3] = o3;
+    }
   }
 }
 
 // 4x4 on slots (A low role, B high role)
-template <typename T, int A, int B>
+// CTRL false: no controls (known when the kernel is compiled), no selects
+template <typename T, int A, int B, bool CTRL = true>
 __device__ __forceinline__ void reg_u2(VT<T> (&x)[kNR], const GateDev& g, bool gon) {
   if constexpr (A >= kRegQubits || B >= kRegQubits) return;
-  const int creg = g.creg;
+  const int creg = CTRL ? g.creg : 0;
 #pragma unroll
   for (int j = 0; j < kNR; ++j) {
     if (j & ((1 << A) | (1 << B))) continue;
     const int id0 = j, id1 = j | (1 << A), id2 = j | (1 << B), id3 = j | (1 << A) | (1 << B);
     const VT<T> in0 = x[id0], in1 = x[id1], in2 = x[id2], in3 = x[id3];
-    const bool on = gon && ((j & creg) == creg);
     const VT<T> o0 = u2_row<T>(g.m, 0, in0, in1, in2, in3);
     const VT<T> o1 = u2_row<T>(g.m, 1, in0, in1, in2, in3);
     const VT<T> o2 = u2_row<T>(g.m, 2, in0, in1, in2, in3);
     const VT<T> o3 = u2_row<T>(g.m, 3, in0, in1, in2, in3);
-    x[id0] = csel(on, o0, in0);
-    x[id1] = csel(on, o1, in1);
-    x[id2] = csel(on, o2, in2);
-    x[id3] = csel(on, o3, in3);
+    if constexpr (CTRL) {
+      const bool on = gon && ((j & creg) == creg);
+      x[id0] = csel(on, o0, in0);
+      x[id1] = csel(on, o1, in1);
+      x[id2] = csel(on, o2, in2);
+      x[id3] = csel(on, o3, in3);
+    } else {
+      x[id0] = o0;
+      x[id1] = o1;
+      x[id2] = o2;
+      x[id3] = o3;
+    }
   }
 }
 

@@ -333,14 +333,21 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
               o << "    reg_u1<double, " << u1_kind_name(G.kind) << ", " << G.a << ", true>(x, "
                 << gd << ", (sf.gl & " << gd << ".cglob) == " << gd << ".cglob);\n";
             break;
-          case GK_U2:
+          case GK_U2: {
+            // uncontrolled gates (the usual case) get no per-lane selects
+            const bool ctrl = (G.creg | G.cglob) != 0;
             if (G.zero)
-              o << "    reg_u2z<double, " << G.a << ", " << G.b << ", " << G.zero << "u>(x, " << gd
-                << ", (sf.gl & " << gd << ".cglob) == " << gd << ".cglob);\n";
+              o << "    reg_u2z<double, " << G.a << ", " << G.b << ", " << G.zero << "u, "
+                << (ctrl ? "true" : "false") << ">(x, " << gd;
             else
-              o << "    reg_u2<double, " << G.a << ", " << G.b << ">(x, " << gd << ", (sf.gl & "
-                << gd << ".cglob) == " << gd << ".cglob);\n";
+              o << "    reg_u2<double, " << G.a << ", " << G.b << ", "
+                << (ctrl ? "true" : "false") << ">(x, " << gd;
+            if (ctrl)
+              o << ", (sf.gl & " << gd << ".cglob) == " << gd << ".cglob);\n";
+            else
+              o << ", true);\n";
             break;
+          }
           case GK_DIAGTAB:
             o << "    reg_diagtab<double, " << G.nu << ">(x, " << gd << ", sf.gl, tabs);\n";
             break;
