@@ -313,7 +313,8 @@ def adjoint_vs_parameter_shift(N=24, layers=20, shift_params=4, batch_n=20, batc
         "batched": {"n_qubits": batch_n, "batch": batch, "params_per_set": tb.n_params,
                     "s_per_batched_gradient": t_batch,
                     "gradients_per_s": batch / t_batch,
-                    "kernels": "per-gate sweep, one launch per gate for the whole batch"},
+                    "kernels": "compiled passes, each pass one launch for the whole batch "
+                               "(every item its own gate records)"},
         "batched_one_launch": {"n_qubits": batch_n, "batch": batch, "s": t_one,
                                "gradients_per_s": batch / t_one,
                                "max_abs_diff_vs_per_gate_batch": d_one,

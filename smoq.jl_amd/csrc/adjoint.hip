@@ -7,6 +7,10 @@
 // λ_g = U_{>g}† H ψ_N; dE/dθ_g = 2 Re<λ_g|(-i/2) G ψ_g> = Im<λ_g|G|ψ_g>,
 // then both vectors are un-rotated by R(-θ_g) in the same pass.
 #include <algorithm>
+#include <string>
+#include <cstring>
+#include <thread>
+#include <atomic>
 #include <cmath>
 
 #include "adjoint_common.h"
@@ -557,6 +561,173 @@ int fused_adjoint(smoq_state* s, const std::vector<Step>& steps, const double* t
   return SMOQ_OK;
 }
 
+// Plans of every item of a batch on host threads: fn(b) for b < B.
+template <typename F>
+static void for_items(int B, F fn) {
+  const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nth = std::min(B, hw);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int b; (b = next.fetch_add(1)) < B;) fn(b);
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nth; ++t) th.emplace_back(work);
+  work();
+  for (auto& t : th) t.join();
+}
+
+// every item's program image, item_bytes apart, in one copy through the
+// pinned staging buffer
+static int upload_items(smoq_ctx* ctx, const std::vector<Program>& progs, const ProgLayout& L,
+                        size_t item_bytes, void* d) {
+  const size_t total = item_bytes * progs.size();
+  int rc = ensure_pinned(ctx, total);
+  if (rc) return rc;
+  SMOQ_HIP(hipStreamSynchronize(ctx->stream));  // the staging buffer may feed an earlier copy
+  unsigned char* h = (unsigned char*)ctx->host_pinned;
+  for (size_t b = 0; b < progs.size(); ++b) {
+    const Program& pr = progs[b];
+    unsigned char* o = h + b * item_bytes;
+    std::memcpy(o, pr.passes.data(), L.pb);
+    std::memcpy(o + L.pb, pr.stages.data(), L.sb);
+    std::memcpy(o + L.pb + L.sb, pr.gates.data(), L.gb);
+    if (L.ntab) std::memcpy(o + L.off_t, pr.tabs.data(), L.ntab * sizeof(double2));
+  }
+  SMOQ_HIP(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, ctx->stream));
+  return SMOQ_OK;
+}
+
+// The batched form of fused_adjoint: B parameter sets of one tape on a
+// batch state.  Every item gets its own copy of the forward and reverse
+// programs (same structure, its own angles; structures that differ between
+// items leave *done false), so each pass is one launch for the whole batch.
+int fused_adjoint_batch(smoq_state* s, const smoq_op* ops, int n_ops,
+                        const std::vector<Step>& steps, const double* theta, int n_terms,
+                        const uint64_t* xm, const uint64_t* zm, const double* coeff,
+                        int n_params, double* energy, double* grad, bool* done) {
+  *done = false;
+  smoq_ctx* ctx = s->ctx;
+  const int B = s->batch, n = s->n, nW = n + 1;
+  if (B < 2 || ctx->jit_mode == 0 || n < kRegQubits || nW < 6 || nW > kMaxQubits)
+    return SMOQ_OK;
+  std::vector<AdjTerm> tv;
+  int rc = adj_terms(n_terms, xm, zm, coeff, n, tv);
+  if (rc) return rc;
+  const int rowbits = s->dtype == SMOQ_C128 ? CT<double>::kRowBits : CT<float>::kRowBits;
+  // every item's forward and reverse programs, planned on host threads
+  std::vector<Program> fp(B), rp(B);
+  std::vector<int> errs(B, 0), slots(B, 0);
+  std::vector<std::string> fkey(B), rkey(B);
+  size_t n_fwd_gates = 0;
+  {
+    std::vector<IGate> g0;
+    if ((rc = lower_tape(ops, n_ops, theta, n_params, n, g0))) return rc;
+    n_fwd_gates = g0.size();
+  }
+  const int ftile = pass_tile(ctx, n, B, n_fwd_gates);
+  const bool fswaps = ctx->lane_swaps != 0 && jit_planned(ctx, n, n_fwd_gates);
+  for_items(B, [&](int b) {
+    std::vector<IGate> gs, rg;
+    const double* th = theta + (size_t)b * n_params;
+    int e = lower_tape(ops, n_ops, th, n_params, n, gs);
+    if (!e) {
+      plan_program(gs, n, ftile, ctx->low_qubits, rowbits, fp[b], ctx->pass_max_gates, fswaps);
+      jit_structure_key(fp[b], s->dtype, fkey[b]);
+      e = reverse_gates(steps, th, n, rg, &slots[b]);
+    }
+    if (!e) {
+      plan_program(rg, nW, pass_tile(ctx, nW, B, rg.size()), ctx->low_qubits, rowbits, rp[b],
+                   ctx->pass_max_gates, ctx->lane_swaps != 0);
+      jit_structure_key(rp[b], s->dtype, rkey[b]);
+    }
+    errs[b] = e;
+  });
+  for (int b = 0; b < B; ++b)
+    if (errs[b]) {
+      // again on this thread, for its error message (thread-local)
+      std::vector<IGate> gs, rg;
+      const double* th = theta + (size_t)b * n_params;
+      if ((rc = lower_tape(ops, n_ops, th, n_params, n, gs))) return rc;
+      int ns = 0;
+      if ((rc = reverse_gates(steps, th, n, rg, &ns))) return rc;
+      return errs[b];
+    }
+  for (int b = 1; b < B; ++b)
+    if (fkey[b] != fkey[0] || rkey[b] != rkey[0]) return SMOQ_OK;  // item structures differ
+  const int n_slots = slots[0];
+  if (fp[0].passes.empty() || rp[0].passes.empty()) return SMOQ_OK;
+  std::shared_ptr<JitModule> jf, jr;
+  if ((rc = jit_get(ctx, fp[0], s->dtype, jf)) || (rc = jit_get(ctx, rp[0], s->dtype, jr)))
+    return ctx->jit_mode >= 2 ? rc : SMOQ_OK;
+  const ProgLayout Lf = program_layout(fp[0], s->dtype), Lr = program_layout(rp[0], s->dtype);
+  const size_t fbytes = (Lf.total + 255) & ~(size_t)255, rbytes = (Lr.total + 255) & ~(size_t)255;
+  // buffers: programs (forward, then reverse) | W (2S) | overlap partials,
+  // slot sums, energy partials | terms
+  void* pbuf = nullptr;
+  if ((rc = ensure_work(ctx, 0, std::max(fbytes, rbytes) * B, &pbuf))) return rc;
+  void* wbuf = nullptr;
+  if ((rc = ensure_work(ctx, 1, 2 * s->bytes, &wbuf))) return rc;
+  const uint64_t tiles_item = 1ull << rp[0].passes[0].n_nonlocal;
+  const uint64_t ntiles = (uint64_t)B * tiles_item;
+  const int bpi = blocks_per_item(s);
+  const size_t n_op = (size_t)n_slots * ntiles, n_sums = (size_t)n_slots * B;
+  void* pb = nullptr;
+  if ((rc = ensure_work(ctx, 2, (n_op + n_sums + (size_t)B * bpi + B + 2) * sizeof(double), &pb)))
+    return rc;
+  double* opart = (double*)pb;
+  double* sums = opart + n_op;
+  double* epart = sums + n_sums;
+  double* esum = epart + (size_t)B * bpi;
+  void* tb = nullptr;
+  if ((rc = ensure_work(ctx, 3, (tv.size() + 1) * sizeof(AdjTerm), &tb))) return rc;
+  AdjTerm* dterms = (AdjTerm*)tb;
+  if (!tv.empty())
+    SMOQ_HIP(hipMemcpyAsync(dterms, tv.data(), tv.size() * sizeof(AdjTerm), hipMemcpyHostToDevice,
+                            ctx->stream));
+  // forward: |0…0> through each item's program
+  if ((rc = smoq_state_set_basis(s, 0))) return rc;
+  if ((rc = upload_items(ctx, fp, Lf, fbytes, pbuf))) return rc;
+  if ((rc = launch_program(s, fp[0], Lf, pbuf, jf.get(), nullptr, -1, fbytes))) return rc;
+  // W and the energies
+  if (s->dtype == SMOQ_C128)
+    hipLaunchKernelGGL(k_adj_init<double>, dim3(bpi, B), dim3(kAdjBlock), 0, ctx->stream,
+                       (const double2*)s->d, (double2*)wbuf, n, (int)tv.size(), dterms, epart);
+  else
+    hipLaunchKernelGGL(k_adj_init<float>, dim3(bpi, B), dim3(kAdjBlock), 0, ctx->stream,
+                       (const float2*)s->d, (float2*)wbuf, n, (int)tv.size(), dterms, epart);
+  SMOQ_CHECK_LAUNCH("k_adj_init");
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, epart,
+                     bpi, B, esum);
+  SMOQ_CHECK_LAUNCH("k_rowsum");
+  // reverse sweep over every item's W
+  smoq_state ws = *s;
+  ws.n = nW;
+  ws.dim = 2 * s->dim;
+  ws.bytes = 2 * s->bytes;
+  ws.d = wbuf;
+  if ((rc = upload_items(ctx, rp, Lr, rbytes, pbuf))) return rc;
+  if ((rc = launch_program(&ws, rp[0], Lr, pbuf, jr.get(), opart, -1, rbytes))) return rc;
+  if (n_slots > 0) {
+    hipLaunchKernelGGL(k_ovl_reduce, dim3((unsigned)n_sums), dim3(kOvlRedBlock), 0, ctx->stream,
+                       opart, ntiles, tiles_item, B, sums);
+    SMOQ_CHECK_LAUNCH("k_ovl_reduce");
+  }
+  std::vector<double> hs(n_sums + 1);
+  if (n_sums > 0)
+    SMOQ_HIP(hipMemcpyAsync(hs.data(), sums, n_sums * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+  SMOQ_HIP(hipMemcpyAsync(energy, esum, B * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  SMOQ_HIP(hipStreamSynchronize(ctx->stream));
+  // each item's parameters from its slots, in gate order
+  std::vector<double> hb(n_slots + 1);
+  for (int b = 0; b < B; ++b) {
+    for (int k = 0; k < n_slots; ++k) hb[k] = hs[(size_t)k * B + b];
+    grad_from_slots(steps, hb.data(), n_slots, n_params, grad + (size_t)b * n_params);
+  }
+  *done = true;
+  return SMOQ_OK;
+}
+
 }  // namespace
 }  // namespace smoq
 
@@ -616,6 +787,12 @@ extern "C" int smoq_adjoint_gradient(smoq_state* s, const smoq_op* ops, int n_op
     if (done) return SMOQ_OK;
     if ((rc = setup(s, theta, n_params, w, hcs))) return rc;
   } else {
+    // a batch of parameter sets: every pass one launch for all items
+    bool done = false;
+    if ((rc = fused_adjoint_batch(s, ops, n_ops, steps, theta, n_terms, xmask, zmask, coeff,
+                                  n_params, energy, grad, &done)))
+      return rc;
+    if (done) return SMOQ_OK;
     if ((rc = setup(s, theta, n_params, w, hcs))) return rc;
     if ((rc = forward(s, ops, n_ops, theta, n_params, steps, w.cs, false))) return rc;
   }

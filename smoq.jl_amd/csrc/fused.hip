@@ -90,7 +90,7 @@ int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayo
 template <typename T>
 static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout& L,
                             const void* dbuf, const JitModule* jit, double* opart,
-                            int only_pass) {
+                            int only_pass, uint64_t item_bytes) {
   smoq_ctx* ctx = s->ctx;
   int rc = SMOQ_OK;
   const unsigned char* d = (const unsigned char*)dbuf;
@@ -105,6 +105,9 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
     for (const GateDev& g : prog.gates)
       if (g.kind == GK_OVL)
         return set_error(SMOQ_EINVAL, "fused pass: overlap gates need compiled passes");
+  if (!jit && item_bytes)
+    return set_error(SMOQ_EINVAL, "fused pass: per-item programs need compiled passes");
+  unsigned long long ib = item_bytes;
   VT<T>* psi = (VT<T>*)s->d;
   int n = s->n;
   for (size_t p = 0; p < prog.passes.size(); ++p) {
@@ -134,7 +137,7 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
     if (ctx->timing && (rc = time_mark(ctx))) return rc;
     if (jit) {
       const PassDev* pp = dp + p;
-      void* args[] = {&psi, &n, &pp, (void*)&ds, (void*)&dg, (void*)&dt, &opart};
+      void* args[] = {&psi, &n, &pp, (void*)&ds, (void*)&dg, (void*)&dt, &opart, &ib};
       SMOQ_HIP(hipModuleLaunchKernel(jit->fn[p], (unsigned)gx, (unsigned)gy, 1, nt, 1, 1,
                                      0u, ctx->stream, args, nullptr));
     } else {
@@ -148,9 +151,10 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
 }
 
 int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf,
-                   const JitModule* jit, double* opart, int only_pass) {
-  return s->dtype == SMOQ_C128 ? launch_program_t<double>(s, prog, L, dbuf, jit, opart, only_pass)
-                               : launch_program_t<float>(s, prog, L, dbuf, jit, opart, only_pass);
+                   const JitModule* jit, double* opart, int only_pass, uint64_t item_bytes) {
+  return s->dtype == SMOQ_C128
+             ? launch_program_t<double>(s, prog, L, dbuf, jit, opart, only_pass, item_bytes)
+             : launch_program_t<float>(s, prog, L, dbuf, jit, opart, only_pass, item_bytes);
 }
 
 int run_program(smoq_state* s, Program& prog, bool fixed_angles) {

@@ -231,7 +231,8 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
       << ", 4) smoq_pass_" << p << "(VT<" << T
       << ">* __restrict__ psi, int n, const PassDev* __restrict__ pass,"
          " const StageDev* __restrict__ stages, const GateDev* __restrict__ gates,"
-         " const double2* __restrict__ tabs, double* __restrict__ opart) {\n"
+         " const double2* __restrict__ tabs, double* __restrict__ opart,"
+         " unsigned long long item_bytes) {\n"
       << "  typedef " << T << " T;\n"
       // the tile as a static LDS array: its address is known when the
       // kernel is compiled, so an LDS access is one XOR of a lane part with
@@ -265,7 +266,14 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
         << g0 << ");\n    for (uint32_t w = threadIdx.x * 32u; w < " << nw
         << "u; w += blockDim.x * 32u) warm ^= gw[w];\n  }\n";
     }
-    o << "  pass_load<T>(lds, psi, n, pass, f);\n";
+    o << "  pass_load<T>(lds, psi, n, pass, f);\n"
+      // batches whose items carry their own gate records and tables (same
+      // structure, other angles): item_bytes apart
+      << "  if (item_bytes) {\n"
+      << "    const unsigned long long item = f.tile >> pass->n_nonlocal;\n"
+      << "    gates = (const GateDev*)((const char*)gates + item * item_bytes);\n"
+      << "    tabs = (const double2*)((const char*)tabs + item * item_bytes);\n"
+      << "  }\n";
     for (int s = pd.s0; s < pd.s1; ++s) {
       const StageDev& st = prog.stages[s];
       const bool seg_end = s + 1 == pd.s1 || !prog.stages[s + 1].cont;
@@ -444,6 +452,7 @@ std::string jit_source(const Program& prog, int dtype) {
 }
 
 // the key of prog into buf (whose capacity is reused across calls)
+void jit_structure_key(const Program& prog, int dtype, std::string& buf);
 static void jit_key(const Program& prog, int dtype, std::string& buf) {
   KeyOut o;
   o.k.swap(buf);
@@ -451,6 +460,10 @@ static void jit_key(const Program& prog, int dtype, std::string& buf) {
   o.k.reserve(256 * prog.gates.size() + 4096);
   jit_emit(prog, dtype, o);
   o.k.swap(buf);
+}
+
+void jit_structure_key(const Program& prog, int dtype, std::string& buf) {
+  jit_key(prog, dtype, buf);
 }
 
 JitModule::~JitModule() {

@@ -297,8 +297,14 @@ int upload_program(smoq_ctx* ctx, const Program& prog, int dtype, const ProgLayo
 // opart: per-tile partials of the program's overlap gates (GK_OVL), at
 // opart[oslot * tiles + tile]; compiled passes only
 // only_pass >= 0: that pass alone
+// item_bytes > 0 (compiled passes only): every batch item has its own
+// uploaded copy of the program, item_bytes apart, whose gate records and
+// tables its tiles read (same structure, other angles)
 int launch_program(smoq_state* s, const Program& prog, const ProgLayout& L, const void* dbuf,
-                   const JitModule* jit, double* opart = nullptr, int only_pass = -1);
+                   const JitModule* jit, double* opart = nullptr, int only_pass = -1,
+                   uint64_t item_bytes = 0);
+// a key of the compiled form of a program (equal keys: same kernels)
+void jit_structure_key(const Program& prog, int dtype, std::string& buf);
 // run-time compiled pass kernels (jit.hip)
 std::string jit_source(const Program& prog, int dtype);
 // rewrites rotation gates into the factored kinds of run-time compiled
