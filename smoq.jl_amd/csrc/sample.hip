@@ -306,21 +306,30 @@ constexpr int kSeqStageBlock = 256;
 constexpr int kSeqStageTerms = 16;
 static_assert(kSeqChunk % kSeqStageTerms == 0, "chunk length");
 
+// the next kSeqStageTerms terms of the workgroup's chunks: each thread's
+// share of the raw amplitudes (loads in flight until seq_put)
 template <typename T>
-__device__ __forceinline__ void seq_stage(const typename CT<T>::V* __restrict__ psi, uint64_t n,
+__device__ __forceinline__ void seq_fetch(const typename CT<T>::V* __restrict__ psi, uint64_t n,
                                           uint64_t diag_step, uint64_t ch0, uint32_t off,
-                                          double (*buf)[kSeqStageTerms + 1]) {
-  double v[kSeqStageTerms];
+                                          typename CT<T>::V (&v)[kSeqStageTerms]) {
 #pragma unroll
   for (int j = 0; j < kSeqStageTerms; ++j) {
     const uint32_t L = (uint32_t)j * kSeqStageBlock + threadIdx.x;
     const uint64_t i = (ch0 + L / kSeqStageTerms) * kSeqChunk + off + L % kSeqStageTerms;
-    v[j] = i < n ? seq_term<T>(psi, i, diag_step) : 0.0;
+    if (i < n) v[j] = psi[diag_step ? i * diag_step : i];
   }
+}
+// ... their terms into LDS (a term past the end is zero)
+template <typename T>
+__device__ __forceinline__ void seq_put(uint64_t n, uint64_t diag_step, uint64_t ch0, uint32_t off,
+                                        const typename CT<T>::V (&v)[kSeqStageTerms],
+                                        double (*buf)[kSeqStageTerms + 1]) {
 #pragma unroll
   for (int j = 0; j < kSeqStageTerms; ++j) {
     const uint32_t L = (uint32_t)j * kSeqStageBlock + threadIdx.x;
-    buf[L / kSeqStageTerms][L % kSeqStageTerms] = v[j];
+    const uint64_t i = (ch0 + L / kSeqStageTerms) * kSeqChunk + off + L % kSeqStageTerms;
+    buf[L / kSeqStageTerms][L % kSeqStageTerms] =
+        i < n ? (diag_step ? (double)v[j].x : prob_of<T>(v[j])) : 0.0;
   }
 }
 
@@ -341,9 +350,14 @@ __global__ void __launch_bounds__(kSeqStageBlock)
   const bool mine = ch < nch;
   double a = MODE == 0 ? 0.0 : (mine ? from[ch] : 0.0);
   bool t = false;
+  typename CT<T>::V v[kSeqStageTerms];
+  seq_fetch<T>(psi, n, diag_step, ch0, 0, v);
   for (uint32_t off = 0; off < kSeqChunk; off += kSeqStageTerms) {
-    seq_stage<T>(psi, n, diag_step, ch0, off, buf);
+    seq_put<T>(n, diag_step, ch0, off, v, buf);
     __syncthreads();
+    // the next round's loads in flight while this round is added
+    if (off + kSeqStageTerms < kSeqChunk)
+      seq_fetch<T>(psi, n, diag_step, ch0, off + kSeqStageTerms, v);
     if (mine) {
 #pragma unroll
       for (int u = 0; u < kSeqStageTerms; ++u) {
