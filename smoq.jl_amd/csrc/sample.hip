@@ -285,7 +285,8 @@ __global__ void k_seq_prefix(const typename CT<T>::V* __restrict__ psi, uint64_t
 constexpr uint32_t kSeqChunk = 4096;
 constexpr uint64_t kSeqParallelMin = 1ull << 18;  // shorter: one thread
 constexpr int kSeqFixThreads = 1024;
-constexpr uint64_t kSeqFixWin = 16 * kSeqFixThreads;  // chunks per scan window
+constexpr uint64_t kSeqFixWin = 4 * kSeqFixThreads;        // chunks per scan window (first)
+constexpr uint64_t kSeqFixWinMax = 256 * kSeqFixThreads;  // ... and at most
 
 template <typename T>
 __device__ __forceinline__ double seq_term(const typename CT<T>::V* __restrict__ psi, uint64_t i,
@@ -459,9 +460,12 @@ __global__ void __launch_bounds__(kSeqFixThreads)
   __syncthreads();
   unsigned long long nrerun = 0;
   uint64_t k0 = 0;
+  // the window doubles after a window without a rerun and starts small
+  // again after one (reruns cluster where the sum is small)
+  uint64_t win = kSeqFixWin;
   while (k0 < nch) {
     const double S0 = s_S;
-    const uint64_t k1 = nch - k0 < kSeqFixWin ? nch : k0 + kSeqFixWin;
+    const uint64_t k1 = nch - k0 < win ? nch : k0 + win;
     const uint64_t per = (k1 - k0 + kSeqFixThreads - 1) / kSeqFixThreads;
     const uint64_t a0 = k0 + (uint64_t)tid * per;
     const uint64_t a = a0 < k1 ? a0 : k1, b = a + per < k1 ? a + per : k1;
@@ -507,8 +511,10 @@ __global__ void __launch_bounds__(kSeqFixThreads)
     __syncthreads();
     if (bad == ~0ull) {
       k0 = k1;
+      win = win < kSeqFixWinMax ? 2 * win : win;
       continue;
     }
+    win = kSeqFixWin;
     // run chunk `bad` again from its exact start
     const uint64_t i0 = bad * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
     for (uint64_t i = i0 + tid; i < i1; i += kSeqFixThreads)
