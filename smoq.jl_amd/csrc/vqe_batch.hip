@@ -28,6 +28,9 @@ namespace smoq {
 namespace {
 
 constexpr int kVqeTile = 12;
+// device memory for one batch of large items with the same tape (ψ plus the
+// two-state W of each item)
+constexpr uint64_t kVqeGroupBytes = 8ull << 30;
 
 // Byte offsets from the launch's base pointers (tables; states) rather than
 // pointers: addresses formed from a kernel argument are known to be global,
@@ -543,6 +546,19 @@ int finish_batch(hipStream_t st, const double* res, const smoq_vqe_item* items, 
   return SMOQ_OK;
 }
 
+// items that differ only in their angles (same qubit count, tape and
+// Hamiltonian, compared by contents)
+static bool same_problem(const smoq_vqe_item& a, const smoq_vqe_item& b) {
+  auto eq = [](const void* x, const void* y, size_t bytes) {
+    return x == y || (x && y && std::memcmp(x, y, bytes) == 0);
+  };
+  return a.n_qubits == b.n_qubits && a.n_ops == b.n_ops && a.n_params == b.n_params &&
+         a.n_terms == b.n_terms && eq(a.ops, b.ops, sizeof(smoq_op) * a.n_ops) &&
+         eq(a.xmask, b.xmask, sizeof(uint64_t) * a.n_terms) &&
+         eq(a.zmask, b.zmask, sizeof(uint64_t) * a.n_terms) &&
+         eq(a.coeff, b.coeff, sizeof(double) * a.n_terms);
+}
+
 // Which items take the one launch and which the whole-device sweep one after
 // another: a workgroup moves an item's 2^n amplitudes through its passes
 // alone (about 8.6e-10 s per amplitude and tape op, measured at n = 20),
@@ -634,16 +650,42 @@ extern "C" int smoq_adjoint_gradient_batch(smoq_ctx* ctx, int dtype, const smoq_
       return rc;
     }
   }
-  // the large items, each through the single-set sweep on the context's
-  // stream, while the launch runs
+  // the large items through the whole-device sweep on the context's stream,
+  // while the launch runs: items with the same tape and Hamiltonian (equal
+  // contents) as batches of parameter sets (each pass one launch for the
+  // batch), up to a memory budget per batch, the others one after another
+  std::vector<char> done(n_items, 0);
   for (int i = 0; i < n_items && !rc; ++i) {
-    if (!big[i]) continue;
+    if (!big[i] || done[i]) continue;
     const smoq_vqe_item& it = items[i];
+    std::vector<int> group{i};
+    const uint64_t state_bytes =
+        ((uint64_t)1 << it.n_qubits) * (dtype == SMOQ_C128 ? 16u : 8u);
+    // ψ plus W (two states) per item
+    const uint64_t max_group = std::max<uint64_t>(1, kVqeGroupBytes / (3 * state_bytes));
+    for (int j = i + 1; j < n_items && group.size() < max_group; ++j)
+      if (big[j] && !done[j] && same_problem(it, items[j])) group.push_back(j);
+    for (int j : group) done[j] = 1;
+    const int B = (int)group.size();
     smoq_state* s = nullptr;
-    rc = smoq_state_alloc(ctx, it.n_qubits, dtype, 1, &s);
-    if (!rc)
-      rc = smoq_adjoint_gradient(s, it.ops, it.n_ops, it.theta, it.n_params, it.n_terms, it.xmask,
-                                 it.zmask, it.coeff, it.energy, it.grad);
+    rc = smoq_state_alloc(ctx, it.n_qubits, dtype, B, &s);
+    if (!rc) {
+      if (B == 1) {
+        rc = smoq_adjoint_gradient(s, it.ops, it.n_ops, it.theta, it.n_params, it.n_terms,
+                                   it.xmask, it.zmask, it.coeff, it.energy, it.grad);
+      } else {
+        const size_t P = (size_t)it.n_params;
+        std::vector<double> th(P * B), en(B), gr(P * B);
+        for (int b = 0; b < B; ++b)
+          std::copy(items[group[b]].theta, items[group[b]].theta + P, th.begin() + b * P);
+        rc = smoq_adjoint_gradient(s, it.ops, it.n_ops, th.data(), it.n_params, it.n_terms,
+                                   it.xmask, it.zmask, it.coeff, en.data(), gr.data());
+        for (int b = 0; b < B && !rc; ++b) {
+          *items[group[b]].energy = en[b];
+          std::copy(gr.begin() + b * P, gr.begin() + (b + 1) * P, items[group[b]].grad);
+        }
+      }
+    }
     if (s) {
       const int frc = smoq_state_free(s);
       if (!rc) rc = frc;

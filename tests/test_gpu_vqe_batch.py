@@ -110,6 +110,41 @@ def test_batch_matches_single_state_adjoint(ctx, tile, route):
         assert np.max(np.abs(g - g1[0])) < 1e-10
 
 
+@pytest.mark.parametrize("route", ["sweep", "auto"])
+def test_sweep_groups_items_of_one_tape(ctx, route):
+    """Large items that share a tape and Hamiltonian go through the sweep as
+    one batch of parameter sets; interleaved with items of other tapes (a
+    copy of the tape in its own arrays counts as the same problem), each
+    against the single-state adjoint."""
+    from smoq_amd.circuits import hardware_efficient_ansatz
+    from smoq_amd.gradients import VQEEnergy, adjoint_gradient_batch
+
+    N = 16
+    tape = mixed_tape(N, 3)
+    other = mixed_tape(N, 4)
+    small = hardware_efficient_ansatz(9, 2)
+    rng = np.random.default_rng(21)
+    items = []
+    for k in range(7):
+        t = other if k == 2 else (small if k == 5 else (tape if k != 4 else mixed_tape(N, 3)))
+        n = 9 if t is small else N
+        terms = tfim_terms(n)
+        items.append((t, rng.uniform(0, 2 * math.pi, t.n_params)) + tuple(terms))
+    print(f"[gpu] vqe batch grouped sweep N={N} route={route}", flush=True)
+    if route != "auto":
+        os.environ["SMOQ_VQE_ROUTE"] = route
+    try:
+        E, G = adjoint_gradient_batch(items)
+    finally:
+        os.environ.pop("SMOQ_VQE_ROUTE", None)
+    for (t, th, xs, zs, cs), e, g in zip(items, E, G):
+        v = VQEEnergy(t, xs, zs, cs)
+        e1, g1 = v.energy_and_grad(th)
+        v.free()
+        assert abs(e - e1[0]) < 1e-11
+        assert np.max(np.abs(g - g1[0])) < 1e-10
+
+
 def test_complex64_batch_against_complex128(ctx):
     from smoq_amd.circuits import hardware_efficient_ansatz
     from smoq_amd.gradients import adjoint_gradient_batch
