@@ -285,8 +285,9 @@ __global__ void k_seq_prefix(const typename CT<T>::V* __restrict__ psi, uint64_t
 constexpr uint32_t kSeqChunk = 4096;
 constexpr uint64_t kSeqParallelMin = 1ull << 18;  // shorter: one thread
 constexpr int kSeqFixThreads = 1024;
-constexpr uint64_t kSeqFixWin = 4 * kSeqFixThreads;        // chunks per scan window (first)
-constexpr uint64_t kSeqFixWinMax = 256 * kSeqFixThreads;  // ... and at most
+constexpr int kSeqFixPer = 8;  // consecutive chunks per thread in a round of the walk
+constexpr uint64_t kSeqFixRound = (uint64_t)kSeqFixPer * kSeqFixThreads;
+constexpr int kSeqSub = 4;  // terms per sub-chunk when a chunk is run again
 
 template <typename T>
 __device__ __forceinline__ double seq_term(const typename CT<T>::V* __restrict__ psi, uint64_t i,
@@ -302,6 +303,21 @@ __device__ __forceinline__ double seq_term(const typename CT<T>::V* __restrict__
 // stores them coalesced.
 __device__ __forceinline__ uint64_t dbits(double x) { return (uint64_t)__double_as_longlong(x); }
 __device__ __forceinline__ int dexp(double x) { return (int)((dbits(x) >> 52) & 0x7ff); }
+
+// a + p, and t |= whether the exact sum lay halfway between grid points
+// (TwoSum error, compared with the spacing above and below the result)
+__device__ __forceinline__ double seq_add_tie(double a, double p, bool& t) {
+#pragma clang fp contract(off)
+  const double s = a + p;
+  const double bp = s - a;
+  const double err = (a - (s - bp)) + (p - bp);
+  if (err != 0.0) {
+    const double e2 = 2.0 * fabs(err);
+    const double up = __longlong_as_double((long long)(dbits(s) + 1)) - s;
+    t |= e2 == up || 2.0 * e2 == up;
+  }
+  return s;
+}
 
 constexpr int kSeqStageBlock = 256;
 constexpr int kSeqStageTerms = 16;
@@ -363,16 +379,7 @@ __global__ void __launch_bounds__(kSeqStageBlock)
 #pragma unroll
       for (int u = 0; u < kSeqStageTerms; ++u) {
         const double p = buf[threadIdx.x][u];
-        const double s = a + p;
-        if (MODE == 1) {
-          const double bp = s - a;
-          const double err = (a - (s - bp)) + (p - bp);
-          if (err != 0.0) {
-            const double e2 = 2.0 * fabs(err);
-            const double up = __longlong_as_double((long long)(dbits(s) + 1)) - s;
-            t |= e2 == up || 2.0 * e2 == up;
-          }
-        }
+        const double s = MODE == 1 ? seq_add_tie(a, p, t) : a + p;
         a = s;
         if (MODE == 2) buf[threadIdx.x][u] = s;
       }
@@ -393,30 +400,54 @@ __global__ void __launch_bounds__(kSeqStageBlock)
   if (mine && MODE == 1) tie[ch] = t ? 1 : 0;
 }
 
-// exclusive prefix of the chunk sums (a guess only: any order will do), one
-// workgroup: each thread sums a range, the workgroup scans the range sums
-__global__ void __launch_bounds__(1024)
-    k_seq_guess(const double* __restrict__ sums, uint64_t nch, double* __restrict__ guess) {
-  __shared__ double part[1024];
-  const uint64_t per = (nch + 1023) / 1024;
-  const uint64_t b0 = threadIdx.x * per, b1 = b0 + per < nch ? b0 + per : nch;
-  double t = 0.0;
-  for (uint64_t c = b0; c < b1; ++c) t += sums[c];
-  part[threadIdx.x] = t;
+// Block-wide scans of one value per thread (every thread of the workgroup
+// takes part; the wave sums pass through `ws`, kSeqFixThreads / 64 slots).
+// Returns the exclusive prefix; *total gets the workgroup's sum.  Two
+// barriers; `ws` may be reused after the second.
+template <typename V>
+__device__ __forceinline__ V seq_block_scan(V v, V* ws, V* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  V incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const V y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) ws[wid] = incl;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double run = 0.0;
-    for (int k = 0; k < 1024; ++k) {
-      const double v = part[k];
-      part[k] = run;
-      run += v;
-    }
+  V base = incl - v, tot = V(0);
+#pragma unroll
+  for (int w = 0; w < kSeqFixThreads / 64; ++w) {
+    if (w < wid) base += ws[w];
+    tot += ws[w];
   }
   __syncthreads();
-  double run = part[threadIdx.x];
-  for (uint64_t c = b0; c < b1; ++c) {
-    guess[c] = run;
-    run += sums[c];
+  *total = tot;
+  return base;
+}
+
+// exclusive prefix of the chunk sums (a guess only: any order will do), one
+// workgroup over rounds of kSeqFixPer consecutive sums per thread
+__global__ void __launch_bounds__(kSeqFixThreads)
+    k_seq_guess(const double* __restrict__ sums, uint64_t nch, double* __restrict__ guess) {
+  __shared__ double ws[kSeqFixThreads / 64];
+  double run = 0.0;
+  for (uint64_t k0 = 0; k0 < nch; k0 += kSeqFixRound) {
+    const uint64_t kb = k0 + (uint64_t)threadIdx.x * kSeqFixPer;
+    double v[kSeqFixPer], loc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kSeqFixPer; ++j) {
+      v[j] = kb + j < nch ? sums[kb + j] : 0.0;
+      loc += v[j];
+    }
+    double tot;
+    double a = run + seq_block_scan(loc, ws, &tot);
+#pragma unroll
+    for (int j = 0; j < kSeqFixPer; ++j) {
+      if (kb + j < nch) guess[kb + j] = a;
+      a += v[j];
+    }
+    run += tot;
   }
 }
 
@@ -424,10 +455,12 @@ __global__ void __launch_bounds__(1024)
 // pass 3: the exact start of every chunk (one workgroup).  Inside a binade
 // the walk is integer arithmetic on bit patterns: B(S_{k+1}) = B(S_k) +
 // (B(E_k) - B(G_k)) for every chunk that meets the conditions above (0 for a
-// chunk of zeros), so the starts over a window of chunks are one prefix sum.
-// The first chunk in the window that fails them is run again from its exact
-// start (its terms staged in LDS, one thread adding them in order), and the
-// next window begins after it.
+// chunk of zeros), so the starts over a round of chunks are one prefix sum.
+// The first chunk in the round that fails them is run again from its exact
+// start, and the next round begins after it.  The rerun applies the same
+// rule to the chunk's kSeqFixThreads sub-chunks of kSeqSub terms (staged in
+// LDS, one per thread, guesses from a scan of their sums), and adds in
+// order, on one thread, only the terms of the sub-chunks that fail it.
 __device__ __forceinline__ bool seq_shift_ok(double S, double g, double e, bool tie, bool zero) {
   if (zero || S == g) return true;
   const int eg = dexp(g);
@@ -440,6 +473,54 @@ __device__ __forceinline__ bool seq_shift_ok(double S, double g, double e, bool 
   return (int)((rb >> 52) & 0x7ff) == eg;
 }
 
+// One round of the walk: thread t holds items t*P .. t*P+P-1 (in order) with
+// their guesses g, chain ends e and flags (1 tie, 2 zero, 4 present); the
+// first present item starts exactly at S.  Returns the first present item
+// whose shift is not exact (or kNone), after put(j, start) for every item of
+// the thread up to it, and leaves in *s_S the exact start of that item, or
+// else the end of the round.  Three barriers.
+constexpr uint32_t kSeqNone = 0xffffffffu;
+template <int P, typename Put>
+__device__ __forceinline__ uint32_t seq_walk_round(double S, const double (&g)[P],
+                                                   const double (&e)[P], const uint8_t (&fl)[P],
+                                                   long long* ws, unsigned* s_bad, double* s_S,
+                                                   Put put) {
+  long long d[P], loc = 0;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    d[j] = (fl[j] & 6) == 4 ? (long long)(dbits(e[j]) - dbits(g[j])) : 0ll;
+    loc += d[j];
+  }
+  if (threadIdx.x == 0) *s_bad = kSeqNone;
+  long long tot;
+  const long long base = seq_block_scan(loc, ws, &tot);  // its barriers order s_bad's reset
+  uint64_t bS = dbits(S) + (uint64_t)base;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    if ((fl[j] & 4) &&
+        !seq_shift_ok(__longlong_as_double((long long)bS), g[j], e[j], fl[j] & 1, fl[j] & 2)) {
+      atomicMin(s_bad, (unsigned)(threadIdx.x * P + j));
+      break;
+    }
+    bS += (uint64_t)d[j];
+  }
+  __syncthreads();
+  const uint32_t bad = *s_bad;
+  bS = dbits(S) + (uint64_t)base;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const uint32_t r = threadIdx.x * P + j;
+    if (r > bad) break;
+    if (fl[j] & 4) put(j, __longlong_as_double((long long)bS));
+    if (r == bad) *s_S = __longlong_as_double((long long)bS);
+    bS += (uint64_t)d[j];
+  }
+  if (bad == kSeqNone && threadIdx.x == kSeqFixThreads - 1)
+    *s_S = __longlong_as_double((long long)(dbits(S) + (uint64_t)tot));
+  __syncthreads();
+  return bad;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kSeqFixThreads)
     k_seq_fix(const typename CT<T>::V* __restrict__ psi, uint64_t n, uint64_t diag_step,
@@ -447,87 +528,72 @@ __global__ void __launch_bounds__(kSeqFixThreads)
               const double* __restrict__ ends, const uint8_t* __restrict__ tie,
               double* __restrict__ start, unsigned long long* __restrict__ reruns) {
 #pragma clang fp contract(off)
+  static_assert(kSeqChunk == kSeqSub * kSeqFixThreads, "one sub-chunk per thread");
   __shared__ double terms[kSeqChunk];
-  __shared__ long long wsum[kSeqFixThreads / 64];
-  __shared__ unsigned long long s_bad;
+  __shared__ long long ws[kSeqFixThreads / 64];
+  __shared__ double wsd[kSeqFixThreads / 64];
+  __shared__ unsigned s_bad;
   __shared__ double s_S;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  // the pattern step of chunk k: 0 for a chunk of zeros
-  auto dstep = [&](uint64_t k) -> long long {
-    return sums[k] == 0.0 ? 0ll : (long long)(dbits(ends[k]) - dbits(guess[k]));
-  };
+  const int tid = threadIdx.x;
   if (tid == 0) s_S = 0.0;
   __syncthreads();
   unsigned long long nrerun = 0;
   uint64_t k0 = 0;
-  // the window doubles after a window without a rerun and starts small
-  // again after one (reruns cluster where the sum is small)
-  uint64_t win = kSeqFixWin;
   while (k0 < nch) {
     const double S0 = s_S;
-    const uint64_t k1 = nch - k0 < win ? nch : k0 + win;
-    const uint64_t per = (k1 - k0 + kSeqFixThreads - 1) / kSeqFixThreads;
-    const uint64_t a0 = k0 + (uint64_t)tid * per;
-    const uint64_t a = a0 < k1 ? a0 : k1, b = a + per < k1 ? a + per : k1;
-    long long loc = 0;
-    for (uint64_t k = a; k < b; ++k) loc += dstep(k);
-    long long incl = loc;
+    const uint64_t kb = k0 + (uint64_t)tid * kSeqFixPer;
+    double g[kSeqFixPer], e[kSeqFixPer];
+    uint8_t fl[kSeqFixPer];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const long long y = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += y;
+    for (int j = 0; j < kSeqFixPer; ++j) {
+      const uint64_t k = kb + j;
+      const bool in = k < nch;
+      g[j] = in ? guess[k] : 0.0;
+      e[j] = in ? ends[k] : 0.0;
+      fl[j] = in ? (uint8_t)(4 | (sums[k] == 0.0 ? 2 : 0) | (tie[k] ? 1 : 0)) : 0;
     }
-    if (lane == 63) wsum[wid] = incl;
-    if (tid == 0) s_bad = ~0ull;
-    __syncthreads();
-    long long base = incl - loc;
-    for (int w = 0; w < wid; ++w) base += wsum[w];
-    // the first chunk whose shift is not exact
-    uint64_t bS = dbits(S0) + (uint64_t)base;
-    for (uint64_t k = a; k < b; ++k) {
-      if (!seq_shift_ok(__longlong_as_double((long long)bS), guess[k], ends[k], tie[k] != 0,
-                        sums[k] == 0.0)) {
-        atomicMin(&s_bad, (unsigned long long)k);
-        break;
-      }
-      bS += (uint64_t)dstep(k);
-    }
-    __syncthreads();
-    const uint64_t bad = s_bad;
-    bS = dbits(S0) + (uint64_t)base;
-    for (uint64_t k = a; k < b && k <= bad; ++k) {
-      start[k] = __longlong_as_double((long long)bS);
-      if (k == bad) {
-        s_S = __longlong_as_double((long long)bS);
-        break;
-      }
-      bS += (uint64_t)dstep(k);
-    }
-    if (bad == ~0ull && tid == kSeqFixThreads - 1) {
-      long long tot = 0;
-      for (int w = 0; w < kSeqFixThreads / 64; ++w) tot += wsum[w];
-      s_S = __longlong_as_double((long long)(dbits(S0) + (uint64_t)tot));
-    }
-    __syncthreads();
-    if (bad == ~0ull) {
-      k0 = k1;
-      win = win < kSeqFixWinMax ? 2 * win : win;
+    const uint32_t bad = seq_walk_round<kSeqFixPer>(
+        S0, g, e, fl, ws, &s_bad, &s_S, [&](int j, double v) { start[kb + j] = v; });
+    if (bad == kSeqNone) {
+      k0 += kSeqFixRound;
       continue;
     }
-    win = kSeqFixWin;
-    // run chunk `bad` again from its exact start
-    const uint64_t i0 = bad * kSeqChunk, i1 = i0 + kSeqChunk < n ? i0 + kSeqChunk : n;
-    for (uint64_t i = i0 + tid; i < i1; i += kSeqFixThreads)
-      terms[i - i0] = seq_term<T>(psi, i, diag_step);
+    // chunk k0 + bad again, from its exact start s_S, in sub-chunks
+    const uint64_t c = k0 + bad;
+    const uint64_t i0 = c * kSeqChunk;
+    for (uint32_t L = tid; L < kSeqChunk; L += kSeqFixThreads)
+      terms[L] = i0 + L < n ? seq_term<T>(psi, i0 + L, diag_step) : 0.0;
     __syncthreads();
-    if (tid == 0) {
-      double S = s_S;
-      for (uint64_t i = 0; i < i1 - i0; ++i) S = S + terms[i];
-      s_S = S;
+    const double Sc = s_S;
+    double u = 0.0;
+#pragma unroll
+    for (int q = 0; q < kSeqSub; ++q) u = u + terms[tid * kSeqSub + q];
+    double utot;
+    const double gs = Sc + seq_block_scan(u, wsd, &utot);
+    bool t = false;
+    double es = gs;
+#pragma unroll
+    for (int q = 0; q < kSeqSub; ++q) es = seq_add_tie(es, terms[tid * kSeqSub + q], t);
+    uint32_t j0 = 0;
+    for (;;) {
+      const double gj[1] = {gs}, ej[1] = {es};
+      const uint8_t fj[1] = {(uint8_t)((uint32_t)tid >= j0 ? 4 | (u == 0.0 ? 2 : 0) | (t ? 1 : 0)
+                                                           : 0)};
+      const uint32_t sb =
+          seq_walk_round<1>(s_S, gj, ej, fj, ws, &s_bad, &s_S, [](int, double) {});
+      if (sb == kSeqNone) break;
+      if (tid == 0) {
+        double S = s_S;
+#pragma unroll
+        for (int q = 0; q < kSeqSub; ++q) S = S + terms[sb * kSeqSub + q];
+        s_S = S;
+      }
+      __syncthreads();
+      j0 = sb + 1;
+      if (j0 >= (uint32_t)kSeqFixThreads) break;
     }
     ++nrerun;
-    __syncthreads();
-    k0 = bad + 1;
+    k0 = c + 1;
   }
   if (tid == 0) *reruns = nrerun;
 }
@@ -859,7 +925,10 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   const size_t off_c = off_o + up(nd * sizeof(uint64_t));
   const size_t off_levels = off_c + up((leaf_ends ? 1 : n) * sizeof(double));
   // the parallel sequential prefix keeps per-chunk records after c
-  const uint64_t nch_seq = (!tree && n >= kSeqParallelMin) ? (n + kSeqChunk - 1) / kSeqChunk : 0;
+  // (probabilities |ψ_i|² only: the chunk rule needs a monotone chain, and
+  // the diagonal of a density matrix may hold small negative values)
+  const bool seq_par = !tree && n >= kSeqParallelMin && !diag_step;
+  const uint64_t nch_seq = seq_par ? (n + kSeqChunk - 1) / kSeqChunk : 0;
   const size_t seq_bytes = nch_seq ? up(nch_seq * (4 * sizeof(double) + 1) + 16) : 0;
   int rc = ensure_scratch(ctx, off_levels + (leaf_ends ? 0 : tree_bytes) + seq_bytes);
   if (rc) return rc;
@@ -870,7 +939,7 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
   if ((rc = stage_draws(ctx, r_host, nd, rd))) return rc;
 
   if (!tree) {
-    if (n < kSeqParallelMin) {
+    if (!seq_par) {
       hipLaunchKernelGGL(k_seq_prefix<T>, dim3(1), dim3(1), 0, ctx->stream, psi, n, c, diag_step);
       SMOQ_CHECK_LAUNCH("k_seq_prefix");
     } else {
@@ -884,7 +953,8 @@ static int sample_t(smoq_state* s, const double* r_host, int64_t nd, int sem, ui
       const unsigned gb = (unsigned)((nch + kSeqStageBlock - 1) / kSeqStageBlock);
       hipLaunchKernelGGL((k_seq_chunks<T, 0>), dim3(gb), dim3(kSeqStageBlock), 0, ctx->stream,
                          psi, n, diag_step, nch, nullptr, sums, nullptr);
-      hipLaunchKernelGGL(k_seq_guess, dim3(1), dim3(1024), 0, ctx->stream, sums, nch, guess);
+      hipLaunchKernelGGL(k_seq_guess, dim3(1), dim3(kSeqFixThreads), 0, ctx->stream, sums, nch,
+                         guess);
       hipLaunchKernelGGL((k_seq_chunks<T, 1>), dim3(gb), dim3(kSeqStageBlock), 0, ctx->stream,
                          psi, n, diag_step, nch, guess, ends, tie);
       hipLaunchKernelGGL(k_seq_fix<T>, dim3(1), dim3(kSeqFixThreads), 0, ctx->stream, psi, n,
