@@ -632,12 +632,14 @@ int fused_adjoint_batch(smoq_state* s, const smoq_op* ops, int n_ops,
     int e = lower_tape(ops, n_ops, th, n_params, n, gs);
     if (!e) {
       plan_program(gs, n, ftile, ctx->low_qubits, rowbits, fp[b], ctx->pass_max_gates, fswaps);
+      fp[b].per_item = true;
       jit_structure_key(fp[b], s->dtype, fkey[b]);
       e = reverse_gates(steps, th, n, rg, &slots[b]);
     }
     if (!e) {
       plan_program(rg, nW, pass_tile(ctx, nW, B, rg.size()), ctx->low_qubits, rowbits, rp[b],
                    ctx->pass_max_gates, ctx->lane_swaps != 0);
+      rp[b].per_item = true;
       jit_structure_key(rp[b], s->dtype, rkey[b]);
     }
     errs[b] = e;

@@ -105,8 +105,8 @@ static int launch_program_t(smoq_state* s, const Program& prog, const ProgLayout
     for (const GateDev& g : prog.gates)
       if (g.kind == GK_OVL)
         return set_error(SMOQ_EINVAL, "fused pass: overlap gates need compiled passes");
-  if (!jit && item_bytes)
-    return set_error(SMOQ_EINVAL, "fused pass: per-item programs need compiled passes");
+  if (item_bytes && (!jit || !prog.per_item))
+    return set_error(SMOQ_EINVAL, "fused pass: per-item programs need passes compiled for them");
   unsigned long long ib = item_bytes;
   VT<T>* psi = (VT<T>*)s->d;
   int n = s->n;

@@ -266,14 +266,17 @@ static void jit_emit(const Program& prog, int dtype, O& o) {
         << g0 << ");\n    for (uint32_t w = threadIdx.x * 32u; w < " << nw
         << "u; w += blockDim.x * 32u) warm ^= gw[w];\n  }\n";
     }
-    o << "  pass_load<T>(lds, psi, n, pass, f);\n"
-      // batches whose items carry their own gate records and tables (same
-      // structure, other angles): item_bytes apart
-      << "  if (item_bytes) {\n"
-      << "    const unsigned long long item = f.tile >> pass->n_nonlocal;\n"
-      << "    gates = (const GateDev*)((const char*)gates + item * item_bytes);\n"
-      << "    tabs = (const double2*)((const char*)tabs + item * item_bytes);\n"
-      << "  }\n";
+    o << "  pass_load<T>(lds, psi, n, pass, f);\n";
+    // batches whose items carry their own gate records and tables (same
+    // structure, other angles): item_bytes apart.  Only in programs built
+    // for it, so that the other passes address their records from the
+    // kernel arguments alone.
+    if (prog.per_item)
+      o << "  {\n"
+        << "    const unsigned long long item = f.tile >> pass->n_nonlocal;\n"
+        << "    gates = (const GateDev*)((const char*)gates + item * item_bytes);\n"
+        << "    tabs = (const double2*)((const char*)tabs + item * item_bytes);\n"
+        << "  }\n";
     for (int s = pd.s0; s < pd.s1; ++s) {
       const StageDev& st = prog.stages[s];
       const bool seg_end = s + 1 == pd.s1 || !prog.stages[s + 1].cont;

@@ -723,6 +723,7 @@ void plan_program(const std::vector<IGate>& gates, int n, int tile_qubits, int l
   prog.stages.clear();
   prog.gates.clear();
   prog.tabs.clear();
+  prog.per_item = false;
   const int k = std::min(tile_qubits, n);
   // keep room for two non-low targets so that every gate fits an empty pass
   const int m = (k == n) ? k : std::min(low_qubits, k - 2);

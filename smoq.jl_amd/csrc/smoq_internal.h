@@ -67,6 +67,9 @@ struct Program {
   std::vector<StageDev> stages;
   std::vector<GateDev> gates;
   std::vector<double> tabs;   // interleaved complex phase tables
+  // compiled passes read their gate records and tables at an offset per
+  // batch item (launched with item_bytes > 0)
+  bool per_item = false;
 };
 
 // Builds the pass/stage program for a gate list on an n-qubit state.
