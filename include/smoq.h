@@ -522,6 +522,11 @@ int smoq_tape_energy(smoq_state* s, const smoq_op* ops, int n_ops, const double*
  * W = (ψ, Hψ) interleaved, the reverse sweep with its overlaps, the
  * per-parameter sums, all by one workgroup that owns the item (the items
  * are dealt to the workgroups largest first); one launch runs every item.
+ * Large items go instead, by a cost model, to the whole-device sweep
+ * (smoq_adjoint_gradient) on a second stream while the launch runs; large
+ * items with equal tape and Hamiltonian (compared by contents) run there as
+ * one batch of parameter sets.  SMOQ_VQE_ROUTE=one_launch / sweep forces
+ * either path.
  * Outputs as smoq_adjoint_gradient for one item: *energy and grad[p].
  * n_qubits 3 .. 30; the states live in the context's work buffers. */
 typedef struct smoq_vqe_item {
